@@ -1,0 +1,370 @@
+"""FP64 CPU oracle for the sTDA / sTD-DFT core (TEST INFRASTRUCTURE ONLY).
+
+Imported only by tests/, by ``__graft_entry__.smoke()`` and by the
+cpu_baseline leg of ``bench.py``: it is the checker the HIP path is compared
+against, never part of the product path.
+
+It restates the reference's hot path (grimme-lab/stda std2 v2.0) in FP64:
+the loop nests live in ``stda_oracle.c`` (compiled to ``liboracle.so``) and
+the LAPACK steps are the numpy/scipy equivalents of the reference's calls:
+
+=========================  ==============================================
+reference                  here
+=========================  ==============================================
+ssymm / sgemm (stda.f)     numpy matmul
+ssyevr 'V','V' (0, thr]    numpy.linalg.eigh + window (stda.f:1134-1150)
+sspevd in smatpow          numpy.linalg.eigh (srpapack.f:201-256)
+ssyevr on the RPA form     numpy.linalg.eigh + window (srpapack.f:56-92)
+ssptrf/ssptri/ssptrs       numpy.linalg.inv / solve (linear_response.f)
+=========================  ==============================================
+
+Every function cites the reference lines it follows.  Pinning: the water
+fixtures of the reference's tests (EXCI_TEST*, 2PA_TEST*) and the
+invariants listed in tests/test_oracle.py.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARTREE_EV = 27.211385050
+HARTREE_EV_PRINT = 27.21139
+ROT = 235.7220
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    so = os.path.join(HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return so
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        so = build()
+        L = ctypes.CDLL(so)
+        P = ctypes.c_void_p
+        i32, i64, f64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.orc_gamma.argtypes = [i32, P, f64, f64, f64, P, P]
+        L.orc_charges_diag.argtypes = [i32, i32, i32, i32, P, P, P, P]
+        L.orc_charges_ia.argtypes = [i32, i32, i32, i32, P, P, P]
+        L.orc_charges_packed.argtypes = [i32, i32, i32, i32, P, P, P, P]
+        L.orc_screen.argtypes = [i32, i32, i32, P, P, P, P, f64, f64, f64, i32, f64, f64, f64,
+                                 i64, P, P, P, P, P, P]
+        L.orc_ptselect.argtypes = [i64, i32, i32, i32, i64, i64, P, P, f64, P, P, P, P, P, P,
+                                   f64, P]
+        L.orc_ptselect.restype = i64
+        L.orc_rtdamat.argtypes = [i64, i32, i32, i32, i64, P, f64, P, P, P, P, P, P]
+        L.orc_rrpamat.argtypes = [i64, i32, i32, i32, i64, P, f64, f64, P, P, P, P, P, P, P]
+        L.orc_rtdacorr.argtypes = [i64, i32, i32, i32, i64, P, f64, f64, P, P, P]
+        L.orc_num_threads.restype = i32
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class OracleResult:
+    nex: int
+    npt: int
+    nci: int
+    iconf: np.ndarray          # (nci, 2) 1-based (occ, absolute virtual MO)
+    ed: np.ndarray             # (nci,) PT2-corrected diagonal
+    omega: np.ndarray          # (nroot,) Hartree
+    X: np.ndarray              # (nci, nroot)
+    Y: np.ndarray = None       # (nci, nroot) RPA only
+    props: np.ndarray = None   # (nroot, 4): fL, fV, RL, RV (or Rv(corr))
+    alpha_sos: np.ndarray = None
+    alpha_resp: list = field(default_factory=list)   # [(omega, 3x3), ...]
+    A: np.ndarray = None
+    apb: np.ndarray = None
+    amb: np.ndarray = None
+    bhalf: np.ndarray = None
+    timings: dict = field(default_factory=dict)
+
+
+def unpack_sym(p: np.ndarray, n: int) -> np.ndarray:
+    """Full symmetric matrix from the packed upper triangle lin(i,j)."""
+    M = np.zeros((n, n))
+    iu = np.triu_indices(n)               # (row j <= col i)
+    idx = iu[0] + iu[1] * (iu[1] + 1) // 2
+    M[iu] = p[idx]
+    M[(iu[1], iu[0])] = p[idx]
+    return M
+
+
+def gamma(ci, ax, alphak, betaj):
+    """stda.f:509-532 through orc_gamma."""
+    L = lib()
+    n = ci.ncent
+    gj = np.zeros((n, n), order="F")
+    gk = np.zeros((n, n), order="F")
+    xyz = np.ascontiguousarray(np.asfortranarray(ci.xyz).T)   # (ncent,4) == xyz(4,ncent) col-major
+    with np.errstate(divide="ignore"):
+        L.orc_gamma(n, _p(xyz), ax, alphak, betaj, _p(gj), _p(gk))
+    return gj, gk
+
+
+def default_params(ax, alphak=-100.0, betaj=-100.0):
+    """stda.f:485-500."""
+    if betaj < -99.0:
+        betaj = 0.20 + 1.830 * ax
+    if alphak < -99.0:
+        alphak = 1.420 + 0.480 * ax
+    return alphak, betaj
+
+
+def intermediates(ci, ax, alphak=-100.0, betaj=-100.0):
+    """gamma, q and p arrays of stda.f:465-763 (all FP64)."""
+    L = lib()
+    alphak, betaj = default_params(ax, alphak, betaj)
+    gj, gk = gamma(ci, ax, alphak, betaj)
+    n, nao, no, nv = ci.ncent, ci.nao, ci.no, ci.nv
+    clow = np.asfortranarray(ci.clow)
+    iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+    qii = np.zeros((n, no), order="F")
+    qaa = np.zeros((n, nv), order="F")
+    L.orc_charges_diag(n, nao, no, nv, _p(iaoat), _p(clow), _p(qii), _p(qaa))
+    pii = gj @ qii
+    jdiag = np.asfortranarray(qaa.T @ pii)          # (nv, no): sgemm of stda.f:564
+    qia = np.zeros((n, no * nv), order="F")
+    L.orc_charges_ia(n, nao, no, nv, _p(iaoat), _p(clow), _p(qia))
+    pia = np.asfortranarray(gk @ qia)
+    qij = np.zeros((n, no * (no + 1) // 2), order="F")
+    qab = np.zeros((n, nv * (nv + 1) // 2), order="F")
+    L.orc_charges_packed(n, nao, no, nv, _p(iaoat), _p(clow), _p(qij), _p(qab))
+    pij = np.asfortranarray(gj @ qij)
+    return dict(gamj=gj, gamk=gk, qii=qii, qaa=qaa, jdiag=jdiag, qia=qia, pia=pia,
+                qij=qij, qab=qab, pij=pij, alphak=alphak, betaj=betaj)
+
+
+def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=None):
+    """Screening (stda.f:593-668) then PT2 selection (stda.f:1773-1863)."""
+    L = lib()
+    no, nv, n = ci.no, ci.nv, ci.ncent
+    ak = 0.0 if triplet else 2.0
+    thr = thr_ev / HARTREE_EV
+    fthr = 1.7976931348623157e308 if fthr_ev is None else fthr_ev / HARTREE_EV
+    mx = no * nv
+    iconf = np.zeros(2 * mx, dtype=np.int32)
+    kconf = np.zeros(2 * mx, dtype=np.int32)
+    ed = np.zeros(mx)
+    edpt = np.zeros(mx)
+    nex = ctypes.c_int64()
+    nexpt = ctypes.c_int64()
+    ks = kshift or (0.5, 0.1, 4.0)
+    L.orc_screen(n, no, nv, _p(np.ascontiguousarray(ci.epsi)), _p(inter["jdiag"]),
+                 _p(inter["pia"]), _p(inter["qia"]), ak, thr, fthr, 1 if kshift else 0,
+                 ks[0], ks[1], ks[2], mx, _p(iconf), _p(ed), _p(kconf), _p(edpt),
+                 ctypes.byref(nex), ctypes.byref(nexpt))
+    nex, nexpt = nex.value, nexpt.value
+    if nex < 1:
+        raise RuntimeError("no CSF, increase energy threshold (-e option)")
+    incl = np.zeros(max(nexpt, 1), dtype=np.uint8)
+    new = L.orc_ptselect(nex, n, no, nv, nexpt, mx, _p(iconf), _p(kconf), ak, _p(ed),
+                         _p(edpt), _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]),
+                         _p(inter["qab"]), thrp, _p(incl))
+    nci = nex + new
+    ic = np.stack([iconf[:nci], iconf[mx:mx + nci]], axis=1)
+    return dict(nex=nex, npt=new, nci=nci, nexpt=nexpt, iconf_raw=iconf, iconf=ic,
+                ed=ed[:nci].copy(), ed_raw=ed, mx=mx, ak=ak, thr=thr,
+                kconf=np.stack([kconf[:nexpt], kconf[mx:mx + nexpt]], axis=1),
+                edpt=edpt[:nexpt].copy(), incl=incl[:nexpt].astype(bool))
+
+
+def build_a(ci, inter, sel):
+    L = lib()
+    nci = sel["nci"]
+    A = np.zeros((nci, nci), order="F")
+    ed = np.zeros(sel["mx"])
+    ed[:nci] = sel["ed"]
+    L.orc_rtdamat(nci, ci.ncent, ci.no, ci.nv, sel["mx"], _p(sel["iconf_raw"]), sel["ak"],
+                  _p(ed), _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]),
+                  _p(inter["qab"]), _p(A))
+    return A
+
+
+def build_apb_amb(ci, inter, sel, ax):
+    L = lib()
+    nci = sel["nci"]
+    npk = nci * (nci + 1) // 2
+    apb = np.zeros(npk)
+    amb = np.zeros(npk)
+    ed = np.zeros(sel["mx"])
+    ed[:nci] = sel["ed"]
+    L.orc_rrpamat(nci, ci.ncent, ci.no, ci.nv, sel["mx"], _p(sel["iconf_raw"]), sel["ak"],
+                  ax, _p(ed), _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]),
+                  _p(inter["qab"]), _p(apb), _p(amb))
+    return apb, amb
+
+
+def build_bhalf(ci, inter, sel, ax):
+    L = lib()
+    nci = sel["nci"]
+    b = np.zeros(nci * (nci + 1) // 2)
+    L.orc_rtdacorr(nci, ci.ncent, ci.no, ci.nv, sel["mx"], _p(sel["iconf_raw"]), sel["ak"],
+                   ax, _p(inter["pia"]), _p(inter["qia"]), _p(b))
+    return b
+
+
+def csf_moments(ci, iconf):
+    """Moments of each CSF: M(io, iv) from the packed arrays (stda.f:1229-1231)."""
+    io = iconf[:, 0].astype(np.int64)
+    iv = iconf[:, 1].astype(np.int64)
+    a = np.maximum(io, iv)
+    b = np.minimum(io, iv)
+    ij = b + a * (a - 1) // 2 - 1
+    return ci.moments[:, ij]       # (9, nci): xl yl zl xm ym zm xv yv zv
+
+
+def properties_tda(ci, iconf, omega, U, ak, Vx=None):
+    """Transition moments, fL, fV, RL, RV (stda.f:1207-1305)."""
+    mom = csf_moments(ci, iconf)
+    l = (mom[0:3] @ U) * np.sqrt(ak)
+    m = (mom[3:6] @ U) * np.sqrt(ak)
+    v = (mom[6:9] @ U) * np.sqrt(ak)
+    if Vx is not None:       # RPA: lengths with X+Y, velocity/magnetic with X-Y
+        m = (mom[3:6] @ Vx) * np.sqrt(ak)
+        v = (mom[6:9] @ Vx) * np.sqrt(ak)
+    de = omega
+    ef = 1.0 / (de + 1.0e-8)
+    fl = de * (2.0 / 3.0) * (l * l).sum(0)
+    fv = ef * (2.0 / 3.0) * (v * v).sum(0)
+    rl = -ROT * (l * m).sum(0)
+    rv = -ROT * (v * m).sum(0) * ef
+    a = np.zeros((3, 3))
+    for k in range(len(de)):
+        a += np.outer(l[:, k], l[:, k]) * ef[k]
+    return np.stack([fl, fv, rl, rv], axis=1), 2.0 * a
+
+
+def properties_velcorr(ci, iconf, omega, X, Bhalf, ak):
+    """apbtrafo (apbtrafo.f:23-298): fL, fV, RL and Rv(corr)."""
+    mom = csf_moments(ci, iconf)
+    ef = 1.0 / (omega + 1.0e-8)
+    Xn = (Bhalf @ X) * ef[None, :]
+    l = mom[0:3] @ X
+    v = mom[6:9] @ X
+    m = mom[3:6] @ X
+    v2 = mom[6:9] @ Xn
+    m2 = mom[3:6] @ Xn
+    p23 = ak * 2.0 / 3.0
+    hilf = 1.0 - np.exp(-150.0 * omega * omega)
+    efmod = 1.0 / (omega ** hilf + 1.0e-8)
+    fl = (l * l).sum(0) * p23 * omega
+    fv = p23 * ef * (v * v).sum(0)
+    rl = -ROT * (l * m).sum(0) * ak
+    rv = -ROT * ((v * m).sum(0) + (v2 * m).sum(0) + (v * m2).sum(0)) * ak * efmod
+    return np.stack([fl, fv, rl, rv], axis=1)
+
+
+def window_eigh(M, lo, hi):
+    """ssyevr RANGE='V': eigenpairs with lo < e <= hi, ascending."""
+    e, V = np.linalg.eigh(M)
+    keep = (e > lo) & (e <= hi)
+    return e[keep], V[:, keep]
+
+
+def run(ci, thr_ev, ax, thrp=1e-4, alphak=-100.0, betaj=-100.0, fthr_ev=None,
+        triplet=False, rpa=False, velcorr=None, nroots=None, kshift=None,
+        resp_omegas=None, keep_matrices=False):
+    """Whole sTDA / sTD-DFT core in FP64 (stda.f:454-1471, srpapack.f, linear_response.f).
+
+    ``nroots=None`` reproduces the reference's window (all roots in
+    (0, Ethr]); an integer keeps the lowest ``nroots`` roots of the same
+    matrix instead.  ``resp_omegas`` (Hartree) runs the linear-response
+    polarizability instead of the eigenvalue problem (lresp, linear_response.f:23-192).
+    """
+    import time
+    t0 = time.perf_counter()
+    if velcorr is None:
+        velcorr = not rpa                        # main.f:102, 662
+    inter = intermediates(ci, ax, alphak, betaj)
+    t1 = time.perf_counter()
+    sel = select(ci, inter, thr_ev, thrp, fthr_ev, triplet, kshift)
+    t2 = time.perf_counter()
+    nci, ak, thr = sel["nci"], sel["ak"], sel["thr"]
+    res = OracleResult(nex=sel["nex"], npt=sel["npt"], nci=nci, iconf=sel["iconf"],
+                       ed=sel["ed"], omega=None, X=None)
+    res.timings = {"intermediates": t1 - t0, "select": t2 - t1}
+    if not rpa:
+        A = build_a(ci, inter, sel)
+        if nroots is None:
+            w, U = window_eigh(A, 0.0, thr)
+        else:
+            w, U = np.linalg.eigh(A)
+            w, U = w[:nroots], U[:, :nroots]
+        res.omega, res.X = w, U
+        if velcorr:
+            Bp = build_bhalf(ci, inter, sel, ax)
+            B = unpack_sym(Bp, nci)
+            res.props = properties_velcorr(ci, sel["iconf"], w, U, B, ak)
+            _, res.alpha_sos = properties_tda(ci, sel["iconf"], w, U, ak)
+            if keep_matrices:
+                res.bhalf = B
+        else:
+            res.props, res.alpha_sos = properties_tda(ci, sel["iconf"], w, U, ak)
+        if keep_matrices:
+            res.A = A
+    else:
+        apbp, ambp = build_apb_amb(ci, inter, sel, ax)
+        apb = unpack_sym(apbp, nci)
+        amb = unpack_sym(ambp, nci)
+        if keep_matrices:
+            res.apb, res.amb = apb, amb
+        if resp_omegas is not None:
+            res.alpha_resp = lresp(ci, sel["iconf"], apb, amb, resp_omegas)
+        else:
+            e, V = np.linalg.eigh(amb)             # smatpow, srpapack.f:201-256
+            if e[0] < 0:
+                raise RuntimeError("matrix power impossible")
+            sq = (V * np.sqrt(e)) @ V.T
+            M = sq @ apb @ sq                      # srpapack.f:56-67
+            w2, Z = window_eigh(M, 0.0, thr * thr) if nroots is None else np.linalg.eigh(M)
+            if nroots is not None:
+                w2, Z = w2[:nroots], Z[:, :nroots]
+            w = np.sqrt(w2)
+            xpy = (sq @ Z) / np.sqrt(w)[None, :]   # srpapack.f:108-122
+            xmy = (apb @ xpy) / w[None, :]         # srpapack.f:125-153
+            X = 0.5 * (xpy + xmy)
+            Y = xpy - X
+            res.omega, res.X, res.Y = w, X, Y
+            res.props, res.alpha_sos = properties_tda(ci, sel["iconf"], w, xpy, ak, Vx=xmy)
+    res.timings["solve"] = time.perf_counter() - t2
+    return res
+
+
+def lresp(ci, iconf, apb, amb, omegas):
+    """alpha(omega) = -2 mu^T (X+Y), ((A+B) - w^2 (A-B)^-1)(X+Y) = -2 mu
+    (linear_response.f:90-172)."""
+    mom = csf_moments(ci, iconf)
+    mu = -2.0 * mom[0:3].T                         # (nci, 3): mu_x = -2 xl
+    inv_amb = np.linalg.inv(amb)
+    out = []
+    for w in omegas:
+        Mw = apb - w * w * inv_amb
+        xpy = np.linalg.solve(Mw, mu)
+        a = -2.0 * mom[0:3] @ xpy                  # (3,3): a[r, c] = -2 sum l_r (X+Y)_c
+        out.append((w, a))
+    return out
+
+
+def tda_dat_rows(res, thr_ev):
+    """DATXY rows as the reference writes them (F10.4, 4f13.6 with 27.21139)."""
+    rows = []
+    for k in range(len(res.omega)):
+        rows.append((k + 1, round(res.omega[k] * HARTREE_EV_PRINT, 4),
+                     *[round(float(x), 6) for x in res.props[k]]))
+    return rows

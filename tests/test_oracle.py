@@ -1,0 +1,185 @@
+"""Pins the FP64 oracle: reference fixtures first, then invariants that a
+plausible mistake (dropped term, wrong sign or index, transposed operand)
+would break."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_datxy, read_table
+import oracle as O
+from stda_amd.prep import prepare
+
+WATER_CASES = {  # tests/run_tests.sh of the reference
+    1: dict(ax=1.0, thr=20.0, alphak=-100.0, betaj=-100.0),
+    2: dict(ax=0.5, thr=30.0, alphak=-100.0, betaj=-100.0),
+    3: dict(ax=1.0, thr=30.0, alphak=3.0, betaj=1.0),
+}
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_exci_fixture(water, case):
+    """EXCI_TEST*: sTDA with velocity correction; energies (F10.4) and fL,
+    fV, RL, Rv(corr) (f13.6).  The FP64 oracle must agree to the printed
+    precision plus the FP32 noise of the reference binary (2e-6).  Water is
+    achiral, so RL and Rv(corr) vanish exactly; the reference prints FP32
+    round-off of up to 2.5e-5 in those columns, the bound used for them."""
+    p = WATER_CASES[case]
+    ci = prepare(water, p["thr"], p["ax"])
+    r = O.run(ci, p["thr"], p["ax"], alphak=p["alphak"], betaj=p["betaj"])
+    exp = read_datxy(os.path.join(GOLDEN, "water", f"EXCI_TEST{case}.dat"))
+    assert len(r.omega) == len(exp)
+    for k, row in enumerate(exp):
+        assert abs(r.omega[k] * O.HARTREE_EV_PRINT - row[1]) <= 0.5e-4 + 1e-6
+        for c in range(2):
+            assert abs(r.props[k, c] - row[2 + c]) <= 0.5e-6 + 2e-6, (k, c)
+        for c in range(2, 4):
+            assert abs(r.props[k, c]) < 1e-10 and abs(row[2 + c]) <= 2.5e-5
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_rpa_fixture(water, case):
+    """2PA_TEST*: sTD-DFT (RPA) roots printed F7.3."""
+    p = WATER_CASES[case]
+    ci = prepare(water, p["thr"], p["ax"])
+    r = O.run(ci, p["thr"], p["ax"], alphak=p["alphak"], betaj=p["betaj"], rpa=True)
+    exp = read_table(os.path.join(GOLDEN, "water", f"2PA_TEST{case}.txt"))
+    for k, row in enumerate(exp):   # -2PA 3: the three lowest roots
+        assert abs(r.omega[k] * O.HARTREE_EV_PRINT - row[1]) <= 0.5e-3 + 1e-6
+
+
+def test_water_velcorr_is_not_plain_properties(water):
+    """fV of the velocity-corrected table equals the uncorrected fV; only Rv changes."""
+    p = WATER_CASES[2]
+    ci = prepare(water, p["thr"], p["ax"])
+    a = O.run(ci, p["thr"], p["ax"], velcorr=True)
+    b = O.run(ci, p["thr"], p["ax"], velcorr=False)
+    np.testing.assert_allclose(a.props[:, :3], b.props[:, :3], atol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def polyene_inputs():
+    from stda_amd import synth
+    mol, ints = synth.polyene()
+    return prepare(mol, 7.0, 0.25, ints)
+
+
+def test_charges_invariants(polyene_inputs):
+    ci = polyene_inputs
+    it = O.intermediates(ci, 0.25)
+    # Loewdin orbitals are orthonormal: sum_A q^A_pq = delta_pq
+    np.testing.assert_allclose(it["qii"].sum(0), 1.0, atol=1e-12)
+    np.testing.assert_allclose(it["qaa"].sum(0), 1.0, atol=1e-12)
+    np.testing.assert_allclose(it["qia"].sum(0), 0.0, atol=1e-12)
+    # packed q_ij: diagonal entries equal q_ii
+    no = ci.no
+    for i in range(no):
+        np.testing.assert_allclose(it["qij"][:, i + i * (i + 1) // 2], it["qii"][:, i], atol=1e-14)
+    # brute force of one q_ia from the AO->atom map
+    C = np.asarray(ci.clow)
+    i, a = 3, 5
+    q = np.zeros(ci.ncent)
+    for mu in range(ci.nao):
+        q[ci.iaoat[mu] - 1] += C[mu, i] * C[mu, no + a]
+    np.testing.assert_allclose(it["qia"][:, i * ci.nv + a], q, atol=1e-15)
+
+
+def test_gamma_limits(polyene_inputs):
+    ci = polyene_inputs
+    from stda_amd.constants import ETA
+    Z = ci.xyz[3].astype(int)
+    for ax in (0.25, 1.0):
+        al, be = O.default_params(ax)
+        gj, gk = O.gamma(ci, ax, al, be)
+        np.testing.assert_allclose(np.diag(gk), ETA[Z - 1], rtol=1e-14)
+        np.testing.assert_allclose(np.diag(gj), ax * ETA[Z - 1], rtol=1e-14)
+        assert np.allclose(gj, gj.T) and np.allclose(gk, gk.T)
+    gj, gk = O.gamma(ci, 0.0, *O.default_params(0.0))
+    assert np.all(gj == 0.0) and np.all(np.isfinite(gk))
+    # far apart: gamma -> 1/R
+    import copy
+    far = copy.deepcopy(ci)
+    far.xyz = np.asfortranarray(ci.xyz.copy())
+    far.xyz[0, 1] += 1.0e4
+    gj, gk = O.gamma(far, 0.5, 1.0, 1.0)
+    R = np.linalg.norm(far.xyz[:3, 0] - far.xyz[:3, 1])
+    assert abs(gk[0, 1] * R - 1.0) < 1e-3 and abs(gj[0, 1] * R - 1.0) < 1e-3
+
+
+def test_matrix_relations(polyene_inputs):
+    """A = 1/2 [(A+B) + (A-B)], A+B - A = 2 (0.5 B), symmetric A, diag = ed."""
+    ci = polyene_inputs
+    ax = 0.25
+    it = O.intermediates(ci, ax)
+    sel = O.select(ci, it, 7.0)
+    A = O.build_a(ci, it, sel)
+    apb, amb = (O.unpack_sym(x, sel["nci"]) for x in O.build_apb_amb(ci, it, sel, ax))
+    bh = O.unpack_sym(O.build_bhalf(ci, it, sel, ax), sel["nci"])
+    np.testing.assert_allclose(A, A.T, atol=0)
+    np.testing.assert_allclose(np.diag(A), sel["ed"], atol=0)
+    np.testing.assert_allclose(0.5 * (apb + amb), A, atol=1e-14)
+    np.testing.assert_allclose(apb - A, 2.0 * bh, atol=1e-14)
+    # J terms cancel in (A+B)-(A-B) = 2B, which has no gamma^J dependence
+    it2 = dict(it)
+    it2["pij"] = it["pij"] * 0.0
+    apb2, amb2 = (O.unpack_sym(x, sel["nci"]) for x in O.build_apb_amb(ci, it2, sel, ax))
+    np.testing.assert_allclose(apb - amb, apb2 - amb2, atol=1e-14)
+
+
+def test_pt2_brute_force_small(polyene_inputs):
+    """PT2 inclusion and energy lowering against a direct evaluation from
+    the explicit coupling matrix of the full pair space."""
+    ci = polyene_inputs
+    ax = 0.25
+    it = O.intermediates(ci, ax)
+    sel = O.select(ci, it, 7.0)
+    no, nv, n = ci.no, ci.nv, ci.ncent
+    qia, pia = it["qia"], it["pia"]
+    # full pair-space coupling ak (ia|jb) - (ij|ab) from full charge tensors
+    C = np.asarray(ci.clow)
+    at = ci.iaoat - 1
+    Q = np.zeros((n, no + nv, no + nv))
+    for mu in range(ci.nao):
+        Q[at[mu]] += np.outer(C[mu], C[mu])
+    P = np.einsum("AB,Bpq->Apq", it["gamj"], Q)
+    K = np.einsum("Ai,Aj->ij", pia, qia)                  # (ia|jb) over pair index
+    Jf = np.einsum("Aij,Aab->iajb", P[:, :no, :no], Q[:, no:, no:]).reshape(no * nv, no * nv)
+    Cpl = 2.0 * K - Jf
+    prim = [(i - 1) * nv + (a - no - 1) for i, a in sel["iconf"][:sel["nex"]]]
+    sec = [(i - 1) * nv + (a - no - 1) for i, a in sel["kconf"]]
+    ed_prim0 = sel["ed_raw"][:sel["nex"]]  # noqa: F841 (overwritten in place by PT2)
+    # recompute screening energies for the primaries independently
+    eps = ci.epsi
+    de_all = (eps[no:][None, :] - eps[:no][:, None]).ravel() - np.einsum(
+        "Aa,Ai->ia", it["qaa"], it["gamj"] @ it["qii"]).ravel() + 2.0 * np.einsum("Ak,Ak->k", pia, qia)
+    edp = de_all[prim]
+    pert = (Cpl[np.ix_(sec, prim)] ** 2) / (de_all[sec][:, None] - edp[None, :] + 1e-10)
+    incl = pert.sum(1) > 1e-4
+    np.testing.assert_array_equal(incl, sel["incl"])
+    lowered = edp - pert[~incl].sum(0)
+    np.testing.assert_allclose(sel["ed"][:sel["nex"]], lowered, rtol=1e-12, atol=1e-14)
+
+
+def test_lresp_closed_forms(polyene_inputs):
+    """Static alpha = 4 mu^T (A+B)^-1 mu and the RPA sum-over-states identity
+    alpha(w) = sum_n 2 w_n mu_n mu_n^T / (w_n^2 - w^2) over all roots."""
+    ci = polyene_inputs
+    ax = 0.25
+    r_all = O.run(ci, 7.0, ax, rpa=True, nroots=10 ** 9)       # every root
+    omegas = [0.0, 0.05, 0.1]
+    r = O.run(ci, 7.0, ax, rpa=True, resp_omegas=omegas, keep_matrices=True)
+    mom = O.csf_moments(ci, r.iconf)[0:3]
+    a0 = 4.0 * mom @ np.linalg.solve(r.apb, mom.T)
+    np.testing.assert_allclose(r.alpha_resp[0][1], a0, rtol=1e-10, atol=1e-12)
+    xpy = r_all.X + r_all.Y
+    mun = np.sqrt(2.0) * (mom @ xpy)                          # (3, nroots)
+    for w, a in r.alpha_resp:
+        sos = (mun * (2.0 * r_all.omega / (r_all.omega ** 2 - w * w))) @ mun.T
+        np.testing.assert_allclose(a, sos, rtol=1e-9, atol=1e-10)
+
+
+def test_rpa_normalisation(polyene_inputs):
+    ci = polyene_inputs
+    r = O.run(ci, 7.0, 0.25, rpa=True, nroots=5)
+    G = (r.X + r.Y).T @ (r.X - r.Y)
+    np.testing.assert_allclose(G, np.eye(5), atol=1e-10)
