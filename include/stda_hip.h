@@ -1,0 +1,157 @@
+/*
+ * stda_hip.h -- C ABI of the MI355X excited-state core (sTDA / sTD-DFT /
+ * linear-response polarizability).
+ *
+ * This is the drop-in boundary for the block of the reference's RKS driver
+ * `stda` (stda.f) that runs from the Loewdin orbitals to the spectrum:
+ * stda.f:454-1471, with the solvers of srpapack.f:32-197 and
+ * linear_response.f:23-369.  The caller (the Fortran driver through an
+ * ISO_C_BINDING module, or the Python host layer through ctypes) hands over
+ * the FP64 quantities the reference holds at stda.f:452 and receives FP64
+ * results.  Plain pointers and sizes only; all arrays are column-major
+ * (Fortran order) and owned by the caller unless stated otherwise.
+ *
+ * Every function returns an int status: STDA_HIP_OK (0) or one of the error
+ * codes below; stda_hip_error_string() gives the message the reference
+ * prints with `stop` in the same situation.
+ *
+ * Threading: a context is used by one host thread at a time.  Calls on
+ * different contexts may run concurrently; each context owns its own HIP
+ * stream and device memory.
+ */
+#ifndef STDA_HIP_H
+#define STDA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    STDA_HIP_OK = 0,
+    STDA_HIP_ERR_ARG = 1,          /* invalid argument / dimension */
+    STDA_HIP_ERR_NO_CSF = 2,       /* 'no CSF, increase energy threshold (-e option)' stda.f:316,644 */
+    STDA_HIP_ERR_DEVICE = 3,       /* HIP runtime or library failure */
+    STDA_HIP_ERR_NOMEM = 4,        /* device allocation failed */
+    STDA_HIP_ERR_STATE = 5,        /* call out of order (e.g. solve before setup) */
+    STDA_HIP_ERR_NOT_CONVERGED = 6,/* iterative solver hit its iteration limit */
+    STDA_HIP_ERR_AMB_NOT_PD = 7,   /* 'matrix power impossible' srpapack.f:229 (A-B not positive definite) */
+    STDA_HIP_ERR_NO_ROOTS = 8,     /* 'internal error in diag' stda.f:1152 / 'RPA diag failed' srpapack.f:92 */
+    STDA_HIP_ERR_TOO_LARGE = 9     /* problem exceeds a documented size limit of an entry point */
+};
+
+/* Moments: order of the 9 packed arrays (each moci*(moci+1)/2 FP64,
+ * element lin(i,j) = min + max*(max-1)/2 (1-based) holding M(min,max)):
+ * xl yl zl (dipole length), xm ym zm (angular momentum), xv yv zv (velocity),
+ * as produced by onetri+shrink at stda.f:346-412. */
+
+typedef struct {
+    int32_t ncent;          /* atoms */
+    int32_t nao;            /* AOs */
+    int32_t no, nv;         /* window occupied / virtual MOs (moci = no + nv) */
+    const double *clow;     /* clow(nao, moci): Loewdin MOs S^1/2 C, stda.f:436-452 */
+    const double *epsi;     /* epsi(moci): orbital energies (Hartree), ascending */
+    const int32_t *iaoat;   /* iaoat(nao): 1-based atom of each AO (intslvm.f:66-77) */
+    const double *xyz;      /* xyz(4, ncent): Bohr; xyz(4,:) = nuclear charge */
+    const double *moments;  /* 9 * moci*(moci+1)/2, see above; may be NULL (no properties) */
+} stda_hip_system;
+
+typedef struct {
+    double thr_ev;          /* Ethr: spectral range (eV), main.f:87 default 7 */
+    double thrp;            /* PT2 selection threshold (Hartree), main.f:93 default 1e-4 */
+    double fthr_ev;         /* max CSF selection range (eV); <= 0 means unlimited (main.f:86) */
+    double ax;              /* Fock-exchange fraction a_x */
+    double alphak, betaj;   /* gamma exponents; < -99 selects 1.42+0.48ax / 0.20+1.83ax (stda.f:490-500) */
+    int32_t triplet;        /* ak = 0 instead of 2 (stda.f:482-483) */
+    int32_t kshift;         /* K(ia,ia)-dependent diagonal shift (xTB mode, stda.f:1750-1768) */
+    double kshift_max_ev, kshift_width_ev, kshift_steep;   /* defaults 0.5, 0.1, 4.0 */
+} stda_hip_params;
+
+typedef struct stda_hip_ctx stda_hip_ctx;
+
+/* Create a context on HIP device `device` (-1: current device). */
+int stda_hip_create(stda_hip_ctx **ctx, int device);
+int stda_hip_destroy(stda_hip_ctx *ctx);
+const char *stda_hip_error_string(int status);
+/* Message of the last error raised on this context (more detail than the status). */
+const char *stda_hip_last_error(const stda_hip_ctx *ctx);
+
+/*
+ * Stage 1 (replaces stda.f:454-790): gamma^J/gamma^K (stda.f:509-532),
+ * Loewdin transition charges (lo12pop, stda.f:465-763), contracted
+ * intermediates (stda.f:555-589,742), CSF screening and ordering
+ * (stda.f:593-668) and PT2 selection (ptselect, stda.f:1773-1863).
+ * Device-resident afterwards: charges, gamma, CSF list, diagonal.
+ * Out: nci = primary + PT2-selected CSFs; nprim = primaries (nex);
+ * nsec = secondaries considered in PT2 (nexpt).
+ */
+int stda_hip_setup(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip_params *par,
+                   int64_t *nci, int64_t *nprim, int64_t *nsec);
+
+/* CSF list after selection: iconf(nci,2) 1-based (occupied MO, absolute
+ * virtual MO index no+1..moci) and ed(nci), the PT2-corrected diagonal
+ * (same order as the reference: primaries sorted by energy, then the
+ * PT2-selected CSFs in pair order). */
+int stda_hip_get_csf(stda_hip_ctx *ctx, int32_t *iconf, double *ed);
+
+/*
+ * TDA eigenproblem (replaces rtdamat + ssyevr, stda.f:1107-1150, and the
+ * property loop stda.f:1207-1378 / apbtrafo.f:23-298).
+ *   nroots > 0 : the nroots lowest roots (block Davidson, FP64);
+ *   nroots <= 0: every root in (0, Ethr], as ssyevr with RANGE='V'.
+ *   capacity   : columns available in x / rows in props (>= roots returned).
+ *   velcorr    : 1 = Rv(corr) from (A + 0.5 B) (default sTDA), 0 = plain RV.
+ *   tol        : Davidson residual-norm tolerance (<= 0: 1e-8).
+ * Out: nfound; omega(nfound) (Hartree); x(nci, nfound) eigenvectors (may be
+ * NULL); props(5, nfound) = fL, fV, RL, RV-or-Rv(corr), (unused 0);
+ * alpha_sos(6) = SOS alpha(0) xx,xy,yy,xz,yz,zz (stda.f:1272-1277,1465).
+ */
+int stda_hip_solve_tda(stda_hip_ctx *ctx, int32_t nroots, int32_t capacity, int32_t velcorr,
+                       double tol, int32_t *nfound, double *omega, double *x, double *props,
+                       double *alpha_sos);
+
+/*
+ * sTD-DFT (RPA) eigenproblem (replaces rrpamat + srpapack, stda.f:800-1005,
+ * srpapack.f:32-197): paired (A+B)/(A-B) Davidson without (A-B)^1/2.
+ * Out: omega, x = X, y = Y (nci, nfound) as srpapack returns them
+ * (X+Y = (A-B)^1/2 Z / sqrt(w), normalised so (X+Y)^T (X-Y) = 1); props as
+ * the RPA property loop (lengths with X+Y, velocity/magnetic with X-Y).
+ */
+int stda_hip_solve_rpa(stda_hip_ctx *ctx, int32_t nroots, int32_t capacity, double tol,
+                       int32_t *nfound, double *omega, double *x, double *y, double *props,
+                       double *alpha_sos);
+
+/*
+ * Linear-response polarizability (replaces lresp, linear_response.f:23-192,
+ * and the alpha part of lresp1): for each frequency w solve
+ * ((A+B) - w^2 (A-B)^-1)(X+Y) = -2 mu for the 3 Cartesian right-hand sides
+ * and return alpha(w)_rc = -2 mu_r^T (X+Y)_c as alpha(3,3,nfreq).
+ * xpy (nci,3,nfreq) may be NULL.
+ */
+int stda_hip_lresp(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double tol,
+                   double *alpha, double *xpy);
+
+/* Per-routine FP64 equivalents of the reference's matrix builders, for
+ * parity checks and small systems (explicit matrices, O(nci^2) memory):
+ *   rtdamat  (stda.f:2010-2056):     a(nci,nci) full;
+ *   rrpamat  (stda.f:1870-2004):     apb, amb packed lin(i,j), the A-B
+ *                                    diagonal un-rooted (resp mode);
+ *   rtdacorr (apbtrafo.f:580-637):   bhalf packed, 0.5 B. */
+int stda_hip_build_a(stda_hip_ctx *ctx, double *a);
+int stda_hip_build_apb_amb(stda_hip_ctx *ctx, double *apb, double *amb);
+int stda_hip_build_bhalf(stda_hip_ctx *ctx, double *bhalf);
+
+/* A (or A+B, A-B, 0.5B) applied to nvec vectors without forming the matrix:
+ * which = 0 A, 1 A+B, 2 A-B, 3 0.5B.  v, av: (nci, nvec). */
+int stda_hip_matvec(stda_hip_ctx *ctx, int32_t which, int32_t nvec, const double *v, double *av);
+
+/* Timing of the last setup/solve phases on this context, in milliseconds:
+ * [0] gamma+charges+intermediates, [1] screening+sort, [2] PT2,
+ * [3] eigensolve / response, [4] properties, [5] total. */
+int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STDA_HIP_H */

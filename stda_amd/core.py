@@ -1,0 +1,273 @@
+"""Python host layer over the C ABI of ``libstda_hip.so``.
+
+:class:`StdaHip` wraps one device context.  :func:`stda` mirrors the
+reference driver routine ``stda`` (stda.f:53-1471): from the molecule's
+boundary inputs (see :mod:`stda_amd.prep`) through CSF selection to the
+spectrum, with the same parameter names and meanings (``thr`` in eV, ``thrp``,
+``ax``, ``alphak``/``betaj`` with -100 meaning "default", ``fthr``), plus
+``nroots`` for the root count (None reproduces the reference's energy
+window).  Errors raise :class:`StdaHipError` carrying the message the
+reference stops with.
+
+There is no CPU fallback: if the library is missing the import of this
+module's loader raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .constants import HARTREE_EV, HARTREE_EV_PRINT
+from .prep import CoreInputs
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libstda_hip.so")
+
+C_API = [
+    "stda_hip_create", "stda_hip_destroy", "stda_hip_error_string", "stda_hip_last_error",
+    "stda_hip_setup", "stda_hip_get_csf", "stda_hip_solve_tda", "stda_hip_solve_rpa",
+    "stda_hip_lresp", "stda_hip_build_a", "stda_hip_build_apb_amb", "stda_hip_build_bhalf",
+    "stda_hip_matvec", "stda_hip_timings",
+]
+
+
+class StdaHipError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class _System(ctypes.Structure):
+    _fields_ = [("ncent", ctypes.c_int32), ("nao", ctypes.c_int32), ("no", ctypes.c_int32),
+                ("nv", ctypes.c_int32), ("clow", ctypes.c_void_p), ("epsi", ctypes.c_void_p),
+                ("iaoat", ctypes.c_void_p), ("xyz", ctypes.c_void_p),
+                ("moments", ctypes.c_void_p)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("thr_ev", ctypes.c_double), ("thrp", ctypes.c_double),
+                ("fthr_ev", ctypes.c_double), ("ax", ctypes.c_double),
+                ("alphak", ctypes.c_double), ("betaj", ctypes.c_double),
+                ("triplet", ctypes.c_int32), ("kshift", ctypes.c_int32),
+                ("kshift_max_ev", ctypes.c_double), ("kshift_width_ev", ctypes.c_double),
+                ("kshift_steep", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load ``libstda_hip.so``; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m stda_amd.build` "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(path)
+    P, I32, I64, F64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    L.stda_hip_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    L.stda_hip_destroy.argtypes = [P]
+    L.stda_hip_error_string.restype = ctypes.c_char_p
+    L.stda_hip_error_string.argtypes = [ctypes.c_int]
+    L.stda_hip_last_error.restype = ctypes.c_char_p
+    L.stda_hip_last_error.argtypes = [P]
+    L.stda_hip_setup.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params),
+                                 ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
+    L.stda_hip_get_csf.argtypes = [P, P, P]
+    L.stda_hip_solve_tda.argtypes = [P, I32, I32, I32, F64, ctypes.POINTER(I32), P, P, P, P]
+    L.stda_hip_solve_rpa.argtypes = [P, I32, I32, F64, ctypes.POINTER(I32), P, P, P, P, P]
+    L.stda_hip_lresp.argtypes = [P, I32, P, F64, P, P]
+    L.stda_hip_build_a.argtypes = [P, P]
+    L.stda_hip_build_apb_amb.argtypes = [P, P, P]
+    L.stda_hip_build_bhalf.argtypes = [P, P]
+    L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
+    L.stda_hip_timings.argtypes = [P, P]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Solution:
+    omega: np.ndarray                    # Hartree
+    X: np.ndarray = None                 # (nci, k)
+    Y: np.ndarray = None                 # (nci, k) RPA only
+    props: np.ndarray = None             # (k, 4): fL, fV, RL, RV or Rv(corr)
+    alpha_sos: np.ndarray = None         # (3, 3)
+    timings_ms: dict = field(default_factory=dict)
+
+    def tda_dat_rows(self):
+        """Rows of tda.dat's DATXY block as the reference formats them."""
+        return [(k + 1, round(float(self.omega[k]) * HARTREE_EV_PRINT, 4),
+                 *[round(float(x), 6) for x in self.props[k]]) for k in range(len(self.omega))]
+
+
+def _alpha3(a6):
+    xx, xy, yy, xz, yz, zz = a6
+    return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]])
+
+
+class StdaHip:
+    """One device context of the excited-state core."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        self._h = ctypes.c_void_p()
+        self._check(self._L.stda_hip_create(ctypes.byref(self._h), device))
+        self.nci = self.nprim = self.nsec = 0
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            self._L.stda_hip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            msg = self._L.stda_hip_error_string(st).decode()
+            detail = self._L.stda_hip_last_error(self._h).decode() if self._h else ""
+            raise StdaHipError(st, f"{msg}: {detail}" if detail else msg)
+
+    # ---------------------------------------------------------------- setup
+    def setup(self, ci: CoreInputs, thr: float, ax: float, thrp: float = 1e-4,
+              alphak: float = -100.0, betaj: float = -100.0, fthr: float | None = None,
+              triplet: bool = False, kshift=None):
+        """Stage 1: gamma, charges, screening, PT2 selection (stda.f:454-790)."""
+        clow = np.asfortranarray(ci.clow, dtype=np.float64)
+        eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
+        iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+        xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
+        mom = np.ascontiguousarray(ci.moments, dtype=np.float64) if ci.moments is not None else None
+        self._keep = [clow, eps, iaoat, xyz, mom]
+        sysd = _System(ci.ncent, ci.nao, ci.no, ci.nv, _ptr(clow), _ptr(eps), _ptr(iaoat),
+                       _ptr(xyz), _ptr(mom))
+        ks = kshift or (0.5, 0.1, 4.0)
+        par = _Params(thr, thrp, -1.0 if fthr is None else fthr, ax, alphak, betaj,
+                      1 if triplet else 0, 1 if kshift else 0, ks[0], ks[1], ks[2])
+        nci, nprim, nsec = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.stda_hip_setup(self._h, ctypes.byref(sysd), ctypes.byref(par),
+                                           ctypes.byref(nci), ctypes.byref(nprim),
+                                           ctypes.byref(nsec)))
+        self.nci, self.nprim, self.nsec = nci.value, nprim.value, nsec.value
+        self.thr = thr / HARTREE_EV
+        return self.nci, self.nprim, self.nsec
+
+    def csf(self):
+        iconf = np.zeros(2 * self.nci, dtype=np.int32)
+        ed = np.zeros(self.nci)
+        self._check(self._L.stda_hip_get_csf(self._h, _ptr(iconf), _ptr(ed)))
+        return np.stack([iconf[:self.nci], iconf[self.nci:]], axis=1), ed
+
+    # ---------------------------------------------------------------- solvers
+    def solve_tda(self, nroots: int | None, velcorr: bool = True, tol: float = 1e-8,
+                  vectors: bool = True) -> Solution:
+        cap = self.nci if nroots is None else min(nroots, self.nci)
+        omega = np.zeros(cap)
+        X = np.zeros((self.nci, cap), order="F") if vectors else None
+        props = np.zeros((cap, 5))
+        a6 = np.zeros(6)
+        nf = ctypes.c_int32()
+        self._check(self._L.stda_hip_solve_tda(self._h, 0 if nroots is None else nroots, cap,
+                                               1 if velcorr else 0, tol, ctypes.byref(nf),
+                                               _ptr(omega), _ptr(X), _ptr(props), _ptr(a6)))
+        k = nf.value
+        return Solution(omega=omega[:k], X=None if X is None else X[:, :k],
+                        props=props[:k, :4], alpha_sos=_alpha3(a6), timings_ms=self.timings())
+
+    def solve_rpa(self, nroots: int | None, tol: float = 1e-8, vectors: bool = True) -> Solution:
+        cap = self.nci if nroots is None else min(nroots, self.nci)
+        omega = np.zeros(cap)
+        X = np.zeros((self.nci, cap), order="F") if vectors else None
+        Y = np.zeros((self.nci, cap), order="F") if vectors else None
+        props = np.zeros((cap, 5))
+        a6 = np.zeros(6)
+        nf = ctypes.c_int32()
+        self._check(self._L.stda_hip_solve_rpa(self._h, 0 if nroots is None else nroots, cap, tol,
+                                               ctypes.byref(nf), _ptr(omega), _ptr(X), _ptr(Y),
+                                               _ptr(props), _ptr(a6)))
+        k = nf.value
+        return Solution(omega=omega[:k], X=None if X is None else X[:, :k],
+                        Y=None if Y is None else Y[:, :k], props=props[:k, :4],
+                        alpha_sos=_alpha3(a6), timings_ms=self.timings())
+
+    def lresp(self, omegas, tol: float = 1e-8, vectors: bool = False):
+        """alpha(w) for each frequency (Hartree): list of (w, 3x3)."""
+        w = np.ascontiguousarray(omegas, dtype=np.float64)
+        nf = len(w)
+        alpha = np.zeros(9 * nf)
+        xpy = np.zeros(self.nci * 3 * nf) if vectors else None
+        self._check(self._L.stda_hip_lresp(self._h, nf, _ptr(w), tol, _ptr(alpha), _ptr(xpy)))
+        out = [(float(w[f]), alpha[9 * f:9 * f + 9].reshape(3, 3, order="F")) for f in range(nf)]
+        if vectors:
+            return out, xpy.reshape(self.nci, 3 * nf, order="F")
+        return out
+
+    # ---------------------------------------------------------------- explicit matrices
+    def build_a(self):
+        A = np.zeros((self.nci, self.nci), order="F")
+        self._check(self._L.stda_hip_build_a(self._h, _ptr(A)))
+        return A
+
+    def build_apb_amb(self):
+        npk = self.nci * (self.nci + 1) // 2
+        apb, amb = np.zeros(npk), np.zeros(npk)
+        self._check(self._L.stda_hip_build_apb_amb(self._h, _ptr(apb), _ptr(amb)))
+        return apb, amb
+
+    def build_bhalf(self):
+        b = np.zeros(self.nci * (self.nci + 1) // 2)
+        self._check(self._L.stda_hip_build_bhalf(self._h, _ptr(b)))
+        return b
+
+    def matvec(self, which: int, V: np.ndarray) -> np.ndarray:
+        V = np.asfortranarray(V, dtype=np.float64).reshape(self.nci, -1, order="F")
+        out = np.zeros_like(V, order="F")
+        self._check(self._L.stda_hip_matvec(self._h, which, V.shape[1], _ptr(V), _ptr(out)))
+        return out
+
+    def timings(self):
+        t = np.zeros(6)
+        self._L.stda_hip_timings(self._h, _ptr(t))
+        return dict(zip(["intermediates", "screen_sort", "pt2", "solve", "properties", "setup_total"],
+                        t.tolist()))
+
+
+def stda(ci: CoreInputs, thr: float, thrp: float, ax: float, alphak: float = -100.0,
+         betaj: float = -100.0, fthr: float | None = None, nroots: int | None = None,
+         rpa: bool = False, triplet: bool = False, velcorr: bool | None = None,
+         resp_omegas=None, tol: float = 1e-8, device: int = 0):
+    """Drop-in for the reference's ``stda`` hot path (stda.f:454-1471).
+
+    Returns ``(Solution | list-of-alpha, csf)`` where ``csf`` = (iconf, ed).
+    ``resp_omegas`` (Hartree) selects the linear-response branch of
+    ``-resp``/``-aresp`` (stda.f:878-903) instead of the eigenvalue problem.
+    """
+    if velcorr is None:
+        velcorr = not rpa                 # main.f:102, 662
+    with StdaHip(device) as h:
+        h.setup(ci, thr, ax, thrp=thrp, alphak=alphak, betaj=betaj, fthr=fthr, triplet=triplet)
+        csf = h.csf()
+        if resp_omegas is not None:
+            return h.lresp(resp_omegas, tol=tol), csf
+        if rpa:
+            return h.solve_rpa(nroots, tol=tol), csf
+        return h.solve_tda(nroots, velcorr=velcorr, tol=tol), csf

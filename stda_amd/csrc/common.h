@@ -1,0 +1,110 @@
+// Shared declarations of the MI355X excited-state core (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace stda {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+#define STDA_HIP_CHECK(expr)                                                              \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::stda::Error(e_ == hipErrorOutOfMemory ? 4 : 3,                        \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// Sizes of the 4-index coupling tiles (see kernels.hip: coupling_tile_kernel).
+// A row tile is TK occupied x TC virtual indices (pairs (k,c)); a column tile
+// is TJ occupied x TB virtual indices (pairs (j,b)).
+constexpr int TK = 4, TC = 8, TJ = 4, TB = 8;
+constexpr int NA_CHUNK = 32;
+
+// --- kernel launchers (kernels.hip) -------------------------------------------------
+void launch_gamma(hipStream_t s, int ncent, const double *xyz4, const double *eta, double ax,
+                  double alphak, double betaj, double *gamj, double *gamk);
+
+// charge modes for launch_charges
+enum ChargeMode { CH_IA = 0, CH_OCC_PACKED = 1, CH_VIR_PACKED = 2, CH_DIAG = 3 };
+// ct: (moci, nao) row-major Loewdin MOs with AOs sorted by atom; atom_ptr(ncent+1).
+void launch_charges(hipStream_t s, ChargeMode mode, int ncent, int nao, int no, int nv,
+                    const double *ct, const int *atom_ptr, double *q);
+
+void launch_screen(hipStream_t s, int ncent, int no, int nv, const double *eps,
+                   const double *jdiag, const double *pia, const double *qia, double ak,
+                   double thr, double fthr, int kshift, double smax, double width,
+                   double steep, double *de, double *kdiag, unsigned char *flags);
+
+// flags bit 0: primary, bit 1: secondary.  Order-preserving compaction.
+void launch_count_flags(hipStream_t s, const unsigned char *flags, int64_t n, int bit,
+                        int *block_counts, int nblocks);
+void launch_compact_flags(hipStream_t s, const unsigned char *flags, int64_t n, int bit,
+                          const int64_t *block_offsets, int nblocks, int32_t *out);
+int compact_block_size();
+
+void launch_gather_f64(hipStream_t s, const double *src, const int32_t *idx, int64_t n,
+                       double *dst);
+void launch_scatter_pairs(hipStream_t s, const double *v, int64_t ldv, const int32_t *pair,
+                          int64_t n, int nvec, double *full, int64_t ldfull);
+void launch_gather_pairs(hipStream_t s, const double *full, int64_t ldfull, const int32_t *pair,
+                         int64_t n, int nvec, const double *diag, const double *v, int64_t ldv,
+                         double *out, int64_t ldout);
+void launch_csf_moments(hipStream_t s, const double *moments, int64_t npk, const int32_t *pair,
+                        int64_t n, int no, int nv, double *mom);
+
+// AO-basis coupling: t_A = sum_{mu in A} D(mu,mu), u = gamk t; per vector.
+void launch_atom_diag(hipStream_t s, int nvec, int nao, int ncent, const double *Dt,
+                      int64_t strideD, const int *atom_of, const double *gamk, double *u);
+// Gt = aJ (gamJ o Dt) + aX (gamK o Dt^T) + diag(aK u_A(mu)), per vector.
+void launch_gbuild(hipStream_t s, int nvec, int nao, int ncent, const double *Dt,
+                   int64_t strideD, const int *atom_of, const double *gamj, const double *gamk,
+                   const double *u, double aJ, double aX, double aK, double *Gt, int64_t strideG);
+
+// Davidson helpers
+void launch_residual(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
+                     const double *theta, double *R);
+void launch_col_norms(hipStream_t s, int64_t n, int k, const double *R, double *norms);
+void launch_precondition(hipStream_t s, int64_t n, int k, const double *R, const double *diag,
+                         const double *theta, double *T);
+void launch_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *scale);
+void launch_set_unit(hipStream_t s, int64_t n, int k, double *V, const int32_t *rows);
+// Y(:, c) = diag o X(:, c)
+void launch_axpby_diag(hipStream_t s, int64_t n, int k, const double *diag, const double *X,
+                       int64_t ldx, double *Y, int64_t ldy);
+
+// 4-index coupling tiles: amat(kc, jb) = ak (kc|jb)_K - (kj|cb)_J.
+struct TileSet {
+    const int32_t *occ;   // ntiles x T_occ occupied indices (-1 = none)
+    const int32_t *vir;   // ntiles x T_vir virtual indices (-1 = none)
+    int ntiles;
+};
+// PT2 pass 1: row sums of pt over all primaries for every secondary row tile.
+void launch_pt2_rows(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak,
+                     double *rowsum);
+// PT2 pass 2: column sums of pt over the rows selected by rowmask/rowbit.
+void launch_pt2_cols(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak,
+                     double *colsum);
+// Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
+// (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
+void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,
+                    const double *q, const double *pJ, const double *qab,
+                    const int32_t *csf_of_pair, double ak, double *A, int64_t lda);
+// Exchange-type term (ib|ja)_K for every CSF pair (explicit A+-B, 0.5B; small systems).
+void launch_exchange(hipStream_t s, int64_t nci, int ncent, int nv, const int32_t *pair,
+                     const double *pK, const double *q, double *E, int64_t lde);
+
+}  // namespace stda

@@ -1,0 +1,1476 @@
+// Host side of the MI355X excited-state core: the setup pipeline that
+// replaces stda.f:454-790, the matrix-free operators, the FP64 block
+// Davidson / paired RPA Davidson / response solvers, and the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/stda_hip.h"
+#include "common.h"
+#include "devbuf.h"
+
+#include <dlfcn.h>
+
+namespace stda {
+
+// hardness table of setrep (stda.f:2060-2167), Hartree
+static const double kEta[94] = {
+    0.472592880, 0.922033910, 0.174528880, 0.257007330, 0.339490860, 0.421954120, 0.504381930,
+    0.586918630, 0.669313510, 0.751916070, 0.179641050, 0.221572760, 0.263485780, 0.305396450,
+    0.347340140, 0.389247250, 0.431156700, 0.473082690, 0.171054690, 0.202762440, 0.210073220,
+    0.217396470, 0.224710390, 0.232015010, 0.239339690, 0.246656380, 0.253982550, 0.261288630,
+    0.268594760, 0.275925650, 0.307629990, 0.339315800, 0.372359850, 0.402735490, 0.434457760,
+    0.466117080, 0.155850790, 0.186493240, 0.193562100, 0.200633110, 0.207705220, 0.214772540,
+    0.221846140, 0.228918720, 0.235986210, 0.243056120, 0.250130180, 0.257199370, 0.287847800,
+    0.318486730, 0.349124310, 0.379765930, 0.410408080, 0.441057770, 0.050193320, 0.067625700,
+    0.085044450, 0.102477360, 0.119911050, 0.137327720, 0.154762970, 0.172182650, 0.189612880,
+    0.207047600, 0.224467520, 0.241896450, 0.259325030, 0.276760940, 0.294182310, 0.311595870,
+    0.329022740, 0.345922980, 0.363880480, 0.381305860, 0.398774760, 0.416142980, 0.433645100,
+    0.451040140, 0.468489860, 0.485845500, 0.125267300, 0.142686770, 0.160116150, 0.177558890,
+    0.194975570, 0.212407780, 0.072635250, 0.094221580, 0.099202950, 0.104186210, 0.142356330,
+    0.163942940, 0.185519410, 0.223701390};
+
+constexpr double kHartreeEv = 27.211385050;   // stda.f:142
+constexpr double kRot = 235.7220;             // stda.f:1300
+
+// Symmetric eigensolver for the small subspace problems.  The image's LAPACK
+// (MKL's dsyevd) is loaded at run time by absolute path with RTLD_LOCAL, so
+// that its directory never enters the library search path of the process
+// (it holds an older libstdc++ than the ROCm libraries need).  Without it, a
+// Householder tridiagonalisation + implicit QL (tred2/tql2 of EISPACK,
+// Wilkinson & Reinsch, Handbook for Automatic Computation II) is used.
+using dsyevd_t = void (*)(const char *, const char *, const int *, double *, const int *, double *,
+                          double *, const int *, int *, const int *, int *);
+static dsyevd_t load_dsyevd() {
+    static bool tried = false;
+    static dsyevd_t fn = nullptr;
+    if (tried) return fn;
+    tried = true;
+    const char *env = getenv("STDA_HIP_LAPACK");
+    const char *cands[] = {env ? env : "", "/opt/conda/lib/libmkl_rt.so.1",
+                           "/opt/conda/lib/libmkl_rt.so"};
+    for (const char *path : cands) {
+        if (!path || !*path) continue;
+        void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+        if (!h) continue;
+        using set_t = int (*)(int);
+        if (auto st = (set_t)dlsym(h, "MKL_Set_Threading_Layer")) st(1);   // sequential
+        fn = (dsyevd_t)dlsym(h, "dsyevd_");
+        if (fn) break;
+    }
+    return fn;
+}
+
+static void tred2_tql2(int n, std::vector<double> &a, std::vector<double> &d) {
+    // a: column-major n x n symmetric on input, eigenvectors on output
+    std::vector<double> e(n, 0.0);
+    d.assign(n, 0.0);
+    auto A = [&](int i, int j) -> double & { return a[i + (size_t)n * j]; };
+    for (int i = n - 1; i > 0; --i) {
+        int l = i - 1;
+        double h = 0.0, scale = 0.0;
+        if (l > 0) {
+            for (int k = 0; k <= l; ++k) scale += std::fabs(A(i, k));
+            if (scale == 0.0) {
+                e[i] = A(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) { A(i, k) /= scale; h += A(i, k) * A(i, k); }
+                double f = A(i, l);
+                double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+                e[i] = scale * g;
+                h -= f * g;
+                A(i, l) = f - g;
+                f = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    A(j, i) = A(i, j) / h;
+                    g = 0.0;
+                    for (int k = 0; k <= j; ++k) g += A(j, k) * A(i, k);
+                    for (int k = j + 1; k <= l; ++k) g += A(k, j) * A(i, k);
+                    e[j] = g / h;
+                    f += e[j] * A(i, j);
+                }
+                double hh = f / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    f = A(i, j);
+                    e[j] = g = e[j] - hh * f;
+                    for (int k = 0; k <= j; ++k) A(j, k) -= (f * e[k] + g * A(i, k));
+                }
+            }
+        } else {
+            e[i] = A(i, l);
+        }
+        d[i] = h;
+    }
+    d[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int l = i - 1;
+        if (d[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += A(i, k) * A(k, j);
+                for (int k = 0; k <= l; ++k) A(k, j) -= g * A(k, i);
+            }
+        }
+        d[i] = A(i, i);
+        A(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) A(j, i) = A(i, j) = 0.0;
+    }
+    // implicit QL on the tridiagonal (d, e)
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-15 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 60) throw Error(STDA_HIP_ERR_DEVICE, "tql2: no convergence");
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    e[i + 1] = (r = std::hypot(f, g));
+                    if (r == 0.0) { d[i + 1] -= p; e[m] = 0.0; break; }
+                    s = f / r; c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    d[i + 1] = g + (p = s * r);
+                    g = c * r - b;
+                    for (int k = 0; k < n; ++k) {
+                        f = A(k, i + 1);
+                        A(k, i + 1) = s * A(k, i) + c * f;
+                        A(k, i) = c * A(k, i) - s * f;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p; e[l] = g; e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    // sort ascending
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+    std::vector<double> a2(a.size()), d2(n);
+    for (int j = 0; j < n; ++j) {
+        d2[j] = d[idx[j]];
+        for (int i = 0; i < n; ++i) a2[i + (size_t)n * j] = a[i + (size_t)n * idx[j]];
+    }
+    a.swap(a2);
+    d.swap(d2);
+}
+
+void host_syev(int n, std::vector<double> &a, std::vector<double> &w) {
+    w.assign(n, 0.0);
+    if (n == 0) return;
+    dsyevd_t fn = load_dsyevd();
+    if (!fn) {
+        tred2_tql2(n, a, w);
+        return;
+    }
+    int lwork = -1, liwork = -1, info = 0, iw = 0;
+    double wq = 0.0;
+    fn("V", "U", &n, a.data(), &n, w.data(), &wq, &lwork, &iw, &liwork, &info);
+    lwork = (int)wq + 1;
+    liwork = iw + 1;
+    std::vector<double> work(lwork);
+    std::vector<int> iwork(liwork);
+    fn("V", "U", &n, a.data(), &n, w.data(), work.data(), &lwork, iwork.data(), &liwork, &info);
+    if (info != 0) throw Error(STDA_HIP_ERR_DEVICE, "dsyevd failed, info=" + std::to_string(info));
+}
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+            .count();
+    }
+};
+
+}  // namespace stda
+
+using namespace stda;
+
+struct stda_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string last_error;
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+
+    // system
+    int ncent = 0, nao = 0, no = 0, nv = 0;
+    int64_t npair = 0;
+    double ak = 2.0, ax = 0.0, thr = 0.0, thrp = 1e-4, fthr = 0.0;
+    bool small_ax = false;
+    bool have_moments = false;
+    std::vector<double> moments_h;   // 9 * npk (host copy, for window-mode helpers)
+    int64_t npk = 0;
+
+    // device-resident intermediates
+    DBuf<double> ct;          // (nao, moci) Loewdin MOs, AOs sorted by atom (column-major)
+    DBuf<int> atom_ptr, atom_of;
+    DBuf<double> gamj, gamk, eta, xyz;
+    DBuf<double> qdiag, jdiag, eps;
+    DBuf<double> qia, pia, qij, pij, qab;
+    DBuf<double> de, kdiag;
+    DBuf<unsigned char> flags;
+    DBuf<double> moments;
+
+    // CSF space
+    int64_t nci = 0, nprim = 0, nsec = 0, npt = 0;
+    std::vector<int32_t> pair_h;   // CSF -> pair index
+    std::vector<double> ed_h;
+    DBuf<int32_t> pair;
+    DBuf<double> ed, corr, corr_small;   // diagonal corrections of the AO-path operators
+    DBuf<double> mom;                    // (nci, 9) CSF moments
+
+    // matrix-free work buffers
+    DBuf<double> xfull, tbuf, dbuf, gbuf, wbuf, rbuf, ubuf;
+    int nb_max = 0;
+    bool xfull_clean = false;
+
+    bool ready = false;
+
+    void sync() { STDA_HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// matrix-free operators
+enum Op { OP_A = 0, OP_APB = 1, OP_AMB = 2, OP_BHALF = 3 };
+
+void ensure_work(stda_hip_ctx *c, int nb) {
+    if (nb <= c->nb_max) return;
+    size_t nao = c->nao, no = c->no;
+    c->xfull.resize((size_t)c->npair * nb);
+    c->xfull.zero(c->stream);
+    c->tbuf.resize(nao * no * nb);
+    c->dbuf.resize(nao * nao * nb);
+    c->gbuf.resize(nao * nao * nb);
+    c->wbuf.resize(nao * no * nb);
+    c->rbuf.resize((size_t)c->npair * nb);
+    c->ubuf.resize((size_t)c->ncent * nb);
+    c->nb_max = nb;
+}
+
+int chunk_vectors(stda_hip_ctx *c) {
+    size_t per = sizeof(double) * ((size_t)2 * c->npair + (size_t)2 * c->nao * c->no +
+                                   (size_t)2 * c->nao * c->nao + c->ncent);
+    size_t freeb = 0, total = 0;
+    (void)hipMemGetInfo(&freeb, &total);
+    size_t budget = std::min<size_t>(freeb / 2, (size_t)24 << 30);
+    int nb = (int)std::max<size_t>(1, budget / per);
+    return std::min(nb, 64);
+}
+
+// out(I, v) = [op V](I, v) for nvec column vectors (n = nci rows).
+// Two operators sharing one transition density can be requested at once
+// (op2 >= 0 writes out2), which is how the RPA solver gets (A+B)V and (A-B)V.
+void apply_ops(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V, int64_t ldv,
+               double *out1, int64_t ld1, double *out2, int64_t ld2) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    const int nao = c->nao, no = c->no, nv = c->nv, na = c->ncent;
+    const double *Co = c->ct.get();
+    const double *Cv = c->ct.get() + (size_t)nao * no;
+    const int nbmax = chunk_vectors(c);
+    for (int v0 = 0; v0 < nvec; v0 += nbmax) {
+        int nb = std::min(nbmax, nvec - v0);
+        ensure_work(c, nb);
+        const double *Vb = V + ldv * v0;
+        // embed the CSF vectors in the full occupied x virtual grid
+        launch_scatter_pairs(s, Vb, ldv, c->pair.get(), n, nb, c->xfull.get(), c->npair);
+        // T = Cv X^T (nao x no) for all vectors at once: X^T is (nv, no) column-major
+        dgemm(c->blas, false, false, nao, (int64_t)no * nb, nv, 1.0, Cv, nao, c->xfull.get(), nv,
+              0.0, c->tbuf.get(), nao);
+        // Dt = T Co^T (nao x nao) per vector
+        dgemm_batched(c->blas, false, true, nao, nao, no, 1.0, c->tbuf.get(), nao,
+                      (int64_t)nao * no, Co, nao, 0, 0.0, c->dbuf.get(), nao, (int64_t)nao * nao,
+                      nb);
+        bool need_u = false;
+        for (int pass = 0; pass < 2; ++pass) {
+            int op = pass == 0 ? op1 : op2;
+            if (op < 0) continue;
+            double *out = pass == 0 ? out1 : out2;
+            int64_t ld = pass == 0 ? ld1 : ld2;
+            double aJ = -1.0, aX = 0.0, aK = 0.0;
+            const double *corr = c->corr.get();
+            switch (op) {
+                case OP_A: aK = c->ak; break;
+                case OP_APB:
+                    if (c->small_ax) { aJ = 0.0; aK = 2.0 * c->ak; corr = c->corr_small.get(); }
+                    else { aX = -c->ax; aK = 2.0 * c->ak; }
+                    break;
+                case OP_AMB:
+                    if (c->small_ax) {   // A-B diagonal (stda.f:1893-1924): out = corr_small o V
+                        launch_axpby_diag(s, n, nb, c->corr_small.get(), Vb, ldv, out + ld * v0, ld);
+                        continue;
+                    }
+                    aX = c->ax;
+                    break;
+                case OP_BHALF:
+                    aJ = 0.0; aX = -0.5 * c->ax; aK = 0.5 * c->ak; corr = nullptr;
+                    break;
+            }
+            if (aK != 0.0 && !need_u) {
+                launch_atom_diag(s, nb, nao, na, c->dbuf.get(), (int64_t)nao * nao,
+                                 c->atom_ptr.get(), c->gamk.get(), c->ubuf.get());
+                need_u = true;
+            }
+            launch_gbuild(s, nb, nao, na, c->dbuf.get(), (int64_t)nao * nao, c->atom_of.get(),
+                          c->gamj.get(), c->gamk.get(), c->ubuf.get(), aJ, aX, aK, c->gbuf.get(),
+                          (int64_t)nao * nao);
+            // W = Gt Co (nao x no) per vector, then R = Cv^T W (nv x no)
+            dgemm_batched(c->blas, false, false, nao, no, nao, 1.0, c->gbuf.get(), nao,
+                          (int64_t)nao * nao, Co, nao, 0, 0.0, c->wbuf.get(), nao,
+                          (int64_t)nao * no, nb);
+            dgemm(c->blas, true, false, nv, (int64_t)no * nb, nao, 1.0, Cv, nao, c->wbuf.get(),
+                  nao, 0.0, c->rbuf.get(), nv);
+            launch_gather_pairs(s, c->rbuf.get(), c->npair, c->pair.get(), n, nb, corr, Vb, ldv,
+                                out + ld * v0, ld);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// small dense helpers (host)
+std::vector<double> sym_sqrt(const std::vector<double> &A, int m, bool inverse, double *minev) {
+    std::vector<double> U(A), w;
+    host_syev(m, U, w);
+    if (minev) *minev = m ? w[0] : 0.0;
+    std::vector<double> R((size_t)m * m, 0.0);
+    for (int k = 0; k < m; ++k) {
+        double f = w[k] > 0 ? std::sqrt(w[k]) : 0.0;
+        if (inverse) f = f > 0 ? 1.0 / f : 0.0;
+        for (int j = 0; j < m; ++j) {
+            double ujk = U[j + (size_t)m * k] * f;
+            for (int i = 0; i < m; ++i) R[i + (size_t)m * j] += U[i + (size_t)m * k] * ujk;
+        }
+    }
+    return R;
+}
+
+std::vector<double> matmul(const std::vector<double> &A, const std::vector<double> &B, int m,
+                           int k, int n) {   // (m x k)(k x n), column-major
+    std::vector<double> C((size_t)m * n, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int l = 0; l < k; ++l) {
+            double b = B[l + (size_t)k * j];
+            if (b == 0.0) continue;
+            for (int i = 0; i < m; ++i) C[i + (size_t)m * j] += A[i + (size_t)m * l] * b;
+        }
+    return C;
+}
+
+// Orthonormalise the columns of T (n x nt, device) against V (n x m, device,
+// orthonormal) and among themselves; returns the number of columns kept,
+// packed at the front of T.
+int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, DBuf<double> &tmp) {
+    const int64_t n = c->nci;
+    if (nt == 0) return 0;
+    tmp.ensure((size_t)std::max(m, 1) * nt + (size_t)nt * nt + (size_t)n * nt);
+    double *C = tmp.get();
+    for (int pass = 0; pass < 2 && m > 0; ++pass) {
+        dgemm(c->blas, true, false, m, nt, n, 1.0, V, n, T, n, 0.0, C, m);
+        dgemm(c->blas, false, false, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
+    }
+    // symmetric orthonormalisation within T: S = T^T T = U s U^T, T <- T U s^-1/2
+    double *S = tmp.get() + (size_t)std::max(m, 1) * nt;
+    double *T2 = S + (size_t)nt * nt;
+    dgemm(c->blas, true, false, nt, nt, n, 1.0, T, n, T, n, 0.0, S, nt);
+    std::vector<double> Sh((size_t)nt * nt), w;
+    STDA_HIP_CHECK(hipMemcpyAsync(Sh.data(), S, Sh.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    host_syev(nt, Sh, w);
+    double wmax = nt ? std::max(w[nt - 1], 0.0) : 0.0;
+    std::vector<double> Z;
+    int kept = 0;
+    for (int k = nt - 1; k >= 0; --k) {
+        if (w[k] <= 1e-14 * std::max(1.0, wmax) || w[k] <= 1e-20) continue;
+        double f = 1.0 / std::sqrt(w[k]);
+        for (int i = 0; i < nt; ++i) Z.push_back(Sh[i + (size_t)nt * k] * f);
+        ++kept;
+    }
+    if (kept == 0) return 0;
+    DBuf<double> Zd;
+    Zd.upload(Z.data(), Z.size(), c->stream);
+    dgemm(c->blas, false, false, n, kept, nt, 1.0, T, n, Zd.get(), nt, 0.0, T2, n);
+    // one more projection against V for safety, then renormalise
+    if (m > 0) {
+        dgemm(c->blas, true, false, m, kept, n, 1.0, V, n, T2, n, 0.0, C, m);
+        dgemm(c->blas, false, false, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
+    }
+    std::vector<double> nrm(kept);
+    DBuf<double> nd(kept);
+    launch_col_norms(c->stream, n, kept, T2, nd.get());
+    nd.download(nrm.data(), kept, c->stream);
+    c->sync();
+    for (auto &x : nrm) x = x > 0 ? 1.0 / x : 0.0;
+    nd.upload(nrm.data(), kept, c->stream);
+    launch_scale_cols(c->stream, n, kept, T2, nd.get());
+    STDA_HIP_CHECK(hipMemcpyAsync(T, T2, (size_t)n * kept * 8, hipMemcpyDeviceToDevice, c->stream));
+    c->sync();
+    return kept;
+}
+
+std::vector<int32_t> lowest_diag_rows(const std::vector<double> &d, int k) {
+    std::vector<int32_t> idx(d.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+    idx.resize(std::min<size_t>(idx.size(), (size_t)k));
+    return idx;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 block Davidson for the k lowest eigenpairs of A (TDA).
+void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta,
+                  DBuf<double> &X) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    k = (int)std::min<int64_t>(k, n);
+    int nguess = (int)std::min<int64_t>(n, std::max(k + 8, k + k / 2));
+    int maxdim = (int)std::min<int64_t>(n, std::max(6 * k, k + 64));
+    if (maxdim < nguess) maxdim = nguess;
+    DBuf<double> V((size_t)n * maxdim), AV((size_t)n * maxdim), R((size_t)n * maxdim);
+    DBuf<double> AX((size_t)n * k), th(k), nrm(k), tmp, Yd;
+    X.resize((size_t)n * k);
+    std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
+    DBuf<int32_t> rowsd;
+    rowsd.upload(rows.data(), rows.size(), s);
+    launch_set_unit(s, n, nguess, V.get(), rowsd.get());
+    std::vector<double> H((size_t)maxdim * maxdim, 0.0);
+    int m = 0, mnew = nguess;
+    const int maxiter = 300;
+    for (int it = 0; it < maxiter; ++it) {
+        apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0);
+        // H(0:m+mnew, m:m+mnew) = V^T AV
+        int mt = m + mnew;
+        DBuf<double> Hc((size_t)mt * mnew);
+        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, AV.get() + n * m, n, 0.0,
+              Hc.get(), mt);
+        std::vector<double> hc((size_t)mt * mnew);
+        Hc.download(hc.data(), hc.size(), s);
+        c->sync();
+        for (int j = 0; j < mnew; ++j)
+            for (int i = 0; i < mt; ++i) {
+                double v = hc[i + (size_t)mt * j];
+                H[i + (size_t)maxdim * (m + j)] = v;
+                H[(m + j) + (size_t)maxdim * i] = v;
+            }
+        m = mt;
+        // Rayleigh-Ritz on the host (FP64)
+        std::vector<double> Hs((size_t)m * m), w;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) Hs[i + (size_t)m * j] = 0.5 * (H[i + (size_t)maxdim * j] + H[j + (size_t)maxdim * i]);
+        host_syev(m, Hs, w);
+        int kk = std::min(k, m);
+        Yd.upload(Hs.data(), (size_t)m * kk, s);
+        theta.assign(w.begin(), w.begin() + kk);
+        th.upload(theta.data(), kk, s);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
+        launch_residual(s, n, kk, AX.get(), X.get(), th.get(), R.get());
+        launch_col_norms(s, n, kk, R.get(), nrm.get());
+        std::vector<double> rn(kk);
+        nrm.download(rn.data(), kk, s);
+        c->sync();
+        std::vector<int> un;
+        for (int j = 0; j < kk; ++j)
+            if (!(rn[j] < tol)) un.push_back(j);
+        if (un.empty() && kk == k) return;
+        if (m == n) return;   // full space: exact
+        // compact the unconverged residuals to the front of R, precondition
+        int nun = (int)un.size();
+        for (int q = 0; q < nun; ++q)
+            if (un[q] != q)
+                STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8,
+                                              hipMemcpyDeviceToDevice, s));
+        std::vector<double> thu(nun);
+        for (int q = 0; q < nun; ++q) thu[q] = theta[un[q]];
+        DBuf<double> thud;
+        thud.upload(thu.data(), nun, s);
+        launch_precondition(s, n, nun, R.get(), c->ed.get(), thud.get(), R.get());
+        if (m + nun > maxdim) {   // restart with the Ritz vectors
+            STDA_HIP_CHECK(hipMemcpyAsync(V.get(), X.get(), (size_t)n * kk * 8,
+                                          hipMemcpyDeviceToDevice, s));
+            STDA_HIP_CHECK(hipMemcpyAsync(AV.get(), AX.get(), (size_t)n * kk * 8,
+                                          hipMemcpyDeviceToDevice, s));
+            std::fill(H.begin(), H.end(), 0.0);
+            for (int j = 0; j < kk; ++j) H[j + (size_t)maxdim * j] = theta[j];
+            m = kk;
+        }
+        int room = std::min(nun, maxdim - m);
+        STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * room * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+        if (mnew == 0) return;   // no new directions: converged to machine precision
+    }
+    throw Error(STDA_HIP_ERR_NOT_CONVERGED, "Davidson did not converge");
+}
+
+// ------------------------------------------------------------------------------------------
+// Paired Davidson for the RPA problem (A-B)(A+B)|X+Y> = w^2 |X+Y>, using
+// the Hermitian reduced form (Am^1/2 Ap Am^1/2) of the subspace problem so
+// that (A-B)^1/2 is never formed in the full space.
+void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega,
+                  DBuf<double> &XpY, DBuf<double> &XmY) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    k = (int)std::min<int64_t>(k, n);
+    int nguess = (int)std::min<int64_t>(n, std::max(k + 8, k + k / 2));
+    int maxdim = (int)std::min<int64_t>(n, std::max(8 * k, k + 96));
+    if (maxdim < nguess) maxdim = nguess;
+    DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim);
+    DBuf<double> R((size_t)n * 2 * k), PX((size_t)n * k), MX((size_t)n * k), tmp, Yd, nrm(2 * k);
+    XpY.resize((size_t)n * k);
+    XmY.resize((size_t)n * k);
+    std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
+    DBuf<int32_t> rowsd;
+    rowsd.upload(rows.data(), rows.size(), s);
+    launch_set_unit(s, n, nguess, V.get(), rowsd.get());
+    std::vector<double> Ap((size_t)maxdim * maxdim, 0.0), Am((size_t)maxdim * maxdim, 0.0);
+    int m = 0, mnew = nguess;
+    for (int it = 0; it < 300; ++it) {
+        apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n, M.get() + n * m,
+                  n);
+        int mt = m + mnew;
+        DBuf<double> Hc((size_t)mt * mnew * 2);
+        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, P.get() + n * m, n, 0.0,
+              Hc.get(), mt);
+        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, M.get() + n * m, n, 0.0,
+              Hc.get() + (size_t)mt * mnew, mt);
+        std::vector<double> hc((size_t)mt * mnew * 2);
+        Hc.download(hc.data(), hc.size(), s);
+        c->sync();
+        for (int j = 0; j < mnew; ++j)
+            for (int i = 0; i < mt; ++i) {
+                double vp = hc[i + (size_t)mt * j], vm = hc[(size_t)mt * mnew + i + (size_t)mt * j];
+                Ap[i + (size_t)maxdim * (m + j)] = vp; Ap[(m + j) + (size_t)maxdim * i] = vp;
+                Am[i + (size_t)maxdim * (m + j)] = vm; Am[(m + j) + (size_t)maxdim * i] = vm;
+            }
+        m = mt;
+        std::vector<double> ap((size_t)m * m), am((size_t)m * m);
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) {
+                ap[i + (size_t)m * j] = 0.5 * (Ap[i + (size_t)maxdim * j] + Ap[j + (size_t)maxdim * i]);
+                am[i + (size_t)m * j] = 0.5 * (Am[i + (size_t)maxdim * j] + Am[j + (size_t)maxdim * i]);
+            }
+        double minev = 0.0;
+        std::vector<double> sq = sym_sqrt(am, m, false, &minev);
+        if (minev <= 0.0)
+            throw Error(STDA_HIP_ERR_AMB_NOT_PD, "matrix power impossible (A-B not positive definite)");
+        std::vector<double> Hm = matmul(matmul(sq, ap, m, m, m), sq, m, m, m), w2;
+        host_syev(m, Hm, w2);
+        int kk = std::min(k, m);
+        omega.assign(kk, 0.0);
+        std::vector<double> xr((size_t)m * kk), yr((size_t)m * kk);   // (X+Y)_red, (X-Y)_red
+        std::vector<double> Z(Hm.begin(), Hm.begin() + (size_t)m * kk);
+        std::vector<double> xp = matmul(sq, Z, m, m, kk);
+        for (int j = 0; j < kk; ++j) {
+            if (w2[j] <= 0.0) throw Error(STDA_HIP_ERR_NO_ROOTS, "RPA diag failed (w^2 <= 0)");
+            omega[j] = std::sqrt(w2[j]);
+            double f = 1.0 / std::sqrt(omega[j]);
+            for (int i = 0; i < m; ++i) xr[i + (size_t)m * j] = xp[i + (size_t)m * j] * f;
+        }
+        std::vector<double> apx = matmul(ap, xr, m, m, kk);
+        for (int j = 0; j < kk; ++j)
+            for (int i = 0; i < m; ++i) yr[i + (size_t)m * j] = apx[i + (size_t)m * j] / omega[j];
+        // full-space Ritz vectors and residuals
+        std::vector<double> both((size_t)m * 2 * kk);
+        std::copy(xr.begin(), xr.end(), both.begin());
+        std::copy(yr.begin(), yr.end(), both.begin() + (size_t)m * kk);
+        Yd.upload(both.data(), both.size(), s);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XpY.get(), n);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
+              XmY.get(), n);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, P.get(), n, Yd.get(), m, 0.0, PX.get(), n);
+        dgemm(c->blas, false, false, n, kk, m, 1.0, M.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
+              MX.get(), n);
+        DBuf<double> om;
+        om.upload(omega.data(), kk, s);
+        launch_residual(s, n, kk, PX.get(), XmY.get(), om.get(), R.get());            // (A+B)(X+Y) - w(X-Y)
+        launch_residual(s, n, kk, MX.get(), XpY.get(), om.get(), R.get() + n * kk);   // (A-B)(X-Y) - w(X+Y)
+        launch_col_norms(s, n, 2 * kk, R.get(), nrm.get());
+        std::vector<double> rn(2 * kk);
+        nrm.download(rn.data(), 2 * kk, s);
+        c->sync();
+        std::vector<int> un;
+        for (int j = 0; j < 2 * kk; ++j)
+            if (!(rn[j] < tol)) un.push_back(j);
+        if (un.empty() && kk == k) return;
+        if (m == n) return;
+        int nun = (int)un.size();
+        for (int q = 0; q < nun; ++q)
+            if (un[q] != q)
+                STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8,
+                                              hipMemcpyDeviceToDevice, s));
+        std::vector<double> thu(nun);
+        for (int q = 0; q < nun; ++q) thu[q] = omega[un[q] % kk];
+        DBuf<double> thud;
+        thud.upload(thu.data(), nun, s);
+        launch_precondition(s, n, nun, R.get(), c->ed.get(), thud.get(), R.get());
+        if (m + nun > maxdim) {
+            // restart: V <- orthonormal basis of span(X+Y, X-Y), P/M transformed alike
+            std::vector<double> G = both;   // (m x 2kk) coefficient block
+            int nc = 2 * kk;
+            std::vector<double> S = matmul([&] {
+                std::vector<double> t((size_t)nc * m);
+                for (int j = 0; j < nc; ++j)
+                    for (int i = 0; i < m; ++i) t[j + (size_t)nc * i] = G[i + (size_t)m * j];
+                return t;
+            }(), G, nc, m, nc), w;
+            host_syev(nc, S, w);
+            std::vector<double> Zc;
+            int kept = 0;
+            for (int q = nc - 1; q >= 0; --q) {
+                if (w[q] <= 1e-12 * w[nc - 1]) continue;
+                std::vector<double> col(nc);
+                for (int i = 0; i < nc; ++i) col[i] = S[i + (size_t)nc * q] / std::sqrt(w[q]);
+                std::vector<double> gz = matmul(G, col, m, nc, 1);
+                Zc.insert(Zc.end(), gz.begin(), gz.end());
+                ++kept;
+            }
+            DBuf<double> Zd, V2((size_t)n * kept), P2((size_t)n * kept), M2((size_t)n * kept);
+            Zd.upload(Zc.data(), Zc.size(), s);
+            dgemm(c->blas, false, false, n, kept, m, 1.0, V.get(), n, Zd.get(), m, 0.0, V2.get(), n);
+            dgemm(c->blas, false, false, n, kept, m, 1.0, P.get(), n, Zd.get(), m, 0.0, P2.get(), n);
+            dgemm(c->blas, false, false, n, kept, m, 1.0, M.get(), n, Zd.get(), m, 0.0, M2.get(), n);
+            STDA_HIP_CHECK(hipMemcpyAsync(V.get(), V2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
+            STDA_HIP_CHECK(hipMemcpyAsync(P.get(), P2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
+            STDA_HIP_CHECK(hipMemcpyAsync(M.get(), M2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
+            // reduced matrices in the new basis: Zc^T Ap Zc
+            std::vector<double> zt((size_t)kept * m);
+            for (int j = 0; j < kept; ++j)
+                for (int i = 0; i < m; ++i) zt[j + (size_t)kept * i] = Zc[i + (size_t)m * j];
+            std::vector<double> ap2 = matmul(zt, matmul(ap, Zc, m, m, kept), kept, m, kept);
+            std::vector<double> am2 = matmul(zt, matmul(am, Zc, m, m, kept), kept, m, kept);
+            std::fill(Ap.begin(), Ap.end(), 0.0);
+            std::fill(Am.begin(), Am.end(), 0.0);
+            for (int j = 0; j < kept; ++j)
+                for (int i = 0; i < kept; ++i) {
+                    Ap[i + (size_t)maxdim * j] = ap2[i + (size_t)kept * j];
+                    Am[i + (size_t)maxdim * j] = am2[i + (size_t)kept * j];
+                }
+            m = kept;
+            c->sync();
+        }
+        int room = std::min(nun, maxdim - m);
+        STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * room * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+        if (mnew == 0) return;
+    }
+    throw Error(STDA_HIP_ERR_NOT_CONVERGED, "RPA Davidson did not converge");
+}
+
+// ------------------------------------------------------------------------------------------
+// properties (stda.f:1207-1305; apbtrafo.f:160-265)
+void gather_moments(stda_hip_ctx *c) {
+    if (!c->have_moments) return;
+    c->mom.resize((size_t)c->nci * 9);
+    launch_csf_moments(c->stream, c->moments.get(), c->npk, c->pair.get(), c->nci, c->no, c->nv,
+                       c->mom.get());
+}
+
+// T(9, k) = mom^T U
+std::vector<double> moment_products(stda_hip_ctx *c, const double *U, int k) {
+    std::vector<double> T((size_t)9 * k, 0.0);
+    if (!c->have_moments || k == 0) return T;
+    DBuf<double> Td((size_t)9 * k);
+    dgemm(c->blas, true, false, 9, k, c->nci, 1.0, c->mom.get(), c->nci, U, c->nci, 0.0, Td.get(), 9);
+    Td.download(T.data(), T.size(), c->stream);
+    c->sync();
+    return T;
+}
+
+}  // namespace
+
+// ============================================================================================
+// setup: stda.f:454-790
+static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par) {
+    if (!sys || !par || sys->ncent <= 0 || sys->nao <= 0 || sys->no < 0 || sys->nv < 0 ||
+        !sys->clow || !sys->epsi || !sys->iaoat || !sys->xyz)
+        throw Error(STDA_HIP_ERR_ARG, "invalid system description");
+    if (sys->no == 0 || sys->nv == 0)
+        throw Error(STDA_HIP_ERR_NO_CSF, "no CSF, increase energy threshold (-e option)");
+    Timer ttot;
+    hipStream_t s = c->stream;
+    c->ready = false;
+    c->nb_max = 0;
+    c->ncent = sys->ncent; c->nao = sys->nao; c->no = sys->no; c->nv = sys->nv;
+    const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
+    c->npair = (int64_t)no * nv;
+    if (c->npair > INT32_MAX) throw Error(STDA_HIP_ERR_TOO_LARGE, "no*nv exceeds int32 pair indexing");
+    c->ak = par->triplet ? 0.0 : 2.0;                    // stda.f:482-483
+    c->ax = par->ax;
+    c->small_ax = std::fabs(par->ax) < 1.0e-6;           // stda.f:1893
+    double betaj = par->betaj < -99.0 ? 0.20 + 1.830 * par->ax : par->betaj;   // stda.f:490-500
+    double alphak = par->alphak < -99.0 ? 1.420 + 0.480 * par->ax : par->alphak;
+    c->thr = par->thr_ev / kHartreeEv;
+    c->thrp = par->thrp;
+    c->fthr = par->fthr_ev > 0.0 ? par->fthr_ev / kHartreeEv : 1.7976931348623157e308;
+    for (int i = 0; i < na; ++i) {
+        int z = (int)sys->xyz[4 * i + 3];
+        if (z < 1 || z > 94) throw Error(STDA_HIP_ERR_ARG, "nuclear charge outside 1..94");
+    }
+
+    // AOs sorted by atom (stable), Loewdin MOs transposed into that order
+    std::vector<int> order(nao);
+    std::iota(order.begin(), order.end(), 0);
+    for (int m = 0; m < nao; ++m)
+        if (sys->iaoat[m] < 1 || sys->iaoat[m] > na) throw Error(STDA_HIP_ERR_ARG, "iaoat out of range");
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return sys->iaoat[a] < sys->iaoat[b]; });
+    std::vector<int> aptr(na + 1, 0), aof(nao);
+    for (int m = 0; m < nao; ++m) aptr[sys->iaoat[m]]++;
+    for (int a = 0; a < na; ++a) aptr[a + 1] += aptr[a];
+    for (int m = 0; m < nao; ++m) aof[m] = sys->iaoat[order[m]] - 1;
+    std::vector<double> cth((size_t)nao * moci);
+    for (int p = 0; p < moci; ++p)
+        for (int m = 0; m < nao; ++m) cth[m + (size_t)nao * p] = sys->clow[order[m] + (size_t)nao * p];
+    Timer t0;
+    c->ct.upload(cth.data(), cth.size(), s);
+    c->atom_ptr.upload(aptr.data(), aptr.size(), s);
+    c->atom_of.upload(aof.data(), aof.size(), s);
+    c->eta.upload(kEta, 94, s);
+    c->xyz.upload(sys->xyz, (size_t)4 * na, s);
+    c->eps.upload(sys->epsi, moci, s);
+    c->have_moments = sys->moments != nullptr;
+    c->npk = (int64_t)moci * (moci + 1) / 2;
+    if (c->have_moments) {
+        c->moments.upload(sys->moments, (size_t)9 * c->npk, s);
+        c->moments_h.assign(sys->moments, sys->moments + (size_t)9 * c->npk);
+    }
+
+    // gamma (stda.f:509-532)
+    c->gamj.resize((size_t)na * na);
+    c->gamk.resize((size_t)na * na);
+    launch_gamma(s, na, c->xyz.get(), c->eta.get(), c->ax, alphak, betaj, c->gamj.get(), c->gamk.get());
+    // q_ii, q_aa and (ii|aa)^J (stda.f:465-566)
+    c->qdiag.resize((size_t)na * moci);
+    launch_charges(s, CH_DIAG, na, nao, no, nv, c->ct.get(), c->atom_ptr.get(), c->qdiag.get());
+    DBuf<double> pii((size_t)na * no);
+    dgemm(c->blas, false, false, na, no, na, 1.0, c->gamj.get(), na, c->qdiag.get(), na, 0.0, pii.get(), na);
+    c->jdiag.resize((size_t)nv * no);
+    dgemm(c->blas, true, false, nv, no, na, 1.0, c->qdiag.get() + (size_t)na * no, na, pii.get(), na, 0.0,
+          c->jdiag.get(), nv);
+    // q_ia over all pairs and p_ia = gamK q_ia (stda.f:576-589)
+    c->qia.resize((size_t)na * c->npair);
+    c->pia.resize((size_t)na * c->npair);
+    launch_charges(s, CH_IA, na, nao, no, nv, c->ct.get(), c->atom_ptr.get(), c->qia.get());
+    dgemm(c->blas, false, false, na, c->npair, na, 1.0, c->gamk.get(), na, c->qia.get(), na, 0.0,
+          c->pia.get(), na);
+    // packed q_ij -> p_ij = gamJ q_ij, packed q_ab (stda.f:728-763)
+    int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
+    c->qij.resize((size_t)na * nij);
+    c->pij.resize((size_t)na * nij);
+    c->qab.resize((size_t)na * nab);
+    launch_charges(s, CH_OCC_PACKED, na, nao, no, nv, c->ct.get(), c->atom_ptr.get(), c->qij.get());
+    launch_charges(s, CH_VIR_PACKED, na, nao, no, nv, c->ct.get(), c->atom_ptr.get(), c->qab.get());
+    dgemm(c->blas, false, false, na, nij, na, 1.0, c->gamj.get(), na, c->qij.get(), na, 0.0,
+          c->pij.get(), na);
+    c->qij.release();
+    c->sync();
+    c->timings[0] = t0.ms();
+
+    // screening (stda.f:593-632)
+    Timer t1;
+    c->de.resize(c->npair);
+    c->kdiag.resize(c->npair);
+    c->flags.resize(c->npair);
+    launch_screen(s, na, no, nv, c->eps.get(), c->jdiag.get(), c->pia.get(), c->qia.get(), c->ak,
+                  c->thr, c->fthr, par->kshift, par->kshift ? par->kshift_max_ev : 0.5,
+                  par->kshift ? par->kshift_width_ev : 0.1, par->kshift ? par->kshift_steep : 4.0,
+                  c->de.get(), c->kdiag.get(), c->flags.get());
+    std::vector<unsigned char> fl(c->npair);
+    std::vector<double> deh(c->npair);
+    c->flags.download(fl.data(), fl.size(), s);
+    c->de.download(deh.data(), deh.size(), s);
+    c->sync();
+    // order-preserving lists (the pair loop order of stda.f:600-631)
+    std::vector<int32_t> prim, sec;
+    for (int64_t p = 0; p < c->npair; ++p) {
+        if (fl[p] & 1) prim.push_back((int32_t)p);
+        if (fl[p] & 2) sec.push_back((int32_t)p);
+    }
+    c->nprim = (int64_t)prim.size();
+    c->nsec = (int64_t)sec.size();
+    if (c->nprim < 1) throw Error(STDA_HIP_ERR_NO_CSF, "no CSF, increase energy threshold (-e option)");
+    // energy order of the primaries (stda.f:651-668; ties keep pair order)
+    std::stable_sort(prim.begin(), prim.end(), [&](int32_t a, int32_t b) { return deh[a] < deh[b]; });
+    c->timings[1] = t1.ms();
+
+    // PT2 selection (stda.f:1773-1863)
+    Timer t2;
+    std::vector<double> pt2(c->npair, 0.0);
+    std::vector<unsigned char> incl(c->npair, 0);
+    if (c->nsec > 0) {
+        // tiles of the (k,c) grid holding secondaries and of the (j,b) grid holding primaries
+        auto make_tiles = [&](int tocc, int tvir, unsigned char bit, std::vector<int32_t> &occ,
+                              std::vector<int32_t> &vir) {
+            int nt = 0;
+            for (int k0 = 0; k0 < no; k0 += tocc)
+                for (int c0 = 0; c0 < nv; c0 += tvir) {
+                    bool any = false;
+                    for (int k = k0; k < std::min(no, k0 + tocc) && !any; ++k)
+                        for (int cc = c0; cc < std::min(nv, c0 + tvir); ++cc)
+                            if (fl[(int64_t)k * nv + cc] & bit) { any = true; break; }
+                    if (!any) continue;
+                    for (int k = 0; k < tocc; ++k) occ.push_back(k0 + k < no ? k0 + k : -1);
+                    for (int cc = 0; cc < tvir; ++cc) vir.push_back(c0 + cc < nv ? c0 + cc : -1);
+                    ++nt;
+                }
+            return nt;
+        };
+        std::vector<int32_t> ro, rv, co, cv;
+        int nrt = make_tiles(TK, TC, 2, ro, rv);
+        int nct = make_tiles(TJ, TB, 1, co, cv);
+        DBuf<int32_t> rod, rvd, cod, cvd;
+        rod.upload(ro.data(), ro.size(), s); rvd.upload(rv.data(), rv.size(), s);
+        cod.upload(co.data(), co.size(), s); cvd.upload(cv.data(), cv.size(), s);
+        TileSet rows{rod.get(), rvd.get(), nrt}, cols{cod.get(), cvd.get(), nct};
+        DBuf<double> rowsum(c->npair);
+        launch_pt2_rows(s, rows, cols, na, no, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                        c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1, c->ak,
+                        rowsum.get());
+        std::vector<double> rs(c->npair);
+        rowsum.download(rs.data(), rs.size(), s);
+        c->sync();
+        int64_t nexcl = 0;
+        for (int32_t p : sec) {
+            if (rs[p] > c->thrp) incl[p] = 1;
+            else { fl[p] |= 4; ++nexcl; }
+        }
+        if (nexcl > 0) {
+            c->flags.upload(fl.data(), fl.size(), s);
+            std::vector<int32_t> ro2, rv2;
+            int nrt2 = make_tiles(TK, TC, 4, ro2, rv2);
+            DBuf<int32_t> ro2d, rv2d;
+            ro2d.upload(ro2.data(), ro2.size(), s); rv2d.upload(rv2.data(), rv2.size(), s);
+            TileSet rows2{ro2d.get(), rv2d.get(), nrt2};
+            DBuf<double> colsum(c->npair);
+            launch_pt2_cols(s, rows2, cols, na, no, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                            c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1, c->ak,
+                            colsum.get());
+            std::vector<double> cs(c->npair);
+            colsum.download(cs.data(), cs.size(), s);
+            c->sync();
+            for (int32_t p : prim) pt2[p] = cs[p];
+        }
+    }
+    c->timings[2] = t2.ms();
+
+    // CSF list: sorted primaries (ed lowered by PT2), then selected secondaries
+    c->pair_h.clear();
+    c->ed_h.clear();
+    for (int32_t p : prim) { c->pair_h.push_back(p); c->ed_h.push_back(deh[p] - pt2[p]); }
+    for (int32_t p : sec)
+        if (incl[p]) { c->pair_h.push_back(p); c->ed_h.push_back(deh[p]); }
+    c->nci = (int64_t)c->pair_h.size();
+    c->npt = c->nci - c->nprim;
+    c->pair.upload(c->pair_h.data(), c->nci, s);
+    c->ed.upload(c->ed_h.data(), c->nci, s);
+    // diagonal corrections of the AO-path operators: ed - ak K_ii + J_ii (and
+    // ed - ak K_ii for the small-ax A+B / diagonal A-B)
+    std::vector<double> kd(c->npair), jd((size_t)nv * no);
+    c->kdiag.download(kd.data(), kd.size(), s);
+    c->jdiag.download(jd.data(), jd.size(), s);
+    c->sync();
+    std::vector<double> corr(c->nci), corr_s(c->nci);
+    for (int64_t I = 0; I < c->nci; ++I) {
+        int64_t p = c->pair_h[I];
+        int64_t i = p / nv, a = p % nv;
+        corr[I] = c->ed_h[I] - c->ak * kd[p] + jd[a + (int64_t)nv * i];
+        corr_s[I] = c->ed_h[I] - c->ak * kd[p];
+    }
+    c->corr.upload(corr.data(), corr.size(), s);
+    c->corr_small.upload(corr_s.data(), corr_s.size(), s);
+    gather_moments(c);
+    c->sync();
+    c->timings[5] = ttot.ms();
+    c->ready = true;
+}
+
+// explicit A (nci x nci) on the device, column-major
+static void build_a_device(stda_hip_ctx *c, DBuf<double> &A) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    std::vector<int32_t> csf(c->npair, -1);
+    for (int64_t I = 0; I < n; ++I) csf[c->pair_h[I]] = (int32_t)I;
+    std::vector<int32_t> occ, vir;
+    int nt = 0;
+    for (int k0 = 0; k0 < c->no; k0 += TK)
+        for (int c0 = 0; c0 < c->nv; c0 += TC) {
+            bool any = false;
+            for (int k = k0; k < std::min(c->no, k0 + TK) && !any; ++k)
+                for (int cc = c0; cc < std::min(c->nv, c0 + TC); ++cc)
+                    if (csf[(int64_t)k * c->nv + cc] >= 0) { any = true; break; }
+            if (!any) continue;
+            for (int k = 0; k < TK; ++k) occ.push_back(k0 + k < c->no ? k0 + k : -1);
+            for (int cc = 0; cc < TC; ++cc) vir.push_back(c0 + cc < c->nv ? c0 + cc : -1);
+            ++nt;
+        }
+    DBuf<int32_t> od, vd, csfd;
+    od.upload(occ.data(), occ.size(), s);
+    vd.upload(vir.data(), vir.size(), s);
+    csfd.upload(csf.data(), csf.size(), s);
+    A.resize((size_t)n * n);
+    A.zero(s);
+    launch_build_a(s, TileSet{od.get(), vd.get(), nt}, c->ncent, c->no, c->nv, c->pia.get(),
+                   c->qia.get(), c->pij.get(), c->qab.get(), csfd.get(), c->ak, A.get(), n);
+    // diagonal = ed (rtdamat: hci(i,i) = ed(i))
+    for (int64_t I = 0; I < n; ++I)
+        STDA_HIP_CHECK(hipMemcpyAsync(A.get() + I * (n + 1), &c->ed_h[I], 8, hipMemcpyHostToDevice, s));
+    c->sync();
+}
+
+// (ia|jb)_K for all CSF pairs (n x n) and (ib|ja)_K (n x n), device
+static void build_k_and_exchange(stda_hip_ctx *c, DBuf<double> &K, DBuf<double> &E) {
+    const int64_t n = c->nci;
+    const int na = c->ncent;
+    hipStream_t s = c->stream;
+    DBuf<double> pq((size_t)na * n), qq((size_t)na * n);
+    for (int64_t I = 0; I < n; ++I) {
+        int64_t p = c->pair_h[I];
+        STDA_HIP_CHECK(hipMemcpyAsync(pq.get() + (size_t)na * I, c->pia.get() + (size_t)na * p, na * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        STDA_HIP_CHECK(hipMemcpyAsync(qq.get() + (size_t)na * I, c->qia.get() + (size_t)na * p, na * 8,
+                                      hipMemcpyDeviceToDevice, s));
+    }
+    K.resize((size_t)n * n);
+    dgemm(c->blas, true, false, n, n, na, 1.0, pq.get(), na, qq.get(), na, 0.0, K.get(), n);
+    E.resize((size_t)n * n);
+    launch_exchange(s, n, na, c->nv, c->pair.get(), c->pia.get(), c->qia.get(), E.get(), n);
+    c->sync();
+}
+
+static std::vector<double> download_matrix(stda_hip_ctx *c, DBuf<double> &A, int64_t n) {
+    std::vector<double> h((size_t)n * n);
+    A.download(h.data(), h.size(), c->stream);
+    c->sync();
+    return h;
+}
+
+// TDA / RPA property tables (5 per root) and alpha_SOS
+static void tda_properties(stda_hip_ctx *c, const double *X, const std::vector<double> &w,
+                           int velcorr, double *props, double *alpha) {
+    int k = (int)w.size();
+    std::vector<double> T = moment_products(c, X, k);
+    std::vector<double> T2;
+    if (velcorr && c->have_moments && k > 0) {
+        DBuf<double> Xn((size_t)c->nci * k);
+        apply_ops(c, OP_BHALF, -1, k, X, c->nci, Xn.get(), c->nci, nullptr, 0);
+        std::vector<double> sc(k);
+        for (int j = 0; j < k; ++j) sc[j] = 1.0 / (w[j] + 1.0e-8);   // apbtrafo.f:63-69
+        DBuf<double> scd;
+        scd.upload(sc.data(), k, c->stream);
+        launch_scale_cols(c->stream, c->nci, k, Xn.get(), scd.get());
+        T2 = moment_products(c, Xn.get(), k);
+    }
+    double sak = std::sqrt(c->ak);
+    double a6[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < k; ++j) {
+        const double *t = &T[(size_t)9 * j];
+        double de = w[j], ef = 1.0 / (de + 1.0e-8);
+        double l[3] = {t[0] * sak, t[1] * sak, t[2] * sak};
+        double m[3] = {t[3] * sak, t[4] * sak, t[5] * sak};
+        double v[3] = {t[6] * sak, t[7] * sak, t[8] * sak};
+        double fl = de * (2.0 / 3.0) * (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+        double fv = ef * (2.0 / 3.0) * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double rl = -kRot * (l[0] * m[0] + l[1] * m[1] + l[2] * m[2]);
+        double rv = -kRot * (v[0] * m[0] + v[1] * m[1] + v[2] * m[2]) * ef;
+        if (velcorr && !T2.empty()) {   // apbtrafo.f:180-262
+            const double *u = &T2[(size_t)9 * j];
+            double hilf = 1.0 - std::exp(-150.0 * de * de);
+            double efmod = 1.0 / (std::pow(de, hilf) + 1.0e-8);
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q)
+                s += t[6 + q] * t[3 + q] + u[6 + q] * t[3 + q] + t[6 + q] * u[3 + q];
+            rv = -kRot * s * c->ak * efmod;
+        }
+        if (props) {
+            props[5 * j + 0] = fl; props[5 * j + 1] = fv; props[5 * j + 2] = rl;
+            props[5 * j + 3] = rv; props[5 * j + 4] = 0.0;
+        }
+        a6[0] += l[0] * l[0] * ef; a6[1] += l[0] * l[1] * ef; a6[2] += l[1] * l[1] * ef;
+        a6[3] += l[0] * l[2] * ef; a6[4] += l[1] * l[2] * ef; a6[5] += l[2] * l[2] * ef;
+    }
+    if (alpha)
+        for (int q = 0; q < 6; ++q) alpha[q] = 2.0 * a6[q];
+}
+
+static void rpa_properties(stda_hip_ctx *c, const double *XpY, const double *XmY,
+                           const std::vector<double> &w, double *props, double *alpha) {
+    int k = (int)w.size();
+    std::vector<double> Tp = moment_products(c, XpY, k), Tm = moment_products(c, XmY, k);
+    double sak = std::sqrt(c->ak);
+    double a6[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < k; ++j) {
+        const double *tp = &Tp[(size_t)9 * j], *tm = &Tm[(size_t)9 * j];
+        double de = w[j], ef = 1.0 / (de + 1.0e-8);
+        double l[3] = {tp[0] * sak, tp[1] * sak, tp[2] * sak};
+        double m[3] = {tm[3] * sak, tm[4] * sak, tm[5] * sak};
+        double v[3] = {tm[6] * sak, tm[7] * sak, tm[8] * sak};
+        if (props) {
+            props[5 * j + 0] = de * (2.0 / 3.0) * (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+            props[5 * j + 1] = ef * (2.0 / 3.0) * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            props[5 * j + 2] = -kRot * (l[0] * m[0] + l[1] * m[1] + l[2] * m[2]);
+            props[5 * j + 3] = -kRot * (v[0] * m[0] + v[1] * m[1] + v[2] * m[2]) * ef;
+            props[5 * j + 4] = 0.0;
+        }
+        a6[0] += l[0] * l[0] * ef; a6[1] += l[0] * l[1] * ef; a6[2] += l[1] * l[1] * ef;
+        a6[3] += l[0] * l[2] * ef; a6[4] += l[1] * l[2] * ef; a6[5] += l[2] * l[2] * ef;
+    }
+    if (alpha)
+        for (int q = 0; q < 6; ++q) alpha[q] = 2.0 * a6[q];
+}
+
+// ============================================================================================
+// C ABI
+#define STDA_TRY(ctx, ...)                                            \
+    try {                                                             \
+        __VA_ARGS__;                                                  \
+        return STDA_HIP_OK;                                           \
+    } catch (const Error &e) {                                        \
+        if (ctx) (ctx)->last_error = e.what();                        \
+        return e.status;                                              \
+    } catch (const std::exception &e) {                               \
+        if (ctx) (ctx)->last_error = e.what();                        \
+        return STDA_HIP_ERR_DEVICE;                                   \
+    }
+
+extern "C" {
+
+int stda_hip_create(stda_hip_ctx **out, int device) {
+    if (!out) return STDA_HIP_ERR_ARG;
+    stda_hip_ctx *c = new stda_hip_ctx();
+    try {
+        if (device >= 0) STDA_HIP_CHECK(hipSetDevice(device));
+        STDA_HIP_CHECK(hipGetDevice(&c->device));
+        STDA_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        STDA_BLAS_CHECK(rocblas_create_handle(&c->blas));
+        STDA_BLAS_CHECK(rocblas_set_stream(c->blas, c->stream));
+        STDA_BLAS_CHECK(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
+    } catch (const Error &e) {
+        delete c;
+        return e.status;
+    }
+    *out = c;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_destroy(stda_hip_ctx *c) {
+    if (!c) return STDA_HIP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return STDA_HIP_OK;
+}
+
+const char *stda_hip_error_string(int st) {
+    switch (st) {
+        case STDA_HIP_OK: return "ok";
+        case STDA_HIP_ERR_ARG: return "invalid argument";
+        case STDA_HIP_ERR_NO_CSF: return "no CSF, increase energy threshold (-e option)";
+        case STDA_HIP_ERR_DEVICE: return "HIP device or library error";
+        case STDA_HIP_ERR_NOMEM: return "device allocation failed";
+        case STDA_HIP_ERR_STATE: return "setup must be called before solving";
+        case STDA_HIP_ERR_NOT_CONVERGED: return "iterative solver did not converge";
+        case STDA_HIP_ERR_AMB_NOT_PD: return "matrix power impossible";
+        case STDA_HIP_ERR_NO_ROOTS: return "internal error in diag";
+        case STDA_HIP_ERR_TOO_LARGE: return "problem too large for this entry point";
+        default: return "unknown status";
+    }
+}
+
+const char *stda_hip_last_error(const stda_hip_ctx *c) { return c ? c->last_error.c_str() : ""; }
+
+int stda_hip_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
+                   int64_t *nci, int64_t *nprim, int64_t *nsec) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        do_setup(c, sys, par);
+        if (nci) *nci = c->nci;
+        if (nprim) *nprim = c->nprim;
+        if (nsec) *nsec = c->nsec;
+    })
+}
+
+int stda_hip_get_csf(stda_hip_ctx *c, int32_t *iconf, double *ed) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    for (int64_t I = 0; I < c->nci; ++I) {
+        int64_t p = c->pair_h[I];
+        if (iconf) {
+            iconf[I] = (int32_t)(p / c->nv) + 1;
+            iconf[I + c->nci] = (int32_t)(c->no + p % c->nv) + 1;
+        }
+        if (ed) ed[I] = c->ed_h[I];
+    }
+    return STDA_HIP_OK;
+}
+
+int stda_hip_solve_tda(stda_hip_ctx *c, int32_t nroots, int32_t capacity, int32_t velcorr,
+                       double tol, int32_t *nfound, double *omega, double *x, double *props,
+                       double *alpha_sos) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        Timer t3;
+        if (tol <= 0.0) tol = 1.0e-8;
+        std::vector<double> w;
+        DBuf<double> X;
+        const int64_t n = c->nci;
+        if (nroots > 0) {
+            davidson_tda(c, nroots, tol, w, X);
+        } else {
+            // every root in (0, Ethr] (ssyevr RANGE='V', stda.f:1143-1150)
+            int64_t cnt = 0;
+            for (double e : c->ed_h) cnt += e <= c->thr;
+            int k = (int)std::min<int64_t>(n, cnt + 8);
+            while (true) {
+                davidson_tda(c, k, tol, w, X);
+                if (w.back() > c->thr || k >= n) break;
+                k = (int)std::min<int64_t>(n, 2 * (int64_t)k);
+            }
+            int keep = 0;
+            for (double e : w) keep += (e > 0.0 && e <= c->thr);
+            int first = 0;
+            while (first < (int)w.size() && w[first] <= 0.0) ++first;
+            if (first > 0) {
+                STDA_HIP_CHECK(hipMemcpyAsync(X.get(), X.get() + n * first, (size_t)n * keep * 8,
+                                              hipMemcpyDeviceToDevice, c->stream));
+            }
+            w = std::vector<double>(w.begin() + first, w.begin() + first + keep);
+        }
+        int k = (int)w.size();
+        if (k < 1) throw Error(STDA_HIP_ERR_NO_ROOTS, "internal error in diag");
+        if (capacity < k) throw Error(STDA_HIP_ERR_ARG, "capacity smaller than the number of roots");
+        c->timings[3] = t3.ms();
+        Timer t4;
+        *nfound = k;
+        std::copy(w.begin(), w.end(), omega);
+        if (x) {
+            X.download(x, (size_t)n * k, c->stream);
+            c->sync();
+        }
+        tda_properties(c, X.get(), w, velcorr, props, alpha_sos);
+        c->timings[4] = t4.ms();
+    })
+}
+
+int stda_hip_solve_rpa(stda_hip_ctx *c, int32_t nroots, int32_t capacity, double tol,
+                       int32_t *nfound, double *omega, double *x, double *y, double *props,
+                       double *alpha_sos) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        Timer t3;
+        if (tol <= 0.0) tol = 1.0e-8;
+        std::vector<double> w;
+        DBuf<double> XpY, XmY;
+        const int64_t n = c->nci;
+        if (nroots > 0) {
+            davidson_rpa(c, nroots, tol, w, XpY, XmY);
+        } else {
+            int64_t cnt = 0;
+            for (double e : c->ed_h) cnt += e <= c->thr;
+            int k = (int)std::min<int64_t>(n, cnt + 8);
+            while (true) {
+                davidson_rpa(c, k, tol, w, XpY, XmY);
+                if (w.back() > c->thr || k >= n) break;
+                k = (int)std::min<int64_t>(n, 2 * (int64_t)k);
+            }
+            int keep = 0;
+            for (double e : w) keep += (e * e <= c->thr * c->thr);   // srpapack.f:84-89
+            w.resize(keep);
+        }
+        int k = (int)w.size();
+        if (k < 1) throw Error(STDA_HIP_ERR_NO_ROOTS, "RPA diag failed");
+        if (capacity < k) throw Error(STDA_HIP_ERR_ARG, "capacity smaller than the number of roots");
+        c->timings[3] = t3.ms();
+        Timer t4;
+        *nfound = k;
+        std::copy(w.begin(), w.end(), omega);
+        if (x || y) {   // X = ((X+Y) + (X-Y))/2, Y = (X+Y) - X (srpapack.f:155-162)
+            std::vector<double> hp((size_t)n * k), hm((size_t)n * k);
+            XpY.download(hp.data(), hp.size(), c->stream);
+            XmY.download(hm.data(), hm.size(), c->stream);
+            c->sync();
+            for (size_t q = 0; q < hp.size(); ++q) {
+                double xv = 0.5 * (hp[q] + hm[q]);
+                if (x) x[q] = xv;
+                if (y) y[q] = hp[q] - xv;
+            }
+        }
+        rpa_properties(c, XpY.get(), XmY.get(), w, props, alpha_sos);
+        c->timings[4] = t4.ms();
+    })
+}
+
+int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double tol,
+                   double *alpha, double *xpy_out) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    if (nfreq < 1 || !omega || !alpha) return STDA_HIP_ERR_ARG;
+    if (!c->have_moments) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        Timer t3;
+        if (tol <= 0.0) tol = 1.0e-8;
+        const int64_t n = c->nci;
+        hipStream_t s = c->stream;
+        const int nrhs = 3 * nfreq;
+        // right-hand sides -2 mu (linear_response.f:98-116): mu = xl of each CSF
+        DBuf<double> rhs((size_t)n * 3);
+        STDA_HIP_CHECK(hipMemcpyAsync(rhs.get(), c->mom.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
+        std::vector<double> m2(3, -2.0);
+        DBuf<double> m2d;
+        m2d.upload(m2.data(), 3, s);
+        launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
+        int maxdim = (int)std::min<int64_t>(n, std::max(24 * nrhs, 128));
+        DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
+        DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd, rhsV;
+        // initial space: preconditioned right-hand sides
+        STDA_HIP_CHECK(hipMemcpyAsync(V.get(), rhs.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
+        std::vector<double> zero3(3, 0.0);
+        DBuf<double> z3;
+        z3.upload(zero3.data(), 3, s);
+        launch_precondition(s, n, 3, V.get(), c->ed.get(), z3.get(), V.get());
+        int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), 3, tmp);
+        std::vector<double> Ap((size_t)maxdim * maxdim), Am((size_t)maxdim * maxdim), Br((size_t)maxdim * 3);
+        std::vector<double> rhsnorm(3);
+        {
+            DBuf<double> rn(3);
+            launch_col_norms(s, n, 3, rhs.get(), rn.get());
+            rn.download(rhsnorm.data(), 3, s);
+            c->sync();
+        }
+        std::vector<double> sol;   // (m x nrhs) reduced X+Y
+        bool done = false;
+        for (int it = 0; it < 300 && !done; ++it) {
+            apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
+                      M.get() + n * m, n);
+            int mt = m + mnew;
+            DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
+            dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, P.get() + n * m, n, 0.0, Hc.get(), mt);
+            dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, M.get() + n * m, n, 0.0,
+                  Hc.get() + (size_t)mt * mnew, mt);
+            dgemm(c->blas, true, false, mnew, 3, n, 1.0, V.get() + n * m, n, rhs.get(), n, 0.0,
+                  Hc.get() + (size_t)2 * mt * mnew, mnew);
+            std::vector<double> hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
+            Hc.download(hc.data(), hc.size(), s);
+            c->sync();
+            for (int j = 0; j < mnew; ++j)
+                for (int i = 0; i < mt; ++i) {
+                    double vp = hc[i + (size_t)mt * j], vm = hc[(size_t)mt * mnew + i + (size_t)mt * j];
+                    Ap[i + (size_t)maxdim * (m + j)] = vp; Ap[(m + j) + (size_t)maxdim * i] = vp;
+                    Am[i + (size_t)maxdim * (m + j)] = vm; Am[(m + j) + (size_t)maxdim * i] = vm;
+                }
+            for (int r = 0; r < 3; ++r)
+                for (int j = 0; j < mnew; ++j) Br[(m + j) + (size_t)maxdim * r] = hc[(size_t)2 * mt * mnew + j + (size_t)mnew * r];
+            m = mt;
+            std::vector<double> ap((size_t)m * m), am((size_t)m * m);
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < m; ++i) {
+                    ap[i + (size_t)m * j] = 0.5 * (Ap[i + (size_t)maxdim * j] + Ap[j + (size_t)maxdim * i]);
+                    am[i + (size_t)m * j] = 0.5 * (Am[i + (size_t)maxdim * j] + Am[j + (size_t)maxdim * i]);
+                }
+            // Am^-1 via its eigen-decomposition (A-B positive definite)
+            std::vector<double> U(am), ev;
+            host_syev(m, U, ev);
+            if (ev[0] <= 0.0) throw Error(STDA_HIP_ERR_AMB_NOT_PD, "A-B not positive definite");
+            std::vector<double> aminv((size_t)m * m, 0.0);
+            for (int q = 0; q < m; ++q)
+                for (int j = 0; j < m; ++j) {
+                    double f = U[j + (size_t)m * q] / ev[q];
+                    for (int i = 0; i < m; ++i) aminv[i + (size_t)m * j] += U[i + (size_t)m * q] * f;
+                }
+            // reduced solves: (Ap - w^2 Am^-1) a = V^T r ; b = w Am^-1 a
+            std::vector<double> coef((size_t)m * 2 * nrhs);   // [a | b] per rhs
+            for (int f = 0; f < nfreq; ++f) {
+                double w = omega[f];
+                std::vector<double> Mr((size_t)m * m);
+                for (size_t q = 0; q < Mr.size(); ++q) Mr[q] = ap[q] - w * w * aminv[q];
+                // symmetric solve through eigen-decomposition
+                std::vector<double> Uq(Mr), eq;
+                host_syev(m, Uq, eq);
+                for (int r = 0; r < 3; ++r) {
+                    std::vector<double> a(m, 0.0);
+                    for (int q = 0; q < m; ++q) {
+                        double proj = 0.0;
+                        for (int i = 0; i < m; ++i) proj += Uq[i + (size_t)m * q] * Br[i + (size_t)maxdim * r];
+                        proj /= eq[q];
+                        for (int i = 0; i < m; ++i) a[i] += Uq[i + (size_t)m * q] * proj;
+                    }
+                    int col = f * 3 + r;
+                    for (int i = 0; i < m; ++i) coef[i + (size_t)m * col] = a[i];
+                    for (int i = 0; i < m; ++i) {
+                        double bsum = 0.0;
+                        for (int j = 0; j < m; ++j) bsum += aminv[i + (size_t)m * j] * a[j];
+                        coef[i + (size_t)m * (nrhs + col)] = w * bsum;
+                    }
+                }
+            }
+            sol.assign(coef.begin(), coef.begin() + (size_t)m * nrhs);
+            // residuals R1 = P a - w V b - r ; R2 = M b - w V a
+            Yd.upload(coef.data(), coef.size(), s);
+            DBuf<double> VB((size_t)n * nrhs), VA((size_t)n * nrhs), MB((size_t)n * nrhs);
+            dgemm(c->blas, false, false, n, nrhs, m, 1.0, P.get(), n, Yd.get(), m, 0.0, R.get(), n);
+            dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0, VB.get(), n);
+            dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, VA.get(), n);
+            dgemm(c->blas, false, false, n, nrhs, m, 1.0, M.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0,
+                  R.get() + n * nrhs, n);
+            std::vector<double> wcol(nrhs);
+            for (int f = 0; f < nfreq; ++f)
+                for (int r = 0; r < 3; ++r) wcol[f * 3 + r] = omega[f];
+            DBuf<double> wd;
+            wd.upload(wcol.data(), nrhs, s);
+            launch_residual(s, n, nrhs, R.get(), VB.get(), wd.get(), R.get());
+            launch_residual(s, n, nrhs, R.get() + n * nrhs, VA.get(), wd.get(), R.get() + n * nrhs);
+            std::vector<double> onev(3, 1.0);
+            DBuf<double> oned;
+            oned.upload(onev.data(), 3, s);
+            for (int f = 0; f < nfreq; ++f)   // subtract the right-hand sides: R1 -= r
+                launch_residual(s, n, 3, R.get() + n * 3 * f, rhs.get(), oned.get(),
+                                R.get() + n * 3 * f);
+            launch_col_norms(s, n, 2 * nrhs, R.get(), nrm.get());
+            std::vector<double> rn(2 * nrhs);
+            nrm.download(rn.data(), rn.size(), s);
+            c->sync();
+            std::vector<int> un;
+            for (int j = 0; j < 2 * nrhs; ++j)
+                if (!(rn[j] < tol * std::max(1.0, rhsnorm[j % 3]))) un.push_back(j);
+            if (un.empty() || m == n) { done = true; break; }
+            int nun = (int)un.size();
+            for (int q = 0; q < nun; ++q)
+                if (un[q] != q)
+                    STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8, hipMemcpyDeviceToDevice, s));
+            std::vector<double> zeros(nun, 0.0);
+            DBuf<double> zd;
+            zd.upload(zeros.data(), nun, s);
+            launch_precondition(s, n, nun, R.get(), c->ed.get(), zd.get(), R.get());
+            if (m + nun > maxdim) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
+            STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * nun * 8, hipMemcpyDeviceToDevice, s));
+            mnew = orthonormalize(c, V.get(), m, V.get() + n * m, nun, tmp);
+            if (mnew == 0) { done = true; break; }
+        }
+        if (!done) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response solver did not converge");
+        // alpha_rc = -2 mu_r^T (X+Y)_c (linear_response.f:147-172)
+        Yd.upload(sol.data(), sol.size(), s);
+        dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
+        DBuf<double> Ad((size_t)3 * nrhs);
+        dgemm(c->blas, true, false, 3, nrhs, n, -2.0, c->mom.get(), n, XP.get(), n, 0.0, Ad.get(), 3);
+        std::vector<double> ah((size_t)3 * nrhs);
+        Ad.download(ah.data(), ah.size(), s);
+        if (xpy_out) XP.download(xpy_out, (size_t)n * nrhs, s);
+        c->sync();
+        for (int f = 0; f < nfreq; ++f)
+            for (int col = 0; col < 3; ++col)
+                for (int r = 0; r < 3; ++r) alpha[r + 3 * col + 9 * f] = ah[r + 3 * (f * 3 + col)];
+        c->timings[3] = t3.ms();
+    })
+}
+
+int stda_hip_build_a(stda_hip_ctx *c, double *a) {
+    if (!c || !a) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        if (c->nci > 60000) throw Error(STDA_HIP_ERR_TOO_LARGE, "explicit A limited to nci <= 60000");
+        DBuf<double> A;
+        build_a_device(c, A);
+        A.download(a, (size_t)c->nci * c->nci, c->stream);
+        c->sync();
+    })
+}
+
+int stda_hip_build_apb_amb(stda_hip_ctx *c, double *apb, double *amb) {
+    if (!c || !apb || !amb) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci;
+        if (n > 20000) throw Error(STDA_HIP_ERR_TOO_LARGE, "explicit A+-B limited to nci <= 20000");
+        DBuf<double> A, K, E;
+        build_a_device(c, A);
+        build_k_and_exchange(c, K, E);
+        std::vector<double> a = download_matrix(c, A, n), k = download_matrix(c, K, n),
+                            e = download_matrix(c, E, n);
+        const double ak = c->ak, ax = c->ax;
+        for (int64_t i = 0; i < n; ++i) {   // rrpamat, stda.f:1896-1962
+            for (int64_t j = 0; j < i; ++j) {
+                int64_t ij = i * (i + 1) / 2 + j;
+                double kk = k[i + n * j], ee = e[i + n * j];
+                double jj = ak * kk - a[i + n * j];   // (ij|ab)
+                if (c->small_ax) { apb[ij] = 2.0 * ak * kk; amb[ij] = 0.0; }
+                else { apb[ij] = 2.0 * ak * kk - ax * ee - jj; amb[ij] = ax * ee - jj; }
+            }
+            int64_t ii = i * (i + 1) / 2 + i;
+            double kd = k[i + n * i];
+            if (c->small_ax) { amb[ii] = c->ed_h[i] - ak * kd; apb[ii] = c->ed_h[i] + ak * kd; }
+            else { amb[ii] = c->ed_h[i] - ak * kd + ax * kd; apb[ii] = c->ed_h[i] - ax * kd + ak * kd; }
+        }
+    })
+}
+
+int stda_hip_build_bhalf(stda_hip_ctx *c, double *bh) {
+    if (!c || !bh) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci;
+        if (n > 20000) throw Error(STDA_HIP_ERR_TOO_LARGE, "explicit 0.5B limited to nci <= 20000");
+        DBuf<double> K, E;
+        build_k_and_exchange(c, K, E);
+        std::vector<double> k = download_matrix(c, K, n), e = download_matrix(c, E, n);
+        for (int64_t i = 0; i < n; ++i) {   // rtdacorr, apbtrafo.f:604-625
+            for (int64_t j = 0; j < i; ++j)
+                bh[i * (i + 1) / 2 + j] = 0.5 * c->ak * k[i + n * j] - 0.5 * c->ax * e[i + n * j];
+            double kd = k[i + n * i];
+            bh[i * (i + 1) / 2 + i] = 0.5 * (c->ak * kd - c->ax * kd);
+        }
+    })
+}
+
+int stda_hip_matvec(stda_hip_ctx *c, int32_t which, int32_t nvec, const double *v, double *av) {
+    if (!c || !v || !av || which < 0 || which > 3 || nvec < 0) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci;
+        DBuf<double> V, AV((size_t)n * nvec);
+        V.upload(v, (size_t)n * nvec, c->stream);
+        apply_ops(c, which, -1, nvec, V.get(), n, AV.get(), n, nullptr, 0);
+        AV.download(av, (size_t)n * nvec, c->stream);
+        c->sync();
+    })
+}
+
+int stda_hip_timings(const stda_hip_ctx *c, double *ms6) {
+    if (!c || !ms6) return STDA_HIP_ERR_ARG;
+    for (int i = 0; i < 6; ++i) ms6[i] = c->timings[i];
+    return STDA_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,758 @@
+// HIP kernels of the MI355X excited-state core (gfx950, wave64).
+//
+// Index conventions (0-based): occupied i in [0,no), virtual a in [0,nv),
+// pair p = i*nv + a; charge arrays are (ncent, npairs) with the atom index
+// fastest; packed occupied/virtual pairs use lin0(x,y) = max*(max+1)/2 + min,
+// the 0-based form of the reference's lin() (stda.f:2174-2181).
+#include "common.h"
+
+#include <hip/hip_runtime.h>
+
+namespace stda {
+
+__device__ __forceinline__ int64_t lin0(int64_t x, int64_t y) {
+    int64_t a = x > y ? x : y, b = x > y ? y : x;
+    return a * (a + 1) / 2 + b;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// ----------------------------------------------------------------------------
+// gamma^J, gamma^K (stda.f:509-532): Mataga-Nishimoto-Ohno-Klopman forms
+//   gJ = (R^bJ + (ax eta)^-bJ)^(-1/bJ),  gK = (R^aK + eta^-aK)^(-1/aK)
+// eta = (eta_ZA + eta_ZB)/2.  ax = 0 gives gJ = 0 exactly (1/0^b = +Inf).
+__global__ void gamma_kernel(int n, const double *__restrict__ xyz4, const double *__restrict__ eta,
+                             double ax, double ak_exp, double bj_exp, double *gamj, double *gamk) {
+    int j = blockIdx.x * 16 + threadIdx.x;
+    int i = blockIdx.y * 16 + threadIdx.y;
+    if (i >= n || j >= n) return;
+    int zi = (int)xyz4[4 * i + 3], zj = (int)xyz4[4 * j + 3];
+    double e = 0.5 * (eta[zi - 1] + eta[zj - 1]);
+    double dx = xyz4[4 * i] - xyz4[4 * j];
+    double dy = xyz4[4 * i + 1] - xyz4[4 * j + 1];
+    double dz = xyz4[4 * i + 2] - xyz4[4 * j + 2];
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double xj = e * ax;
+    double gj = (xj == 0.0) ? 0.0 : 1.0 / pow(pow(r, bj_exp) + 1.0 / pow(xj, bj_exp), 1.0 / bj_exp);
+    double gk = 1.0 / pow(pow(r, ak_exp) + 1.0 / pow(e, ak_exp), 1.0 / ak_exp);
+    gamj[i + (int64_t)n * j] = gj;
+    gamk[i + (int64_t)n * j] = gk;
+}
+
+void launch_gamma(hipStream_t s, int ncent, const double *xyz4, const double *eta, double ax,
+                  double alphak, double betaj, double *gamj, double *gamk) {
+    dim3 b(16, 16), g((ncent + 15) / 16, (ncent + 15) / 16);
+    hipLaunchKernelGGL(gamma_kernel, g, b, 0, s, ncent, xyz4, eta, ax, alphak, betaj, gamj, gamk);
+}
+
+// ----------------------------------------------------------------------------
+// Loewdin transition charges q^A_pq = sum_{mu on A} C'(mu,p) C'(mu,q)
+// (lo12pop, stda.f:1727-1744).  ct is (moci, nao) row-major with the AOs
+// sorted by atom, so the AOs of atom A are ct[p][atom_ptr[A]:atom_ptr[A+1]).
+// One block per "outer" MO p, staged in LDS; the block covers CH_TI inner MOs.
+constexpr int CH_TI = 32;
+constexpr int CH_LDS = 8192;   // doubles of the outer MO kept in LDS (64 KiB)
+
+__global__ void __launch_bounds__(256) charges_kernel(int mode, int ncent, int nao, int no, int nv,
+                                                      const double *__restrict__ ct,
+                                                      const int *__restrict__ atom_ptr,
+                                                      double *__restrict__ q) {
+    __shared__ double sp[CH_LDS];
+    int outer = blockIdx.y;
+    int p, qbeg, qend;
+    int64_t colbase;
+    if (mode == CH_IA) {
+        p = outer; qbeg = no; qend = no + nv; colbase = (int64_t)outer * nv - no;
+    } else if (mode == CH_OCC_PACKED) {
+        p = outer; qbeg = 0; qend = outer + 1; colbase = (int64_t)outer * (outer + 1) / 2;
+    } else if (mode == CH_VIR_PACKED) {
+        p = no + outer; qbeg = no; qend = no + outer + 1;
+        colbase = (int64_t)outer * (outer + 1) / 2 - no;
+    } else {
+        p = outer; qbeg = outer; qend = outer + 1; colbase = 0;
+    }
+    int q0 = qbeg + blockIdx.x * CH_TI;
+    if (q0 >= qend) return;
+    int nq = min(CH_TI, qend - q0);
+    const double *cp = ct + (int64_t)p * nao;
+    bool in_lds = nao <= CH_LDS;
+    if (in_lds) {
+        for (int m = threadIdx.x; m < nao; m += blockDim.x) sp[m] = cp[m];
+        __syncthreads();
+    }
+    int total = nq * ncent;
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        int A = u % ncent;
+        int qi = q0 + u / ncent;
+        const double *cq = ct + (int64_t)qi * nao;
+        int m0 = atom_ptr[A], m1 = atom_ptr[A + 1];
+        double s = 0.0;
+        if (in_lds)
+            for (int m = m0; m < m1; ++m) s += sp[m] * cq[m];
+        else
+            for (int m = m0; m < m1; ++m) s += cp[m] * cq[m];
+        q[A + (int64_t)ncent * (colbase + qi)] = s;
+    }
+}
+
+void launch_charges(hipStream_t s, ChargeMode mode, int ncent, int nao, int no, int nv,
+                    const double *ct, const int *atom_ptr, double *q) {
+    int nouter, maxinner;
+    switch (mode) {
+        case CH_IA: nouter = no; maxinner = nv; break;
+        case CH_OCC_PACKED: nouter = no; maxinner = no; break;
+        case CH_VIR_PACKED: nouter = nv; maxinner = nv; break;
+        default: nouter = no + nv; maxinner = 1; break;
+    }
+    if (nouter == 0) return;
+    dim3 g((maxinner + CH_TI - 1) / CH_TI, nouter);
+    hipLaunchKernelGGL(charges_kernel, g, dim3(256), 0, s, (int)mode, ncent, nao, no, nv, ct,
+                       atom_ptr, q);
+}
+
+// ----------------------------------------------------------------------------
+// CSF screening (stda.f:593-632): de = (e_a - e_i) - (ii|aa)_J + ak (ia|ia)_K
+// (+ the optional K-dependent shift, stda.f:1750-1768).  One wavefront per pair.
+// flags: bit0 primary (de <= thr), bit1 secondary (thr < de < fthr).
+__global__ void __launch_bounds__(256) screen_kernel(int ncent, int no, int nv,
+                                                     const double *__restrict__ eps,
+                                                     const double *__restrict__ jdiag,
+                                                     const double *__restrict__ pia,
+                                                     const double *__restrict__ qia, double ak,
+                                                     double thr, double fthr, int kshift,
+                                                     double sau, double wau, double steep,
+                                                     double *de_out, double *kdiag,
+                                                     unsigned char *flags) {
+    int lane = threadIdx.x & 63;
+    int64_t npair = (int64_t)no * nv;
+    int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t p = wave; p < npair; p += nwaves) {
+        const double *a = pia + (int64_t)ncent * p;
+        const double *b = qia + (int64_t)ncent * p;
+        double s = 0.0;
+        for (int k = lane; k < ncent; k += 64) s += a[k] * b[k];
+        s = wave_sum(s);
+        if (lane == 0) {
+            int i = (int)(p / nv), l = (int)(p % nv);
+            double de = eps[no + l] - eps[i];
+            de = de - jdiag[l + (int64_t)nv * i];
+            de = de + ak * s;
+            if (kshift) de = de + sau / (1.0 + pow(s / wau, steep));
+            de_out[p] = de;
+            kdiag[p] = s;
+            unsigned char f = 0;
+            if (de <= thr) f |= 1;
+            if (de > thr && de < fthr) f |= 2;
+            flags[p] = f;
+        }
+    }
+}
+
+void launch_screen(hipStream_t s, int ncent, int no, int nv, const double *eps,
+                   const double *jdiag, const double *pia, const double *qia, double ak,
+                   double thr, double fthr, int kshift, double smax, double width,
+                   double steep, double *de, double *kdiag, unsigned char *flags) {
+    int64_t npair = (int64_t)no * nv;
+    int64_t blocks = (npair + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    double sau = smax * 0.036749320, wau = width * 0.036749320;   // stda.f:1764-1765
+    hipLaunchKernelGGL(screen_kernel, dim3((unsigned)blocks), dim3(256), 0, s, ncent, no, nv, eps,
+                       jdiag, pia, qia, ak, thr, fthr, kshift, sau, wau, steep, de, kdiag, flags);
+}
+
+// ----------------------------------------------------------------------------
+// Order-preserving stream compaction of the pair indices whose flag has `bit`.
+constexpr int CMP_THREADS = 256, CMP_ITEMS = 4, CMP_BLOCK = CMP_THREADS * CMP_ITEMS;
+int compact_block_size() { return CMP_BLOCK; }
+
+__global__ void __launch_bounds__(CMP_THREADS) count_kernel(const unsigned char *flags, int64_t n,
+                                                            int bit, int *counts) {
+    __shared__ int ws[CMP_THREADS / 64];
+    int64_t base = (int64_t)blockIdx.x * CMP_BLOCK;
+    int c = 0;
+    for (int r = 0; r < CMP_ITEMS; ++r) {
+        int64_t idx = base + (int64_t)r * CMP_THREADS + threadIdx.x;
+        if (idx < n && (flags[idx] >> bit & 1)) ++c;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < CMP_THREADS / 64; ++w) t += ws[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(CMP_THREADS) compact_kernel(const unsigned char *flags, int64_t n,
+                                                              int bit, const int64_t *offsets,
+                                                              int32_t *out) {
+    // items of thread t: base + t*ITEMS + r (contiguous per thread keeps order)
+    __shared__ int scan[CMP_THREADS];
+    int64_t base = (int64_t)blockIdx.x * CMP_BLOCK + (int64_t)threadIdx.x * CMP_ITEMS;
+    int c = 0;
+    for (int r = 0; r < CMP_ITEMS; ++r) {
+        int64_t idx = base + r;
+        if (idx < n && (flags[idx] >> bit & 1)) ++c;
+    }
+    scan[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < CMP_THREADS; off <<= 1) {
+        int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int64_t pos = offsets[blockIdx.x] + scan[threadIdx.x] - c;
+    for (int r = 0; r < CMP_ITEMS; ++r) {
+        int64_t idx = base + r;
+        if (idx < n && (flags[idx] >> bit & 1)) out[pos++] = (int32_t)idx;
+    }
+}
+
+void launch_count_flags(hipStream_t s, const unsigned char *flags, int64_t n, int bit,
+                        int *block_counts, int nblocks) {
+    hipLaunchKernelGGL(count_kernel, dim3(nblocks), dim3(CMP_THREADS), 0, s, flags, n, bit,
+                       block_counts);
+}
+
+void launch_compact_flags(hipStream_t s, const unsigned char *flags, int64_t n, int bit,
+                          const int64_t *block_offsets, int nblocks, int32_t *out) {
+    hipLaunchKernelGGL(compact_kernel, dim3(nblocks), dim3(CMP_THREADS), 0, s, flags, n, bit,
+                       block_offsets, out);
+}
+
+// ----------------------------------------------------------------------------
+// Gathers / scatters between the CSF list and the full occupied x virtual grid.
+__global__ void gather_f64_kernel(const double *src, const int32_t *idx, int64_t n, double *dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+void launch_gather_f64(hipStream_t s, const double *src, const int32_t *idx, int64_t n,
+                       double *dst) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(gather_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src,
+                       idx, n, dst);
+}
+
+__global__ void scatter_pairs_kernel(const double *v, int64_t ldv, const int32_t *pair, int64_t n,
+                                     double *full, int64_t ldfull) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vec = blockIdx.y;
+    if (i < n) full[pair[i] + ldfull * vec] = v[i + ldv * vec];
+}
+
+void launch_scatter_pairs(hipStream_t s, const double *v, int64_t ldv, const int32_t *pair,
+                          int64_t n, int nvec, double *full, int64_t ldfull) {
+    if (n == 0 || nvec == 0) return;
+    hipLaunchKernelGGL(scatter_pairs_kernel, dim3((unsigned)((n + 255) / 256), nvec), dim3(256), 0,
+                       s, v, ldv, pair, n, full, ldfull);
+}
+
+// out = full[pair] + diag o v
+__global__ void gather_pairs_kernel(const double *full, int64_t ldfull, const int32_t *pair,
+                                    int64_t n, const double *diag, const double *v, int64_t ldv,
+                                    double *out, int64_t ldout) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vec = blockIdx.y;
+    if (i >= n) return;
+    double r = full[pair[i] + ldfull * vec];
+    if (diag) r += diag[i] * v[i + ldv * vec];
+    out[i + ldout * vec] = r;
+}
+
+void launch_gather_pairs(hipStream_t s, const double *full, int64_t ldfull, const int32_t *pair,
+                         int64_t n, int nvec, const double *diag, const double *v, int64_t ldv,
+                         double *out, int64_t ldout) {
+    if (n == 0 || nvec == 0) return;
+    hipLaunchKernelGGL(gather_pairs_kernel, dim3((unsigned)((n + 255) / 256), nvec), dim3(256), 0,
+                       s, full, ldfull, pair, n, diag, v, ldv, out, ldout);
+}
+
+// mom(n, 9): moments of each CSF, element M(io, iv) of the packed arrays
+// (stda.f:1229-1231: ij = lin(io, iv)).
+__global__ void csf_moments_kernel(const double *moments, int64_t npk, const int32_t *pair,
+                                   int64_t n, int no, int nv, double *mom) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    int64_t p = pair[I];
+    int64_t io = p / nv;            // 0-based occupied MO
+    int64_t iv = no + p % nv;       // 0-based absolute virtual MO
+    int64_t ij = lin0(io, iv);
+    for (int c = 0; c < 9; ++c) mom[I + n * c] = moments[c * npk + ij];
+}
+
+void launch_csf_moments(hipStream_t s, const double *moments, int64_t npk, const int32_t *pair,
+                        int64_t n, int no, int nv, double *mom) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(csf_moments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       moments, npk, pair, n, no, nv, mom);
+}
+
+// ----------------------------------------------------------------------------
+// AO-basis (Loewdin-basis) form of the coupling operators.  For a trial
+// vector X (no x nv) the transition density is D = Co X Cv^T (nao x nao),
+// stored transposed as Dt = Cv X^T Co^T.  Then
+//   sum_jb (ia|jb)_K X_jb = [Co^T diag(u_A(mu)) Cv]_ia,  u = gamK t,
+//                          t_A = sum_{mu on A} D(mu,mu)
+//   sum_jb (ij|ab)_J X_jb = [Co^T (GamJ o D) Cv]_ia
+//   sum_jb (ib|ja)_K X_jb = [Co^T (GamK o D^T) Cv]_ia
+// with Gam_{mu nu} = gamma_{A(mu) A(nu)}, because the Loewdin charges
+// factor per AO: q^A_pq = sum_{mu on A} C'(mu,p) C'(mu,q).
+__global__ void atom_diag_kernel(int nao, int ncent, const double *Dt, int64_t strideD,
+                                 const int *atom_ptr, const double *gamk, double *u) {
+    extern __shared__ double st[];
+    int v = blockIdx.x;
+    const double *D = Dt + strideD * v;
+    for (int A = threadIdx.x; A < ncent; A += blockDim.x) {
+        double s = 0.0;
+        for (int m = atom_ptr[A]; m < atom_ptr[A + 1]; ++m) s += D[m + (int64_t)nao * m];
+        st[A] = s;
+    }
+    __syncthreads();
+    for (int A = threadIdx.x; A < ncent; A += blockDim.x) {
+        double s = 0.0;
+        const double *g = gamk + (int64_t)ncent * A;   // symmetric
+        for (int B = 0; B < ncent; ++B) s += g[B] * st[B];
+        u[A + (int64_t)ncent * v] = s;
+    }
+}
+
+void launch_atom_diag(hipStream_t s, int nvec, int nao, int ncent, const double *Dt,
+                      int64_t strideD, const int *atom_ptr, const double *gamk, double *u) {
+    if (nvec == 0) return;
+    hipLaunchKernelGGL(atom_diag_kernel, dim3(nvec), dim3(256), ncent * sizeof(double), s, nao,
+                       ncent, Dt, strideD, atom_ptr, gamk, u);
+}
+
+constexpr int GB_T = 32;
+__global__ void __launch_bounds__(256) gbuild_kernel(int nao, int ncent, const double *Dt,
+                                                     int64_t strideD, const int *atom_of,
+                                                     const double *gamj, const double *gamk,
+                                                     const double *u, double aJ, double aX,
+                                                     double aK, double *Gt, int64_t strideG) {
+    __shared__ double tr[GB_T][GB_T + 1];
+    int v = blockIdx.z;
+    const double *D = Dt + strideD * v;
+    double *G = Gt + strideG * v;
+    int nu0 = blockIdx.x * GB_T, mu0 = blockIdx.y * GB_T;   // Gt(nu, mu), column-major
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;       // 32 x 8
+    if (aX != 0.0) {
+        // stage Dt(mu0+.., nu0+..) so that tr[nu][mu] = Dt(mu, nu)
+        for (int r = ty; r < GB_T; r += 8) {
+            int mu = mu0 + tx, nu = nu0 + r;
+            tr[r][tx] = (mu < nao && nu < nao) ? D[mu + (int64_t)nao * nu] : 0.0;
+        }
+        __syncthreads();
+    }
+    for (int r = ty; r < GB_T; r += 8) {
+        int nu = nu0 + tx, mu = mu0 + r;
+        if (nu >= nao || mu >= nao) continue;
+        int An = atom_of[nu], Am = atom_of[mu];
+        int64_t g = An + (int64_t)ncent * Am;
+        double val = 0.0;
+        if (aJ != 0.0) val += aJ * gamj[g] * D[nu + (int64_t)nao * mu];
+        if (aX != 0.0) val += aX * gamk[g] * tr[tx][r];
+        if (nu == mu && aK != 0.0) val += aK * u[Am + (int64_t)ncent * v];
+        G[nu + (int64_t)nao * mu] = val;
+    }
+}
+
+void launch_gbuild(hipStream_t s, int nvec, int nao, int ncent, const double *Dt,
+                   int64_t strideD, const int *atom_of, const double *gamj, const double *gamk,
+                   const double *u, double aJ, double aX, double aK, double *Gt, int64_t strideG) {
+    if (nvec == 0) return;
+    dim3 g((nao + GB_T - 1) / GB_T, (nao + GB_T - 1) / GB_T, nvec);
+    hipLaunchKernelGGL(gbuild_kernel, g, dim3(256), 0, s, nao, ncent, Dt, strideD, atom_of, gamj,
+                       gamk, u, aJ, aX, aK, Gt, strideG);
+}
+
+// ----------------------------------------------------------------------------
+// Davidson vector operations on (n x k) column-major blocks.
+__global__ void residual_kernel(int64_t n, const double *AX, const double *X, const double *theta,
+                                double *R) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) R[i + n * c] = AX[i + n * c] - theta[c] * X[i + n * c];
+}
+
+void launch_residual(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
+                     const double *theta, double *R) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, n,
+                       AX, X, theta, R);
+}
+
+__global__ void col_norms_kernel(int64_t n, const double *R, double *norms) {
+    __shared__ double ws[4];
+    int c = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double v = R[i + n * c];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) norms[c] = sqrt(ws[0] + ws[1] + ws[2] + ws[3]);
+}
+
+void launch_col_norms(hipStream_t s, int64_t n, int k, const double *R, double *norms) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(col_norms_kernel, dim3(k), dim3(256), 0, s, n, R, norms);
+}
+
+// T = R / (theta - diag): diagonal (Davidson) preconditioner, guarded.
+__global__ void precondition_kernel(int64_t n, const double *R, const double *diag,
+                                    const double *theta, double *T) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= n) return;
+    double d = theta[c] - diag[i];
+    if (fabs(d) < 1.0e-4) d = d < 0.0 ? -1.0e-4 : 1.0e-4;
+    T[i + n * c] = R[i + n * c] / d;
+}
+
+void launch_precondition(hipStream_t s, int64_t n, int k, const double *R, const double *diag,
+                         const double *theta, double *T) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(precondition_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s,
+                       n, R, diag, theta, T);
+}
+
+__global__ void scale_cols_kernel(int64_t n, double *V, const double *scale) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) V[i + n * c] *= scale[c];
+}
+
+void launch_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *scale) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, n,
+                       V, scale);
+}
+
+__global__ void set_unit_kernel(int64_t n, double *V, const int32_t *rows) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) V[i + n * c] = (i == rows[c]) ? 1.0 : 0.0;
+}
+
+void launch_set_unit(hipStream_t s, int64_t n, int k, double *V, const int32_t *rows) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(set_unit_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, n, V,
+                       rows);
+}
+
+// Y(:, c) = diag o X(:, c)
+__global__ void axpby_diag_kernel(int64_t n, const double *diag, const double *X, int64_t ldx,
+                                  double *Y, int64_t ldy) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) Y[i + ldy * c] = diag[i] * X[i + ldx * c];
+}
+
+void launch_axpby_diag(hipStream_t s, int64_t n, int k, const double *diag, const double *X,
+                       int64_t ldx, double *Y, int64_t ldy) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(axpby_diag_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, n,
+                       diag, X, ldx, Y, ldy);
+}
+
+// ----------------------------------------------------------------------------
+// 4-index coupling tiles.
+//
+// For a row tile R = Kset x Cset (secondary-type pairs (k,c)) and a column
+// tile C = Jset x Bset (primary-type pairs (j,b)) the block forms, for every
+// 4-tuple (k,c,j,b), both contractions over atoms
+//   K(kc,jb) = sum_A pK(A,kc) q(A,jb)           ((kc|jb), stda.f:1811)
+//   J(kc,jb) = sum_A pJ(A,lin(k,j)) qab(A,lin(c,b))   ((kj|cb), stda.f:1814-1815)
+// i.e. two small GEMMs over the atom index whose output layouts differ by a
+// permutation of the tuple; here each thread owns the same tuples in both,
+// so the permutation is implicit.  Operands are staged through LDS in
+// atom chunks of NA_CHUNK, stored [atom][vector] so that a wavefront reads
+// consecutive vectors.
+constexpr int NT = 256;
+constexpr int NRV = TK * TC;     // row-tile vectors (pK)
+constexpr int NCV = TJ * TB;     // col-tile vectors (q)
+constexpr int NJV = TK * TJ;     // pJ vectors
+constexpr int NQV = TC * TB;     // qab vectors
+static_assert(TK * TC * TJ * TB == 4 * NT, "each thread owns TK tuples");
+static_assert(TK == 4, "thread mapping assumes TK == 4");
+
+struct TileSmem {
+    double pk[NA_CHUNK][NRV + 1];
+    double qc[NA_CHUNK][NCV + 1];
+    double pj[NA_CHUNK][NJV + 1];
+    double qv[NA_CHUNK][NQV + 1];
+    int occr[TK], virr[TC], occc[TJ], virc[TB];
+};
+
+__device__ __forceinline__ void stage_tile(TileSmem &sm, int a0, int na, int ncent, int nv,
+                                           const double *__restrict__ pK,
+                                           const double *__restrict__ q,
+                                           const double *__restrict__ pJ,
+                                           const double *__restrict__ qab) {
+    // each (vector, atom) element loaded by one thread; atoms fastest for coalescing
+    for (int e = threadIdx.x; e < NRV * NA_CHUNK; e += NT) {
+        int v = e / NA_CHUNK, a = e % NA_CHUNK;
+        int k = sm.occr[v / TC], c = sm.virr[v % TC];
+        double x = 0.0;
+        if (a < na && k >= 0 && c >= 0) x = pK[a0 + a + (int64_t)ncent * ((int64_t)k * nv + c)];
+        sm.pk[a][v] = x;
+    }
+    for (int e = threadIdx.x; e < NCV * NA_CHUNK; e += NT) {
+        int v = e / NA_CHUNK, a = e % NA_CHUNK;
+        int j = sm.occc[v / TB], b = sm.virc[v % TB];
+        double x = 0.0;
+        if (a < na && j >= 0 && b >= 0) x = q[a0 + a + (int64_t)ncent * ((int64_t)j * nv + b)];
+        sm.qc[a][v] = x;
+    }
+    for (int e = threadIdx.x; e < NJV * NA_CHUNK; e += NT) {
+        int v = e / NA_CHUNK, a = e % NA_CHUNK;
+        int k = sm.occr[v / TJ], j = sm.occc[v % TJ];
+        double x = 0.0;
+        if (a < na && k >= 0 && j >= 0) x = pJ[a0 + a + (int64_t)ncent * lin0(k, j)];
+        sm.pj[a][v] = x;
+    }
+    for (int e = threadIdx.x; e < NQV * NA_CHUNK; e += NT) {
+        int v = e / NA_CHUNK, a = e % NA_CHUNK;
+        int c = sm.virr[v / TB], b = sm.virc[v % TB];
+        double x = 0.0;
+        if (a < na && c >= 0 && b >= 0) x = qab[a0 + a + (int64_t)ncent * lin0(c, b)];
+        sm.qv[a][v] = x;
+    }
+}
+
+// thread t owns tuples (k=0..3, c, j, b) with b = t%TB, j = (t/TB)%TJ, c = t/(TB*TJ)
+__device__ __forceinline__ void tile_contract(TileSmem &sm, int ncent, int nv,
+                                              const double *pK, const double *q,
+                                              const double *pJ, const double *qab, double accK[TK],
+                                              double accJ[TK]) {
+    int t = threadIdx.x;
+    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
+    for (int k = 0; k < TK; ++k) { accK[k] = 0.0; accJ[k] = 0.0; }
+    for (int a0 = 0; a0 < ncent; a0 += NA_CHUNK) {
+        int na = min(NA_CHUNK, ncent - a0);
+        __syncthreads();
+        stage_tile(sm, a0, na, ncent, nv, pK, q, pJ, qab);
+        __syncthreads();
+        for (int a = 0; a < na; ++a) {
+            double qjb = sm.qc[a][j * TB + b];
+            double qcb = sm.qv[a][c * TB + b];
+#pragma unroll
+            for (int k = 0; k < TK; ++k) {
+                accK[k] += sm.pk[a][k * TC + c] * qjb;
+                accJ[k] += sm.pj[a][k * TJ + j] * qcb;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void load_tile_ids(TileSmem &sm, TileSet rows, int rt, TileSet cols,
+                                              int ctile) {
+    __syncthreads();
+    if (threadIdx.x < TK) sm.occr[threadIdx.x] = rows.occ[rt * TK + threadIdx.x];
+    if (threadIdx.x < TC) sm.virr[threadIdx.x] = rows.vir[rt * TC + threadIdx.x];
+    if (threadIdx.x < TJ) sm.occc[threadIdx.x] = cols.occ[ctile * TJ + threadIdx.x];
+    if (threadIdx.x < TB) sm.virc[threadIdx.x] = cols.vir[ctile * TB + threadIdx.x];
+    __syncthreads();
+}
+
+// PT2 pass 1 (row sums) -- one block per row tile, looping over column tiles.
+__global__ void __launch_bounds__(NT) pt2_rows_kernel(TileSet rows, TileSet cols, int ncent,
+                                                      int no, int nv, const double *pK,
+                                                      const double *q, const double *pJ,
+                                                      const double *qab, const double *de,
+                                                      const unsigned char *rowmask,
+                                                      unsigned char rowbit,
+                                                      const unsigned char *colmask,
+                                                      unsigned char colbit, double ak,
+                                                      double *rowsum) {
+    __shared__ TileSmem sm;
+    int rt = blockIdx.x;
+    int t = threadIdx.x;
+    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
+    double part[TK] = {0.0, 0.0, 0.0, 0.0};
+    for (int ct = 0; ct < cols.ntiles; ++ct) {
+        load_tile_ids(sm, rows, rt, cols, ct);
+        double accK[TK], accJ[TK];
+        tile_contract(sm, ncent, nv, pK, q, pJ, qab, accK, accJ);
+        int jj = sm.occc[j], bb = sm.virc[b];
+        int cc = sm.virr[c];
+        if (jj < 0 || bb < 0 || cc < 0) continue;
+        int64_t pc = (int64_t)jj * nv + bb;
+        if (!(colmask[pc] & colbit)) continue;
+        double dcol = de[pc];
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            int kk = sm.occr[k];
+            if (kk < 0) continue;
+            int64_t pr = (int64_t)kk * nv + cc;
+            if (!(rowmask[pr] & rowbit)) continue;
+            double amat = ak * accK[k] - accJ[k];
+            part[k] += amat * amat / (de[pr] - dcol + 1.0e-10);
+        }
+    }
+    // reduce over the TJ*TB = 32 threads sharing c (consecutive lanes)
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        double v = part[k];
+        for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 32);
+        part[k] = v;
+    }
+    if ((t % (TB * TJ)) == 0) {
+        int cc = rows.vir[rt * TC + c];
+        for (int k = 0; k < TK; ++k) {
+            int kk = rows.occ[rt * TK + k];
+            if (kk < 0 || cc < 0) continue;
+            int64_t pr = (int64_t)kk * nv + cc;
+            if (rowmask[pr] & rowbit) rowsum[pr] = part[k];
+        }
+    }
+}
+
+// PT2 pass 2 (column sums over the selected rows) -- one block per column tile.
+__global__ void __launch_bounds__(NT) pt2_cols_kernel(TileSet rows, TileSet cols, int ncent,
+                                                      int no, int nv, const double *pK,
+                                                      const double *q, const double *pJ,
+                                                      const double *qab, const double *de,
+                                                      const unsigned char *rowmask,
+                                                      unsigned char rowbit,
+                                                      const unsigned char *colmask,
+                                                      unsigned char colbit, double ak,
+                                                      double *colsum) {
+    __shared__ TileSmem sm;
+    __shared__ double red[NT];
+    int ctile = blockIdx.x;
+    int t = threadIdx.x;
+    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
+    double part = 0.0;
+    for (int rt = 0; rt < rows.ntiles; ++rt) {
+        load_tile_ids(sm, rows, rt, cols, ctile);
+        double accK[TK], accJ[TK];
+        tile_contract(sm, ncent, nv, pK, q, pJ, qab, accK, accJ);
+        int jj = sm.occc[j], bb = sm.virc[b];
+        int cc = sm.virr[c];
+        if (jj < 0 || bb < 0 || cc < 0) continue;
+        int64_t pc = (int64_t)jj * nv + bb;
+        if (!(colmask[pc] & colbit)) continue;
+        double dcol = de[pc];
+#pragma unroll
+        for (int k = 0; k < TK; ++k) {
+            int kk = sm.occr[k];
+            if (kk < 0) continue;
+            int64_t pr = (int64_t)kk * nv + cc;
+            if (!(rowmask[pr] & rowbit)) continue;
+            double amat = ak * accK[k] - accJ[k];
+            part += amat * amat / (de[pr] - dcol + 1.0e-10);
+        }
+    }
+    // reduce over the TC threads sharing (j, b): stride TB*TJ
+    red[t] = part;
+    __syncthreads();
+    if (t < TB * TJ) {
+        double s = 0.0;
+        for (int cI = 0; cI < TC; ++cI) s += red[t + cI * TB * TJ];
+        int jj = cols.occ[ctile * TJ + j], bb = cols.vir[ctile * TB + b];
+        if (jj >= 0 && bb >= 0) {
+            int64_t pc = (int64_t)jj * nv + bb;
+            if (colmask[pc] & colbit) colsum[pc] = s;
+        }
+    }
+}
+
+void launch_pt2_rows(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak,
+                     double *rowsum) {
+    if (rows.ntiles == 0) return;
+    hipLaunchKernelGGL(pt2_rows_kernel, dim3(rows.ntiles), dim3(NT), 0, s, rows, cols, ncent, no,
+                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, rowsum);
+}
+
+void launch_pt2_cols(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak,
+                     double *colsum) {
+    if (cols.ntiles == 0) return;
+    hipLaunchKernelGGL(pt2_cols_kernel, dim3(cols.ntiles), dim3(NT), 0, s, rows, cols, ncent, no,
+                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
+}
+
+// Explicit A: one block per (row tile, column tile); stores the coupling of
+// every pair of CSFs (the diagonal is overwritten afterwards with ed).
+__global__ void __launch_bounds__(NT) build_a_kernel(TileSet tiles, int ncent, int no, int nv,
+                                                     const double *pK, const double *q,
+                                                     const double *pJ, const double *qab,
+                                                     const int32_t *csf_of_pair, double ak,
+                                                     double *A, int64_t lda) {
+    __shared__ TileSmem sm;
+    int rt = blockIdx.x, ctile = blockIdx.y;
+    int t = threadIdx.x;
+    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
+    TileSet cols{tiles.occ, tiles.vir, tiles.ntiles};
+    // the same tile list serves rows and columns; TK == TJ and TC == TB here
+    load_tile_ids(sm, tiles, rt, cols, ctile);
+    double accK[TK], accJ[TK];
+    tile_contract(sm, ncent, nv, pK, q, pJ, qab, accK, accJ);
+    int jj = sm.occc[j], bb = sm.virc[b], cc = sm.virr[c];
+    if (jj < 0 || bb < 0 || cc < 0) return;
+    int32_t col = csf_of_pair[(int64_t)jj * nv + bb];
+    if (col < 0) return;
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+        int kk = sm.occr[k];
+        if (kk < 0) continue;
+        int32_t row = csf_of_pair[(int64_t)kk * nv + cc];
+        if (row < 0) continue;
+        A[row + lda * col] = ak * accK[k] - accJ[k];
+    }
+}
+
+void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,
+                    const double *q, const double *pJ, const double *qab,
+                    const int32_t *csf_of_pair, double ak, double *A, int64_t lda) {
+    if (tiles.ntiles == 0) return;
+    hipLaunchKernelGGL(build_a_kernel, dim3(tiles.ntiles, tiles.ntiles), dim3(NT), 0, s, tiles,
+                       ncent, no, nv, pK, q, pJ, qab, csf_of_pair, ak, A, lda);
+}
+
+// E(I,J) = (ib|ja)_K = pK(:, i*nv+b) . q(:, j*nv+a) for CSFs I=(i,a), J=(j,b)
+// (stda.f:1952-1956).  One wavefront per (I, J) pair; small systems only.
+__global__ void __launch_bounds__(256) exchange_kernel(int64_t nci, int ncent, int nv,
+                                                       const int32_t *pair, const double *pK,
+                                                       const double *q, double *E, int64_t lde) {
+    int lane = threadIdx.x & 63;
+    int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t e = w; e < nci * nci; e += nw) {
+        int64_t I = e % nci, J = e / nci;
+        int64_t pI = pair[I], pJ_ = pair[J];
+        int64_t i = pI / nv, a = pI % nv, j = pJ_ / nv, b = pJ_ % nv;
+        const double *x = pK + (int64_t)ncent * (i * nv + b);
+        const double *y = q + (int64_t)ncent * (j * nv + a);
+        double s = 0.0;
+        for (int k = lane; k < ncent; k += 64) s += x[k] * y[k];
+        s = wave_sum(s);
+        if (lane == 0) E[I + lde * J] = s;
+    }
+}
+
+void launch_exchange(hipStream_t s, int64_t nci, int ncent, int nv, const int32_t *pair,
+                     const double *pK, const double *q, double *E, int64_t lde) {
+    if (nci == 0) return;
+    int64_t blocks = (nci * nci + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(exchange_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nci, ncent, nv,
+                       pair, pK, q, E, lde);
+}
+
+}  // namespace stda

@@ -1,0 +1,211 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Tolerances (plain FP64 path): excitation energies 1e-6 eV, oscillator and
+rotatory strengths 1e-6 absolute, polarizabilities 1e-6 relative; the CSF
+selection (iconf) must be identical and the PT2-corrected diagonal equal to
+1e-10 Hartree.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_datxy
+import oracle as O
+from stda_amd.prep import prepare
+
+pytestmark = pytest.mark.gpu
+
+EV = 27.211385050
+E_TOL = 1e-6 / EV          # 1e-6 eV in Hartree
+F_TOL = 1e-6
+
+
+def _hip():
+    from stda_amd.core import StdaHip
+    return StdaHip(0)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+WATER = {1: (1.0, 20.0, -100.0, -100.0), 2: (0.5, 30.0, -100.0, -100.0), 3: (1.0, 30.0, 3.0, 1.0)}
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_tda_window(water, case):
+    ax, thr, al, be = WATER[case]
+    say(f"water TDA case {case}")
+    ci = prepare(water, thr, ax)
+    ref = O.run(ci, thr, ax, alphak=al, betaj=be)
+    with _hip() as h:
+        h.setup(ci, thr, ax, alphak=al, betaj=be)
+        iconf, ed = h.csf()
+        sol = h.solve_tda(None)
+    np.testing.assert_array_equal(iconf, ref.iconf)
+    np.testing.assert_allclose(ed, ref.ed, atol=1e-10)
+    assert len(sol.omega) == len(ref.omega)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+    exp = read_datxy(os.path.join(GOLDEN, "water", f"EXCI_TEST{case}.dat"))
+    for k, row in enumerate(exp):
+        assert abs(sol.omega[k] * 27.21139 - row[1]) <= 0.5e-4 + 1e-6
+        assert abs(sol.props[k, 0] - row[2]) <= 2.5e-6
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_rpa_window(water, case):
+    ax, thr, al, be = WATER[case]
+    say(f"water RPA case {case}")
+    ci = prepare(water, thr, ax)
+    ref = O.run(ci, thr, ax, alphak=al, betaj=be, rpa=True)
+    with _hip() as h:
+        h.setup(ci, thr, ax, alphak=al, betaj=be)
+        sol = h.solve_rpa(None)
+    assert len(sol.omega) == len(ref.omega)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+
+
+@pytest.fixture(scope="module")
+def polyene():
+    from stda_amd import synth
+    mol, ints = synth.polyene()
+    return prepare(mol, 7.0, 0.25, ints)
+
+
+@pytest.fixture(scope="module")
+def cluster30():
+    from stda_amd import synth
+    mol, ints = synth.cluster(30, seed=3, basis="minimal")
+    return prepare(mol, 10.0, 0.25, ints)
+
+
+def _compare_selection(h, ref):
+    iconf, ed = h.csf()
+    assert h.nprim == ref.nex and h.nci == ref.nci
+    np.testing.assert_array_equal(iconf, ref.iconf)
+    np.testing.assert_allclose(ed, ref.ed, atol=1e-10)
+
+
+def _compare_vectors(X, Xref):
+    ov = np.abs(np.sum(X * Xref, axis=0))
+    np.testing.assert_allclose(ov, 1.0, atol=1e-6)
+
+
+def test_polyene_tda_lowest_roots(polyene):
+    say("polyene TDA k=10")
+    ci = polyene
+    ref = O.run(ci, 7.0, 0.25, nroots=10)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        _compare_selection(h, ref)
+        sol = h.solve_tda(10)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+    np.testing.assert_allclose(sol.alpha_sos, ref.alpha_sos, rtol=1e-6, atol=1e-9)
+    _compare_vectors(sol.X, ref.X)
+
+
+def test_polyene_tda_window(polyene):
+    say("polyene TDA window")
+    ci = polyene
+    ref = O.run(ci, 7.0, 0.25)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        sol = h.solve_tda(None)
+    assert len(sol.omega) == len(ref.omega)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+
+
+def test_polyene_explicit_matrices(polyene):
+    say("polyene explicit A, A+-B, 0.5B and matrix-free products")
+    ci = polyene
+    ax = 0.25
+    it = O.intermediates(ci, ax)
+    sel = O.select(ci, it, 7.0)
+    A = O.build_a(ci, it, sel)
+    apb, amb = O.build_apb_amb(ci, it, sel, ax)
+    bh = O.build_bhalf(ci, it, sel, ax)
+    n = sel["nci"]
+    rng = np.random.default_rng(0)
+    V = rng.standard_normal((n, 5))
+    with _hip() as h:
+        h.setup(ci, 7.0, ax)
+        Ah = h.build_a()
+        apbh, ambh = h.build_apb_amb()
+        bhh = h.build_bhalf()
+        mv = [h.matvec(w, V) for w in range(4)]
+    np.testing.assert_allclose(Ah, A, atol=1e-12)
+    np.testing.assert_allclose(apbh, apb, atol=1e-12)
+    np.testing.assert_allclose(ambh, amb, atol=1e-12)
+    np.testing.assert_allclose(bhh, bh, atol=1e-12)
+    mats = [A, O.unpack_sym(apb, n), O.unpack_sym(amb, n), O.unpack_sym(bh, n)]
+    for w in range(4):
+        np.testing.assert_allclose(mv[w], mats[w] @ V, atol=1e-11, err_msg=f"operator {w}")
+
+
+def test_polyene_rpa(polyene):
+    say("polyene RPA k=8")
+    ci = polyene
+    ref = O.run(ci, 7.0, 0.25, rpa=True, nroots=8)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        sol = h.solve_rpa(8)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+    _compare_vectors(sol.X + sol.Y, ref.X + ref.Y)
+
+
+def test_polyene_lresp(polyene):
+    say("polyene lresp at 4 frequencies")
+    ci = polyene
+    omegas = [0.0, 0.05, 0.1, 0.15]
+    ref = O.run(ci, 7.0, 0.25, rpa=True, resp_omegas=omegas)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        res = h.lresp(omegas)
+    for (w, a), (wr, ar) in zip(res, ref.alpha_resp):
+        assert w == wr
+        np.testing.assert_allclose(a, ar, rtol=1e-6, atol=1e-9)
+
+
+def test_cluster_selection_and_roots(cluster30):
+    say("30-atom cluster TDA k=12")
+    ci = cluster30
+    ref = O.run(ci, 10.0, 0.25, nroots=12)
+    with _hip() as h:
+        h.setup(ci, 10.0, 0.25)
+        _compare_selection(h, ref)
+        sol = h.solve_tda(12)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol.props, ref.props, atol=F_TOL)
+
+
+def test_triplet_and_ax0(polyene):
+    say("polyene triplet TDA and ax=0 RPA")
+    ci = polyene
+    ref = O.run(ci, 7.0, 0.25, nroots=6, triplet=True)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25, triplet=True)
+        _compare_selection(h, ref)
+        sol = h.solve_tda(6)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    ref0 = O.run(ci, 7.0, 0.0, rpa=True, nroots=6)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.0)
+        sol0 = h.solve_rpa(6)
+    np.testing.assert_allclose(sol0.omega, ref0.omega, atol=E_TOL)
+    np.testing.assert_allclose(sol0.props, ref0.props, atol=F_TOL)
+
+
+def test_no_csf_error(polyene):
+    from stda_amd.core import StdaHipError
+    with _hip() as h:
+        with pytest.raises(StdaHipError) as e:
+            h.setup(polyene, 0.01, 0.25)
+    assert e.value.status == 2
