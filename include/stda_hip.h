@@ -89,6 +89,12 @@ const char *stda_hip_last_error(const stda_hip_ctx *ctx);
 int stda_hip_setup(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip_params *par,
                    int64_t *nci, int64_t *nprim, int64_t *nsec);
 
+/* stda_hip_setup in two halves: load copies the system to the device (host
+ * -> HBM transfers, AO reordering on the host) and select runs the device
+ * pipeline on the resident data.  select may be called again (same result). */
+int stda_hip_load(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip_params *par);
+int stda_hip_select(stda_hip_ctx *ctx, int64_t *nci, int64_t *nprim, int64_t *nsec);
+
 /* CSF list after selection: iconf(nci,2) 1-based (occupied MO, absolute
  * virtual MO index no+1..moci) and ed(nci), the PT2-corrected diagonal
  * (same order as the reference: primaries sorted by energy, then the
@@ -150,6 +156,12 @@ int stda_hip_matvec(stda_hip_ctx *ctx, int32_t which, int32_t nvec, const double
  * [0] gamma+charges+intermediates, [1] screening+sort, [2] PT2,
  * [3] eigensolve / response, [4] properties, [5] total. */
 int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
+
+/* Per-kernel statistics accumulated since the last select: HIP-event time
+ * (ms) on the context's stream, launch count and algorithmic work (flops).
+ * Names: "pt2_rows", "pt2_cols", "ao_matvec".  Unknown names give zeros. */
+int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
+                          double *launches, double *work);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,7 @@ def lib():
         L.orc_rtdamat.argtypes = [i64, i32, i32, i32, i64, P, f64, P, P, P, P, P, P]
         L.orc_rrpamat.argtypes = [i64, i32, i32, i32, i64, P, f64, f64, P, P, P, P, P, P, P]
         L.orc_rtdacorr.argtypes = [i64, i32, i32, i32, i64, P, f64, f64, P, P, P]
+        L.orc_rtdamat_rows.argtypes = [i64, i64, i64, i32, i32, i32, i64, P, f64, P, P, P, P, P]
         L.orc_num_threads.restype = i32
         _lib = L
     return _lib
@@ -368,3 +369,97 @@ def tda_dat_rows(res, thr_ev):
         rows.append((k + 1, round(res.omega[k] * HARTREE_EV_PRINT, 4),
                      *[round(float(x), 6) for x in res.props[k]]))
     return rows
+
+
+def cpu_baseline_sample(ci, thr_ev, ax, nroots, budget_s=20.0, seed=0):
+    """Time the reference algorithm on a bounded sample of one solve.
+
+    Phases timed in full: gamma, charges and contracted intermediates
+    (stda.f:465-763), screening and ordering (stda.f:593-668).  Phases
+    timed on a sample and scaled: the PT2 loop (stda.f:1798-1838) over a
+    random subset of secondaries (scaled by nexpt / sample), the A build
+    (rtdamat) over a subset of rows (scaled by the triangle size), and the
+    dense window eigensolve (ssyevr, stda.f:1145) on a random symmetric
+    matrix of size m (scaled by (nci/m)^3).  The number of CSFs entering
+    the last two is the PT2-selected count of the sample extrapolated to
+    all secondaries.  Returns (seconds per solve, description, threads).
+    """
+    import time
+    L = lib()
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    inter = intermediates(ci, ax)
+    t_inter = time.perf_counter() - t0
+    no, nv, n = ci.no, ci.nv, ci.ncent
+    ak = 2.0
+    thr = thr_ev / HARTREE_EV
+    mx = no * nv
+    iconf = np.zeros(2 * mx, dtype=np.int32)
+    kconf = np.zeros(2 * mx, dtype=np.int32)
+    ed = np.zeros(mx)
+    edpt = np.zeros(mx)
+    nex = ctypes.c_int64()
+    nexpt = ctypes.c_int64()
+    t0 = time.perf_counter()
+    L.orc_screen(n, no, nv, _p(np.ascontiguousarray(ci.epsi)), _p(inter["jdiag"]),
+                 _p(inter["pia"]), _p(inter["qia"]), ak, thr, 1.7976931348623157e308, 0,
+                 0.5, 0.1, 4.0, mx, _p(iconf), _p(ed), _p(kconf), _p(edpt),
+                 ctypes.byref(nex), ctypes.byref(nexpt))
+    t_screen = time.perf_counter() - t0
+    nex, nexpt = nex.value, nexpt.value
+    # PT2 on a growing random sample of secondaries until ~budget/2 is spent
+    per_row = None
+    frac_incl = 0.0
+    srows = 16
+    spent = 0.0
+    while True:
+        pick = np.sort(rng.choice(nexpt, size=min(srows, nexpt), replace=False))
+        kc = np.zeros(2 * mx, dtype=np.int32)
+        kc[:len(pick)] = kconf[pick]
+        kc[mx:mx + len(pick)] = kconf[mx + pick]
+        ep = np.zeros(mx)
+        ep[:len(pick)] = edpt[pick]
+        ic = iconf.copy()
+        e2 = ed.copy()
+        incl = np.zeros(len(pick), dtype=np.uint8)
+        t0 = time.perf_counter()
+        L.orc_ptselect(nex, n, no, nv, len(pick), mx, _p(ic), _p(kc), ak, _p(e2), _p(ep),
+                       _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]), _p(inter["qab"]),
+                       1e-4, _p(incl))
+        dt = time.perf_counter() - t0
+        spent += dt
+        per_row = dt / len(pick)
+        frac_incl = float(incl.mean()) if len(pick) else 0.0
+        if spent > budget_s / 2 or len(pick) >= nexpt or dt > budget_s / 4:
+            break
+        srows *= 4
+    t_pt2 = per_row * nexpt
+    nci = int(nex + frac_incl * nexpt)
+    # A build on a sample of rows of a CSF list of the primaries (same per-element loop)
+    rows_n = max(1, min(nex - 1, 64))
+    buf = np.zeros(rows_n * nex)
+    t0 = time.perf_counter()
+    L.orc_rtdamat_rows(nex - rows_n, nex, nex, n, no, nv, mx, _p(iconf), ak, _p(inter["pia"]),
+                       _p(inter["qia"]), _p(inter["pij"]), _p(inter["qab"]), _p(buf))
+    dt = time.perf_counter() - t0
+    elems = rows_n * (nex - rows_n / 2.0)
+    t_a = dt / elems * (nci * (nci - 1) / 2.0)
+    # dense eigensolve cost
+    m = 1500
+    M = rng.standard_normal((m, m))
+    M = M + M.T
+    t0 = time.perf_counter()
+    np.linalg.eigh(M)
+    t_eig = (time.perf_counter() - t0) * (nci / m) ** 3
+    total = t_inter + t_screen + t_pt2 + t_a + t_eig
+    desc = (f"oracle restatement of the reference algorithm: intermediates {t_inter:.1f}s and "
+            f"screening {t_screen:.1f}s timed in full; PT2 timed on {len(pick)} of {nexpt} "
+            f"secondaries (x{nexpt/len(pick):.0f}); rtdamat on {rows_n} rows scaled to "
+            f"nci={nci} (PT2-selected fraction {frac_incl:.2f} of the sample); dense eigh "
+            f"m={m} scaled by (nci/m)^3; extrapolated seconds/solve {total:.3g} "
+            f"(pt2 {t_pt2:.3g}, A {t_a:.3g}, eig {t_eig:.3g})")
+    return total, desc, orc_num_threads()
+
+
+def orc_num_threads():
+    return int(lib().orc_num_threads())

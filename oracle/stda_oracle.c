@@ -376,3 +376,27 @@ void orc_rtdacorr(int64_t nci, int ncent, int no, int nv, int64_t mx, const int 
         bmat[ii] = fact * (ak * ek - ax * ek);
     }
 }
+
+/* Rows i0..i1-1 of the TDA matrix (couplings with every j < i), the loop of
+ * rtdamat for a slice of rows; used to time the A build on a sample. */
+void orc_rtdamat_rows(int64_t i0, int64_t i1, int64_t nci, int ncent, int no, int nv,
+                      int64_t mx, const int *iconf, double ak, const double *pia,
+                      const double *qia, const double *pij, const double *qab, double *rows) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = i0; i < i1; ++i) {
+        int io = iconf[i], iv = iconf[i + mx];
+        int iiv = iv - no;
+        int64_t iwrk = (int64_t)(io - 1) * nv + iiv - 1;
+        const double *qk = pia + (int64_t)ncent * iwrk;
+        double *out = rows + (i - i0) * nci;
+        for (int64_t j = 0; j < i; ++j) {
+            int jo = iconf[j], jv = iconf[j + mx];
+            int jjv = jv - no;
+            int64_t jwrk = (int64_t)(jo - 1) * nv + jjv - 1;
+            double ek = dot(ncent, qk, qia + (int64_t)ncent * jwrk);
+            double ej = dot(ncent, pij + (int64_t)ncent * (lin(io, jo) - 1),
+                            qab + (int64_t)ncent * (lin(iiv, jjv) - 1));
+            out[j] = ak * ek - ej;
+        }
+    }
+}

@@ -30,7 +30,8 @@ C_API = [
     "stda_hip_create", "stda_hip_destroy", "stda_hip_error_string", "stda_hip_last_error",
     "stda_hip_setup", "stda_hip_get_csf", "stda_hip_solve_tda", "stda_hip_solve_rpa",
     "stda_hip_lresp", "stda_hip_build_a", "stda_hip_build_apb_amb", "stda_hip_build_bhalf",
-    "stda_hip_matvec", "stda_hip_timings",
+    "stda_hip_matvec", "stda_hip_timings", "stda_hip_load", "stda_hip_select",
+    "stda_hip_kernel_stats",
 ]
 
 
@@ -86,6 +87,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_build_bhalf.argtypes = [P, P]
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
     L.stda_hip_timings.argtypes = [P, P]
+    L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
+    L.stda_hip_select.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
+    L.stda_hip_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
+                                        ctypes.POINTER(F64), ctypes.POINTER(F64)]
     _lib = L
     return L
 
@@ -163,13 +168,42 @@ class StdaHip:
         ks = kshift or (0.5, 0.1, 4.0)
         par = _Params(thr, thrp, -1.0 if fthr is None else fthr, ax, alphak, betaj,
                       1 if triplet else 0, 1 if kshift else 0, ks[0], ks[1], ks[2])
-        nci, nprim, nsec = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        self._check(self._L.stda_hip_setup(self._h, ctypes.byref(sysd), ctypes.byref(par),
-                                           ctypes.byref(nci), ctypes.byref(nprim),
-                                           ctypes.byref(nsec)))
-        self.nci, self.nprim, self.nsec = nci.value, nprim.value, nsec.value
+        self._check(self._L.stda_hip_load(self._h, ctypes.byref(sysd), ctypes.byref(par)))
         self.thr = thr / HARTREE_EV
+        return self.select()
+
+    def load(self, ci: CoreInputs, thr: float, ax: float, **kw):
+        """Copy the system to the device only (see :meth:`select`)."""
+        clow = np.asfortranarray(ci.clow, dtype=np.float64)
+        eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
+        iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+        xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
+        mom = np.ascontiguousarray(ci.moments, dtype=np.float64) if ci.moments is not None else None
+        self._keep = [clow, eps, iaoat, xyz, mom]
+        sysd = _System(ci.ncent, ci.nao, ci.no, ci.nv, _ptr(clow), _ptr(eps), _ptr(iaoat),
+                       _ptr(xyz), _ptr(mom))
+        ks = kw.get("kshift") or (0.5, 0.1, 4.0)
+        fthr = kw.get("fthr")
+        par = _Params(thr, kw.get("thrp", 1e-4), -1.0 if fthr is None else fthr, ax,
+                      kw.get("alphak", -100.0), kw.get("betaj", -100.0),
+                      1 if kw.get("triplet") else 0, 1 if kw.get("kshift") else 0,
+                      ks[0], ks[1], ks[2])
+        self._check(self._L.stda_hip_load(self._h, ctypes.byref(sysd), ctypes.byref(par)))
+        self.thr = thr / HARTREE_EV
+
+    def select(self):
+        """Device pipeline on the loaded system: gamma, charges, screening, PT2."""
+        nci, nprim, nsec = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.stda_hip_select(self._h, ctypes.byref(nci), ctypes.byref(nprim),
+                                            ctypes.byref(nsec)))
+        self.nci, self.nprim, self.nsec = nci.value, nprim.value, nsec.value
         return self.nci, self.nprim, self.nsec
+
+    def kernel_stats(self, name: str):
+        ms, n, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._L.stda_hip_kernel_stats(self._h, name.encode(), ctypes.byref(ms),
+                                                  ctypes.byref(n), ctypes.byref(w)))
+        return {"ms": ms.value, "launches": n.value, "work": w.value}
 
     def csf(self):
         iconf = np.zeros(2 * self.nci, dtype=np.int32)

@@ -98,6 +98,19 @@ void launch_pt2_cols(hipStream_t s, TileSet rows, TileSet cols, int ncent, int n
                      const double *de, const unsigned char *rowmask, unsigned char rowbit,
                      const unsigned char *colmask, unsigned char colbit, double ak,
                      double *colsum);
+// The same two passes on the matrix cores (FP64 MFMA) with 8 x 8 x 8 x 8 tiles:
+// rows/cols tile lists hold PT2_T occupied and PT2_T virtual indices each.
+constexpr int PT2_T = 8;
+void launch_pt2_rows_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                          const double *pK, const double *q, const double *pJ, const double *qab,
+                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                          const unsigned char *colmask, unsigned char colbit, double ak,
+                          double *rowsum);
+void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                          const double *pK, const double *q, const double *pJ, const double *qab,
+                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                          const unsigned char *colmask, unsigned char colbit, double ak,
+                          double *colsum);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

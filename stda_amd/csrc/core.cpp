@@ -244,9 +244,65 @@ struct stda_hip_ctx {
     bool xfull_clean = false;
 
     bool ready = false;
+    bool loaded = false;
+    double alphak = 0.0, betaj = 0.0;
+    int kshift = 0;
+    double ks_max = 0.5, ks_width = 0.1, ks_steep = 4.0;
+
+    // per-kernel statistics: HIP-event time on the context's stream and the
+    // algorithmic work (flops or bytes) of the launches
+    struct Stat { double ms = 0, launches = 0, work = 0; };
+    std::vector<std::pair<std::string, Stat>> stats;
+    Stat &stat(const std::string &name) {
+        for (auto &kv : stats)
+            if (kv.first == name) return kv.second;
+        stats.emplace_back(name, Stat{});
+        return stats.back().second;
+    }
 
     void sync() { STDA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
+
+namespace {
+// Adds host wall time (ms) since t0 to the named statistic.
+struct HostSpan {
+    stda_hip_ctx *c;
+    const char *name;
+    Timer t;
+    HostSpan(stda_hip_ctx *ctx, const char *n) : c(ctx), name(n) {}
+    ~HostSpan() {
+        auto &st = c->stat(name);
+        st.ms += t.ms();
+        st.launches += 1;
+    }
+};
+
+// Records the HIP-event duration of the work enqueued on the context's stream
+// between construction and finish().
+struct EventTimer {
+    stda_hip_ctx *c;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit EventTimer(stda_hip_ctx *ctx) : c(ctx) {
+        STDA_HIP_CHECK(hipEventCreate(&a));
+        STDA_HIP_CHECK(hipEventCreate(&b));
+        STDA_HIP_CHECK(hipEventRecord(a, c->stream));
+    }
+    void finish(const char *name, double work) {
+        STDA_HIP_CHECK(hipEventRecord(b, c->stream));
+        STDA_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        STDA_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        auto &st = c->stat(name);
+        st.ms += ms;
+        st.launches += 1;
+        st.work += work;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
 
 namespace {
 
@@ -293,6 +349,7 @@ void apply_ops(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V, int
         int nb = std::min(nbmax, nvec - v0);
         ensure_work(c, nb);
         const double *Vb = V + ldv * v0;
+        EventTimer evm(c);
         // embed the CSF vectors in the full occupied x virtual grid
         launch_scatter_pairs(s, Vb, ldv, c->pair.get(), n, nb, c->xfull.get(), c->npair);
         // T = Cv X^T (nao x no) for all vectors at once: X^T is (nv, no) column-major
@@ -344,6 +401,10 @@ void apply_ops(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V, int
             launch_gather_pairs(s, c->rbuf.get(), c->npair, c->pair.get(), n, nb, corr, Vb, ldv,
                                 out + ld * v0, ld);
         }
+        // algorithmic flops of the four transforms per vector (and per extra operator)
+        double per = 2.0 * nao * (double)no * nv + 2.0 * (double)nao * nao * no;
+        int nops = (op1 >= 0) + (op2 >= 0 && !(op2 == OP_AMB && c->small_ax));
+        evm.finish("ao_matvec", nb * (per + nops * per));
     }
 }
 
@@ -456,11 +517,17 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     std::vector<double> H((size_t)maxdim * maxdim, 0.0);
     int m = 0, mnew = nguess;
     const int maxiter = 300;
+    DBuf<double> Hc((size_t)maxdim * maxdim), thud(maxdim);
+    Yd.resize((size_t)maxdim * k);
     for (int it = 0; it < maxiter; ++it) {
-        apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0);
+        {
+            HostSpan hs(c, "dav_matvec");
+            apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0);
+            c->sync();
+        }
+        HostSpan hs_it(c, "dav_iter_other");
         // H(0:m+mnew, m:m+mnew) = V^T AV
         int mt = m + mnew;
-        DBuf<double> Hc((size_t)mt * mnew);
         dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, AV.get() + n * m, n, 0.0,
               Hc.get(), mt);
         std::vector<double> hc((size_t)mt * mnew);
@@ -477,11 +544,14 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         std::vector<double> Hs((size_t)m * m), w;
         for (int j = 0; j < m; ++j)
             for (int i = 0; i < m; ++i) Hs[i + (size_t)m * j] = 0.5 * (H[i + (size_t)maxdim * j] + H[j + (size_t)maxdim * i]);
-        host_syev(m, Hs, w);
+        {
+            HostSpan hs(c, "dav_rr_eig");
+            host_syev(m, Hs, w);
+        }
         int kk = std::min(k, m);
-        Yd.upload(Hs.data(), (size_t)m * kk, s);
+        STDA_HIP_CHECK(hipMemcpyAsync(Yd.get(), Hs.data(), (size_t)m * kk * 8, hipMemcpyHostToDevice, s));
         theta.assign(w.begin(), w.begin() + kk);
-        th.upload(theta.data(), kk, s);
+        STDA_HIP_CHECK(hipMemcpyAsync(th.get(), theta.data(), kk * 8, hipMemcpyHostToDevice, s));
         dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
         dgemm(c->blas, false, false, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
         launch_residual(s, n, kk, AX.get(), X.get(), th.get(), R.get());
@@ -502,9 +572,9 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
                                               hipMemcpyDeviceToDevice, s));
         std::vector<double> thu(nun);
         for (int q = 0; q < nun; ++q) thu[q] = theta[un[q]];
-        DBuf<double> thud;
-        thud.upload(thu.data(), nun, s);
+        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu.data(), nun * 8, hipMemcpyHostToDevice, s));
         launch_precondition(s, n, nun, R.get(), c->ed.get(), thud.get(), R.get());
+        c->sync();
         if (m + nun > maxdim) {   // restart with the Ritz vectors
             STDA_HIP_CHECK(hipMemcpyAsync(V.get(), X.get(), (size_t)n * kk * 8,
                                           hipMemcpyDeviceToDevice, s));
@@ -517,7 +587,10 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         int room = std::min(nun, maxdim - m);
         STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * room * 8,
                                       hipMemcpyDeviceToDevice, s));
-        mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+        {
+            HostSpan hs(c, "dav_ortho");
+            mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+        }
         if (mnew == 0) return;   // no new directions: converged to machine precision
     }
     throw Error(STDA_HIP_ERR_NOT_CONVERGED, "Davidson did not converge");
@@ -702,15 +775,15 @@ std::vector<double> moment_products(stda_hip_ctx *c, const double *U, int k) {
 
 // ============================================================================================
 // setup: stda.f:454-790
-static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par) {
+static void do_load(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par) {
     if (!sys || !par || sys->ncent <= 0 || sys->nao <= 0 || sys->no < 0 || sys->nv < 0 ||
         !sys->clow || !sys->epsi || !sys->iaoat || !sys->xyz)
         throw Error(STDA_HIP_ERR_ARG, "invalid system description");
     if (sys->no == 0 || sys->nv == 0)
         throw Error(STDA_HIP_ERR_NO_CSF, "no CSF, increase energy threshold (-e option)");
-    Timer ttot;
     hipStream_t s = c->stream;
     c->ready = false;
+    c->loaded = false;
     c->nb_max = 0;
     c->ncent = sys->ncent; c->nao = sys->nao; c->no = sys->no; c->nv = sys->nv;
     const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
@@ -721,6 +794,12 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
     c->small_ax = std::fabs(par->ax) < 1.0e-6;           // stda.f:1893
     double betaj = par->betaj < -99.0 ? 0.20 + 1.830 * par->ax : par->betaj;   // stda.f:490-500
     double alphak = par->alphak < -99.0 ? 1.420 + 0.480 * par->ax : par->alphak;
+    c->alphak = alphak;
+    c->betaj = betaj;
+    c->kshift = par->kshift;
+    if (par->kshift) {
+        c->ks_max = par->kshift_max_ev; c->ks_width = par->kshift_width_ev; c->ks_steep = par->kshift_steep;
+    }
     c->thr = par->thr_ev / kHartreeEv;
     c->thrp = par->thrp;
     c->fthr = par->fthr_ev > 0.0 ? par->fthr_ev / kHartreeEv : 1.7976931348623157e308;
@@ -743,7 +822,6 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
     std::vector<double> cth((size_t)nao * moci);
     for (int p = 0; p < moci; ++p)
         for (int m = 0; m < nao; ++m) cth[m + (size_t)nao * p] = sys->clow[order[m] + (size_t)nao * p];
-    Timer t0;
     c->ct.upload(cth.data(), cth.size(), s);
     c->atom_ptr.upload(aptr.data(), aptr.size(), s);
     c->atom_of.upload(aof.data(), aof.size(), s);
@@ -756,6 +834,20 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
         c->moments.upload(sys->moments, (size_t)9 * c->npk, s);
         c->moments_h.assign(sys->moments, sys->moments + (size_t)9 * c->npk);
     }
+    c->sync();
+    c->loaded = true;
+}
+
+static void do_select(stda_hip_ctx *c) {
+    if (!c->loaded) throw Error(STDA_HIP_ERR_STATE, "load must be called before select");
+    Timer ttot;
+    hipStream_t s = c->stream;
+    c->ready = false;
+    c->nb_max = 0;
+    c->stats.clear();
+    const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
+    const double alphak = c->alphak, betaj = c->betaj;
+    Timer t0;
 
     // gamma (stda.f:509-532)
     c->gamj.resize((size_t)na * na);
@@ -794,8 +886,7 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
     c->kdiag.resize(c->npair);
     c->flags.resize(c->npair);
     launch_screen(s, na, no, nv, c->eps.get(), c->jdiag.get(), c->pia.get(), c->qia.get(), c->ak,
-                  c->thr, c->fthr, par->kshift, par->kshift ? par->kshift_max_ev : 0.5,
-                  par->kshift ? par->kshift_width_ev : 0.1, par->kshift ? par->kshift_steep : 4.0,
+                  c->thr, c->fthr, c->kshift, c->ks_max, c->ks_width, c->ks_steep,
                   c->de.get(), c->kdiag.get(), c->flags.get());
     std::vector<unsigned char> fl(c->npair);
     std::vector<double> deh(c->npair);
@@ -838,16 +929,18 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
             return nt;
         };
         std::vector<int32_t> ro, rv, co, cv;
-        int nrt = make_tiles(TK, TC, 2, ro, rv);
-        int nct = make_tiles(TJ, TB, 1, co, cv);
+        int nrt = make_tiles(PT2_T, PT2_T, 2, ro, rv);
+        int nct = make_tiles(PT2_T, PT2_T, 1, co, cv);
         DBuf<int32_t> rod, rvd, cod, cvd;
         rod.upload(ro.data(), ro.size(), s); rvd.upload(rv.data(), rv.size(), s);
         cod.upload(co.data(), co.size(), s); cvd.upload(cv.data(), cv.size(), s);
         TileSet rows{rod.get(), rvd.get(), nrt}, cols{cod.get(), cvd.get(), nct};
         DBuf<double> rowsum(c->npair);
-        launch_pt2_rows(s, rows, cols, na, no, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                        c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1, c->ak,
-                        rowsum.get());
+        EventTimer ev1(c);
+        launch_pt2_rows_mfma(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                             c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1,
+                             c->ak, rowsum.get());
+        ev1.finish("pt2_rows", 4.0 * na * (double)c->nsec * (double)c->nprim);
         std::vector<double> rs(c->npair);
         rowsum.download(rs.data(), rs.size(), s);
         c->sync();
@@ -859,14 +952,16 @@ static void do_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip
         if (nexcl > 0) {
             c->flags.upload(fl.data(), fl.size(), s);
             std::vector<int32_t> ro2, rv2;
-            int nrt2 = make_tiles(TK, TC, 4, ro2, rv2);
+            int nrt2 = make_tiles(PT2_T, PT2_T, 4, ro2, rv2);
             DBuf<int32_t> ro2d, rv2d;
             ro2d.upload(ro2.data(), ro2.size(), s); rv2d.upload(rv2.data(), rv2.size(), s);
             TileSet rows2{ro2d.get(), rv2d.get(), nrt2};
             DBuf<double> colsum(c->npair);
-            launch_pt2_cols(s, rows2, cols, na, no, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                            c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1, c->ak,
-                            colsum.get());
+            EventTimer ev2(c);
+            launch_pt2_cols_mfma(s, rows2, cols, na, nv, c->pia.get(), c->qia.get(),
+                                 c->pij.get(), c->qab.get(), c->de.get(), c->flags.get(), 4,
+                                 c->flags.get(), 1, c->ak, colsum.get());
+            ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs(c->npair);
             colsum.download(cs.data(), cs.size(), s);
             c->sync();
@@ -1102,16 +1197,47 @@ const char *stda_hip_error_string(int st) {
 
 const char *stda_hip_last_error(const stda_hip_ctx *c) { return c ? c->last_error.c_str() : ""; }
 
-int stda_hip_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
-                   int64_t *nci, int64_t *nprim, int64_t *nsec) {
+int stda_hip_load(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par) {
     if (!c) return STDA_HIP_ERR_ARG;
     STDA_TRY(c, {
         STDA_HIP_CHECK(hipSetDevice(c->device));
-        do_setup(c, sys, par);
+        do_load(c, sys, par);
+    })
+}
+
+int stda_hip_select(stda_hip_ctx *c, int64_t *nci, int64_t *nprim, int64_t *nsec) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->loaded) return STDA_HIP_ERR_STATE;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        do_select(c);
         if (nci) *nci = c->nci;
         if (nprim) *nprim = c->nprim;
         if (nsec) *nsec = c->nsec;
     })
+}
+
+int stda_hip_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
+                   int64_t *nci, int64_t *nprim, int64_t *nsec) {
+    int st = stda_hip_load(c, sys, par);
+    if (st != STDA_HIP_OK) return st;
+    return stda_hip_select(c, nci, nprim, nsec);
+}
+
+int stda_hip_kernel_stats(const stda_hip_ctx *c, const char *name, double *ms, double *launches,
+                          double *work) {
+    if (!c || !name) return STDA_HIP_ERR_ARG;
+    for (const auto &kv : c->stats)
+        if (kv.first == name) {
+            if (ms) *ms = kv.second.ms;
+            if (launches) *launches = kv.second.launches;
+            if (work) *work = kv.second.work;
+            return STDA_HIP_OK;
+        }
+    if (ms) *ms = 0;
+    if (launches) *launches = 0;
+    if (work) *work = 0;
+    return STDA_HIP_OK;
 }
 
 int stda_hip_get_csf(stda_hip_ctx *c, int32_t *iconf, double *ed) {

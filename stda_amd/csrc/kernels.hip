@@ -756,3 +756,281 @@ void launch_exchange(hipStream_t s, int64_t nci, int ncent, int nv, const int32_
 }
 
 }  // namespace stda
+
+namespace stda {
+
+// ----------------------------------------------------------------------------
+// PT2 couplings on the matrix cores (FP64 MFMA, v_mfma_f64_16x16x4f64).
+//
+// Work-group tile: 8 occupied (k) x 8 virtual (c) secondary-side pairs times
+// 8 occupied (j) x 8 virtual (b) primary-side pairs = 4096 tuples.  Wave w
+// owns k in [4(w/2), +4) and j in [4(w%2), +4) and all c, b: four 16x16
+// output blocks per coupling, indexed by the virtual halves (ch, bh).
+//   J block: rows (k', j'), cols (c', b')   A = pJ(atom, lin(k,j)), B = qab(atom, lin(c,b))
+//   K block: rows (k', c'), cols (j', b')   A = pK(atom, kc),       B = q(atom, jb)
+// MFMA f64 16x16x4 lane layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; register i of lane l holds D[4i + l/16][l%16].  The J
+// values are moved to the K layout through LDS before the epilogue
+//   amat = ak K - J,  pt = amat^2 / (de(kc) - de(jb) + 1e-10).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int M_T = 8;          // per-dimension tile size
+constexpr int M_CH = 16;        // atoms per staged chunk (4 MFMA k-steps)
+constexpr int M_LD = 64 + 1;    // padded LDS row (doubles)
+
+struct MfmaSmem {
+    double pk[M_CH][M_LD];      // (k,c) = 8k + c
+    double qc[M_CH][M_LD];      // (j,b) = 8j + b
+    double pj[M_CH][M_LD];      // (k,j) = 8k + j
+    double qv[M_CH][M_LD];      // (c,b) = 8c + b
+    double xch[4][256];         // per-wave J block in tuple order
+    double de_r[64], de_c[64];
+    unsigned char ok_r[64], ok_c[64];
+    int occr[M_T], virr[M_T], occc[M_T], virc[M_T];
+    double red[4][64];
+};
+
+__device__ __forceinline__ void mfma_load_ids(MfmaSmem &sm, TileSet rows, int rt, TileSet cols,
+                                              int ct, int nv, const double *de,
+                                              const unsigned char *rowmask, unsigned char rowbit,
+                                              const unsigned char *colmask, unsigned char colbit) {
+    __syncthreads();
+    int t = threadIdx.x;
+    if (t < M_T) {
+        sm.occr[t] = rows.occ[rt * M_T + t];
+        sm.virr[t] = rows.vir[rt * M_T + t];
+        sm.occc[t] = cols.occ[ct * M_T + t];
+        sm.virc[t] = cols.vir[ct * M_T + t];
+    }
+    __syncthreads();
+    if (t < 64) {
+        int k = sm.occr[t / M_T], c = sm.virr[t % M_T];
+        bool ok = k >= 0 && c >= 0;
+        int64_t p = ok ? (int64_t)k * nv + c : 0;
+        ok = ok && (rowmask[p] & rowbit);
+        sm.ok_r[t] = ok;
+        sm.de_r[t] = ok ? de[p] : 0.0;
+    } else if (t < 128) {
+        int u = t - 64;
+        int j = sm.occc[u / M_T], b = sm.virc[u % M_T];
+        bool ok = j >= 0 && b >= 0;
+        int64_t p = ok ? (int64_t)j * nv + b : 0;
+        ok = ok && (colmask[p] & colbit);
+        sm.ok_c[u] = ok;
+        sm.de_c[u] = ok ? de[p] : 0.0;
+    }
+}
+
+// stage atoms [a0, a0+na) of the four operand sets into LDS (zero-padded)
+__device__ __forceinline__ void mfma_stage(MfmaSmem &sm, int a0, int na, int ncent, int nv,
+                                           const double *__restrict__ pK,
+                                           const double *__restrict__ q,
+                                           const double *__restrict__ pJ,
+                                           const double *__restrict__ qab) {
+    int t = threadIdx.x;
+    int a = t % M_CH;            // atom within the chunk (fastest: coalesced)
+    int vb = t / M_CH;           // 0..15
+    bool in = a < na;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int v = vb + 16 * r;     // 0..63
+        int x = v / M_T, y = v % M_T;
+        {   // pK (k,c)
+            int k = sm.occr[x], c = sm.virr[y];
+            sm.pk[a][v] = (in && k >= 0 && c >= 0)
+                              ? pK[a0 + a + (int64_t)ncent * ((int64_t)k * nv + c)] : 0.0;
+        }
+        {   // q (j,b)
+            int j = sm.occc[x], b = sm.virc[y];
+            sm.qc[a][v] = (in && j >= 0 && b >= 0)
+                              ? q[a0 + a + (int64_t)ncent * ((int64_t)j * nv + b)] : 0.0;
+        }
+        {   // pJ (k,j)
+            int k = sm.occr[x], j = sm.occc[y];
+            sm.pj[a][v] = (in && k >= 0 && j >= 0) ? pJ[a0 + a + (int64_t)ncent * lin0(k, j)] : 0.0;
+        }
+        {   // qab (c,b)
+            int c = sm.virr[x], b = sm.virc[y];
+            sm.qv[a][v] = (in && c >= 0 && b >= 0) ? qab[a0 + a + (int64_t)ncent * lin0(c, b)] : 0.0;
+        }
+    }
+}
+
+// accJ[blk], accK[blk]: blk = 2*ch + bh
+__device__ __forceinline__ void mfma_contract(MfmaSmem &sm, int ncent, int nv, const double *pK,
+                                              const double *q, const double *pJ,
+                                              const double *qab, v4d accJ[4], v4d accK[4]) {
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int kh = w >> 1, jh = w & 1;
+    int r16 = lane & 15, g = lane >> 4;
+    int hi = r16 >> 2, lo = r16 & 3;
+    for (int b = 0; b < 4; ++b) {
+        accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
+        accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+    // operand column indices (constant over atoms)
+    int iPJ = M_T * (4 * kh + hi) + (4 * jh + lo);            // (k,j)
+    int iPK0 = M_T * (4 * kh + hi) + lo, iPK1 = iPK0 + 4;     // (k,c) for ch = 0,1
+    int iQ0 = M_T * (4 * jh + hi) + lo, iQ1 = iQ0 + 4;        // (j,b) for bh = 0,1
+    int iQV[4];
+    for (int ch = 0; ch < 2; ++ch)
+        for (int bh = 0; bh < 2; ++bh) iQV[2 * ch + bh] = M_T * (4 * ch + hi) + 4 * bh + lo;
+    for (int a0 = 0; a0 < ncent; a0 += M_CH) {
+        int na = min(M_CH, ncent - a0);
+        __syncthreads();
+        mfma_stage(sm, a0, na, ncent, nv, pK, q, pJ, qab);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < M_CH / 4; ++s) {
+            int arow = 4 * s + g;
+            double aJ = sm.pj[arow][iPJ];
+            double aK0 = sm.pk[arow][iPK0], aK1 = sm.pk[arow][iPK1];
+            double bQ0 = sm.qc[arow][iQ0], bQ1 = sm.qc[arow][iQ1];
+#pragma unroll
+            for (int blk = 0; blk < 4; ++blk) {
+                double bV = sm.qv[arow][iQV[blk]];
+                accJ[blk] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, bV, accJ[blk], 0, 0, 0);
+            }
+            accK[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ0, accK[0], 0, 0, 0);
+            accK[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ1, accK[1], 0, 0, 0);
+            accK[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ0, accK[2], 0, 0, 0);
+            accK[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ1, accK[3], 0, 0, 0);
+        }
+    }
+}
+
+// pt values of this lane in the K layout: pt[blk][i] for row (k,c) =
+// (4kh + i, 4ch + g) and column (j,b) = (4jh + hi, 4bh + lo).
+__device__ __forceinline__ void mfma_epilogue(MfmaSmem &sm, const v4d accJ[4], const v4d accK[4],
+                                              double ak, double pt[4][4]) {
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int kh = w >> 1, jh = w & 1;
+    int r16 = lane & 15, g = lane >> 4;
+    int hi = r16 >> 2, lo = r16 & 3;
+    for (int blk = 0; blk < 4; ++blk) {
+        int ch = blk >> 1, bh = blk & 1;
+        // J block in tuple order (k', c', j', b'): lane holds k' = i, j' = g, c' = hi, b' = lo
+        for (int i = 0; i < 4; ++i) sm.xch[w][((i * 4 + hi) * 4 + g) * 4 + lo] = accJ[blk][i];
+        __builtin_amdgcn_wave_barrier();
+        for (int i = 0; i < 4; ++i) {
+            double J = sm.xch[w][((i * 4 + g) * 4 + hi) * 4 + lo];   // k'=i, c'=g, j'=hi, b'=lo
+            int rr = M_T * (4 * kh + i) + 4 * ch + g;                 // (k,c)
+            int cc = M_T * (4 * jh + hi) + 4 * bh + lo;               // (j,b)
+            double amat = ak * accK[blk][i] - J;
+            pt[blk][i] = (sm.ok_r[rr] && sm.ok_c[cc])
+                             ? amat * amat / (sm.de_r[rr] - sm.de_c[cc] + 1.0e-10) : 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void __launch_bounds__(256) pt2_rows_mfma_kernel(
+    TileSet rows, TileSet cols, int ncent, int nv, const double *pK, const double *q,
+    const double *pJ, const double *qab, const double *de, const unsigned char *rowmask,
+    unsigned char rowbit, const unsigned char *colmask, unsigned char colbit, double ak,
+    double *rowsum) {
+    __shared__ MfmaSmem sm;
+    int rt = blockIdx.x;
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int kh = w >> 1;
+    int g = lane >> 4;
+    double racc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};     // rows (4kh+i, 4ch+g)
+    for (int ct = 0; ct < cols.ntiles; ++ct) {
+        mfma_load_ids(sm, rows, rt, cols, ct, nv, de, rowmask, rowbit, colmask, colbit);
+        v4d accJ[4], accK[4];
+        mfma_contract(sm, ncent, nv, pK, q, pJ, qab, accJ, accK);
+        double pt[4][4];
+        mfma_epilogue(sm, accJ, accK, ak, pt);
+        for (int blk = 0; blk < 4; ++blk)
+            for (int i = 0; i < 4; ++i) racc[blk >> 1][i] += pt[blk][i];
+    }
+    // reduce over the 16 lanes of a row group (columns), then over the two j-halves
+    for (int ch = 0; ch < 2; ++ch)
+        for (int i = 0; i < 4; ++i) {
+            double v = racc[ch][i];
+            for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+            racc[ch][i] = v;
+        }
+    __syncthreads();
+    if ((lane & 15) == 0)
+        for (int ch = 0; ch < 2; ++ch)
+            for (int i = 0; i < 4; ++i) sm.red[w][g * 8 + ch * 4 + i] = racc[ch][i];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int t = threadIdx.x;           // row (k,c) = 8k + c
+        int k = t / M_T, c = t % M_T;
+        int kh2 = k / 4, i = k % 4, ch = c / 4, gg = c % 4;
+        double s = sm.red[2 * kh2][gg * 8 + ch * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + ch * 4 + i];
+        int kk = rows.occ[rt * M_T + k], cc = rows.vir[rt * M_T + c];
+        if (kk >= 0 && cc >= 0) {
+            int64_t p = (int64_t)kk * nv + cc;
+            if (rowmask[p] & rowbit) rowsum[p] = s;
+        }
+    }
+    (void)kh;
+}
+
+__global__ void __launch_bounds__(256) pt2_cols_mfma_kernel(
+    TileSet rows, TileSet cols, int ncent, int nv, const double *pK, const double *q,
+    const double *pJ, const double *qab, const double *de, const unsigned char *rowmask,
+    unsigned char rowbit, const unsigned char *colmask, unsigned char colbit, double ak,
+    double *colsum) {
+    __shared__ MfmaSmem sm;
+    int ct = blockIdx.x;
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int r16 = lane & 15;
+    double cacc[2] = {0.0, 0.0};     // columns (4jh+hi, 4bh+lo), bh = 0,1
+    for (int rt = 0; rt < rows.ntiles; ++rt) {
+        mfma_load_ids(sm, rows, rt, cols, ct, nv, de, rowmask, rowbit, colmask, colbit);
+        v4d accJ[4], accK[4];
+        mfma_contract(sm, ncent, nv, pK, q, pJ, qab, accJ, accK);
+        double pt[4][4];
+        mfma_epilogue(sm, accJ, accK, ak, pt);
+        for (int blk = 0; blk < 4; ++blk)
+            for (int i = 0; i < 4; ++i) cacc[blk & 1] += pt[blk][i];
+    }
+    // reduce over the 4 row groups (lanes 16 apart), then over the two k-halves
+    for (int bh = 0; bh < 2; ++bh) {
+        double v = cacc[bh];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        cacc[bh] = v;
+    }
+    __syncthreads();
+    if (lane < 16)
+        for (int bh = 0; bh < 2; ++bh) sm.red[w][r16 * 2 + bh] = cacc[bh];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int t = threadIdx.x;           // column (j,b) = 8j + b
+        int j = t / M_T, b = t % M_T;
+        int jh = j / 4, hi = j % 4, bh = b / 4, lo = b % 4;
+        int r = hi * 4 + lo;
+        double s = sm.red[jh][r * 2 + bh] + sm.red[2 + jh][r * 2 + bh];
+        int jj = cols.occ[ct * M_T + j], bb = cols.vir[ct * M_T + b];
+        if (jj >= 0 && bb >= 0) {
+            int64_t p = (int64_t)jj * nv + bb;
+            if (colmask[p] & colbit) colsum[p] = s;
+        }
+    }
+}
+
+void launch_pt2_rows_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                          const double *pK, const double *q, const double *pJ, const double *qab,
+                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                          const unsigned char *colmask, unsigned char colbit, double ak,
+                          double *rowsum) {
+    if (rows.ntiles == 0) return;
+    hipLaunchKernelGGL(pt2_rows_mfma_kernel, dim3(rows.ntiles), dim3(256), 0, s, rows, cols, ncent,
+                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, rowsum);
+}
+
+void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                          const double *pK, const double *q, const double *pJ, const double *qab,
+                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                          const unsigned char *colmask, unsigned char colbit, double ak,
+                          double *colsum) {
+    if (cols.ntiles == 0) return;
+    hipLaunchKernelGGL(pt2_cols_mfma_kernel, dim3(cols.ntiles), dim3(256), 0, s, rows, cols, ncent,
+                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
+}
+
+}  // namespace stda
