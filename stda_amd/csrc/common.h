@@ -80,6 +80,11 @@ void launch_set_unit(hipStream_t s, int64_t n, int k, double *V, const int32_t *
 void launch_axpby_diag(hipStream_t s, int64_t n, int k, const double *diag, const double *X,
                        int64_t ldx, double *Y, int64_t ldy);
 
+// response preconditioner (2x2 diagonal blocks) and slice reduction
+void launch_precond_pair(hipStream_t s, int64_t n, int ncol, double *R1, double *R2,
+                         const double *d, const double *w);
+void launch_sum_slices(hipStream_t s, int64_t len, int nchunk, const double *W, double *C);
+
 // 4-index coupling tiles: amat(kc, jb) = ak (kc|jb)_K - (kj|cb)_J.
 struct TileSet {
     const int32_t *occ;   // ntiles x T_occ occupied indices (-1 = none)

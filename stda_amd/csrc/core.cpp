@@ -240,6 +240,7 @@ struct stda_hip_ctx {
 
     // matrix-free work buffers
     DBuf<double> xfull, tbuf, dbuf, gbuf, wbuf, rbuf, ubuf;
+    DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
     int nb_max = 0;
     bool xfull_clean = false;
 
@@ -309,6 +310,32 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // matrix-free operators
 enum Op { OP_A = 0, OP_APB = 1, OP_AMB = 2, OP_BHALF = 3 };
+
+// C (m x p, column-major, leading dimension m) = A^T B with A (n x m, lda),
+// B (n x p, ldb).  For the tall-skinny products of the solvers (n up to
+// millions, m and p at most a few hundred) the reduction dimension is split
+// into up to 256 slices computed as one batched GEMM, then the slices are
+// summed in a fixed order: all CUs work on the product and the result is
+// deterministic.
+void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A, int64_t lda,
+              const double *B, int64_t ldb, double *C) {
+    if (m == 0 || p == 0) return;
+    const int64_t KC_MIN = 2048;
+    if (n < 4 * KC_MIN) {
+        dgemm(c->blas, true, false, m, p, n, 1.0, A, lda, B, ldb, 0.0, C, m);
+        return;
+    }
+    int64_t kc = std::max<int64_t>(KC_MIN, (n + 255) / 256);
+    int64_t nch = n / kc, rem = n - nch * kc;
+    int64_t slots = nch + (rem > 0 ? 1 : 0);
+    c->tnbuf.ensure((size_t)(slots * m * p));
+    dgemm_batched(c->blas, true, false, m, p, kc, 1.0, A, lda, kc, B, ldb, kc, 0.0,
+                  c->tnbuf.get(), m, m * p, nch);
+    if (rem > 0)
+        dgemm(c->blas, true, false, m, p, rem, 1.0, A + nch * kc, lda, B + nch * kc, ldb, 0.0,
+              c->tnbuf.get() + nch * m * p, m);
+    launch_sum_slices(c->stream, m * p, (int)slots, c->tnbuf.get(), C);
+}
 
 void ensure_work(stda_hip_ctx *c, int nb) {
     if (nb <= c->nb_max) return;
@@ -447,13 +474,13 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     tmp.ensure((size_t)std::max(m, 1) * nt + (size_t)nt * nt + (size_t)n * nt);
     double *C = tmp.get();
     for (int pass = 0; pass < 2 && m > 0; ++pass) {
-        dgemm(c->blas, true, false, m, nt, n, 1.0, V, n, T, n, 0.0, C, m);
+        dgemm_tn(c, m, nt, n, V, n, T, n, C);
         dgemm(c->blas, false, false, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
     }
     // symmetric orthonormalisation within T: S = T^T T = U s U^T, T <- T U s^-1/2
     double *S = tmp.get() + (size_t)std::max(m, 1) * nt;
     double *T2 = S + (size_t)nt * nt;
-    dgemm(c->blas, true, false, nt, nt, n, 1.0, T, n, T, n, 0.0, S, nt);
+    dgemm_tn(c, nt, nt, n, T, n, T, n, S);
     std::vector<double> Sh((size_t)nt * nt), w;
     STDA_HIP_CHECK(hipMemcpyAsync(Sh.data(), S, Sh.size() * 8, hipMemcpyDeviceToHost, c->stream));
     c->sync();
@@ -473,7 +500,7 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     dgemm(c->blas, false, false, n, kept, nt, 1.0, T, n, Zd.get(), nt, 0.0, T2, n);
     // one more projection against V for safety, then renormalise
     if (m > 0) {
-        dgemm(c->blas, true, false, m, kept, n, 1.0, V, n, T2, n, 0.0, C, m);
+        dgemm_tn(c, m, kept, n, V, n, T2, n, C);
         dgemm(c->blas, false, false, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
     }
     std::vector<double> nrm(kept);
@@ -528,8 +555,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         HostSpan hs_it(c, "dav_iter_other");
         // H(0:m+mnew, m:m+mnew) = V^T AV
         int mt = m + mnew;
-        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, AV.get() + n * m, n, 0.0,
-              Hc.get(), mt);
+        dgemm_tn(c, mt, mnew, n, V.get(), n, AV.get() + n * m, n, Hc.get());
         std::vector<double> hc((size_t)mt * mnew);
         Hc.download(hc.data(), hc.size(), s);
         c->sync();
@@ -623,10 +649,8 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
                   n);
         int mt = m + mnew;
         DBuf<double> Hc((size_t)mt * mnew * 2);
-        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, P.get() + n * m, n, 0.0,
-              Hc.get(), mt);
-        dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, M.get() + n * m, n, 0.0,
-              Hc.get() + (size_t)mt * mnew, mt);
+        dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
+        dgemm_tn(c, mt, mnew, n, V.get(), n, M.get() + n * m, n, Hc.get() + (size_t)mt * mnew);
         std::vector<double> hc((size_t)mt * mnew * 2);
         Hc.download(hc.data(), hc.size(), s);
         c->sync();
@@ -765,7 +789,7 @@ std::vector<double> moment_products(stda_hip_ctx *c, const double *U, int k) {
     std::vector<double> T((size_t)9 * k, 0.0);
     if (!c->have_moments || k == 0) return T;
     DBuf<double> Td((size_t)9 * k);
-    dgemm(c->blas, true, false, 9, k, c->nci, 1.0, c->mom.get(), c->nci, U, c->nci, 0.0, Td.get(), 9);
+    dgemm_tn(c, 9, k, c->nci, c->mom.get(), c->nci, U, c->nci, Td.get());
     Td.download(T.data(), T.size(), c->stream);
     c->sync();
     return T;
@@ -1374,7 +1398,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         DBuf<double> m2d;
         m2d.upload(m2.data(), 3, s);
         launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
-        int maxdim = (int)std::min<int64_t>(n, std::max(24 * nrhs, 128));
+        int maxdim = (int)std::min<int64_t>(n, std::max(40 * nrhs, 256));
         DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
         DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd, rhsV;
         // initial space: preconditioned right-hand sides
@@ -1399,11 +1423,9 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                       M.get() + n * m, n);
             int mt = m + mnew;
             DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
-            dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, P.get() + n * m, n, 0.0, Hc.get(), mt);
-            dgemm(c->blas, true, false, mt, mnew, n, 1.0, V.get(), n, M.get() + n * m, n, 0.0,
-                  Hc.get() + (size_t)mt * mnew, mt);
-            dgemm(c->blas, true, false, mnew, 3, n, 1.0, V.get() + n * m, n, rhs.get(), n, 0.0,
-                  Hc.get() + (size_t)2 * mt * mnew, mnew);
+            dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
+            dgemm_tn(c, mt, mnew, n, V.get(), n, M.get() + n * m, n, Hc.get() + (size_t)mt * mnew);
+            dgemm_tn(c, mnew, 3, n, V.get() + n * m, n, rhs.get(), n, Hc.get() + (size_t)2 * mt * mnew);
             std::vector<double> hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
             Hc.download(hc.data(), hc.size(), s);
             c->sync();
@@ -1488,15 +1510,15 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             for (int j = 0; j < 2 * nrhs; ++j)
                 if (!(rn[j] < tol * std::max(1.0, rhsnorm[j % 3]))) un.push_back(j);
             if (un.empty() || m == n) { done = true; break; }
+            // 2x2 diagonal preconditioner of the coupled equations, then keep
+            // the unconverged columns
+            launch_precond_pair(s, n, nrhs, R.get(), R.get() + n * nrhs, c->ed.get(), wd.get());
             int nun = (int)un.size();
             for (int q = 0; q < nun; ++q)
                 if (un[q] != q)
                     STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8, hipMemcpyDeviceToDevice, s));
-            std::vector<double> zeros(nun, 0.0);
-            DBuf<double> zd;
-            zd.upload(zeros.data(), nun, s);
-            launch_precondition(s, n, nun, R.get(), c->ed.get(), zd.get(), R.get());
-            if (m + nun > maxdim) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
+            if (m + nun > maxdim) nun = maxdim - m;
+            if (nun <= 0) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
             STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * nun * 8, hipMemcpyDeviceToDevice, s));
             mnew = orthonormalize(c, V.get(), m, V.get() + n * m, nun, tmp);
             if (mnew == 0) { done = true; break; }
@@ -1506,9 +1528,11 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         Yd.upload(sol.data(), sol.size(), s);
         dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
         DBuf<double> Ad((size_t)3 * nrhs);
-        dgemm(c->blas, true, false, 3, nrhs, n, -2.0, c->mom.get(), n, XP.get(), n, 0.0, Ad.get(), 3);
+        dgemm_tn(c, 3, nrhs, n, c->mom.get(), n, XP.get(), n, Ad.get());
         std::vector<double> ah((size_t)3 * nrhs);
         Ad.download(ah.data(), ah.size(), s);
+        c->sync();
+        for (auto &v : ah) v *= -2.0;
         if (xpy_out) XP.download(xpy_out, (size_t)n * nrhs, s);
         c->sync();
         for (int f = 0; f < nfreq; ++f)

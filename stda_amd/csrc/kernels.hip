@@ -1034,3 +1034,45 @@ void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, 
 }
 
 }  // namespace stda
+
+namespace stda {
+
+// Preconditioner of the response equations: with d the diagonal (ed) solve
+//   d x1 - w x2 = R1,  -w x1 + d x2 = R2
+// per element; w is the frequency of each right-hand side column.
+__global__ void precond_pair_kernel(int64_t n, int ncol, double *R1, double *R2, const double *d,
+                                    const double *w) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= n) return;
+    double dd = d[i], ww = w[c];
+    double det = dd * dd - ww * ww;
+    if (fabs(det) < 1.0e-6) det = det < 0.0 ? -1.0e-6 : 1.0e-6;
+    double r1 = R1[i + n * c], r2 = R2[i + n * c];
+    R1[i + n * c] = (dd * r1 + ww * r2) / det;
+    R2[i + n * c] = (ww * r1 + dd * r2) / det;
+}
+
+void launch_precond_pair(hipStream_t s, int64_t n, int ncol, double *R1, double *R2,
+                         const double *d, const double *w) {
+    if (ncol == 0) return;
+    hipLaunchKernelGGL(precond_pair_kernel, dim3((unsigned)((n + 255) / 256), ncol), dim3(256), 0,
+                       s, n, ncol, R1, R2, d, w);
+}
+
+// C(i) = sum_c W(i, c) over nchunk slices of length len, in slice order.
+__global__ void sum_slices_kernel(int64_t len, int nchunk, const double *W, double *C) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += W[i + len * c];
+    C[i] = s;
+}
+
+void launch_sum_slices(hipStream_t s, int64_t len, int nchunk, const double *W, double *C) {
+    if (len == 0) return;
+    hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len,
+                       nchunk, W, C);
+}
+
+}  // namespace stda

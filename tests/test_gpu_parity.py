@@ -102,7 +102,9 @@ def _compare_selection(h, ref):
 
 
 def _compare_vectors(X, Xref):
-    ov = np.abs(np.sum(X * Xref, axis=0))
+    """|cos| of the angle between matching columns (sign and norm free)."""
+    ov = np.abs(np.sum(X * Xref, axis=0)) / (np.linalg.norm(X, axis=0) *
+                                             np.linalg.norm(Xref, axis=0))
     np.testing.assert_allclose(ov, 1.0, atol=1e-6)
 
 
