@@ -116,6 +116,11 @@ void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, 
                           const double *de, const unsigned char *rowmask, unsigned char rowbit,
                           const unsigned char *colmask, unsigned char colbit, double ak,
                           double *colsum);
+// Software-pipelined PT2 passes (mode 0: row sums, 1: column sums).
+void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int ncent, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak, double *out);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

@@ -961,9 +961,9 @@ static void do_select(stda_hip_ctx *c) {
         TileSet rows{rod.get(), rvd.get(), nrt}, cols{cod.get(), cvd.get(), nct};
         DBuf<double> rowsum(c->npair);
         EventTimer ev1(c);
-        launch_pt2_rows_mfma(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                             c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1,
-                             c->ak, rowsum.get());
+        launch_pt2_pipe(s, 0, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                        c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1, c->ak,
+                        rowsum.get());
         ev1.finish("pt2_rows", 4.0 * na * (double)c->nsec * (double)c->nprim);
         std::vector<double> rs(c->npair);
         rowsum.download(rs.data(), rs.size(), s);
@@ -982,9 +982,9 @@ static void do_select(stda_hip_ctx *c) {
             TileSet rows2{ro2d.get(), rv2d.get(), nrt2};
             DBuf<double> colsum(c->npair);
             EventTimer ev2(c);
-            launch_pt2_cols_mfma(s, rows2, cols, na, nv, c->pia.get(), c->qia.get(),
-                                 c->pij.get(), c->qab.get(), c->de.get(), c->flags.get(), 4,
-                                 c->flags.get(), 1, c->ak, colsum.get());
+            launch_pt2_pipe(s, 1, rows2, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                            c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1,
+                            c->ak, colsum.get());
             ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs(c->npair);
             colsum.download(cs.data(), cs.size(), s);

@@ -1076,3 +1076,236 @@ void launch_sum_slices(hipStream_t s, int64_t len, int nchunk, const double *W, 
 }
 
 }  // namespace stda
+
+namespace stda {
+
+// ----------------------------------------------------------------------------
+// Software-pipelined form of the PT2 passes.  A work-group owns one tile of
+// its side (a secondary row tile in pass 1, a primary column tile in pass 2)
+// and streams the tiles of the other side.  The (tile, atom chunk) steps are
+// flattened; while the matrix cores work on step s from one LDS buffer, the
+// operands of step s+1 are fetched into registers and then written to the
+// other buffer, so a single barrier per step orders everything.
+struct PipeBuf {
+    double pk[M_CH][M_LD];
+    double qc[M_CH][M_LD];
+    double pj[M_CH][M_LD];
+    double qv[M_CH][M_LD];
+};
+struct PipeSmem {
+    PipeBuf buf[2];
+    double xch[4][256];
+    double red[4][64];
+};
+
+// Column index (pair or packed pair) of this thread's 16 staged elements, or -1.
+// o = 0 pK(k,c), 1 q(j,b), 2 pJ(k,j), 3 qab(c,b); the row tile gives k, c and
+// the column tile j, b.
+__device__ __forceinline__ void stage_cols(TileSet rows, int rt, TileSet cols, int ct, int nv,
+                                           int32_t idx[4][4]) {
+    int vb = threadIdx.x / M_CH;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int v = vb + 16 * r, x = v / M_T, y = v % M_T;
+        int kx = rows.occ[rt * M_T + x], cy = rows.vir[rt * M_T + y];
+        int jx = cols.occ[ct * M_T + x], by = cols.vir[ct * M_T + y];
+        int jy = cols.occ[ct * M_T + y], cx = rows.vir[rt * M_T + x];
+        idx[0][r] = (kx >= 0 && cy >= 0) ? kx * nv + cy : -1;
+        idx[1][r] = (jx >= 0 && by >= 0) ? jx * nv + by : -1;
+        idx[2][r] = (kx >= 0 && jy >= 0) ? (int32_t)lin0(kx, jy) : -1;
+        idx[3][r] = (cx >= 0 && by >= 0) ? (int32_t)lin0(cx, by) : -1;
+    }
+}
+
+// de of a row pair (k,c) or column pair (j,b) of a tile; NaN when the pair is
+// outside the tile or not selected by the mask.
+__device__ __forceinline__ double pair_de(int o, int v, int nv, const double *de,
+                                          const unsigned char *mask, unsigned char bit) {
+    if (o < 0 || v < 0) return __builtin_nan("");
+    int64_t p = (int64_t)o * nv + v;
+    return (mask[p] & bit) ? de[p] : __builtin_nan("");
+}
+
+template <int MODE>   // 0: row sums (pass 1), 1: column sums (pass 2)
+__global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
+    TileSet rows, TileSet cols, int ncent, int nv, const double *__restrict__ pK,
+    const double *__restrict__ q, const double *__restrict__ pJ, const double *__restrict__ qab,
+    const double *__restrict__ de, const unsigned char *__restrict__ rowmask, unsigned char rowbit,
+    const unsigned char *__restrict__ colmask, unsigned char colbit, double ak, double *out) {
+    __shared__ PipeSmem sm;
+    const int own = blockIdx.x;
+    const int nother = MODE == 0 ? cols.ntiles : rows.ntiles;
+    const int nch = (ncent + M_CH - 1) / M_CH;
+    const int64_t nsteps = (int64_t)nother * nch;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int kh = w >> 1, jh = w & 1;
+    const int r16 = lane & 15, g = lane >> 4, hi = r16 >> 2, lo = r16 & 3;
+    const int sa = threadIdx.x % M_CH, svb = threadIdx.x / M_CH;
+    const int iPJ = M_T * (4 * kh + hi) + (4 * jh + lo);
+    const int iPK0 = M_T * (4 * kh + hi) + lo;
+    const int iQ0 = M_T * (4 * jh + hi) + lo;
+    const int iQV0 = M_T * hi + lo;   // + 32 ch + 4 bh
+    const double *src[4] = {pK, q, pJ, qab};
+
+    int32_t idx[4][4];
+    double pre[4][4];
+    auto fetch = [&](int ch) {
+        int a = ch * M_CH + sa;
+        bool in = a < ncent;
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                pre[o][r] = (in && idx[o][r] >= 0) ? src[o][(int64_t)ncent * idx[o][r] + a] : 0.0;
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int v = svb + 16 * r;
+            sm.buf[b].pk[sa][v] = pre[0][r];
+            sm.buf[b].qc[sa][v] = pre[1][r];
+            sm.buf[b].pj[sa][v] = pre[2][r];
+            sm.buf[b].qv[sa][v] = pre[3][r];
+        }
+    };
+    auto set_tile = [&](int other) {
+        if (MODE == 0) stage_cols(rows, own, cols, other, nv, idx);
+        else stage_cols(rows, other, cols, own, nv, idx);
+    };
+
+    set_tile(0);
+    fetch(0);
+    store(0);
+    __syncthreads();
+
+    v4d accJ[4], accK[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
+        accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+    // MODE 0: partial row sums of rows (4kh+i, 4ch+g); MODE 1: column sums (4jh+hi, 4bh+lo)
+    double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    for (int64_t step = 0; step < nsteps; ++step) {
+        const int ch = (int)(step % nch);
+        const int other = (int)(step / nch);
+        const int cur = (int)(step & 1);
+        const bool more = step + 1 < nsteps;
+        if (more) {
+            int ch2 = (int)((step + 1) % nch);
+            if (ch2 == 0) set_tile(other + 1);
+            fetch(ch2);
+        }
+        const PipeBuf &B = sm.buf[cur];
+#pragma unroll
+        for (int s4 = 0; s4 < M_CH / 4; ++s4) {
+            int arow = 4 * s4 + g;
+            double aJ = B.pj[arow][iPJ];
+            double aK0 = B.pk[arow][iPK0], aK1 = B.pk[arow][iPK0 + 4];
+            double bQ0 = B.qc[arow][iQ0], bQ1 = B.qc[arow][iQ0 + 4];
+            accJ[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0], accJ[0], 0, 0, 0);
+            accJ[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 4], accJ[1], 0, 0, 0);
+            accJ[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 32], accJ[2], 0, 0, 0);
+            accJ[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 36], accJ[3], 0, 0, 0);
+            accK[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ0, accK[0], 0, 0, 0);
+            accK[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ1, accK[1], 0, 0, 0);
+            accK[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ0, accK[2], 0, 0, 0);
+            accK[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ1, accK[3], 0, 0, 0);
+        }
+        if (ch == nch - 1) {
+            const int rt = MODE == 0 ? own : other, ct = MODE == 0 ? other : own;
+            const int jj = cols.occ[ct * M_T + 4 * jh + hi];
+#pragma unroll
+            for (int blk = 0; blk < 4; ++blk) {
+                const int chv = blk >> 1, bh = blk & 1;
+                const double dc = pair_de(jj, cols.vir[ct * M_T + 4 * bh + lo], nv, de, colmask,
+                                          colbit);
+                const int cc = rows.vir[rt * M_T + 4 * chv + g];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    sm.xch[w][((i * 4 + hi) * 4 + g) * 4 + lo] = accJ[blk][i];
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    double J = sm.xch[w][((i * 4 + g) * 4 + hi) * 4 + lo];
+                    double amat = ak * accK[blk][i] - J;
+                    double dr = pair_de(rows.occ[rt * M_T + 4 * kh + i], cc, nv, de, rowmask, rowbit);
+                    double pt = amat * amat / (dr - dc + 1.0e-10);
+                    if (!__builtin_isnan(pt)) {
+                        if (MODE == 0) part[chv * 4 + i] += pt;
+                        else part[bh] += pt;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                accJ[blk] = v4d{0.0, 0.0, 0.0, 0.0};
+                accK[blk] = v4d{0.0, 0.0, 0.0, 0.0};
+            }
+        }
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+
+    if (MODE == 0) {
+#pragma unroll
+        for (int q8 = 0; q8 < 8; ++q8) {
+            double v = part[q8];
+            for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+            part[q8] = v;
+        }
+        if (r16 == 0)
+            for (int q8 = 0; q8 < 8; ++q8) sm.red[w][g * 8 + q8] = part[q8];   // q8 = 4 c2 + i
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            int t = threadIdx.x;
+            int k = t / M_T, c = t % M_T;
+            int kh2 = k / 4, i = k % 4, c2 = c / 4, gg = c % 4;
+            double sum = sm.red[2 * kh2][gg * 8 + c2 * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + c2 * 4 + i];
+            int kk = rows.occ[own * M_T + k], cv = rows.vir[own * M_T + c];
+            if (kk >= 0 && cv >= 0) {
+                int64_t p = (int64_t)kk * nv + cv;
+                if (rowmask[p] & rowbit) out[p] = sum;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int bh = 0; bh < 2; ++bh) {
+            double v = part[bh];
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            part[bh] = v;
+        }
+        if (lane < 16)
+            for (int bh = 0; bh < 2; ++bh) sm.red[w][r16 * 2 + bh] = part[bh];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            int t = threadIdx.x;
+            int j = t / M_T, b = t % M_T;
+            int jh2 = j / 4, hh = j % 4, bh = b / 4, ll = b % 4;
+            int r = hh * 4 + ll;
+            double sum = sm.red[jh2][r * 2 + bh] + sm.red[2 + jh2][r * 2 + bh];
+            int jv = cols.occ[own * M_T + j], bv = cols.vir[own * M_T + b];
+            if (jv >= 0 && bv >= 0) {
+                int64_t p = (int64_t)jv * nv + bv;
+                if (colmask[p] & colbit) out[p] = sum;
+            }
+        }
+    }
+}
+
+void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int ncent, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                     const unsigned char *colmask, unsigned char colbit, double ak, double *out) {
+    int nown = mode == 0 ? rows.ntiles : cols.ntiles;
+    int noth = mode == 0 ? cols.ntiles : rows.ntiles;
+    if (nown == 0 || noth == 0) return;
+    if (mode == 0)
+        hipLaunchKernelGGL(pt2_pipe_kernel<0>, dim3(nown), dim3(256), 0, s, rows, cols, ncent, nv,
+                           pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, out);
+    else
+        hipLaunchKernelGGL(pt2_pipe_kernel<1>, dim3(nown), dim3(256), 0, s, rows, cols, ncent, nv,
+                           pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, out);
+}
+
+}  // namespace stda
