@@ -49,6 +49,19 @@ def workload(name: str):
     return mol, ci, p
 
 
+def copy_probe(nbytes: int):
+    """One device-to-device copy of nbytes (a kernel of known HBM traffic:
+    nbytes read + nbytes written), used to scale the TCC byte counters."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,7 +70,12 @@ def main():
     ap.add_argument("--workload", default="cluster400")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (calibrates HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, not a measured path in this build.", file=sys.stderr)

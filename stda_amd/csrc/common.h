@@ -121,6 +121,17 @@ void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int nc
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *de, const unsigned char *rowmask, unsigned char rowbit,
                      const unsigned char *colmask, unsigned char colbit, double ak, double *out);
+// PT2 pass 2 over a sparse row set: row tile = one occupied k (rk) with 16
+// virtual c (rc, -1 padded); column tile = 16 occupied j (cj) x 16 virtual b (cb).
+void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int32_t *rc, int nct,
+                            const int32_t *cj, const int32_t *cb, int ncent, int nv,
+                            const double *pK, const double *q, const double *pJ, const double *qab,
+                            const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                            const unsigned char *colmask, unsigned char colbit, double ak,
+                            double *colsum);
+// column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
+void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
+                            int nsl, double *norms);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

@@ -241,6 +241,7 @@ struct stda_hip_ctx {
     // matrix-free work buffers
     DBuf<double> xfull, tbuf, dbuf, gbuf, wbuf, rbuf, ubuf;
     DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
+    DBuf<double> normbuf; // slice partials of column norms
     int nb_max = 0;
     bool xfull_clean = false;
 
@@ -335,6 +336,13 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
         dgemm(c->blas, true, false, m, p, rem, 1.0, A + nch * kc, lda, B + nch * kc, ldb, 0.0,
               c->tnbuf.get() + nch * m * p, m);
     launch_sum_slices(c->stream, m * p, (int)slots, c->tnbuf.get(), C);
+}
+
+// column 2-norms of R (n x k) into norms (device), deterministic
+void col_norms(stda_hip_ctx *c, int64_t n, int k, const double *R, double *norms) {
+    int nsl = (int)std::min<int64_t>(256, std::max<int64_t>(1, n / 8192));
+    c->normbuf.ensure((size_t)nsl * std::max(k, 1));
+    launch_col_norms_split(c->stream, n, k, R, c->normbuf.get(), nsl, norms);
 }
 
 void ensure_work(stda_hip_ctx *c, int nb) {
@@ -505,7 +513,7 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     }
     std::vector<double> nrm(kept);
     DBuf<double> nd(kept);
-    launch_col_norms(c->stream, n, kept, T2, nd.get());
+    col_norms(c, n, kept, T2, nd.get());
     nd.download(nrm.data(), kept, c->stream);
     c->sync();
     for (auto &x : nrm) x = x > 0 ? 1.0 / x : 0.0;
@@ -581,7 +589,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
         dgemm(c->blas, false, false, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
         launch_residual(s, n, kk, AX.get(), X.get(), th.get(), R.get());
-        launch_col_norms(s, n, kk, R.get(), nrm.get());
+        col_norms(c, n, kk, R.get(), nrm.get());
         std::vector<double> rn(kk);
         nrm.download(rn.data(), kk, s);
         c->sync();
@@ -702,7 +710,7 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
         om.upload(omega.data(), kk, s);
         launch_residual(s, n, kk, PX.get(), XmY.get(), om.get(), R.get());            // (A+B)(X+Y) - w(X-Y)
         launch_residual(s, n, kk, MX.get(), XpY.get(), om.get(), R.get() + n * kk);   // (A-B)(X-Y) - w(X+Y)
-        launch_col_norms(s, n, 2 * kk, R.get(), nrm.get());
+        col_norms(c, n, 2 * kk, R.get(), nrm.get());
         std::vector<double> rn(2 * kk);
         nrm.download(rn.data(), 2 * kk, s);
         c->sync();
@@ -975,16 +983,42 @@ static void do_select(stda_hip_ctx *c) {
         }
         if (nexcl > 0) {
             c->flags.upload(fl.data(), fl.size(), s);
-            std::vector<int32_t> ro2, rv2;
-            int nrt2 = make_tiles(PT2_T, PT2_T, 4, ro2, rv2);
-            DBuf<int32_t> ro2d, rv2d;
-            ro2d.upload(ro2.data(), ro2.size(), s); rv2d.upload(rv2.data(), rv2.size(), s);
-            TileSet rows2{ro2d.get(), rv2d.get(), nrt2};
+            // rejected secondaries grouped by occupied k, 16 virtual c per row tile
+            std::vector<int32_t> rk, rc;
+            for (int64_t idx = 0; idx < (int64_t)sec.size();) {
+                int32_t p = sec[idx];
+                if (!(fl[p] & 4)) { ++idx; continue; }
+                int k = p / nv;
+                rk.push_back(k);
+                int filled = 0;
+                while (idx < (int64_t)sec.size() && sec[idx] / nv == k && filled < 16) {
+                    if (fl[sec[idx]] & 4) { rc.push_back(sec[idx] % nv); ++filled; }
+                    ++idx;
+                }
+                for (; filled < 16; ++filled) rc.push_back(-1);
+            }
+            std::vector<int32_t> cjv, cbv;
+            int nct2 = 0;
+            for (int j0 = 0; j0 < no; j0 += 16)
+                for (int b0 = 0; b0 < nv; b0 += 16) {
+                    bool any = false;
+                    for (int j = j0; j < std::min(no, j0 + 16) && !any; ++j)
+                        for (int b = b0; b < std::min(nv, b0 + 16); ++b)
+                            if (fl[(int64_t)j * nv + b] & 1) { any = true; break; }
+                    if (!any) continue;
+                    for (int x = 0; x < 16; ++x) cjv.push_back(j0 + x < no ? j0 + x : -1);
+                    for (int x = 0; x < 16; ++x) cbv.push_back(b0 + x < nv ? b0 + x : -1);
+                    ++nct2;
+                }
+            DBuf<int32_t> rkd, rcd, cjd, cbd;
+            rkd.upload(rk.data(), rk.size(), s); rcd.upload(rc.data(), rc.size(), s);
+            cjd.upload(cjv.data(), cjv.size(), s); cbd.upload(cbv.data(), cbv.size(), s);
             DBuf<double> colsum(c->npair);
             EventTimer ev2(c);
-            launch_pt2_pipe(s, 1, rows2, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                            c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1,
-                            c->ak, colsum.get());
+            launch_pt2_cols_sparse(s, (int)rk.size(), rkd.get(), rcd.get(), nct2, cjd.get(),
+                                   cbd.get(), na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                                   c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1,
+                                   c->ak, colsum.get());
             ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs(c->npair);
             colsum.download(cs.data(), cs.size(), s);
@@ -1412,7 +1446,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         std::vector<double> rhsnorm(3);
         {
             DBuf<double> rn(3);
-            launch_col_norms(s, n, 3, rhs.get(), rn.get());
+            col_norms(c, n, 3, rhs.get(), rn.get());
             rn.download(rhsnorm.data(), 3, s);
             c->sync();
         }
@@ -1502,7 +1536,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             for (int f = 0; f < nfreq; ++f)   // subtract the right-hand sides: R1 -= r
                 launch_residual(s, n, 3, R.get() + n * 3 * f, rhs.get(), oned.get(),
                                 R.get() + n * 3 * f);
-            launch_col_norms(s, n, 2 * nrhs, R.get(), nrm.get());
+            col_norms(c, n, 2 * nrhs, R.get(), nrm.get());
             std::vector<double> rn(2 * nrhs);
             nrm.download(rn.data(), rn.size(), s);
             c->sync();

@@ -1309,3 +1309,179 @@ void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int nc
 }
 
 }  // namespace stda
+
+namespace stda {
+
+// ----------------------------------------------------------------------------
+// PT2 pass 2 for sparse row sets.  A row tile is ONE occupied k with up to
+// 16 virtual c (the rejected secondaries of k); a column tile is 16 occupied
+// j x 16 virtual b.  Wave w takes b in [4w, 4w+4):
+//   J blocks nb = 0..3: rows j (16), cols (c = 4nb + r16/4, b = 4w + r16%4)
+//   K blocks mb = 0..3: rows c (16), cols (j = 4mb + r16/4, b = 4w + r16%4)
+// The J values are exchanged through LDS (J[c][j][b]) into the K layout.
+// A work-group owns a column tile and streams the row tiles (deterministic).
+constexpr int S_CH = 8;          // atoms per staged chunk (2 MFMA k-steps)
+constexpr int S_LD = 16 * 16 + 1;
+struct SparseSmem {
+    double pj[S_CH][17];         // (k, j)        16
+    double pk[S_CH][17];         // (k, c)        16
+    double qv[S_CH][S_LD];       // (c, b)        256 = 16 c x 16 b
+    double qc[S_CH][S_LD];       // (j, b)        256 = 16 j x 16 b
+    double xch[4][16 * 16 * 4];  // per wave: [c][j][b_local]
+    double red[4][64];
+};
+
+__global__ void __launch_bounds__(256, 2) pt2_cols_sparse_kernel(
+    int nrt, const int32_t *__restrict__ rk, const int32_t *__restrict__ rc, int nct,
+    const int32_t *__restrict__ cj, const int32_t *__restrict__ cb, int ncent, int nv,
+    const double *__restrict__ pK, const double *__restrict__ q, const double *__restrict__ pJ,
+    const double *__restrict__ qab, const double *__restrict__ de,
+    const unsigned char *__restrict__ rowmask, unsigned char rowbit,
+    const unsigned char *__restrict__ colmask, unsigned char colbit, double ak,
+    double *__restrict__ colsum) {
+    __shared__ SparseSmem sm;
+    const int ct = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4, hi = r16 >> 2, lo = r16 & 3;
+    const int t = threadIdx.x;
+    double cpart[4] = {0.0, 0.0, 0.0, 0.0};   // columns (j = 4mb + hi, b = 4w + lo)
+    // this lane's columns: de (NaN when not a selected primary)
+    double dcol[4];
+    const int bcol = cb[ct * 16 + 4 * w + lo];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) dcol[mb] = pair_de(cj[ct * 16 + 4 * mb + hi], bcol, nv, de, colmask, colbit);
+
+    for (int rt = 0; rt < nrt; ++rt) {
+        const int k = rk[rt];
+        v4d accJ[4], accK[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
+            accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int a0 = 0; a0 < ncent; a0 += S_CH) {
+            __syncthreads();
+            // stage: 544 vectors x 8 atoms; thread -> (atom = t % 8, vector slot = t / 8)
+            const int sa = t % S_CH, sv = t / S_CH;   // sv in 0..31
+            const int a = a0 + sa;
+            const bool in = a < ncent;
+            if (sv < 16) {
+                int j = cj[ct * 16 + sv];
+                sm.pj[sa][sv] = (in && j >= 0) ? pJ[(int64_t)ncent * lin0(k, j) + a] : 0.0;
+            } else {
+                int c = rc[rt * 16 + sv - 16];
+                sm.pk[sa][sv - 16] = (in && c >= 0) ? pK[(int64_t)ncent * ((int64_t)k * nv + c) + a] : 0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                int v = sv + 32 * r;   // 0..255: (x = v / 16, y = v % 16)
+                int x = v >> 4, y = v & 15;
+                int c = rc[rt * 16 + x], b = cb[ct * 16 + y], j = cj[ct * 16 + x];
+                sm.qv[sa][v] = (in && c >= 0 && b >= 0) ? qab[(int64_t)ncent * lin0(c, b) + a] : 0.0;
+                sm.qc[sa][v] = (in && j >= 0 && b >= 0) ? q[(int64_t)ncent * ((int64_t)j * nv + b) + a] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < S_CH / 4; ++s4) {
+                const int arow = 4 * s4 + g;
+                const double aJ = sm.pj[arow][r16];
+                const double aK = sm.pk[arow][r16];
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    // J block nb: column (c = 4nb + hi, b = 4w + lo) -> qab slot 16 c + b
+                    double bV = sm.qv[arow][16 * (4 * nb + hi) + 4 * w + lo];
+                    accJ[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, bV, accJ[nb], 0, 0, 0);
+                    // K block nb: column (j = 4nb + hi, b = 4w + lo) -> q slot 16 j + b
+                    double bQ = sm.qc[arow][16 * (4 * nb + hi) + 4 * w + lo];
+                    accK[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK, bQ, accK[nb], 0, 0, 0);
+                }
+            }
+        }
+        // J to the K layout: J block nb, lane, reg i -> (c = 4nb + hi, j = 4i + g, b = lo)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                sm.xch[w][((4 * nb + hi) * 16 + 4 * i + g) * 4 + lo] = accJ[nb][i];
+        __builtin_amdgcn_wave_barrier();
+        // rows of this lane in the K layout: c = 4i + g
+        double drow[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) drow[i] = pair_de(k, rc[rt * 16 + 4 * i + g], nv, de, rowmask, rowbit);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // K block mb, lane, reg i -> (c = 4i + g, j = 4mb + hi, b = lo)
+                double J = sm.xch[w][((4 * i + g) * 16 + 4 * mb + hi) * 4 + lo];
+                double amat = ak * accK[mb][i] - J;
+                double pt = amat * amat / (drow[i] - dcol[mb] + 1.0e-10);
+                if (!__builtin_isnan(pt)) cpart[mb] += pt;
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // reduce over the four row groups g (lanes 16 apart)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        double v = cpart[mb];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        cpart[mb] = v;
+    }
+    if (g == 0) {
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            int j = cj[ct * 16 + 4 * mb + hi], b = cb[ct * 16 + 4 * w + lo];
+            if (j >= 0 && b >= 0) {
+                int64_t p = (int64_t)j * nv + b;
+                if (colmask[p] & colbit) colsum[p] = cpart[mb];
+            }
+        }
+    }
+}
+
+void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int32_t *rc, int nct,
+                            const int32_t *cj, const int32_t *cb, int ncent, int nv,
+                            const double *pK, const double *q, const double *pJ, const double *qab,
+                            const double *de, const unsigned char *rowmask, unsigned char rowbit,
+                            const unsigned char *colmask, unsigned char colbit, double ak,
+                            double *colsum) {
+    if (nrt == 0 || nct == 0) return;
+    hipLaunchKernelGGL(pt2_cols_sparse_kernel, dim3(nct), dim3(256), 0, s, nrt, rk, rc, nct, cj, cb,
+                       ncent, nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
+}
+
+// Column norms: one block per (column, slice of rows); partial sums of squares
+// are reduced in slice order by a second kernel (deterministic).
+__global__ void col_sumsq_partial_kernel(int64_t n, int64_t slice, const double *R, double *part) {
+    __shared__ double ws[4];
+    int c = blockIdx.y, sl = blockIdx.x;
+    int64_t i0 = (int64_t)sl * slice, i1 = min(n, i0 + slice);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        double v = R[i + n * c];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[sl + (int64_t)gridDim.x * c] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ void col_norm_final_kernel(int nsl, int k, const double *part, double *norms) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (int sl = 0; sl < nsl; ++sl) s += part[sl + (int64_t)nsl * c];
+    norms[c] = sqrt(s);
+}
+
+void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
+                            int nsl, double *norms) {
+    if (k == 0) return;
+    int64_t slice = (n + nsl - 1) / nsl;
+    hipLaunchKernelGGL(col_sumsq_partial_kernel, dim3(nsl, k), dim3(256), 0, s, n, slice, R, part);
+    hipLaunchKernelGGL(col_norm_final_kernel, dim3((k + 63) / 64), dim3(64), 0, s, nsl, k, part, norms);
+}
+
+}  // namespace stda
