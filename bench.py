@@ -121,6 +121,14 @@ def main():
                 "avg_launch_ms": round(st["ms"] / max(st["launches"], 1), 3),
                 "work_per_launch": st["work"] / max(st["launches"], 1)}
 
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # counters for this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    prof = os.path.join(ROOT, "profiles", "pt2_pipe_cluster400_pmc.md")
+    if dom == "pt2_rows" and args.workload == "cluster400" and os.path.exists(prof):
+        for ln in open(prof):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                roofline["traffic"] = float(ln.split(":")[1])
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -1126,6 +1126,14 @@ __device__ __forceinline__ double pair_de(int o, int v, int nv, const double *de
     return (mask[p] & bit) ? de[p] : __builtin_nan("");
 }
 
+// block b of n is dispatched to XCD b % 8; map it to the tile range of that XCD
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int nx = 8;
+    int q = n / nx, r = n % nx;
+    int x = b % nx, i = b / nx;
+    return x * q + min(x, r) + i;
+}
+
 template <int MODE>   // 0: row sums (pass 1), 1: column sums (pass 2)
 __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     TileSet rows, TileSet cols, int ncent, int nv, const double *__restrict__ pK,
@@ -1133,7 +1141,10 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     const double *__restrict__ de, const unsigned char *__restrict__ rowmask, unsigned char rowbit,
     const unsigned char *__restrict__ colmask, unsigned char colbit, double ak, double *out) {
     __shared__ PipeSmem sm;
-    const int own = blockIdx.x;
+    // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so the
+    // tiles are regrouped to give each XCD a contiguous range of owned tiles;
+    // neighbouring tiles share operand columns, which then hit that XCD's L2.
+    const int own = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int nother = MODE == 0 ? cols.ntiles : rows.ntiles;
     const int nch = (ncent + M_CH - 1) / M_CH;
     const int64_t nsteps = (int64_t)nother * nch;
