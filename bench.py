@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--workload", default="cluster400")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters)")
     args = ap.parse_args()
@@ -89,6 +90,7 @@ def main():
     from stda_amd.core import StdaHip
     h = StdaHip(0)
     h.load(ci, p["thr"], p["ax"])
+    h.set_precision(args.precision)
     for w in range(args.warmup):
         h.select()
         h.solve_tda(p["nroots"], velcorr=True, vectors=False)
@@ -101,13 +103,23 @@ def main():
         nci, nprim, nsec = h.select()
         sol = h.solve_tda(p["nroots"], velcorr=True, vectors=False)
         lat.append(time.perf_counter() - ts)
-        stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_cols", "ao_matvec")})
+        stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_cols", "ao_matvec",
+                                                       "ao_matvec_f32")})
         host = {k: h.kernel_stats(k) for k in ("dav_matvec", "dav_iter_other", "dav_rr_eig",
-                                                "dav_ortho")}
+                                                "dav_ortho", "mixed_fp32_phase",
+                                                "mixed_fp64_refine")}
         print(f"# host spans: {host}", file=sys.stderr, flush=True)
         print(f"# step {s + 1}/{args.steps}: {lat[-1]:.3f}s nci={nci} "
               f"timings={sol.timings_ms}", file=sys.stderr, flush=True)
     total = time.perf_counter() - tstart
+    root_err = None
+    if args.precision != "fp64":
+        # per-root error of the mixed-precision roots against the plain FP64 GPU
+        # path (itself checked against the FP64 oracle by the parity tests)
+        h.set_precision("fp64")
+        ref = h.solve_tda(p["nroots"], velcorr=True, vectors=False)
+        root_err = {"max_abs_dE_eV": float(np.abs(sol.omega - ref.omega).max() * 27.211385050),
+                    "max_abs_dfL": float(np.abs(sol.props[:, 0] - ref.props[:, 0]).max())}
     h.close()
 
     # dominant kernel of the last step
@@ -152,7 +164,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 contractions + f64 Davidson",
+        "precision": args.precision,
+        "root_error_vs_fp64": root_err,
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: plain FP64 sTDA drop-in, "
                                f"{p['nroots']} lowest roots",
@@ -163,7 +177,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
                         "tflops": round(v["work"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else 0}
-                    for k, v in last.items()},
+                    for k, v in last.items() if v["launches"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

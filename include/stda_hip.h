@@ -152,6 +152,16 @@ int stda_hip_build_bhalf(stda_hip_ctx *ctx, double *bhalf);
  * which = 0 A, 1 A+B, 2 A-B, 3 0.5B.  v, av: (nci, nvec). */
 int stda_hip_matvec(stda_hip_ctx *ctx, int32_t which, int32_t nvec, const double *v, double *av);
 
+/* Arithmetic of the coupling operators in the solvers of this context:
+ *   0 (default) FP64 throughout;
+ *   1 mixed: the transition-density contractions (Q gamma Q^T products) in
+ *     FP32 with FP32 accumulation, Davidson orthogonalisation, Rayleigh-Ritz
+ *     and residual norms in FP64, converged to a residual of 1e-6, followed
+ *     by one FP64 Rayleigh-Ritz refinement of the converged roots.  The
+ *     response solver uses the FP32 operator with a 1e-6 tolerance.
+ * No counterpart in the reference (which is FP32 throughout, stda.f:82-107). */
+int stda_hip_set_precision(stda_hip_ctx *ctx, int32_t precision);
+
 /* Timing of the last setup/solve phases on this context, in milliseconds:
  * [0] gamma+charges+intermediates, [1] screening+sort, [2] PT2,
  * [3] eigensolve / response, [4] properties, [5] total. */

@@ -31,7 +31,7 @@ C_API = [
     "stda_hip_setup", "stda_hip_get_csf", "stda_hip_solve_tda", "stda_hip_solve_rpa",
     "stda_hip_lresp", "stda_hip_build_a", "stda_hip_build_apb_amb", "stda_hip_build_bhalf",
     "stda_hip_matvec", "stda_hip_timings", "stda_hip_load", "stda_hip_select",
-    "stda_hip_kernel_stats",
+    "stda_hip_kernel_stats", "stda_hip_set_precision",
 ]
 
 
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_build_bhalf.argtypes = [P, P]
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
     L.stda_hip_timings.argtypes = [P, P]
+    L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
     L.stda_hip_select.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
     L.stda_hip_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
@@ -198,6 +199,12 @@ class StdaHip:
                                             ctypes.byref(nsec)))
         self.nci, self.nprim, self.nsec = nci.value, nprim.value, nsec.value
         return self.nci, self.nprim, self.nsec
+
+    def set_precision(self, precision: str):
+        """'fp64' (default) or 'mixed' (FP32 contractions, FP64 Davidson numerics
+        and a final FP64 Rayleigh-Ritz refinement)."""
+        code = {"fp64": 0, "mixed": 1}[precision]
+        self._check(self._L.stda_hip_set_precision(self._h, code))
 
     def kernel_stats(self, name: str):
         ms, n, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -132,6 +132,18 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);
+// FP32 operator kernels (mixed precision)
+void launch_scatter_pairs_f32(hipStream_t s, const double *v, int64_t ldv, const int32_t *pair,
+                              int64_t n, int nvec, float *full, int64_t ldfull);
+void launch_gather_pairs_f32(hipStream_t s, const float *full, int64_t ldfull, const int32_t *pair,
+                             int64_t n, int nvec, const double *diag, const double *v, int64_t ldv,
+                             double *out, int64_t ldout);
+void launch_atom_diag_f32(hipStream_t s, int nvec, int nao, int ncent, const float *Dt,
+                          int64_t strideD, const int *atom_ptr, const float *gamk, float *u);
+void launch_gbuild_f32(hipStream_t s, int nvec, int nao, int ncent, const float *Dt, int64_t strideD,
+                       const int *atom_of, const float *gamj, const float *gamk, const float *u,
+                       float aJ, float aX, float aK, float *Gt, int64_t strideG);
+void launch_to_f32(hipStream_t s, const double *x, int64_t n, float *y);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

@@ -240,6 +240,12 @@ struct stda_hip_ctx {
 
     // matrix-free work buffers
     DBuf<double> xfull, tbuf, dbuf, gbuf, wbuf, rbuf, ubuf;
+    // mixed precision: FP32 copies of the operator data and FP32 work buffers
+    int precision = 0;    // 0 FP64, 1 FP32 operator + FP64 Davidson numerics and refinement
+    DBuf<float> ct32, gamj32, gamk32;
+    DBuf<float> xfull32, tbuf32, dbuf32, gbuf32, wbuf32, rbuf32, ubuf32;
+    int nb_max32 = 0;
+    bool have32 = false;
     DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
     DBuf<double> normbuf; // slice partials of column norms
     int nb_max = 0;
@@ -369,11 +375,111 @@ int chunk_vectors(stda_hip_ctx *c) {
     return std::min(nb, 64);
 }
 
+void ensure_fp32(stda_hip_ctx *c) {
+    if (c->have32) return;
+    const size_t moci = (size_t)c->no + c->nv, na = c->ncent;
+    c->ct32.resize((size_t)c->nao * moci);
+    c->gamj32.resize(na * na);
+    c->gamk32.resize(na * na);
+    launch_to_f32(c->stream, c->ct.get(), (int64_t)c->nao * moci, c->ct32.get());
+    launch_to_f32(c->stream, c->gamj.get(), (int64_t)(na * na), c->gamj32.get());
+    launch_to_f32(c->stream, c->gamk.get(), (int64_t)(na * na), c->gamk32.get());
+    c->have32 = true;
+}
+
+void ensure_work32(stda_hip_ctx *c, int nb) {
+    if (nb <= c->nb_max32) return;
+    size_t nao = c->nao, no = c->no;
+    c->xfull32.resize((size_t)c->npair * nb);
+    c->xfull32.zero(c->stream);
+    c->tbuf32.resize(nao * no * nb);
+    c->dbuf32.resize(nao * nao * nb);
+    c->gbuf32.resize(nao * nao * nb);
+    c->wbuf32.resize(nao * no * nb);
+    c->rbuf32.resize((size_t)c->npair * nb);
+    c->ubuf32.resize((size_t)c->ncent * nb);
+    c->nb_max32 = nb;
+}
+
+// The same operators with the contractions in FP32 (FP32 Loewdin MOs, gamma
+// and GEMMs, FP32 accumulation); the diagonal correction term stays FP64.
+void apply_ops_f32(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V, int64_t ldv,
+                   double *out1, int64_t ld1, double *out2, int64_t ld2) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    const int nao = c->nao, no = c->no, nv = c->nv, na = c->ncent;
+    ensure_fp32(c);
+    const float *Co = c->ct32.get();
+    const float *Cv = c->ct32.get() + (size_t)nao * no;
+    const int nbmax = std::min(128, 2 * chunk_vectors(c));
+    for (int v0 = 0; v0 < nvec; v0 += nbmax) {
+        int nb = std::min(nbmax, nvec - v0);
+        ensure_work32(c, nb);
+        const double *Vb = V + ldv * v0;
+        EventTimer evm(c);
+        launch_scatter_pairs_f32(s, Vb, ldv, c->pair.get(), n, nb, c->xfull32.get(), c->npair);
+        sgemm(c->blas, false, false, nao, (int64_t)no * nb, nv, 1.0f, Cv, nao, c->xfull32.get(), nv,
+              0.0f, c->tbuf32.get(), nao);
+        sgemm_batched(c->blas, false, true, nao, nao, no, 1.0f, c->tbuf32.get(), nao,
+                      (int64_t)nao * no, Co, nao, 0, 0.0f, c->dbuf32.get(), nao, (int64_t)nao * nao,
+                      nb);
+        bool need_u = false;
+        int nops = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            int op = pass == 0 ? op1 : op2;
+            if (op < 0) continue;
+            double *out = pass == 0 ? out1 : out2;
+            int64_t ld = pass == 0 ? ld1 : ld2;
+            float aJ = -1.0f, aX = 0.0f, aK = 0.0f;
+            const double *corr = c->corr.get();
+            switch (op) {
+                case 0: aK = (float)c->ak; break;
+                case 1:
+                    if (c->small_ax) { aJ = 0.0f; aK = (float)(2.0 * c->ak); corr = c->corr_small.get(); }
+                    else { aX = (float)-c->ax; aK = (float)(2.0 * c->ak); }
+                    break;
+                case 2:
+                    if (c->small_ax) {
+                        launch_axpby_diag(s, n, nb, c->corr_small.get(), Vb, ldv, out + ld * v0, ld);
+                        continue;
+                    }
+                    aX = (float)c->ax;
+                    break;
+                case 3:
+                    aJ = 0.0f; aX = (float)(-0.5 * c->ax); aK = (float)(0.5 * c->ak); corr = nullptr;
+                    break;
+            }
+            ++nops;
+            if (aK != 0.0f && !need_u) {
+                launch_atom_diag_f32(s, nb, nao, na, c->dbuf32.get(), (int64_t)nao * nao,
+                                     c->atom_ptr.get(), c->gamk32.get(), c->ubuf32.get());
+                need_u = true;
+            }
+            launch_gbuild_f32(s, nb, nao, na, c->dbuf32.get(), (int64_t)nao * nao, c->atom_of.get(),
+                              c->gamj32.get(), c->gamk32.get(), c->ubuf32.get(), aJ, aX, aK,
+                              c->gbuf32.get(), (int64_t)nao * nao);
+            sgemm_batched(c->blas, false, false, nao, no, nao, 1.0f, c->gbuf32.get(), nao,
+                          (int64_t)nao * nao, Co, nao, 0, 0.0f, c->wbuf32.get(), nao,
+                          (int64_t)nao * no, nb);
+            sgemm(c->blas, true, false, nv, (int64_t)no * nb, nao, 1.0f, Cv, nao, c->wbuf32.get(),
+                  nao, 0.0f, c->rbuf32.get(), nv);
+            launch_gather_pairs_f32(s, c->rbuf32.get(), c->npair, c->pair.get(), n, nb, corr, Vb,
+                                    ldv, out + ld * v0, ld);
+        }
+        double per = 2.0 * nao * (double)no * nv + 2.0 * (double)nao * nao * no;
+        evm.finish("ao_matvec_f32", nb * (per + nops * per));
+    }
+}
+
 // out(I, v) = [op V](I, v) for nvec column vectors (n = nci rows).
 // Two operators sharing one transition density can be requested at once
 // (op2 >= 0 writes out2), which is how the RPA solver gets (A+B)V and (A-B)V.
 void apply_ops(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V, int64_t ldv,
-               double *out1, int64_t ld1, double *out2, int64_t ld2) {
+               double *out1, int64_t ld1, double *out2, int64_t ld2, bool fp32 = false) {
+    if (fp32) {
+        apply_ops_f32(c, op1, op2, nvec, V, ldv, out1, ld1, out2, ld2);
+        return;
+    }
     const int64_t n = c->nci;
     hipStream_t s = c->stream;
     const int nao = c->nao, no = c->no, nv = c->nv, na = c->ncent;
@@ -534,8 +640,16 @@ std::vector<int32_t> lowest_diag_rows(const std::vector<double> &d, int k) {
 
 // ------------------------------------------------------------------------------------------
 // FP64 block Davidson for the k lowest eigenpairs of A (TDA).
+struct DavOpts {
+    bool fp32 = false;             // FP32 operator
+    const double *V0 = nullptr;    // device (n x nv0) start vectors instead of unit vectors
+    int nv0 = 0;
+    int maxiter = 300;
+    bool allow_unconverged = false;
+};
+
 void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta,
-                  DBuf<double> &X) {
+                  DBuf<double> &X, const DavOpts &o = DavOpts()) {
     const int64_t n = c->nci;
     hipStream_t s = c->stream;
     k = (int)std::min<int64_t>(k, n);
@@ -545,19 +659,25 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     DBuf<double> V((size_t)n * maxdim), AV((size_t)n * maxdim), R((size_t)n * maxdim);
     DBuf<double> AX((size_t)n * k), th(k), nrm(k), tmp, Yd;
     X.resize((size_t)n * k);
-    std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
-    DBuf<int32_t> rowsd;
-    rowsd.upload(rows.data(), rows.size(), s);
-    launch_set_unit(s, n, nguess, V.get(), rowsd.get());
-    std::vector<double> H((size_t)maxdim * maxdim, 0.0);
     int m = 0, mnew = nguess;
-    const int maxiter = 300;
+    if (o.V0) {
+        int nv0 = std::min(o.nv0, maxdim);
+        STDA_HIP_CHECK(hipMemcpyAsync(V.get(), o.V0, (size_t)n * nv0 * 8, hipMemcpyDeviceToDevice, s));
+        mnew = orthonormalize(c, V.get(), 0, V.get(), nv0, tmp);
+    } else {
+        std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
+        DBuf<int32_t> rowsd;
+        rowsd.upload(rows.data(), rows.size(), s);
+        launch_set_unit(s, n, nguess, V.get(), rowsd.get());
+    }
+    std::vector<double> H((size_t)maxdim * maxdim, 0.0);
+    const int maxiter = o.maxiter;
     DBuf<double> Hc((size_t)maxdim * maxdim), thud(maxdim);
     Yd.resize((size_t)maxdim * k);
     for (int it = 0; it < maxiter; ++it) {
         {
             HostSpan hs(c, "dav_matvec");
-            apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0);
+            apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0, o.fp32);
             c->sync();
         }
         HostSpan hs_it(c, "dav_iter_other");
@@ -598,6 +718,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
             if (!(rn[j] < tol)) un.push_back(j);
         if (un.empty() && kk == k) return;
         if (m == n) return;   // full space: exact
+        if (it + 1 == maxiter && o.allow_unconverged) return;
         // compact the unconverged residuals to the front of R, precondition
         int nun = (int)un.size();
         for (int q = 0; q < nun; ++q)
@@ -635,7 +756,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
 // the Hermitian reduced form (Am^1/2 Ap Am^1/2) of the subspace problem so
 // that (A-B)^1/2 is never formed in the full space.
 void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega,
-                  DBuf<double> &XpY, DBuf<double> &XmY) {
+                  DBuf<double> &XpY, DBuf<double> &XmY, const DavOpts &o = DavOpts()) {
     const int64_t n = c->nci;
     hipStream_t s = c->stream;
     k = (int)std::min<int64_t>(k, n);
@@ -646,15 +767,21 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
     DBuf<double> R((size_t)n * 2 * k), PX((size_t)n * k), MX((size_t)n * k), tmp, Yd, nrm(2 * k);
     XpY.resize((size_t)n * k);
     XmY.resize((size_t)n * k);
-    std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
-    DBuf<int32_t> rowsd;
-    rowsd.upload(rows.data(), rows.size(), s);
-    launch_set_unit(s, n, nguess, V.get(), rowsd.get());
-    std::vector<double> Ap((size_t)maxdim * maxdim, 0.0), Am((size_t)maxdim * maxdim, 0.0);
     int m = 0, mnew = nguess;
-    for (int it = 0; it < 300; ++it) {
+    if (o.V0) {
+        int nv0 = std::min(o.nv0, maxdim);
+        STDA_HIP_CHECK(hipMemcpyAsync(V.get(), o.V0, (size_t)n * nv0 * 8, hipMemcpyDeviceToDevice, s));
+        mnew = orthonormalize(c, V.get(), 0, V.get(), nv0, tmp);
+    } else {
+        std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
+        DBuf<int32_t> rowsd;
+        rowsd.upload(rows.data(), rows.size(), s);
+        launch_set_unit(s, n, nguess, V.get(), rowsd.get());
+    }
+    std::vector<double> Ap((size_t)maxdim * maxdim, 0.0), Am((size_t)maxdim * maxdim, 0.0);
+    for (int it = 0; it < o.maxiter; ++it) {
         apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n, M.get() + n * m,
-                  n);
+                  n, o.fp32);
         int mt = m + mnew;
         DBuf<double> Hc((size_t)mt * mnew * 2);
         dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
@@ -719,6 +846,7 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
             if (!(rn[j] < tol)) un.push_back(j);
         if (un.empty() && kk == k) return;
         if (m == n) return;
+        if (it + 1 == o.maxiter && o.allow_unconverged) return;
         int nun = (int)un.size();
         for (int q = 0; q < nun; ++q)
             if (un[q] != q)
@@ -876,6 +1004,8 @@ static void do_select(stda_hip_ctx *c) {
     hipStream_t s = c->stream;
     c->ready = false;
     c->nb_max = 0;
+    c->nb_max32 = 0;
+    c->have32 = false;
     c->stats.clear();
     const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
     const double alphak = c->alphak, betaj = c->betaj;
@@ -1324,7 +1454,24 @@ int stda_hip_solve_tda(stda_hip_ctx *c, int32_t nroots, int32_t capacity, int32_
         std::vector<double> w;
         DBuf<double> X;
         const int64_t n = c->nci;
-        if (nroots > 0) {
+        if (nroots > 0 && c->precision == 1) {
+            // FP32 operator to a residual of 1e-6, then one FP64 Rayleigh-Ritz
+            // step in the span of the converged vectors (final refinement)
+            DavOpts lo;
+            lo.fp32 = true;
+            {
+                HostSpan hs(c, "mixed_fp32_phase");
+                davidson_tda(c, nroots, std::max(tol, 1.0e-6), w, X, lo);
+            }
+            DavOpts ref;
+            ref.V0 = X.get();
+            ref.nv0 = (int)w.size();
+            ref.maxiter = 1;
+            ref.allow_unconverged = true;
+            DBuf<double> X0 = std::move(X);
+            HostSpan hs(c, "mixed_fp64_refine");
+            davidson_tda(c, nroots, tol, w, X, ref);
+        } else if (nroots > 0) {
             davidson_tda(c, nroots, tol, w, X);
         } else {
             // every root in (0, Ethr] (ssyevr RANGE='V', stda.f:1143-1150)
@@ -1374,7 +1521,26 @@ int stda_hip_solve_rpa(stda_hip_ctx *c, int32_t nroots, int32_t capacity, double
         std::vector<double> w;
         DBuf<double> XpY, XmY;
         const int64_t n = c->nci;
-        if (nroots > 0) {
+        if (nroots > 0 && c->precision == 1) {
+            DavOpts lo;
+            lo.fp32 = true;
+            {
+                HostSpan hs(c, "mixed_fp32_phase");
+                davidson_rpa(c, nroots, std::max(tol, 1.0e-6), w, XpY, XmY, lo);
+            }
+            // refinement: FP64 paired Rayleigh-Ritz in span(X+Y, X-Y)
+            int k2 = (int)w.size();
+            DBuf<double> V0((size_t)n * 2 * k2);
+            STDA_HIP_CHECK(hipMemcpyAsync(V0.get(), XpY.get(), (size_t)n * k2 * 8, hipMemcpyDeviceToDevice, c->stream));
+            STDA_HIP_CHECK(hipMemcpyAsync(V0.get() + n * k2, XmY.get(), (size_t)n * k2 * 8, hipMemcpyDeviceToDevice, c->stream));
+            DavOpts ref;
+            ref.V0 = V0.get();
+            ref.nv0 = 2 * k2;
+            ref.maxiter = 1;
+            ref.allow_unconverged = true;
+            HostSpan hs(c, "mixed_fp64_refine");
+            davidson_rpa(c, nroots, tol, w, XpY, XmY, ref);
+        } else if (nroots > 0) {
             davidson_rpa(c, nroots, tol, w, XpY, XmY);
         } else {
             int64_t cnt = 0;
@@ -1422,6 +1588,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         STDA_HIP_CHECK(hipSetDevice(c->device));
         Timer t3;
         if (tol <= 0.0) tol = 1.0e-8;
+        if (c->precision == 1) tol = std::max(tol, 1.0e-6);   // FP32 operator noise floor
         const int64_t n = c->nci;
         hipStream_t s = c->stream;
         const int nrhs = 3 * nfreq;
@@ -1454,7 +1621,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         bool done = false;
         for (int it = 0; it < 300 && !done; ++it) {
             apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
-                      M.get() + n * m, n);
+                      M.get() + n * m, n, c->precision == 1);
             int mt = m + mnew;
             DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
             dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
@@ -1649,6 +1816,12 @@ int stda_hip_matvec(stda_hip_ctx *c, int32_t which, int32_t nvec, const double *
         AV.download(av, (size_t)n * nvec, c->stream);
         c->sync();
     })
+}
+
+int stda_hip_set_precision(stda_hip_ctx *c, int32_t precision) {
+    if (!c || precision < 0 || precision > 1) return STDA_HIP_ERR_ARG;
+    c->precision = precision;
+    return STDA_HIP_OK;
 }
 
 int stda_hip_timings(const stda_hip_ctx *c, double *ms6) {

@@ -84,6 +84,29 @@ inline void dgemm_batched(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t
         (rocblas_int)ldc, sc, (rocblas_int)batch));
 }
 
+inline void sgemm(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
+                  const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+                  int64_t ldc) {
+    if (m == 0 || n == 0) return;
+    STDA_BLAS_CHECK(rocblas_sgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                  tb ? rocblas_operation_transpose : rocblas_operation_none,
+                                  (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
+                                  (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C,
+                                  (rocblas_int)ldc));
+}
+
+inline void sgemm_batched(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                          float alpha, const float *A, int64_t lda, int64_t sa, const float *B,
+                          int64_t ldb, int64_t sb, float beta, float *C, int64_t ldc, int64_t sc,
+                          int64_t batch) {
+    if (m == 0 || n == 0 || batch == 0) return;
+    STDA_BLAS_CHECK(rocblas_sgemm_strided_batched(
+        h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+        tb ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
+        (rocblas_int)k, &alpha, A, (rocblas_int)lda, sa, B, (rocblas_int)ldb, sb, &beta, C,
+        (rocblas_int)ldc, sc, (rocblas_int)batch));
+}
+
 // Host symmetric eigensolver (LAPACK dsyevd): eigenvalues ascending, vectors
 // in place of the column-major n x n matrix a.
 void host_syev(int n, std::vector<double> &a, std::vector<double> &w);

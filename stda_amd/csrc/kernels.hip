@@ -1496,3 +1496,118 @@ void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, do
 }
 
 }  // namespace stda
+
+namespace stda {
+
+// ----------------------------------------------------------------------------
+// FP32 variants of the AO-basis operator kernels (mixed-precision solves).
+__global__ void scatter_pairs_f32_kernel(const double *v, int64_t ldv, const int32_t *pair,
+                                         int64_t n, float *full, int64_t ldfull) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vec = blockIdx.y;
+    if (i < n) full[pair[i] + ldfull * vec] = (float)v[i + ldv * vec];
+}
+
+void launch_scatter_pairs_f32(hipStream_t s, const double *v, int64_t ldv, const int32_t *pair,
+                              int64_t n, int nvec, float *full, int64_t ldfull) {
+    if (n == 0 || nvec == 0) return;
+    hipLaunchKernelGGL(scatter_pairs_f32_kernel, dim3((unsigned)((n + 255) / 256), nvec), dim3(256),
+                       0, s, v, ldv, pair, n, full, ldfull);
+}
+
+// out = (double) full[pair] + diag o v   (diagonal term in FP64)
+__global__ void gather_pairs_f32_kernel(const float *full, int64_t ldfull, const int32_t *pair,
+                                        int64_t n, const double *diag, const double *v,
+                                        int64_t ldv, double *out, int64_t ldout) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int vec = blockIdx.y;
+    if (i >= n) return;
+    double r = (double)full[pair[i] + ldfull * vec];
+    if (diag) r += diag[i] * v[i + ldv * vec];
+    out[i + ldout * vec] = r;
+}
+
+void launch_gather_pairs_f32(hipStream_t s, const float *full, int64_t ldfull, const int32_t *pair,
+                             int64_t n, int nvec, const double *diag, const double *v, int64_t ldv,
+                             double *out, int64_t ldout) {
+    if (n == 0 || nvec == 0) return;
+    hipLaunchKernelGGL(gather_pairs_f32_kernel, dim3((unsigned)((n + 255) / 256), nvec), dim3(256), 0,
+                       s, full, ldfull, pair, n, diag, v, ldv, out, ldout);
+}
+
+__global__ void atom_diag_f32_kernel(int nao, int ncent, const float *Dt, int64_t strideD,
+                                     const int *atom_ptr, const float *gamk, float *u) {
+    extern __shared__ float stf[];
+    int v = blockIdx.x;
+    const float *D = Dt + strideD * v;
+    for (int A = threadIdx.x; A < ncent; A += blockDim.x) {
+        float s = 0.0f;
+        for (int m = atom_ptr[A]; m < atom_ptr[A + 1]; ++m) s += D[m + (int64_t)nao * m];
+        stf[A] = s;
+    }
+    __syncthreads();
+    for (int A = threadIdx.x; A < ncent; A += blockDim.x) {
+        float s = 0.0f;
+        const float *g = gamk + (int64_t)ncent * A;
+        for (int B = 0; B < ncent; ++B) s += g[B] * stf[B];
+        u[A + (int64_t)ncent * v] = s;
+    }
+}
+
+void launch_atom_diag_f32(hipStream_t s, int nvec, int nao, int ncent, const float *Dt,
+                          int64_t strideD, const int *atom_ptr, const float *gamk, float *u) {
+    if (nvec == 0) return;
+    hipLaunchKernelGGL(atom_diag_f32_kernel, dim3(nvec), dim3(256), ncent * sizeof(float), s, nao,
+                       ncent, Dt, strideD, atom_ptr, gamk, u);
+}
+
+__global__ void __launch_bounds__(256) gbuild_f32_kernel(int nao, int ncent, const float *Dt,
+                                                         int64_t strideD, const int *atom_of,
+                                                         const float *gamj, const float *gamk,
+                                                         const float *u, float aJ, float aX,
+                                                         float aK, float *Gt, int64_t strideG) {
+    __shared__ float tr[GB_T][GB_T + 1];
+    int v = blockIdx.z;
+    const float *D = Dt + strideD * v;
+    float *G = Gt + strideG * v;
+    int nu0 = blockIdx.x * GB_T, mu0 = blockIdx.y * GB_T;
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (aX != 0.0f) {
+        for (int r = ty; r < GB_T; r += 8) {
+            int mu = mu0 + tx, nu = nu0 + r;
+            tr[r][tx] = (mu < nao && nu < nao) ? D[mu + (int64_t)nao * nu] : 0.0f;
+        }
+        __syncthreads();
+    }
+    for (int r = ty; r < GB_T; r += 8) {
+        int nu = nu0 + tx, mu = mu0 + r;
+        if (nu >= nao || mu >= nao) continue;
+        int64_t g = atom_of[nu] + (int64_t)ncent * atom_of[mu];
+        float val = 0.0f;
+        if (aJ != 0.0f) val += aJ * gamj[g] * D[nu + (int64_t)nao * mu];
+        if (aX != 0.0f) val += aX * gamk[g] * tr[tx][r];
+        if (nu == mu && aK != 0.0f) val += aK * u[atom_of[mu] + (int64_t)ncent * v];
+        G[nu + (int64_t)nao * mu] = val;
+    }
+}
+
+void launch_gbuild_f32(hipStream_t s, int nvec, int nao, int ncent, const float *Dt, int64_t strideD,
+                       const int *atom_of, const float *gamj, const float *gamk, const float *u,
+                       float aJ, float aX, float aK, float *Gt, int64_t strideG) {
+    if (nvec == 0) return;
+    dim3 g((nao + GB_T - 1) / GB_T, (nao + GB_T - 1) / GB_T, nvec);
+    hipLaunchKernelGGL(gbuild_f32_kernel, g, dim3(256), 0, s, nao, ncent, Dt, strideD, atom_of, gamj,
+                       gamk, u, aJ, aX, aK, Gt, strideG);
+}
+
+__global__ void to_f32_kernel(const double *x, int64_t n, float *y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+
+void launch_to_f32(hipStream_t s, const double *x, int64_t n, float *y) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, y);
+}
+
+}  // namespace stda

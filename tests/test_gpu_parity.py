@@ -222,3 +222,34 @@ def test_no_csf_error(polyene):
         with pytest.raises(StdaHipError) as e:
             h.setup(polyene, 0.01, 0.25)
     assert e.value.status == 2
+
+
+# ---------------------------------------------------------------- mixed precision
+MIXED_E_TOL = 1e-3 / EV      # every root within 1 meV of the FP64 oracle
+MIXED_F_TOL = 1e-4           # oscillator strengths within 1e-4
+
+
+def test_mixed_precision_tda_rpa(polyene, cluster30):
+    say("mixed precision TDA / RPA / response")
+    for ci, thr, k in ((polyene, 7.0, 10), (cluster30, 10.0, 12)):
+        ref = O.run(ci, thr, 0.25, nroots=k)
+        with _hip() as h:
+            h.setup(ci, thr, 0.25)
+            h.set_precision("mixed")
+            sol = h.solve_tda(k)
+        err = np.abs(sol.omega - ref.omega)
+        say(f"  TDA nci={ref.nci}: max |dE| = {err.max() * EV:.3e} eV, "
+            f"max |df| = {np.abs(sol.props[:, 0] - ref.props[:, 0]).max():.3e}")
+        assert err.max() < MIXED_E_TOL
+        np.testing.assert_allclose(sol.props[:, :2], ref.props[:, :2], atol=MIXED_F_TOL, rtol=0)
+    ref = O.run(polyene, 7.0, 0.25, rpa=True, nroots=8)
+    with _hip() as h:
+        h.setup(polyene, 7.0, 0.25)
+        h.set_precision("mixed")
+        sol = h.solve_rpa(8)
+        res = h.lresp([0.0, 0.05])
+    assert np.abs(sol.omega - ref.omega).max() < MIXED_E_TOL
+    np.testing.assert_allclose(sol.props[:, :2], ref.props[:, :2], atol=MIXED_F_TOL, rtol=0)
+    refa = O.run(polyene, 7.0, 0.25, rpa=True, resp_omegas=[0.0, 0.05])
+    for (w, a), (wr, ar) in zip(res, refa.alpha_resp):
+        np.testing.assert_allclose(a, ar, rtol=0, atol=1e-4 * np.abs(ar).max())
