@@ -1,18 +1,26 @@
 """Benchmark of the MI355X excited-state core.
 
-One step = one full excited-state solve of the workload on the GPU: the
-device pipeline from the FP64 Loewdin orbitals (gamma, transition charges,
-screening, PT2 CSF selection) through the FP64 block Davidson to the
-transition properties with the velocity correction.  The system arrays are
-copied to HBM before the timed region (``StdaHip.load``); the host-side
-generation of the synthetic molecule and its AO integrals is outside it.
+Default workload ``mixed512``: 512 independent synthetic molecules, atom
+counts log-uniform in [10, 200] (C:H:N:O = 45:45:5:5 clusters, minimal s/p
+basis, half the electrons occupied, seeds 1000 + i), Ethr = 7 eV, a_x = 0.25,
+the 20 lowest sTDA roots of each with transition properties and the
+velocity correction.  One step = one batched mixed-precision solve of all
+512 molecules (``StdaHipBatch``: setup of every molecule on concurrent device
+contexts, one launch of the batched Davidson kernel for the molecules with at
+most 4096 CSFs, the single-molecule mixed path for the larger ones).  The
+plain FP64 GPU path (one molecule per solve, sequentially, as the drop-in
+replaces the reference routine) is timed once over the same 512 molecules;
+every batched root is compared with it (<= 1 meV, |df| <= 1e-4).  The
+molecules' boundary arrays are generated on the host before timing; the
+timed region starts from them (host arrays, copied by each solve).
 
-Default workload: a 400-atom organic-like cluster (C:H:N:O = 45:45:5:5,
-random positions >= 1.0 Angstrom apart, seed 1) in a 3-21G-type
-split-valence s/p basis with 800 occupied orbitals, Ethr = 10 eV,
-a_x = 0.25, the 50 lowest sTDA roots, Davidson residual 1e-8.
+Other workloads: ``cluster400`` (400-atom split-valence cluster, 800
+occupied, Ethr = 10 eV, 50 roots, plain FP64 single-molecule drop-in, step =
+one solve with the inputs already in HBM), ``cluster100``, ``polyene``
+(C14H16, the CPU-runnable case) and ``serve`` (open-loop Poisson stream of
+70% spectrum / 30% polarizability jobs, sizes log-uniform in [10, 400]).
 
-Prints one JSON line (see README of the metric in the output keys).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -29,9 +37,11 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+EV = 27.211385050
+METRIC = "excited-state solves/s (batched mixed vs plain FP64) at <=1 meV; p50/p99 latency"
 
 
-def workload(name: str):
+def single_workload(name: str):
     from stda_amd import synth
     from stda_amd.prep import prepare
     if name == "cluster400":
@@ -46,7 +56,7 @@ def workload(name: str):
     else:
         raise SystemExit(f"unknown workload {name}")
     ci = prepare(mol, p["thr"], p["ax"], ints)
-    return mol, ci, p
+    return ci, p
 
 
 def copy_probe(nbytes: int):
@@ -62,31 +72,20 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_probe_bytes": n * 8}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cluster400")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="only copy this many bytes device-to-device (calibrates HBM counters)")
-    args = ap.parse_args()
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
-              "decomposition is a design, not a measured path in this build.", file=sys.stderr)
-        sys.exit(2)
+def gen_workers():
+    return max(1, min(16, os.cpu_count() or 1))
 
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------------------- single molecule
+def run_single(args):
     t0 = time.perf_counter()
-    mol, ci, p = workload(args.workload)
-    print(f"# workload {args.workload}: natoms={ci.ncent} nao={ci.nao} no={ci.no} nv={ci.nv} "
-          f"(inputs {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
-
+    ci, p = single_workload(args.workload)
+    log(f"# workload {args.workload}: natoms={ci.ncent} nao={ci.nao} no={ci.no} nv={ci.nv} "
+        f"(inputs {time.perf_counter() - t0:.1f}s)")
     from stda_amd.core import StdaHip
     h = StdaHip(0)
     h.load(ci, p["thr"], p["ax"])
@@ -94,9 +93,8 @@ def main():
     for w in range(args.warmup):
         h.select()
         h.solve_tda(p["nroots"], velcorr=True, vectors=False)
-        print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    lat = []
-    stats = []
+        log(f"# warmup {w + 1}/{args.warmup} done")
+    lat, stats = [], []
     tstart = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
@@ -105,24 +103,16 @@ def main():
         lat.append(time.perf_counter() - ts)
         stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_cols", "ao_matvec",
                                                        "ao_matvec_f32")})
-        host = {k: h.kernel_stats(k) for k in ("dav_matvec", "dav_iter_other", "dav_rr_eig",
-                                                "dav_ortho", "mixed_fp32_phase",
-                                                "mixed_fp64_refine")}
-        print(f"# host spans: {host}", file=sys.stderr, flush=True)
-        print(f"# step {s + 1}/{args.steps}: {lat[-1]:.3f}s nci={nci} "
-              f"timings={sol.timings_ms}", file=sys.stderr, flush=True)
+        log(f"# step {s + 1}/{args.steps}: {lat[-1]:.3f}s nci={nci} timings={sol.timings_ms}")
     total = time.perf_counter() - tstart
     root_err = None
     if args.precision != "fp64":
-        # per-root error of the mixed-precision roots against the plain FP64 GPU
-        # path (itself checked against the FP64 oracle by the parity tests)
         h.set_precision("fp64")
         ref = h.solve_tda(p["nroots"], velcorr=True, vectors=False)
-        root_err = {"max_abs_dE_eV": float(np.abs(sol.omega - ref.omega).max() * 27.211385050),
+        root_err = {"max_abs_dE_eV": float(np.abs(sol.omega - ref.omega).max() * EV),
                     "max_abs_dfL": float(np.abs(sol.props[:, 0] - ref.props[:, 0]).max())}
     h.close()
 
-    # dominant kernel of the last step
     last = stats[-1]
     dom = max(last, key=lambda k: last[k]["ms"])
     st = last[dom]
@@ -132,15 +122,11 @@ def main():
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
                 "avg_launch_ms": round(st["ms"] / max(st["launches"], 1), 3),
                 "work_per_launch": st["work"] / max(st["launches"], 1)}
-
-    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
-    # counters for this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
     prof = os.path.join(ROOT, "profiles", "pt2_pipe_cluster400_pmc.md")
     if dom == "pt2_rows" and args.workload == "cluster400" and os.path.exists(prof):
         for ln in open(prof):
             if ln.startswith("traffic_bytes_per_launch:"):
                 roofline["traffic"] = float(ln.split(":")[1])
-
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -149,26 +135,18 @@ def main():
                                                          budget_s=args.cpu_budget)
         cpu = {"value": 1.0 / secs, "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": desc}
-
     lat_ms = np.array(lat) * 1e3
-    line = {
-        "metric": "excited-state solves/s (plain FP64 sTDA, 1 molecule per solve); p50/p99 latency",
+    return {
+        "metric": METRIC,
         "value": args.steps / total,
         "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": total / args.steps * 1e3,
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 contractions + f64 Davidson",
-        "precision": args.precision,
-        "root_error_vs_fp64": root_err,
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: plain FP64 sTDA drop-in, "
+        "precision": args.precision, "root_error_vs_fp64": root_err, "data": "synthetic",
+        "config": {"workload": f"{args.workload}: single-molecule sTDA drop-in, "
                                f"{p['nroots']} lowest roots",
                    "natoms": ci.ncent, "nao": ci.nao, "no": ci.no, "nv": ci.nv,
                    "nci": int(nci), "nprim": int(nprim), "nsec": int(nsec),
@@ -180,6 +158,206 @@ def main():
                     for k, v in last.items() if v["launches"]},
         "cpu_baseline": cpu,
     }
+
+
+# --------------------------------------------------------------------------- batched
+def plain_fp64(cis, thr, ax, nroots):
+    """Sequential plain FP64 solves, one molecule per solve (the drop-in path)."""
+    from stda_amd.core import StdaHip, StdaHipError
+    out, lat = [], []
+    with StdaHip(0) as h:
+        h.set_precision("fp64")
+        t0 = time.perf_counter()
+        for ci in cis:
+            ts = time.perf_counter()
+            try:
+                h.setup(ci, thr, ax)
+                s = h.solve_tda(nroots, velcorr=True, tol=1e-8, vectors=False)
+                out.append((s.omega.copy(), s.props.copy()))
+            except StdaHipError as e:
+                out.append(e.status)
+            lat.append(time.perf_counter() - ts)
+        wall = time.perf_counter() - t0
+    return out, wall, np.array(lat)
+
+
+def compare(res, ref):
+    dE = dF = 0.0
+    mism = 0
+    for (om, pr, st), r in zip(res, ref):
+        if isinstance(r, int):
+            mism += st != r
+            continue
+        if st != 0 or len(om) != len(r[0]):
+            mism += 1
+            continue
+        dE = max(dE, float(np.abs(om - r[0]).max()) * EV)
+        dF = max(dF, float(np.abs(pr[:, :2] - r[1][:, :2]).max()))
+    return {"max_abs_dE_eV": dE, "max_abs_df": dF, "mismatched_molecules": mism}
+
+
+def cpu_baseline_batch(cis, thr, ax, nroots, budget_s):
+    """Oracle restatement on a seeded sample of 8 molecules (with CSFs),
+    extrapolated to the mean seconds per solve of the whole set."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(cis), size=min(8, len(cis)), replace=False)
+    secs, descs, threads = [], [], 1
+    for i in idx:
+        try:
+            s, d, threads = oracle.cpu_baseline_sample(cis[i], thr, ax, nroots,
+                                                       budget_s=budget_s / len(idx))
+        except Exception:
+            continue          # no CSF: nothing to time
+        secs.append(s)
+        descs.append(f"mol {int(i)} (natoms {cis[i].ncent}): {s:.3g} s")
+    mean = float(np.mean(secs)) if secs else float("nan")
+    return {"value": 1.0 / mean if secs else None, "unit": "solves/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle restatement (phases timed in full or on row samples and "
+                       f"extrapolated, as for one molecule) on {len(secs)} seeded molecules of "
+                       f"the set; mean seconds per solve {mean:.3g}; " + "; ".join(descs))}
+
+
+def run_mixed(args):
+    from stda_amd import synth
+    from stda_amd.core import StdaHipBatch
+    n, thr, ax, nroots = args.nmol, 7.0, 0.25, 20
+    t0 = time.perf_counter()
+    cis = synth.mixed_batch(n, lo=10, hi=200, base_seed=1000, basis="minimal", thr=thr, ax=ax,
+                            workers=gen_workers())
+    log(f"# workload mixed{n}: {n} molecules, natoms {min(c.ncent for c in cis)}-"
+        f"{max(c.ncent for c in cis)} (inputs {time.perf_counter() - t0:.1f}s)")
+    ref, plain_wall, plain_lat = plain_fp64(cis, thr, ax, nroots)
+    nok = sum(1 for r in ref if not isinstance(r, int))
+    log(f"# plain FP64 sequential: {plain_wall:.3f}s, {nok}/{n} molecules with CSFs")
+    with StdaHipBatch(0, workers=args.workers) as b:
+        for w in range(args.warmup):
+            b.solve_tda(cis, thr, ax, nroots)
+            log(f"# warmup {w + 1}/{args.warmup} done")
+        walls, phases = [], []
+        tstart = time.perf_counter()
+        for s in range(args.steps):
+            ts = time.perf_counter()
+            res = b.solve_tda(cis, thr, ax, nroots)
+            walls.append(time.perf_counter() - ts)
+            phases.append(b.timings())
+            log(f"# step {s + 1}/{args.steps}: {walls[-1]:.3f}s {phases[-1]}")
+        total = time.perf_counter() - tstart
+        kbytes = b.kernel_bytes()
+    err = compare(res, ref)
+    t, cnt = phases[-1]
+    kms = t["batch_kernel"]
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"kernel": "davidson_batch", "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_ms": round(kms, 3), "work_per_launch": kbytes,
+                "note": "one launch per step; bytes = FP32 A per iteration + FP64 A and 0.5B once"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batch(cis, thr, ax, nroots, args.cpu_budget)
+    value = nok * args.steps / total
+    plain = nok / plain_wall
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "p50_ms": float(np.percentile(np.array(walls) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(walls) * 1e3, 99)),
+        "latency_note": "closed batch: every request of a step completes with the step",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 operator products + f64 Davidson numerics and refinement",
+        "plain_fp64": {"value": plain, "unit": "solves/s", "wall_s": plain_wall,
+                       "p50_ms": float(np.percentile(plain_lat * 1e3, 50)),
+                       "p99_ms": float(np.percentile(plain_lat * 1e3, 99))},
+        "speedup_vs_plain_fp64": value / plain,
+        "root_error_vs_fp64": err,
+        "data": "synthetic",
+        "config": {"workload": f"mixed{n}: {n} molecules, natoms log-uniform [10,200], minimal "
+                               f"basis, Ethr 7 eV, ax 0.25, 20 roots each, batched mixed precision",
+                   "molecules": n, "molecules_with_csf": nok, "workers": args.workers,
+                   "paths": cnt, "phases_ms": t},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+# --------------------------------------------------------------------------- serving
+def run_serve(args):
+    from stda_amd import synth
+    from stda_amd.serve import Server, Job, poisson_arrivals
+    n = args.nmol
+    t0 = time.perf_counter()
+    cis = synth.mixed_batch(n, lo=10, hi=400, base_seed=2000, basis="minimal", thr=7.0, ax=0.25,
+                            workers=gen_workers())
+    log(f"# serve workload: {n} molecules (inputs {time.perf_counter() - t0:.1f}s)")
+    rng = np.random.default_rng(2000)
+    kinds = np.where(rng.uniform(size=n) < 0.7, "spectrum", "polar")
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = []
+    with Server(workers=4, batch_workers=args.workers) as srv:
+        warm = [Job(jid=i, kind=str(kinds[i]), ci=cis[i], omegas=(0.0, 0.05, 0.1, 0.15))
+                for i in range(min(16, n))]
+        srv.run(warm, realtime=False)
+        for rate in rates:
+            arr = poisson_arrivals(n, rate, seed=int(rate))
+            jobs = [Job(jid=i, kind=str(kinds[i]), ci=cis[i], arrival=float(arr[i]),
+                        omegas=(0.0, 0.05, 0.1, 0.15)) for i in range(n)]
+            st = srv.run(jobs, realtime=True)
+            sweep.append({"rate": rate, "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,
+                          "p99_ms": st.p99_ms, "ok": st.ok, "n": st.n, "batches": st.batches,
+                          "mean_batch": st.mean_batch})
+            log(f"# rate {rate}: {sweep[-1]}")
+    best = max(sweep, key=lambda r: r["solves_per_s"])
+    return {
+        "metric": METRIC, "value": best["solves_per_s"], "unit": "solves/s",
+        "n_gpus": 1, "steps": len(rates), "warmup": 1,
+        "ms_per_step": None, "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 operator products + f64 Davidson numerics and refinement",
+        "data": "synthetic",
+        "config": {"workload": f"serve: open-loop Poisson, {n} requests per rate, 70% spectrum "
+                               f"(20 roots) / 30% polarizability (4 frequencies), natoms "
+                               f"log-uniform [10,400]", "sweep": sweep},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed512",
+                    help="mixed512 (default), cluster400, cluster100, polyene, serve")
+    ap.add_argument("--nmol", type=int, default=512)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--rates", default="50,200,800,2000")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="single-molecule workloads only")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (calibrates HBM counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is a design, not a measured path in this build.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload.startswith("mixed"):
+        if args.workload != "mixed512":
+            args.nmol = int(args.workload[5:])
+        line = run_mixed(args)
+    elif args.workload == "serve":
+        line = run_serve(args)
+    else:
+        line = run_single(args)
     print(json.dumps(line), flush=True)
 
 

@@ -173,6 +173,55 @@ int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
 int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
                           double *launches, double *work);
 
+/* ------------------------------------------------------------------------
+ * Batched solves of many independent molecules.  The reference solves one
+ * molecule per process (the `stda` routine, stda.f:53-1471); a driver that
+ * loops over many molecules replaces that loop with one call here.
+ *
+ * A batch object owns `nworkers` contexts, each with its own host thread and
+ * HIP stream (give the process at least as many hardware queues, e.g.
+ * GPU_MAX_HW_QUEUES=16, so that the streams overlap).  stda_hip_batch_solve_tda
+ * runs the setup pipeline of every molecule (as stda_hip_setup) on the
+ * workers concurrently; molecules with at most nmax_batched CSFs (default
+ * 2048) and nroots <= 24 are solved in groups of `group` (default 16) per
+ * launch of a batched Davidson kernel (one work-group per molecule) in mixed
+ * precision (as stda_hip_set_precision(ctx, 1): FP32 operator products, FP64
+ * subspace numerics, residual norms 1e-6, one FP64 Rayleigh-Ritz
+ * refinement), launched as soon as a group is ready; larger molecules take
+ * the single-molecule mixed path on their worker.
+ * Results per molecule i (arrays of nmol rows, row-major by molecule):
+ *   nfound[i]                       roots found (min(nroots, nci))
+ *   omega[i*nroots + j]             excitation energies (Hartree)
+ *   props[(i*nroots + j)*5 + 0..4]  fL, fV, RL, RV or Rv(corr), 0
+ *   status[i]                       STDA_HIP_OK or the molecule's error code
+ * The call itself fails only for invalid arguments or device errors. */
+typedef struct stda_hip_batch stda_hip_batch;
+
+int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers);
+int stda_hip_batch_destroy(stda_hip_batch *b);
+const char *stda_hip_batch_last_error(const stda_hip_batch *b);
+int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_system *sys,
+                             const stda_hip_params *par, int32_t nroots, double tol,
+                             int32_t velcorr, int32_t *nfound, double *omega, double *props,
+                             int32_t *status);
+/* Limits of the batched kernel: molecules with at most nmax_batched CSFs go
+ * to it, `group` molecules per launch. */
+int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t group);
+/* ms5: host time of [0] the worker phase (setup, single-path solves, batched
+ * launches), [1] 0, [2] property tables, [3] total, and [4] the summed
+ * HIP-event times of the batched launches; counts3: molecules solved by the
+ * batched kernel, by the single path, failed. */
+int stda_hip_batch_timings(const stda_hip_batch *b, double *ms5, int64_t *counts3);
+/* Per-molecule phase times of the batched kernel summed over the molecules
+ * of the last call (microseconds): operator products, Gram products,
+ * subspace eigensolve, Ritz vectors, expansion, refinement + moments; [6]
+ * Jacobi sweeps. */
+int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us7);
+/* Algorithmic HBM bytes of the last batched Davidson launch: per molecule,
+ * the FP32 A (4 n^2) once per iteration plus the FP64 A and 0.5 B (8 n^2
+ * each) once. */
+int stda_hip_batch_kernel_bytes(const stda_hip_batch *b, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

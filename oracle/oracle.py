@@ -444,8 +444,8 @@ def cpu_baseline_sample(ci, thr_ev, ax, nroots, budget_s=20.0, seed=0):
     dt = time.perf_counter() - t0
     elems = rows_n * (nex - rows_n / 2.0)
     t_a = dt / elems * (nci * (nci - 1) / 2.0)
-    # dense eigensolve cost
-    m = 1500
+    # dense eigensolve cost (timed at nci itself up to 1500, scaled beyond)
+    m = int(min(1500, max(nci, 2)))
     M = rng.standard_normal((m, m))
     M = M + M.T
     t0 = time.perf_counter()

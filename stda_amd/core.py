@@ -31,7 +31,10 @@ C_API = [
     "stda_hip_setup", "stda_hip_get_csf", "stda_hip_solve_tda", "stda_hip_solve_rpa",
     "stda_hip_lresp", "stda_hip_build_a", "stda_hip_build_apb_amb", "stda_hip_build_bhalf",
     "stda_hip_matvec", "stda_hip_timings", "stda_hip_load", "stda_hip_select",
-    "stda_hip_kernel_stats", "stda_hip_set_precision",
+    "stda_hip_kernel_stats", "stda_hip_set_precision", "stda_hip_batch_create",
+    "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
+    "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
+    "stda_hip_batch_phase_times",
 ]
 
 
@@ -92,6 +95,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_select.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
     L.stda_hip_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
                                         ctypes.POINTER(F64), ctypes.POINTER(F64)]
+    L.stda_hip_batch_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int]
+    L.stda_hip_batch_destroy.argtypes = [P]
+    L.stda_hip_batch_last_error.restype = ctypes.c_char_p
+    L.stda_hip_batch_last_error.argtypes = [P]
+    L.stda_hip_batch_solve_tda.argtypes = [P, I32, ctypes.POINTER(_System),
+                                           ctypes.POINTER(_Params), I32, F64, I32, P, P, P, P]
+    L.stda_hip_batch_timings.argtypes = [P, P, P]
+    L.stda_hip_batch_kernel_bytes.argtypes = [P, ctypes.POINTER(F64)]
+    L.stda_hip_batch_configure.argtypes = [P, I64, I32]
+    L.stda_hip_batch_phase_times.argtypes = [P, P]
     _lib = L
     return L
 
@@ -312,3 +325,112 @@ def stda(ci: CoreInputs, thr: float, thrp: float, ax: float, alphak: float = -10
         if rpa:
             return h.solve_rpa(nroots, tol=tol), csf
         return h.solve_tda(nroots, velcorr=velcorr, tol=tol), csf
+
+
+class StdaHipBatch:
+    """Batched solves of many independent molecules (``stda_hip_batch_*``).
+
+    ``workers`` device contexts, each on its own host thread and stream,
+    prepare the molecules concurrently (selection, explicit FP64/FP32 A,
+    0.5 B, CSF moments); every group of molecules with at most
+    ``nmax_batched`` CSFs and at most 24 roots is solved by one launch of the
+    batched Davidson kernel (one work-group per molecule; FP32 operator
+    products, FP64 subspace numerics, one FP64 Rayleigh-Ritz refinement) while
+    the workers go on.  Larger molecules take the single-molecule mixed path
+    on their worker.  See :meth:`configure`.
+    """
+
+    def __init__(self, device: int = 0, workers: int = 8):
+        self._L = load_library()
+        self._b = ctypes.c_void_p()
+        st = self._L.stda_hip_batch_create(ctypes.byref(self._b), device, workers)
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+
+    def close(self):
+        if self._b:
+            self._L.stda_hip_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solve_tda(self, cis, thr: float, ax: float, nroots: int, tol: float = 1e-8,
+                  velcorr: bool = True, thrp: float = 1e-4, fthr: float | None = None,
+                  triplet: bool = False):
+        """Lowest ``nroots`` sTDA roots of every molecule in ``cis``.
+
+        Returns a list with, per molecule, ``(omega, props, status)``: omega in
+        Hartree (nfound,), props (nfound, 4) = fL, fV, RL, Rv(corr), status 0
+        or an ``STDA_HIP_ERR_*`` code.
+        """
+        nmol = len(cis)
+        keep = []
+        sysarr = (_System * max(nmol, 1))()
+        pararr = (_Params * max(nmol, 1))()
+        for i, ci in enumerate(cis):
+            clow = np.asfortranarray(ci.clow, dtype=np.float64)
+            eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
+            iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+            xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
+            mom = (np.ascontiguousarray(ci.moments, dtype=np.float64)
+                   if ci.moments is not None else None)
+            keep.append((clow, eps, iaoat, xyz, mom))
+            sysarr[i] = _System(ci.ncent, ci.nao, ci.no, ci.nv, _ptr(clow), _ptr(eps),
+                                _ptr(iaoat), _ptr(xyz), _ptr(mom))
+            pararr[i] = _Params(thr, thrp, -1.0 if fthr is None else fthr, ax, -100.0, -100.0,
+                                1 if triplet else 0, 0, 0.5, 0.1, 4.0)
+        nf = np.zeros(max(nmol, 1), dtype=np.int32)
+        om = np.zeros((max(nmol, 1), nroots))
+        pr = np.zeros((max(nmol, 1), nroots, 5))
+        stt = np.zeros(max(nmol, 1), dtype=np.int32)
+        st = self._L.stda_hip_batch_solve_tda(self._b, nmol, sysarr, pararr, nroots, tol,
+                                              1 if velcorr else 0, _ptr(nf), _ptr(om), _ptr(pr),
+                                              _ptr(stt))
+        if st != 0:
+            raise StdaHipError(st, f"{self._L.stda_hip_error_string(st).decode()}: "
+                                   f"{self._L.stda_hip_batch_last_error(self._b).decode()}")
+        del keep
+        return [(om[i, :nf[i]].copy(), pr[i, :nf[i], :4].copy(), int(stt[i]))
+                for i in range(nmol)]
+
+    def last_error(self) -> str:
+        return self._L.stda_hip_batch_last_error(self._b).decode()
+
+    def configure(self, nmax_batched: int = 2048, group: int = 16):
+        """CSF limit of the batched kernel and molecules per batched launch."""
+        st = self._L.stda_hip_batch_configure(self._b, nmax_batched, group)
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+
+    def phase_times(self):
+        """Batched-kernel phase times summed over the molecules (us) and the
+        total Jacobi sweeps of the last call."""
+        t = np.zeros(7)
+        self._L.stda_hip_batch_phase_times(self._b, _ptr(t))
+        names = ["matvec", "gram", "jacobi", "ritz", "expand", "final", "sweeps"]
+        return dict(zip(names, t.tolist()))
+
+    def kernel_bytes(self) -> float:
+        """Algorithmic bytes of the last batched Davidson launch."""
+        v = ctypes.c_double()
+        self._L.stda_hip_batch_kernel_bytes(self._b, ctypes.byref(v))
+        return v.value
+
+    def timings(self):
+        """Host times (ms) of the last call's phases, the batched kernel's
+        HIP-event time, and how many molecules took which path."""
+        t = np.zeros(5)
+        cnt = np.zeros(3, dtype=np.int64)
+        self._L.stda_hip_batch_timings(self._b, _ptr(t), _ptr(cnt))
+        return ({"workers": t[0], "properties": t[2], "total": t[3], "batch_kernel": t[4]},
+                {"batched": int(cnt[0]), "single": int(cnt[1]), "failed": int(cnt[2])})

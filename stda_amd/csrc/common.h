@@ -128,7 +128,7 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
                             const double *pK, const double *q, const double *pJ, const double *qab,
                             const double *de, const unsigned char *rowmask, unsigned char rowbit,
                             const unsigned char *colmask, unsigned char colbit, double ak,
-                            double *colsum);
+                            int nsplit, double *part, double *colsum);
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);
@@ -144,6 +144,37 @@ void launch_gbuild_f32(hipStream_t s, int nvec, int nao, int ncent, const float 
                        const int *atom_of, const float *gamj, const float *gamk, const float *u,
                        float aJ, float aX, float aK, float *Gt, int64_t strideG);
 void launch_to_f32(hipStream_t s, const double *x, int64_t n, float *y);
+// dst(:, I) = src(:, idx[I]) for I < n, columns of `rows` doubles
+void launch_gather_cols(hipStream_t s, int rows, int64_t n, const int32_t *idx, const double *src,
+                        double *dst);
+// A(I, I) = d[I] for the n x n column-major A
+void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A);
+
+// --- batched Davidson (batch.hip): one workgroup per molecule -------------------------
+constexpr int BATCH_MD = 64;                        // subspace limit
+constexpr int BATCH_KMAX = 24;                      // roots per molecule
+constexpr int BATCH_OUT = 20 * BATCH_KMAX + 12;     // doubles of output per molecule
+// Output of one molecule: omega[KMAX] | T[9 KMAX] = mom^T X | T2[9 KMAX] = mom^T Xnew
+// | FP64 residual norms[KMAX] | info[12]: iterations, converged, roots found, last
+// FP32 residual, wall-clock ticks of the phases (matvec, Gram, subspace eigensolve,
+// Ritz vectors, expansion, refinement + moments), Jacobi sweeps, unused.
+struct BatchMol {
+    int64_t n;                 // CSFs
+    int k, nguess, maxdim, maxiter;
+    int velcorr, pad;
+    double tol;                // residual tolerance of the FP32-operator iterations
+    const float *A32;          // n x n FP32 copy of A (iterations)
+    const double *A64;         // n x n FP64 A (final Rayleigh-Ritz)
+    const double *B64;         // n x n 0.5 B (velocity correction) or null
+    const double *diag;        // n, diagonal of A (preconditioner)
+    const double *mom;         // n x 9 CSF moments or null
+    const int32_t *guess;      // nguess start rows
+    double *V, *AV;            // n x maxdim
+    double *X, *AX, *W;        // n x BATCH_KMAX
+    double *H;                 // BATCH_MD x BATCH_MD
+    double *out;               // BATCH_OUT
+};
+void launch_davidson_batch(hipStream_t s, int nmol, const BatchMol *mols);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

@@ -4,12 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/stda_hip.h"
@@ -48,11 +50,8 @@ constexpr double kRot = 235.7220;             // stda.f:1300
 // Wilkinson & Reinsch, Handbook for Automatic Computation II) is used.
 using dsyevd_t = void (*)(const char *, const char *, const int *, double *, const int *, double *,
                           double *, const int *, int *, const int *, int *);
-static dsyevd_t load_dsyevd() {
-    static bool tried = false;
-    static dsyevd_t fn = nullptr;
-    if (tried) return fn;
-    tried = true;
+static dsyevd_t find_dsyevd() {
+    dsyevd_t fn = nullptr;
     const char *env = getenv("STDA_HIP_LAPACK");
     const char *cands[] = {env ? env : "", "/opt/conda/lib/libmkl_rt.so.1",
                            "/opt/conda/lib/libmkl_rt.so"};
@@ -65,6 +64,12 @@ static dsyevd_t load_dsyevd() {
         fn = (dsyevd_t)dlsym(h, "dsyevd_");
         if (fn) break;
     }
+    return fn;
+}
+
+// Looked up once per process; contexts driven from several host threads share it.
+static dsyevd_t load_dsyevd() {
+    static const dsyevd_t fn = find_dsyevd();
     return fn;
 }
 
@@ -205,6 +210,20 @@ struct Timer {
 using namespace stda;
 
 struct stda_hip_ctx {
+    // Owns the stream and the BLAS handle; declared first so that it is
+    // destroyed last, after every buffer below has been freed in the order
+    // of this stream.
+    struct Owner {
+        hipStream_t stream = nullptr;
+        rocblas_handle blas = nullptr;
+        ~Owner() {
+            if (blas) rocblas_destroy_handle(blas);
+            if (stream) {
+                (void)hipStreamSynchronize(stream);
+                (void)hipStreamDestroy(stream);
+            }
+        }
+    } owner;
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
@@ -249,6 +268,7 @@ struct stda_hip_ctx {
     DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
     DBuf<double> normbuf; // slice partials of column norms
     int nb_max = 0;
+    int nb_chunk = 0;     // vectors per operator block (from the free memory at first use)
     bool xfull_clean = false;
 
     bool ready = false;
@@ -366,13 +386,15 @@ void ensure_work(stda_hip_ctx *c, int nb) {
 }
 
 int chunk_vectors(stda_hip_ctx *c) {
+    if (c->nb_chunk > 0) return c->nb_chunk;
     size_t per = sizeof(double) * ((size_t)2 * c->npair + (size_t)2 * c->nao * c->no +
                                    (size_t)2 * c->nao * c->nao + c->ncent);
     size_t freeb = 0, total = 0;
     (void)hipMemGetInfo(&freeb, &total);
     size_t budget = std::min<size_t>(freeb / 2, (size_t)24 << 30);
     int nb = (int)std::max<size_t>(1, budget / per);
-    return std::min(nb, 64);
+    c->nb_chunk = std::min(nb, 64);
+    return c->nb_chunk;
 }
 
 void ensure_fp32(stda_hip_ctx *c) {
@@ -945,6 +967,7 @@ static void do_load(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_
     c->ready = false;
     c->loaded = false;
     c->nb_max = 0;
+    c->nb_chunk = 0;
     c->ncent = sys->ncent; c->nao = sys->nao; c->no = sys->no; c->nv = sys->nv;
     const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
     c->npair = (int64_t)no * nv;
@@ -1144,11 +1167,15 @@ static void do_select(stda_hip_ctx *c) {
             rkd.upload(rk.data(), rk.size(), s); rcd.upload(rc.data(), rc.size(), s);
             cjd.upload(cjv.data(), cjv.size(), s); cbd.upload(cbv.data(), cbv.size(), s);
             DBuf<double> colsum(c->npair);
+            // split the row stream so that about 1024 work-groups are in flight
+            const int nrt2 = (int)rk.size();
+            const int nsplit = std::max(1, std::min(nrt2, (1024 + nct2 - 1) / std::max(nct2, 1)));
+            DBuf<double> part((size_t)nsplit * nct2 * 256);
             EventTimer ev2(c);
-            launch_pt2_cols_sparse(s, (int)rk.size(), rkd.get(), rcd.get(), nct2, cjd.get(),
-                                   cbd.get(), na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                                   c->qab.get(), c->de.get(), c->flags.get(), 4, c->flags.get(), 1,
-                                   c->ak, colsum.get());
+            launch_pt2_cols_sparse(s, nrt2, rkd.get(), rcd.get(), nct2, cjd.get(), cbd.get(), na,
+                                   nv, c->pia.get(), c->qia.get(), c->pij.get(), c->qab.get(),
+                                   c->de.get(), c->flags.get(), 4, c->flags.get(), 1, c->ak, nsplit,
+                                   part.get(), colsum.get());
             ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs(c->npair);
             colsum.download(cs.data(), cs.size(), s);
@@ -1217,8 +1244,7 @@ static void build_a_device(stda_hip_ctx *c, DBuf<double> &A) {
     launch_build_a(s, TileSet{od.get(), vd.get(), nt}, c->ncent, c->no, c->nv, c->pia.get(),
                    c->qia.get(), c->pij.get(), c->qab.get(), csfd.get(), c->ak, A.get(), n);
     // diagonal = ed (rtdamat: hci(i,i) = ed(i))
-    for (int64_t I = 0; I < n; ++I)
-        STDA_HIP_CHECK(hipMemcpyAsync(A.get() + I * (n + 1), &c->ed_h[I], 8, hipMemcpyHostToDevice, s));
+    launch_set_diag(s, n, c->ed.get(), A.get());
     c->sync();
 }
 
@@ -1228,13 +1254,8 @@ static void build_k_and_exchange(stda_hip_ctx *c, DBuf<double> &K, DBuf<double> 
     const int na = c->ncent;
     hipStream_t s = c->stream;
     DBuf<double> pq((size_t)na * n), qq((size_t)na * n);
-    for (int64_t I = 0; I < n; ++I) {
-        int64_t p = c->pair_h[I];
-        STDA_HIP_CHECK(hipMemcpyAsync(pq.get() + (size_t)na * I, c->pia.get() + (size_t)na * p, na * 8,
-                                      hipMemcpyDeviceToDevice, s));
-        STDA_HIP_CHECK(hipMemcpyAsync(qq.get() + (size_t)na * I, c->qia.get() + (size_t)na * p, na * 8,
-                                      hipMemcpyDeviceToDevice, s));
-    }
+    launch_gather_cols(s, na, n, c->pair.get(), c->pia.get(), pq.get());
+    launch_gather_cols(s, na, n, c->pair.get(), c->qia.get(), qq.get());
     K.resize((size_t)n * n);
     dgemm(c->blas, true, false, n, n, na, 1.0, pq.get(), na, qq.get(), na, 0.0, K.get(), n);
     E.resize((size_t)n * n);
@@ -1250,6 +1271,44 @@ static std::vector<double> download_matrix(stda_hip_ctx *c, DBuf<double> &A, int
 }
 
 // TDA / RPA property tables (5 per root) and alpha_SOS
+// Property table (fL, fV, RL, RV or Rv(corr), 0 per root) and alpha_SOS from
+// the moment products T = mom^T X (9 per root: xl yl zl xm ym zm xv yv zv)
+// and, with the velocity correction, T2 = mom^T Xnew.
+static void tda_props_from_moments(double ak, const std::vector<double> &w, const double *T,
+                                   const double *T2, double *props, double *alpha) {
+    const int k = (int)w.size();
+    double sak = std::sqrt(ak);
+    double a6[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < k; ++j) {
+        const double *t = &T[(size_t)9 * j];
+        double de = w[j], ef = 1.0 / (de + 1.0e-8);
+        double l[3] = {t[0] * sak, t[1] * sak, t[2] * sak};
+        double m[3] = {t[3] * sak, t[4] * sak, t[5] * sak};
+        double v[3] = {t[6] * sak, t[7] * sak, t[8] * sak};
+        double fl = de * (2.0 / 3.0) * (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+        double fv = ef * (2.0 / 3.0) * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double rl = -kRot * (l[0] * m[0] + l[1] * m[1] + l[2] * m[2]);
+        double rv = -kRot * (v[0] * m[0] + v[1] * m[1] + v[2] * m[2]) * ef;
+        if (T2) {   // apbtrafo.f:180-262
+            const double *u = &T2[(size_t)9 * j];
+            double hilf = 1.0 - std::exp(-150.0 * de * de);
+            double efmod = 1.0 / (std::pow(de, hilf) + 1.0e-8);
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q)
+                s += t[6 + q] * t[3 + q] + u[6 + q] * t[3 + q] + t[6 + q] * u[3 + q];
+            rv = -kRot * s * ak * efmod;
+        }
+        if (props) {
+            props[5 * j + 0] = fl; props[5 * j + 1] = fv; props[5 * j + 2] = rl;
+            props[5 * j + 3] = rv; props[5 * j + 4] = 0.0;
+        }
+        a6[0] += l[0] * l[0] * ef; a6[1] += l[0] * l[1] * ef; a6[2] += l[1] * l[1] * ef;
+        a6[3] += l[0] * l[2] * ef; a6[4] += l[1] * l[2] * ef; a6[5] += l[2] * l[2] * ef;
+    }
+    if (alpha)
+        for (int q = 0; q < 6; ++q) alpha[q] = 2.0 * a6[q];
+}
+
 static void tda_properties(stda_hip_ctx *c, const double *X, const std::vector<double> &w,
                            int velcorr, double *props, double *alpha) {
     int k = (int)w.size();
@@ -1265,36 +1324,27 @@ static void tda_properties(stda_hip_ctx *c, const double *X, const std::vector<d
         launch_scale_cols(c->stream, c->nci, k, Xn.get(), scd.get());
         T2 = moment_products(c, Xn.get(), k);
     }
-    double sak = std::sqrt(c->ak);
-    double a6[6] = {0, 0, 0, 0, 0, 0};
-    for (int j = 0; j < k; ++j) {
-        const double *t = &T[(size_t)9 * j];
-        double de = w[j], ef = 1.0 / (de + 1.0e-8);
-        double l[3] = {t[0] * sak, t[1] * sak, t[2] * sak};
-        double m[3] = {t[3] * sak, t[4] * sak, t[5] * sak};
-        double v[3] = {t[6] * sak, t[7] * sak, t[8] * sak};
-        double fl = de * (2.0 / 3.0) * (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
-        double fv = ef * (2.0 / 3.0) * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        double rl = -kRot * (l[0] * m[0] + l[1] * m[1] + l[2] * m[2]);
-        double rv = -kRot * (v[0] * m[0] + v[1] * m[1] + v[2] * m[2]) * ef;
-        if (velcorr && !T2.empty()) {   // apbtrafo.f:180-262
-            const double *u = &T2[(size_t)9 * j];
-            double hilf = 1.0 - std::exp(-150.0 * de * de);
-            double efmod = 1.0 / (std::pow(de, hilf) + 1.0e-8);
-            double s = 0.0;
-            for (int q = 0; q < 3; ++q)
-                s += t[6 + q] * t[3 + q] + u[6 + q] * t[3 + q] + t[6 + q] * u[3 + q];
-            rv = -kRot * s * c->ak * efmod;
-        }
-        if (props) {
-            props[5 * j + 0] = fl; props[5 * j + 1] = fv; props[5 * j + 2] = rl;
-            props[5 * j + 3] = rv; props[5 * j + 4] = 0.0;
-        }
-        a6[0] += l[0] * l[0] * ef; a6[1] += l[0] * l[1] * ef; a6[2] += l[1] * l[1] * ef;
-        a6[3] += l[0] * l[2] * ef; a6[4] += l[1] * l[2] * ef; a6[5] += l[2] * l[2] * ef;
+    tda_props_from_moments(c->ak, w, T.data(), T2.empty() ? nullptr : T2.data(), props, alpha);
+}
+
+// Mixed precision TDA: FP32 operator to a residual of max(tol, 1e-6), then one
+// FP64 Rayleigh-Ritz step in the span of the converged vectors.
+static void solve_tda_mixed(stda_hip_ctx *c, int nroots, double tol, std::vector<double> &w,
+                            DBuf<double> &X) {
+    DavOpts lo;
+    lo.fp32 = true;
+    {
+        HostSpan hs(c, "mixed_fp32_phase");
+        davidson_tda(c, nroots, std::max(tol, 1.0e-6), w, X, lo);
     }
-    if (alpha)
-        for (int q = 0; q < 6; ++q) alpha[q] = 2.0 * a6[q];
+    DavOpts ref;
+    ref.V0 = X.get();
+    ref.nv0 = (int)w.size();
+    ref.maxiter = 1;
+    ref.allow_unconverged = true;
+    DBuf<double> X0 = std::move(X);
+    HostSpan hs(c, "mixed_fp64_refine");
+    davidson_tda(c, nroots, tol, w, X, ref);
 }
 
 static void rpa_properties(stda_hip_ctx *c, const double *XpY, const double *XmY,
@@ -1327,6 +1377,7 @@ static void rpa_properties(stda_hip_ctx *c, const double *XpY, const double *XmY
 // C ABI
 #define STDA_TRY(ctx, ...)                                            \
     try {                                                             \
+        AllocScope alloc_scope_((ctx)->stream);                       \
         __VA_ARGS__;                                                  \
         return STDA_HIP_OK;                                           \
     } catch (const Error &e) {                                        \
@@ -1346,9 +1397,19 @@ int stda_hip_create(stda_hip_ctx **out, int device) {
         if (device >= 0) STDA_HIP_CHECK(hipSetDevice(device));
         STDA_HIP_CHECK(hipGetDevice(&c->device));
         STDA_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->owner.stream = c->stream;
         STDA_BLAS_CHECK(rocblas_create_handle(&c->blas));
+        c->owner.blas = c->blas;
         STDA_BLAS_CHECK(rocblas_set_stream(c->blas, c->stream));
         STDA_BLAS_CHECK(rocblas_set_pointer_mode(c->blas, rocblas_pointer_mode_host));
+        // bitwise-reproducible GEMMs: a job's result must not depend on scheduling
+        STDA_BLAS_CHECK(rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed));
+        // keep freed blocks in the device pool instead of returning them at
+        // every synchronisation
+        hipMemPool_t pool;
+        STDA_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, c->device));
+        uint64_t keep = UINT64_MAX;
+        STDA_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
     } catch (const Error &e) {
         delete c;
         return e.status;
@@ -1361,9 +1422,7 @@ int stda_hip_destroy(stda_hip_ctx *c) {
     if (!c) return STDA_HIP_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->blas) rocblas_destroy_handle(c->blas);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
+    delete c;   // buffers first (freed on the stream), then stream and handle (Owner)
     return STDA_HIP_OK;
 }
 
@@ -1455,22 +1514,7 @@ int stda_hip_solve_tda(stda_hip_ctx *c, int32_t nroots, int32_t capacity, int32_
         DBuf<double> X;
         const int64_t n = c->nci;
         if (nroots > 0 && c->precision == 1) {
-            // FP32 operator to a residual of 1e-6, then one FP64 Rayleigh-Ritz
-            // step in the span of the converged vectors (final refinement)
-            DavOpts lo;
-            lo.fp32 = true;
-            {
-                HostSpan hs(c, "mixed_fp32_phase");
-                davidson_tda(c, nroots, std::max(tol, 1.0e-6), w, X, lo);
-            }
-            DavOpts ref;
-            ref.V0 = X.get();
-            ref.nv0 = (int)w.size();
-            ref.maxiter = 1;
-            ref.allow_unconverged = true;
-            DBuf<double> X0 = std::move(X);
-            HostSpan hs(c, "mixed_fp64_refine");
-            davidson_tda(c, nroots, tol, w, X, ref);
+            solve_tda_mixed(c, nroots, tol, w, X);
         } else if (nroots > 0) {
             davidson_tda(c, nroots, tol, w, X);
         } else {
@@ -1828,6 +1872,385 @@ int stda_hip_timings(const stda_hip_ctx *c, double *ms6) {
     if (!c || !ms6) return STDA_HIP_ERR_ARG;
     for (int i = 0; i < 6; ++i) ms6[i] = c->timings[i];
     return STDA_HIP_OK;
+}
+
+}  // extern "C"
+// ============================================================================================
+// Batched solves of many independent molecules (C ABI: stda_hip_batch_*).
+//
+// `nworkers` contexts, each driven by its own host thread and HIP stream,
+// take the molecules in ascending size.  For each one a worker runs the setup
+// pipeline (load, gamma, charges, screening, PT2 selection).  A molecule
+// with at most nmax_batched CSFs and at most BATCH_KMAX roots then gets its
+// explicit FP64 A, the FP32 copy, 0.5 B (velocity correction), the diagonal
+// and the CSF moments built on the device and joins the worker's open group;
+// a full group (or the worker's last one) is solved by one launch of the
+// batched Davidson kernel on the worker's stream, which runs while the
+// worker goes on with the next molecules.  Larger molecules take the
+// single-molecule mixed path on their worker.  Allocations are stream-ordered
+// on each worker's stream, so the workers never synchronise the device.
+
+struct stda_hip_batch {
+    int device = 0;
+    std::vector<stda_hip_ctx *> workers;
+    std::string last_error;
+    int64_t nmax_batched = 2048;         // CSF limit of the batched kernel
+    int group = 16;                      // molecules per batched launch
+    // ms: setup of all molecules incl. single-path solves (wall), 0, property
+    // tables, total, and the sum of the batched launches' HIP-event times
+    double timings[5] = {0, 0, 0, 0, 0};
+    int64_t counts[3] = {0, 0, 0};       // molecules: batched kernel, single path, failed
+    double kernel_bytes = 0.0;           // algorithmic bytes of the batched launches
+    double phase_us[7] = {0, 0, 0, 0, 0, 0, 0};   // summed per-molecule phase times, sweeps
+    double wall_tick_hz = 1.0e8;
+};
+
+namespace {
+
+struct BatchItem {
+    int64_t n = 0;
+    int k = 0, maxdim = 0, nguess = 0;
+    double ak = 2.0;
+    bool moments = false, velcorr = false;
+    DBuf<float> A32;
+    DBuf<double> A64, B64, diag, mom, work;
+    DBuf<int32_t> guess;
+};
+
+// One batched launch of a worker: its molecules, descriptors and outputs.
+struct BatchGroup {
+    std::vector<int> ids;
+    DBuf<BatchMol> desc;
+    DBuf<double> out;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BatchGroup() = default;
+    BatchGroup(BatchGroup &&o) noexcept
+        : ids(std::move(o.ids)), desc(std::move(o.desc)), out(std::move(o.out)), e0(o.e0),
+          e1(o.e1) {
+        o.e0 = o.e1 = nullptr;
+    }
+    BatchGroup(const BatchGroup &) = delete;
+    BatchGroup &operator=(const BatchGroup &) = delete;
+    BatchGroup &operator=(BatchGroup &&) = delete;
+    ~BatchGroup() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// 0.5 B = 0.5 ak (ia|jb)_K - 0.5 ax (ib|ja)_K for all CSF pairs (rtdacorr,
+// apbtrafo.f:580-637), n x n column-major on the device
+void build_bhalf_device(stda_hip_ctx *c, DBuf<double> &B) {
+    const int64_t n = c->nci;
+    const int na = c->ncent;
+    hipStream_t s = c->stream;
+    DBuf<double> pq((size_t)na * n), qq((size_t)na * n);
+    launch_gather_cols(s, na, n, c->pair.get(), c->pia.get(), pq.get());
+    launch_gather_cols(s, na, n, c->pair.get(), c->qia.get(), qq.get());
+    B.resize((size_t)n * n);
+    launch_exchange(s, n, na, c->nv, c->pair.get(), c->pia.get(), c->qia.get(), B.get(), n);
+    dgemm(c->blas, true, false, n, n, na, 0.5 * c->ak, pq.get(), na, qq.get(), na, -0.5 * c->ax,
+          B.get(), n);
+}
+
+// one molecule of the setup phase; returns true when it goes to the batched kernel
+bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
+                   int nroots, double tol, int velcorr, int64_t nmax, BatchItem &it,
+                   int32_t *nfound, double *omega, double *props) {
+    do_load(c, sys, par);
+    do_select(c);
+    const int64_t n = c->nci;
+    const int k = (int)std::min<int64_t>(nroots, n);
+    hipStream_t s = c->stream;
+    if (n > nmax || k > BATCH_KMAX) {
+        std::vector<double> w;
+        DBuf<double> X;
+        solve_tda_mixed(c, nroots, tol, w, X);
+        std::vector<double> pr((size_t)5 * w.size());
+        tda_properties(c, X.get(), w, velcorr, pr.data(), nullptr);
+        *nfound = (int32_t)w.size();
+        for (size_t j = 0; j < w.size(); ++j) {
+            omega[j] = w[j];
+            for (int q = 0; q < 5; ++q) props[5 * j + q] = pr[5 * j + q];
+        }
+        return false;
+    }
+    it.n = n;
+    it.k = k;
+    it.ak = c->ak;
+    it.moments = c->have_moments;
+    it.velcorr = velcorr && c->have_moments;
+    it.maxdim = (int)std::min<int64_t>(n, BATCH_MD);
+    it.nguess = std::min(it.maxdim, std::max(k + 8, k + k / 2));
+    build_a_device(c, it.A64);
+    it.A32.resize((size_t)n * n);
+    launch_to_f32(s, it.A64.get(), n * n, it.A32.get());
+    if (it.velcorr) build_bhalf_device(c, it.B64);
+    it.diag.resize(n);
+    STDA_HIP_CHECK(hipMemcpyAsync(it.diag.get(), c->ed.get(), n * 8, hipMemcpyDeviceToDevice, s));
+    if (it.moments) {
+        it.mom.resize((size_t)9 * n);
+        STDA_HIP_CHECK(hipMemcpyAsync(it.mom.get(), c->mom.get(), (size_t)9 * n * 8,
+                                      hipMemcpyDeviceToDevice, s));
+    }
+    std::vector<int32_t> g = lowest_diag_rows(c->ed_h, it.nguess);
+    it.guess.upload(g.data(), g.size(), s);
+    it.work.resize((size_t)n * (2 * it.maxdim + 3 * BATCH_KMAX) + (size_t)BATCH_MD * BATCH_MD);
+    return true;
+}
+
+// launch the batched kernel for the molecules `ids` on the worker's stream
+void batch_launch(stda_hip_ctx *c, std::vector<BatchItem> &items, std::vector<int> ids,
+                  double tol32, BatchGroup &g) {
+    const int nb = (int)ids.size();
+    // largest first: the biggest molecules get their compute units first
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return items[a].n > items[b].n; });
+    std::vector<BatchMol> desc(nb);
+    g.out.resize((size_t)nb * BATCH_OUT);
+    for (int q = 0; q < nb; ++q) {
+        BatchItem &it = items[ids[q]];
+        BatchMol &d = desc[q];
+        const int64_t n = it.n;
+        d.n = n;
+        d.k = it.k;
+        d.nguess = it.nguess;
+        d.maxdim = it.maxdim;
+        d.maxiter = 300;
+        d.velcorr = it.velcorr ? 1 : 0;
+        d.pad = 0;
+        d.tol = tol32;
+        d.A32 = it.A32.get();
+        d.A64 = it.A64.get();
+        d.B64 = it.velcorr ? it.B64.get() : nullptr;
+        d.diag = it.diag.get();
+        d.mom = it.moments ? it.mom.get() : nullptr;
+        d.guess = it.guess.get();
+        double *w = it.work.get();
+        d.V = w;
+        d.AV = w + n * it.maxdim;
+        d.X = w + 2 * n * it.maxdim;
+        d.AX = d.X + n * BATCH_KMAX;
+        d.W = d.AX + n * BATCH_KMAX;
+        d.H = d.W + n * BATCH_KMAX;
+        d.out = g.out.get() + (size_t)q * BATCH_OUT;
+    }
+    g.ids = std::move(ids);
+    g.desc.upload(desc.data(), desc.size(), c->stream);
+    STDA_HIP_CHECK(hipEventCreate(&g.e0));
+    STDA_HIP_CHECK(hipEventCreate(&g.e1));
+    STDA_HIP_CHECK(hipEventRecord(g.e0, c->stream));
+    launch_davidson_batch(c->stream, nb, g.desc.get());
+    STDA_HIP_CHECK(hipEventRecord(g.e1, c->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers) {
+    if (!out || nworkers < 1 || nworkers > 256) return STDA_HIP_ERR_ARG;
+    auto *b = new stda_hip_batch();
+    for (int w = 0; w < nworkers; ++w) {
+        stda_hip_ctx *c = nullptr;
+        int st = stda_hip_create(&c, device);
+        if (st != STDA_HIP_OK) {
+            stda_hip_batch_destroy(b);
+            return st;
+        }
+        c->precision = 1;
+        b->workers.push_back(c);
+    }
+    b->device = b->workers[0]->device;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess &&
+        khz > 0)
+        b->wall_tick_hz = 1.0e3 * khz;
+    *out = b;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_destroy(stda_hip_batch *b) {
+    if (!b) return STDA_HIP_OK;
+    for (auto *c : b->workers) stda_hip_destroy(c);
+    delete b;
+    return STDA_HIP_OK;
+}
+
+const char *stda_hip_batch_last_error(const stda_hip_batch *b) {
+    return b ? b->last_error.c_str() : "";
+}
+
+int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t group) {
+    if (!b || nmax_batched < 0 || group < 1) return STDA_HIP_ERR_ARG;
+    b->nmax_batched = nmax_batched;
+    b->group = group;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_kernel_bytes(const stda_hip_batch *b, double *bytes) {
+    if (!b || !bytes) return STDA_HIP_ERR_ARG;
+    *bytes = b->kernel_bytes;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us7) {
+    if (!b || !us7) return STDA_HIP_ERR_ARG;
+    std::copy(b->phase_us, b->phase_us + 7, us7);
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_timings(const stda_hip_batch *b, double *ms5, int64_t *counts3) {
+    if (!b) return STDA_HIP_ERR_ARG;
+    if (ms5) std::copy(b->timings, b->timings + 5, ms5);
+    if (counts3) std::copy(b->counts, b->counts + 3, counts3);
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_system *sys,
+                             const stda_hip_params *par, int32_t nroots, double tol,
+                             int32_t velcorr, int32_t *nfound, double *omega, double *props,
+                             int32_t *status) {
+    if (!b || nmol < 0 || (nmol > 0 && (!sys || !par || !nfound || !omega || !props || !status)) ||
+        nroots < 1)
+        return STDA_HIP_ERR_ARG;
+    if (tol <= 0.0) tol = 1.0e-8;
+    const double tol32 = std::max(tol, 1.0e-6);
+    try {
+        Timer t_all;
+        for (auto &x : b->counts) x = 0;
+        for (auto &x : b->timings) x = 0.0;
+        for (auto &x : b->phase_us) x = 0.0;
+        b->kernel_bytes = 0.0;
+        std::vector<BatchItem> items(nmol);
+        std::vector<char> batched(nmol, 0);
+        std::vector<std::string> errs(nmol);
+        std::fill(nfound, nfound + nmol, 0);
+        std::fill(omega, omega + (size_t)nmol * nroots, 0.0);
+        std::fill(props, props + (size_t)nmol * nroots * 5, 0.0);
+        std::vector<int> order(nmol);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return (int64_t)sys[a].no * sys[a].nv < (int64_t)sys[c].no * sys[c].nv;
+        });
+        const int nw = (int)b->workers.size();
+        std::vector<std::vector<BatchGroup>> groups(nw);
+        std::vector<std::string> werr(nw);
+        std::atomic<int> next{0};
+        auto work = [&](int wi) {
+            stda_hip_ctx *c = b->workers[wi];
+            AllocScope scope(c->stream);
+            try {
+                STDA_HIP_CHECK(hipSetDevice(c->device));
+                std::vector<int> open;
+                for (int q = next.fetch_add(1); q < nmol; q = next.fetch_add(1)) {
+                    const int i = order[q];
+                    try {
+                        batched[i] = batch_prepare(c, &sys[i], &par[i], nroots, tol, velcorr,
+                                                   b->nmax_batched, items[i], &nfound[i],
+                                                   omega + (size_t)i * nroots,
+                                                   props + (size_t)i * nroots * 5);
+                        status[i] = STDA_HIP_OK;
+                    } catch (const Error &e) {
+                        if (e.status == STDA_HIP_ERR_DEVICE) throw;
+                        status[i] = e.status;
+                        errs[i] = e.what();
+                        continue;
+                    }
+                    if (batched[i]) {
+                        open.push_back(i);
+                        if ((int)open.size() >= b->group) {
+                            groups[wi].emplace_back();
+                            batch_launch(c, items, std::move(open), tol32, groups[wi].back());
+                            open.clear();
+                        }
+                    }
+                }
+                if (!open.empty()) {
+                    groups[wi].emplace_back();
+                    batch_launch(c, items, std::move(open), tol32, groups[wi].back());
+                }
+                c->sync();
+            } catch (const std::exception &e) {
+                werr[wi] = e.what();
+            }
+        };
+        {
+            std::vector<std::thread> th;
+            for (int w = 0; w < nw; ++w) th.emplace_back(work, w);
+            for (auto &t : th) t.join();
+        }
+        for (int w = 0; w < nw; ++w)
+            if (!werr[w].empty()) throw Error(STDA_HIP_ERR_DEVICE, werr[w]);
+        b->timings[0] = t_all.ms();
+
+        // property tables of the batched molecules
+        Timer t3;
+        const double us_per_tick = 1.0e6 / b->wall_tick_hz;
+        for (int w = 0; w < nw; ++w) {
+            stda_hip_ctx *c = b->workers[w];
+            AllocScope scope(c->stream);
+            STDA_HIP_CHECK(hipSetDevice(c->device));
+            for (BatchGroup &g : groups[w]) {
+                const int nb = (int)g.ids.size();
+                std::vector<double> outh((size_t)nb * BATCH_OUT);
+                g.out.download(outh.data(), outh.size(), c->stream);
+                c->sync();
+                float kms = 0.f;
+                STDA_HIP_CHECK(hipEventElapsedTime(&kms, g.e0, g.e1));
+                b->timings[4] += kms;
+                for (int q = 0; q < nb; ++q) {
+                    const int i = g.ids[q];
+                    const double *o = outh.data() + (size_t)q * BATCH_OUT;
+                    const double *info = o + 20 * BATCH_KMAX;
+                    const int kk = (int)info[2];
+                    const bool conv = info[1] != 0.0;
+                    // the FP32 A once per iteration, the FP64 A and 0.5 B once
+                    const double n2 = (double)items[i].n * (double)items[i].n;
+                    b->kernel_bytes += info[0] * 4.0 * n2 + 8.0 * n2 +
+                                       (items[i].velcorr ? 8.0 * n2 : 0.0);
+                    for (int ph = 0; ph < 6; ++ph) b->phase_us[ph] += info[4 + ph] * us_per_tick;
+                    b->phase_us[6] += info[10];
+                    std::vector<double> w(o, o + kk);
+                    std::vector<double> pr((size_t)5 * kk);
+                    tda_props_from_moments(items[i].ak, w, o + BATCH_KMAX,
+                                           items[i].velcorr ? o + 10 * BATCH_KMAX : nullptr,
+                                           pr.data(), nullptr);
+                    nfound[i] = kk;
+                    for (int j = 0; j < kk; ++j) {
+                        omega[(size_t)i * nroots + j] = w[j];
+                        for (int r = 0; r < 5; ++r)
+                            props[((size_t)i * nroots + j) * 5 + r] = pr[5 * j + r];
+                    }
+                    if (!conv) {
+                        status[i] = STDA_HIP_ERR_NOT_CONVERGED;
+                        errs[i] = "batched Davidson did not converge";
+                    }
+                    ++b->counts[0];
+                }
+            }
+            groups[w].clear();
+        }
+        items.clear();
+        b->timings[2] = t3.ms();
+        b->timings[3] = t_all.ms();
+        b->last_error.clear();
+        for (int i = 0; i < nmol; ++i) {
+            if (status[i] != STDA_HIP_OK) {
+                ++b->counts[2];
+                if (b->last_error.empty())
+                    b->last_error = "molecule " + std::to_string(i) + ": " + errs[i];
+            } else if (!batched[i]) {
+                ++b->counts[1];
+            }
+        }
+        return STDA_HIP_OK;
+    } catch (const Error &e) {
+        b->last_error = e.what();
+        return e.status;
+    } catch (const std::exception &e) {
+        b->last_error = e.what();
+        return STDA_HIP_ERR_DEVICE;
+    }
 }
 
 }  // extern "C"

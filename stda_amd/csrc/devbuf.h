@@ -10,6 +10,24 @@
 
 namespace stda {
 
+// Stream that orders the device allocations of the calling thread.  Every
+// entry point sets it to its context's stream (AllocScope), so that buffers
+// come from the device's stream-ordered pool (hipMallocAsync/hipFreeAsync)
+// and allocating or freeing never synchronises the whole device: contexts
+// driven from several host threads keep overlapping.
+inline hipStream_t &alloc_stream() {
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+    ~AllocScope() { alloc_stream() = prev; }
+    AllocScope(const AllocScope &) = delete;
+    AllocScope &operator=(const AllocScope &) = delete;
+};
+
 template <typename T>
 class DBuf {
   public:
@@ -18,20 +36,28 @@ class DBuf {
     ~DBuf() { release(); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            release();
+            p_ = o.p_; n_ = o.n_; s_ = o.s_;
+            o.p_ = nullptr; o.n_ = 0;
+        }
         return *this;
     }
     void resize(size_t n) {
         if (n == n_) return;
         release();
-        if (n) STDA_HIP_CHECK(hipMalloc(&p_, n * sizeof(T)));
+        if (n) {
+            s_ = alloc_stream();
+            STDA_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p_), n * sizeof(T), s_));
+        }
         n_ = n;
     }
     void ensure(size_t n) { if (n > n_) resize(n); }
+    // freed in the order of the stream it was allocated on
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) (void)hipFreeAsync(p_, s_);
         p_ = nullptr;
         n_ = 0;
     }
@@ -51,6 +77,7 @@ class DBuf {
   private:
     T *p_ = nullptr;
     size_t n_ = 0;
+    hipStream_t s_ = nullptr;
 };
 
 #define STDA_BLAS_CHECK(expr)                                                           \

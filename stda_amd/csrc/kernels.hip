@@ -1330,7 +1330,10 @@ namespace stda {
 //   J blocks nb = 0..3: rows j (16), cols (c = 4nb + r16/4, b = 4w + r16%4)
 //   K blocks mb = 0..3: rows c (16), cols (j = 4mb + r16/4, b = 4w + r16%4)
 // The J values are exchanged through LDS (J[c][j][b]) into the K layout.
-// A work-group owns a column tile and streams the row tiles (deterministic).
+// Work-group (ct, sp) owns column tile ct and streams the row tiles of split
+// sp (a contiguous range); its 256 partial column sums go to
+// part[(sp * nct + ct) * 256 + 16 j_local + b_local] and are summed over the
+// splits in order by pt2_cols_finish_kernel (deterministic).
 constexpr int S_CH = 8;          // atoms per staged chunk (2 MFMA k-steps)
 constexpr int S_LD = 16 * 16 + 1;
 struct SparseSmem {
@@ -1349,9 +1352,11 @@ __global__ void __launch_bounds__(256, 2) pt2_cols_sparse_kernel(
     const double *__restrict__ qab, const double *__restrict__ de,
     const unsigned char *__restrict__ rowmask, unsigned char rowbit,
     const unsigned char *__restrict__ colmask, unsigned char colbit, double ak,
-    double *__restrict__ colsum) {
+    double *__restrict__ part) {
     __shared__ SparseSmem sm;
     const int ct = blockIdx.x;
+    const int nsp = gridDim.y, sp = blockIdx.y;
+    const int rt0 = (int)((int64_t)nrt * sp / nsp), rt1 = (int)((int64_t)nrt * (sp + 1) / nsp);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r16 = lane & 15, g = lane >> 4, hi = r16 >> 2, lo = r16 & 3;
     const int t = threadIdx.x;
@@ -1362,7 +1367,7 @@ __global__ void __launch_bounds__(256, 2) pt2_cols_sparse_kernel(
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) dcol[mb] = pair_de(cj[ct * 16 + 4 * mb + hi], bcol, nv, de, colmask, colbit);
 
-    for (int rt = 0; rt < nrt; ++rt) {
+    for (int rt = rt0; rt < rt1; ++rt) {
         const int k = rk[rt];
         v4d accJ[4], accK[4];
 #pragma unroll
@@ -1440,15 +1445,27 @@ __global__ void __launch_bounds__(256, 2) pt2_cols_sparse_kernel(
         cpart[mb] = v;
     }
     if (g == 0) {
+        double *dst = part + ((int64_t)sp * nct + ct) * 256;
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            int j = cj[ct * 16 + 4 * mb + hi], b = cb[ct * 16 + 4 * w + lo];
-            if (j >= 0 && b >= 0) {
-                int64_t p = (int64_t)j * nv + b;
-                if (colmask[p] & colbit) colsum[p] = cpart[mb];
-            }
-        }
+        for (int mb = 0; mb < 4; ++mb) dst[(4 * mb + hi) * 16 + 4 * w + lo] = cpart[mb];
     }
+}
+
+__global__ void pt2_cols_finish_kernel(int nct, int nsp, const int32_t *__restrict__ cj,
+                                       const int32_t *__restrict__ cb, int nv,
+                                       const unsigned char *__restrict__ colmask,
+                                       unsigned char colbit, const double *__restrict__ part,
+                                       double *__restrict__ colsum) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nct * 256) return;
+    const int ct = (int)(e / 256), slot = (int)(e % 256);
+    const int j = cj[ct * 16 + slot / 16], b = cb[ct * 16 + slot % 16];
+    if (j < 0 || b < 0) return;
+    const int64_t p = (int64_t)j * nv + b;
+    if (!(colmask[p] & colbit)) return;
+    double s = 0.0;
+    for (int sp = 0; sp < nsp; ++sp) s += part[((int64_t)sp * nct + ct) * 256 + slot];
+    colsum[p] = s;
 }
 
 void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int32_t *rc, int nct,
@@ -1456,10 +1473,16 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
                             const double *pK, const double *q, const double *pJ, const double *qab,
                             const double *de, const unsigned char *rowmask, unsigned char rowbit,
                             const unsigned char *colmask, unsigned char colbit, double ak,
-                            double *colsum) {
+                            int nsplit, double *part, double *colsum) {
     if (nrt == 0 || nct == 0) return;
-    hipLaunchKernelGGL(pt2_cols_sparse_kernel, dim3(nct), dim3(256), 0, s, nrt, rk, rc, nct, cj, cb,
-                       ncent, nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
+    if (nsplit < 1) nsplit = 1;
+    if (nsplit > nrt) nsplit = nrt;
+    hipLaunchKernelGGL(pt2_cols_sparse_kernel, dim3(nct, nsplit), dim3(256), 0, s, nrt, rk, rc, nct,
+                       cj, cb, ncent, nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak,
+                       part);
+    const int64_t tot = (int64_t)nct * 256;
+    hipLaunchKernelGGL(pt2_cols_finish_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       nct, nsplit, cj, cb, nv, colmask, colbit, part, colsum);
 }
 
 // Column norms: one block per (column, slice of rows); partial sums of squares
@@ -1608,6 +1631,36 @@ __global__ void to_f32_kernel(const double *x, int64_t n, float *y) {
 void launch_to_f32(hipStream_t s, const double *x, int64_t n, float *y) {
     if (n == 0) return;
     hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, y);
+}
+
+}  // namespace stda
+
+namespace stda {
+
+__global__ void gather_cols_kernel(int rows, int64_t n, const int32_t *idx, const double *src,
+                                   double *dst) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * n) return;
+    const int64_t I = e / rows, a = e - I * rows;
+    dst[e] = src[a + (int64_t)rows * idx[I]];
+}
+
+void launch_gather_cols(hipStream_t s, int rows, int64_t n, const int32_t *idx, const double *src,
+                        double *dst) {
+    const int64_t tot = (int64_t)rows * n;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       rows, n, idx, src, dst);
+}
+
+__global__ void set_diag_kernel(int64_t n, const double *d, double *A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[i * (n + 1)] = d[i];
+}
+
+void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(set_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d, A);
 }
 
 }  // namespace stda

@@ -1,0 +1,256 @@
+"""Continuous-batching service for many excited-state jobs on one GPU.
+
+Jobs are sTDA spectrum requests (lowest ``nroots`` roots with transition
+properties) or sTD-DFT polarizability requests (alpha(w) at a list of
+frequencies).  A :class:`Server` holds
+
+* ``batchers`` batch engines (:class:`stda_amd.core.StdaHipBatch`, each with
+  ``batch_workers`` device contexts): spectrum jobs that have arrived are
+  taken from the queue as one micro-batch by whichever engine is free and
+  solved together (setup on the engine's contexts, one launch of the batched
+  Davidson kernel); requests arriving meanwhile form the next micro-batch,
+  which the other engine runs while the first one is still on the device;
+* ``workers`` contexts for polarizability jobs, each job on its own context
+  and host thread.
+
+Admission is by HBM footprint: a job enters only when its upper-bound device
+footprint fits in what the running jobs leave of ``hbm_budget``.  All device
+state of a job (charges, gamma, CSF space, matrices, Davidson subspaces)
+stays resident from setup to result.  The C calls release the interpreter
+lock, so the engines, the polarizability workers and the device overlap.
+
+Every kernel of a job touches only that job's data and reduces in a fixed
+order, so a job's result does not depend on what else is running; the GPU
+tests check this (alone vs. in the mix).
+"""
+
+from __future__ import annotations
+
+import math
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+
+from .core import StdaHip, StdaHipBatch
+from .prep import CoreInputs
+
+
+@dataclass
+class Job:
+    jid: int
+    kind: str                     # "spectrum" or "polar"
+    ci: CoreInputs
+    thr: float = 7.0              # Ethr (eV)
+    ax: float = 0.25
+    nroots: int = 20
+    omegas: tuple = (0.0,)        # polarizability frequencies (Hartree)
+    arrival: float = 0.0          # seconds after the start of the run
+    start: float = math.nan
+    end: float = math.nan
+    result: object = None         # spectrum: (omega, props); polar: [(w, alpha 3x3)]
+    error: str = ""
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+def footprint_bytes(job: Job) -> int:
+    """Upper bound of the device memory one job needs (bytes).
+
+    Charges and contracted charges over all pairs (q_ia, p_ia, packed
+    occupied/virtual pair charges), the explicit FP64/FP32 A and 0.5 B of the
+    batched solver when the CSF space can be small enough for it, the Davidson
+    or response subspace for the largest possible CSF space (every occupied x
+    virtual pair) and the work buffers of one block of operator applications.
+    """
+    ci = job.ci
+    na, nao, no, nv = ci.ncent, ci.nao, ci.no, ci.nv
+    P = no * nv
+    charges = 8 * na * (2 * P + no * (no + 1) + nv * (nv + 1) // 2 + no + nv)
+    if job.kind == "spectrum":
+        k = job.nroots
+        maxdim = max(6 * k, k + 64)
+    else:
+        k = 3 * len(job.omegas)
+        maxdim = min(P, max(40 * k, 256))
+    dav = 8 * P * 3 * min(maxdim, P)
+    nsmall = min(P, 4096)
+    explicit = 20 * nsmall * nsmall
+    per_vec = 8 * (2 * P + 2 * nao * no + 2 * nao * nao + na)
+    work = per_vec * min(64, max(1, min(maxdim, P)))
+    return int(charges + dav + explicit + work + 64 * 2 ** 20)
+
+
+@dataclass
+class RunStats:
+    n: int
+    ok: int
+    wall: float
+    solves_per_s: float
+    p50_ms: float
+    p99_ms: float
+    mean_ms: float
+    batches: int = 0
+    mean_batch: float = 0.0
+
+
+class Server:
+    """Runs jobs with continuous batching (see the module documentation)."""
+
+    def __init__(self, workers: int = 4, hbm_budget: float = 160e9, precision: str = "mixed",
+                 device: int = 0, tol: float = 1e-8, batchers: int = 2, batch_workers: int = 8,
+                 max_batch: int = 512):
+        self.ctx = [StdaHip(device) for _ in range(workers)]
+        for h in self.ctx:
+            h.set_precision(precision)
+        self.engines = [StdaHipBatch(device, batch_workers) for _ in range(batchers)]
+        self.budget = float(hbm_budget)
+        self.tol = tol
+        self.max_batch = max_batch
+        self._cv = threading.Condition()
+        self._used = 0.0
+
+    def close(self):
+        for h in self.ctx:
+            h.close()
+        for e in self.engines:
+            e.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # admission by footprint (called with self._cv held)
+    def _fits(self, fp: float) -> bool:
+        return self._used + fp <= self.budget or self._used == 0.0
+
+    def run(self, jobs: list, realtime: bool = True) -> RunStats:
+        """Serve ``jobs``.  With ``realtime`` the arrival times are honoured
+        (open loop); otherwise every job is queued at t = 0 (closed batch)."""
+        jobs = sorted(jobs, key=lambda j: j.arrival)
+        if not realtime:
+            for j in jobs:
+                j.arrival = 0.0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        spec_q: deque = deque()
+        polar_q: deque = deque()
+        free_ctx = list(self.ctx)
+        done_feeding = threading.Event()
+        nbatches = [0, 0]
+        threads = []
+
+        def fp_of(j):
+            return min(float(footprint_bytes(j)), self.budget)
+
+        def engine_loop(eng: StdaHipBatch):
+            while True:
+                with self._cv:
+                    while True:
+                        if spec_q and self._fits(fp_of(spec_q[0])):
+                            break
+                        if done_feeding.is_set() and not spec_q:
+                            return
+                        self._cv.wait(0.05)
+                    batch, fps = [], 0.0
+                    while spec_q and len(batch) < self.max_batch:
+                        f = fp_of(spec_q[0])
+                        if batch and not self._fits(f + fps):
+                            break
+                        batch.append(spec_q.popleft())
+                        fps += f
+                    self._used += fps
+                ts = now()
+                for j in batch:
+                    j.start = ts
+                try:
+                    groups = {}
+                    for j in batch:
+                        groups.setdefault((j.thr, j.ax, j.nroots), []).append(j)
+                    for (thr, ax, nroots), grp in groups.items():
+                        res = eng.solve_tda([j.ci for j in grp], thr, ax, nroots, tol=self.tol)
+                        for j, (om, pr, st) in zip(grp, res):
+                            if st == 0:
+                                j.result = (om, pr)
+                            else:
+                                j.error = f"status {st}"
+                except Exception as e:       # reported per job, the service keeps running
+                    for j in batch:
+                        j.error = j.error or str(e)
+                te = now()
+                for j in batch:
+                    j.end = te
+                with self._cv:
+                    self._used -= fps
+                    nbatches[0] += 1
+                    nbatches[1] += len(batch)
+                    self._cv.notify_all()
+
+        def polar_job(h: StdaHip, j: Job, fp: float):
+            j.start = now()
+            try:
+                h.setup(j.ci, j.thr, j.ax)
+                j.result = h.lresp(list(j.omegas), tol=self.tol)
+            except Exception as e:
+                j.error = str(e)
+            j.end = now()
+            with self._cv:
+                self._used -= fp
+                free_ctx.append(h)
+                self._cv.notify_all()
+
+        def polar_loop():
+            while True:
+                with self._cv:
+                    while True:
+                        if polar_q and free_ctx and self._fits(fp_of(polar_q[0])):
+                            break
+                        if done_feeding.is_set() and not polar_q:
+                            return
+                        self._cv.wait(0.05)
+                    j = polar_q.popleft()
+                    fp = fp_of(j)
+                    self._used += fp
+                    h = free_ctx.pop()
+                t = threading.Thread(target=polar_job, args=(h, j, fp), daemon=True)
+                t.start()
+                threads.append(t)
+
+        loops = [threading.Thread(target=engine_loop, args=(e,), daemon=True) for e in self.engines]
+        loops.append(threading.Thread(target=polar_loop, daemon=True))
+        for t in loops:
+            t.start()
+        for j in jobs:                      # the arrival process
+            if realtime:
+                delay = j.arrival - now()
+                if delay > 0:
+                    time.sleep(delay)
+            with self._cv:
+                (spec_q if j.kind == "spectrum" else polar_q).append(j)
+                self._cv.notify_all()
+        done_feeding.set()
+        with self._cv:
+            self._cv.notify_all()
+        for t in loops:
+            t.join()
+        for t in list(threads):
+            t.join()
+        wall = now()
+        lat = np.array([j.latency for j in jobs]) * 1e3 if jobs else np.zeros(1)
+        ok = sum(1 for j in jobs if not j.error)
+        return RunStats(n=len(jobs), ok=ok, wall=wall, solves_per_s=ok / wall if wall > 0 else 0.0,
+                        p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                        mean_ms=float(lat.mean()), batches=nbatches[0],
+                        mean_batch=nbatches[1] / max(nbatches[0], 1))
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
+    """Arrival times (s) of an open-loop Poisson stream of ``rate`` requests/s."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

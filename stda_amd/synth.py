@@ -104,3 +104,48 @@ def cluster(natoms: int, seed: int = 0, basis: str = "split", nocc: int | None =
     if nocc is None:
         nocc = int(Z.sum()) // 2
     return _finish(mol, nocc, rng)
+
+
+def mixed_sizes(n: int, lo: int = 10, hi: int = 200, seed: int = 0) -> np.ndarray:
+    """Atom counts log-uniform in [lo, hi] (rounded), one per molecule."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), n))).astype(int)
+
+
+def _mixed_one(args):
+    from .prep import prepare
+    natoms, seed, basis, thr, ax = args
+    mol, ints = cluster(int(natoms), seed=int(seed), basis=basis)
+    return prepare(mol, thr, ax, ints)
+
+
+def _mixed_worker_init():
+    # one BLAS thread per worker process: the pool itself is the parallelism
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+
+
+def mixed_batch(n: int, lo: int = 10, hi: int = 200, base_seed: int = 1000,
+                basis: str = "minimal", thr: float = 7.0, ax: float = 0.25, workers: int = 0):
+    """``n`` independent molecules for batched/served solves.
+
+    Molecule ``i`` is :func:`cluster` with ``natoms = mixed_sizes(n, lo, hi,
+    base_seed)[i]``, ``seed = base_seed + i``, the given basis and half the
+    electron count occupied, prepared for the energy window ``thr`` (eV) and
+    ``ax``.  Returns the list of boundary inputs
+    (:class:`stda_amd.prep.CoreInputs`) in molecule order.  ``workers`` > 1
+    generates them in that many processes (the AO integrals dominate).
+    """
+    sizes = mixed_sizes(n, lo, hi, base_seed)
+    tasks = [(int(sizes[i]), base_seed + i, basis, thr, ax) for i in range(n)]
+    if workers and workers > 1:
+        import multiprocessing as mp
+        # largest first so the pool's tail is short
+        order = sorted(range(n), key=lambda i: -sizes[i])
+        with mp.get_context("fork").Pool(workers, initializer=_mixed_worker_init) as pool:
+            res = pool.map(_mixed_one, [tasks[i] for i in order], chunksize=1)
+        out = [None] * n
+        for i, r in zip(order, res):
+            out[i] = r
+        return out
+    return [_mixed_one(t) for t in tasks]

@@ -1,0 +1,132 @@
+"""Batched solves of many molecules (stda_hip_batch_*) and the serving loop.
+
+Mixed-precision bar (batched kernel and single-molecule mixed path): every
+root within 1 meV and every oscillator strength (fL, fV) within 1e-4 of the
+FP64 reference -- the FP64 oracle for the small molecules, the plain FP64 GPU
+path (itself held to the oracle by test_gpu_parity.py) for all of them.  The
+same molecule solved alone and inside a batch must give the same result
+bit for bit: each molecule's kernels touch only its own data and reduce in a
+fixed order.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EV = 27.211385050
+E_MIX = 1e-3 / EV          # 1 meV in Hartree
+F_MIX = 1e-4
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def molecules():
+    from stda_amd import synth
+    # 10 molecules of 10-40 atoms plus two of ~80 atoms (nci > 4096: single path)
+    cis = synth.mixed_batch(10, lo=10, hi=40, base_seed=500, workers=4)
+    from stda_amd.prep import prepare
+    for na, seed in ((80, 77), (84, 78)):
+        mol, ints = synth.cluster(na, seed=seed, basis="minimal")
+        cis.append(prepare(mol, 7.0, 0.25, ints))
+    return cis
+
+
+@pytest.fixture(scope="module")
+def plain(molecules):
+    """Plain FP64 GPU results per molecule: (omega, props) or the error status."""
+    from stda_amd.core import StdaHip, StdaHipError
+    out = []
+    with StdaHip(0) as h:
+        for ci in molecules:
+            try:
+                h.setup(ci, 7.0, 0.25)
+                s = h.solve_tda(20, velcorr=True, tol=1e-8, vectors=False)
+                out.append((s.omega.copy(), s.props.copy(), h.nci))
+            except StdaHipError as e:
+                out.append(e.status)
+    return out
+
+
+def test_batch_matches_fp64(molecules, plain):
+    from stda_amd.core import StdaHipBatch
+    say(f"batch of {len(molecules)} molecules, 4 workers")
+    with StdaHipBatch(0, workers=4) as b:
+        res = b.solve_tda(molecules, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
+        t, cnt = b.timings()
+    say(f"  paths {cnt}, timings {t}")
+    assert cnt["batched"] >= 8 and cnt["single"] == 2
+    for i, ((om, pr, st), ref) in enumerate(zip(res, plain)):
+        if isinstance(ref, int):
+            assert st == ref, (i, st, ref)
+            continue
+        assert st == 0, (i, st)
+        rom, rpr, nci = ref
+        assert len(om) == len(rom) == min(20, nci)
+        say(f"  mol {i}: nci={nci} max|dE|={np.abs(om - rom).max() * EV:.2e} eV "
+            f"max|dfL|={np.abs(pr[:, 0] - rpr[:, 0]).max():.2e}")
+        np.testing.assert_allclose(om, rom, atol=E_MIX, rtol=0)
+        np.testing.assert_allclose(pr[:, :2], rpr[:, :2], atol=F_MIX, rtol=0)
+
+
+def test_batch_matches_oracle(molecules):
+    from stda_amd.core import StdaHipBatch
+    small = [ci for ci in molecules if ci.ncent <= 25]
+    assert small
+    with StdaHipBatch(0, workers=2) as b:
+        res = b.solve_tda(small, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
+    for ci, (om, pr, st) in zip(small, res):
+        try:
+            ref = O.run(ci, 7.0, 0.25, nroots=20)
+        except RuntimeError:      # no CSF
+            assert st == 2
+            continue
+        assert st == 0
+        np.testing.assert_allclose(om, ref.omega, atol=E_MIX, rtol=0)
+        np.testing.assert_allclose(pr[:, :2], ref.props[:, :2], atol=F_MIX, rtol=0)
+
+
+def test_alone_equals_in_batch(molecules):
+    from stda_amd.core import StdaHipBatch
+    with StdaHipBatch(0, workers=3) as b:
+        mix = b.solve_tda(molecules, 7.0, 0.25, nroots=20)
+        for i in (0, 3, len(molecules) - 1):
+            alone = b.solve_tda([molecules[i]], 7.0, 0.25, nroots=20)[0]
+            np.testing.assert_array_equal(alone[0], mix[i][0])
+            np.testing.assert_array_equal(alone[1], mix[i][1])
+            assert alone[2] == mix[i][2]
+
+
+def test_server_alone_equals_in_mix(molecules, plain):
+    """Continuous batching: spectrum jobs and polarizability jobs admitted as
+    they arrive; every result equals the same job run alone."""
+    from stda_amd.serve import Server, Job
+    jobs = []
+    for i, ci in enumerate(molecules):
+        kind = "polar" if i % 3 == 2 else "spectrum"
+        jobs.append(Job(jid=i, kind=kind, ci=ci, omegas=(0.0, 0.05)))
+    with Server(workers=4, hbm_budget=64e9, batch_workers=4) as srv:
+        st = srv.run(jobs, realtime=False)
+        say(f"  served {st.n} jobs in {st.wall:.3f}s; p50 {st.p50_ms:.1f} ms")
+        for j in jobs:
+            alone = Job(jid=j.jid, kind=j.kind, ci=j.ci, omegas=j.omegas)
+            srv.run([alone], realtime=False)
+            assert j.error == alone.error
+            if j.error:
+                continue
+            if j.kind == "spectrum":
+                np.testing.assert_array_equal(j.result[0], alone.result[0])
+                np.testing.assert_array_equal(j.result[1], alone.result[1])
+                ref = plain[j.jid]
+                np.testing.assert_allclose(j.result[0], ref[0], atol=E_MIX, rtol=0)
+            else:
+                for (w1, a1), (w2, a2) in zip(j.result, alone.result):
+                    assert w1 == w2
+                    np.testing.assert_array_equal(a1, a2)
