@@ -179,15 +179,16 @@ int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
  * loops over many molecules replaces that loop with one call here.
  *
  * A batch object owns `nworkers` contexts, each with its own host thread and
- * HIP stream (give the process at least as many hardware queues, e.g.
- * GPU_MAX_HW_QUEUES=16, so that the streams overlap).  stda_hip_batch_solve_tda
- * runs the setup pipeline of every molecule (as stda_hip_setup) on the
- * workers concurrently; molecules with at most nmax_batched CSFs (default
- * 2048) and nroots <= 24 are solved in groups of `group` (default 16) per
- * launch of a batched Davidson kernel (one work-group per molecule) in mixed
- * precision (as stda_hip_set_precision(ctx, 1): FP32 operator products, FP64
- * subspace numerics, residual norms 1e-6, one FP64 Rayleigh-Ritz
- * refinement), launched as soon as a group is ready; larger molecules take
+ * HIP stream (give the process at least that many hardware queues, e.g.
+ * GPU_MAX_HW_QUEUES=32, so that the streams overlap), and a driver stream.
+ * stda_hip_batch_solve_tda runs the setup pipeline of every molecule (as
+ * stda_hip_setup) on the workers concurrently.  Molecules with at most
+ * nmax_batched CSFs (default 8192) and nroots <= 24 are solved in lock step
+ * by the calling thread in mixed precision (as stda_hip_set_precision(ctx,
+ * 1): FP32 operator products, FP64 subspace numerics, residual norms 1e-6,
+ * one FP64 Rayleigh-Ritz refinement): per iteration one launch for the
+ * operator products of all of them and one for a Davidson step of each,
+ * molecules joining as soon as their setup is done; larger molecules take
  * the single-molecule mixed path on their worker.
  * Results per molecule i (arrays of nmol rows, row-major by molecule):
  *   nfound[i]                       roots found (min(nroots, nci))
@@ -204,19 +205,25 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                              const stda_hip_params *par, int32_t nroots, double tol,
                              int32_t velcorr, int32_t *nfound, double *omega, double *props,
                              int32_t *status);
-/* Limits of the batched kernel: molecules with at most nmax_batched CSFs go
- * to it, `group` molecules per launch. */
-int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t group);
-/* ms5: host time of [0] the worker phase (setup, single-path solves, batched
- * launches), [1] 0, [2] property tables, [3] total, and [4] the summed
- * HIP-event times of the batched launches; counts3: molecules solved by the
- * batched kernel, by the single path, failed. */
+/* Molecules with at most nmax_batched CSFs go to the lock-step solver, whose
+ * driver reads the state table back every `poll` iterations. */
+int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t poll);
+/* ms5: host time of [0] the setup and solve phase (wall), [1] the driver
+ * loop (wall), [2] property tables, [3] total, and [4] the summed HIP-event
+ * times of the batched launches; counts3: molecules solved by the batched
+ * solver, by the single path, failed. */
 int stda_hip_batch_timings(const stda_hip_batch *b, double *ms5, int64_t *counts3);
-/* Per-molecule phase times of the batched kernel summed over the molecules
- * of the last call (microseconds): operator products, Gram products,
- * subspace eigensolve, Ritz vectors, expansion, refinement + moments; [6]
- * Jacobi sweeps. */
-int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us7);
+/* Per-molecule phase times of the batched step kernel summed over the
+ * molecules of the last call (microseconds): Gram products, subspace
+ * eigensolve, Ritz vectors, expansion, refinement + moments, [5] 0; [6]
+ * Jacobi sweeps; then the workers' host time summed over the workers (us):
+ * [7] setup pipeline, [8] explicit matrices, [9] single-path solves. */
+int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us10);
+/* stda_hip_kernel_stats summed over the workers and the molecules of the last
+ * call (names as there, plus "batch_matvec": the batched operator products,
+ * work = algorithmic bytes). */
+int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms, double *launches,
+                                double *work);
 /* Algorithmic HBM bytes of the last batched Davidson launch: per molecule,
  * the FP32 A (4 n^2) once per iteration plus the FP64 A and 0.5 B (8 n^2
  * each) once. */

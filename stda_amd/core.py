@@ -34,7 +34,7 @@ C_API = [
     "stda_hip_kernel_stats", "stda_hip_set_precision", "stda_hip_batch_create",
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
-    "stda_hip_batch_phase_times",
+    "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats",
 ]
 
 
@@ -105,6 +105,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_batch_kernel_bytes.argtypes = [P, ctypes.POINTER(F64)]
     L.stda_hip_batch_configure.argtypes = [P, I64, I32]
     L.stda_hip_batch_phase_times.argtypes = [P, P]
+    L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
+                                              ctypes.POINTER(F64), ctypes.POINTER(F64)]
     _lib = L
     return L
 
@@ -332,12 +334,13 @@ class StdaHipBatch:
 
     ``workers`` device contexts, each on its own host thread and stream,
     prepare the molecules concurrently (selection, explicit FP64/FP32 A,
-    0.5 B, CSF moments); every group of molecules with at most
-    ``nmax_batched`` CSFs and at most 24 roots is solved by one launch of the
-    batched Davidson kernel (one work-group per molecule; FP32 operator
-    products, FP64 subspace numerics, one FP64 Rayleigh-Ritz refinement) while
-    the workers go on.  Larger molecules take the single-molecule mixed path
-    on their worker.  See :meth:`configure`.
+    0.5 B, CSF moments).  Molecules with at most ``nmax_batched`` CSFs and at
+    most 24 roots join a lock-step solver as soon as they are ready: per
+    iteration one launch computes the operator products of all of them and
+    one advances each by a Davidson step (FP32 operator products, FP64
+    subspace numerics, one FP64 Rayleigh-Ritz refinement).  Larger molecules
+    take the single-molecule mixed path on their worker.  See
+    :meth:`configure`.
     """
 
     def __init__(self, device: int = 0, workers: int = 8):
@@ -406,19 +409,28 @@ class StdaHipBatch:
     def last_error(self) -> str:
         return self._L.stda_hip_batch_last_error(self._b).decode()
 
-    def configure(self, nmax_batched: int = 2048, group: int = 16):
-        """CSF limit of the batched kernel and molecules per batched launch."""
-        st = self._L.stda_hip_batch_configure(self._b, nmax_batched, group)
+    def configure(self, nmax_batched: int = 8192, poll: int = 4):
+        """CSF limit of the lock-step solver and iterations between polls."""
+        st = self._L.stda_hip_batch_configure(self._b, nmax_batched, poll)
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
 
     def phase_times(self):
         """Batched-kernel phase times summed over the molecules (us) and the
         total Jacobi sweeps of the last call."""
-        t = np.zeros(7)
+        t = np.zeros(10)
         self._L.stda_hip_batch_phase_times(self._b, _ptr(t))
-        names = ["matvec", "gram", "jacobi", "ritz", "expand", "final", "sweeps"]
+        names = ["gram", "jacobi", "ritz", "expand", "final", "unused", "sweeps",
+                 "worker_setup", "worker_matrices", "worker_single"]
         return dict(zip(names, t.tolist()))
+
+    def kernel_stats(self, name: str):
+        """Per-kernel statistics summed over the workers and molecules of the
+        last call: HIP-event ms, launches, algorithmic work."""
+        ms, n, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._L.stda_hip_batch_kernel_stats(self._b, name.encode(), ctypes.byref(ms),
+                                            ctypes.byref(n), ctypes.byref(w))
+        return {"ms": ms.value, "launches": n.value, "work": w.value}
 
     def kernel_bytes(self) -> float:
         """Algorithmic bytes of the last batched Davidson launch."""
@@ -432,5 +444,6 @@ class StdaHipBatch:
         t = np.zeros(5)
         cnt = np.zeros(3, dtype=np.int64)
         self._L.stda_hip_batch_timings(self._b, _ptr(t), _ptr(cnt))
-        return ({"workers": t[0], "properties": t[2], "total": t[3], "batch_kernel": t[4]},
+        return ({"solve": t[0], "driver": t[1], "properties": t[2], "total": t[3],
+                 "batch_kernels": t[4]},
                 {"batched": int(cnt[0]), "single": int(cnt[1]), "failed": int(cnt[2])})

@@ -1,8 +1,9 @@
-// Batched sTDA eigensolver for many small molecules: one workgroup runs the
-// whole block Davidson of one molecule (matvecs, Gram products, subspace
+// Batched sTDA eigensolver for many molecules in lock step: per iteration one
+// launch computes the operator products of every active molecule (grid over
+// the molecules' row blocks) and one launch advances every molecule's
+// Davidson by a step (one work-group per molecule: Gram products, subspace
 // eigensolve, Ritz vectors, residuals, preconditioning, orthonormalisation,
-// FP64 refinement and transition moments) inside a single launch for the
-// whole batch.  Mixed precision as in the single-molecule mixed path: the
+// and at the end the FP64 refinement and transition moments).  Mixed precision as in the single-molecule mixed path: the
 // operator products use the FP32 copy of A with FP32 accumulation; the
 // subspace, Gram products, Rayleigh-Ritz, residual norms and
 // orthonormalisation are FP64, and the converged block gets one FP64
@@ -28,15 +29,15 @@ constexpr int JB = 128;            // vector rows staged per LDS pass of a matve
 constexpr int CW32 = 32;           // columns per FP32 matvec pass
 constexpr int CW64 = KM;           // columns per FP64 matvec pass
 constexpr int DI = 4, DC = 8;      // tile of the dot-product routine
-constexpr int JU = (MD / 2) * MD / BT;   // Jacobi items per thread and phase
-static_assert(JU * BT == (MD / 2) * MD, "Jacobi items tile the phase");
+constexpr int LDU = MD + 1;        // leading dimension of the LDS matrices (odd: no bank conflicts)
+static_assert((MD / 2) * (MD / 2) % BT == 0 && MD * (MD / 2) % BT == 0, "Jacobi items tile the step");
 
 struct BatchSmem {
-    double u[MD * MD];     // subspace matrix under Jacobi rotation; staged vector rows
-    double y[MD * MD];     // subspace eigenvectors; projection coefficients
+    double u[MD * LDU];    // subspace matrix under Jacobi rotation; staged vector rows
+    double y[MD * LDU];    // subspace eigenvectors; projection coefficients
     double theta[MD];      // eigenvalues in Jacobi order
     double ths[MD];        // eigenvalues of the kept roots, ascending
-    double rc[MD / 2], rs[MD / 2];
+    double rc[MD / 2], rs[MD / 2], rt[MD / 2];
     double red[NWV * 64];
     double nrm[MD];
     int pa[MD / 2], pb[MD / 2];
@@ -45,21 +46,49 @@ struct BatchSmem {
     int ctl[4];
 };
 
-__device__ __forceinline__ double wsum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+__device__ __forceinline__ int bitrev(int x, int bits) {
+    return bits == 0 ? 0 : (int)(__builtin_bitreverse32((unsigned)x) >> (32 - bits));
+}
+
+// Sums NV (a power of two <= 64) per-lane values over the wave by recursive
+// halving: step s exchanges half of the remaining values with lane ^ 2^s, so
+// log2(NV) steps move NV - 1 values in all; the last value is then summed
+// over the remaining lane groups.  Lane l returns the total of value
+// bitrev(l mod NV, log2 NV).  v is destroyed.
+template <int NV>
+__device__ __forceinline__ double wave_reduce(double (&v)[NV]) {
+    constexpr int L = ilog2(NV);
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        const int half = NV >> (s + 1);
+        // all-ones mask in the upper lanes; the halves are chosen with bit
+        // operations so that v stays in registers (no indexed access)
+        const long long um = -(long long)((lane >> s) & 1);
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const long long lo = __double_as_longlong(v[i]);
+            const long long hi = __double_as_longlong(v[i + half]);
+            const double send = __longlong_as_double((lo & um) | (hi & ~um));
+            const double keep = __longlong_as_double((hi & um) | (lo & ~um));
+            v[i] = keep + __shfl_xor(send, 1 << s, 64);
+        }
+    }
+    double t = v[0];
+#pragma unroll
+    for (int s = L; s < 6; ++s) t += __shfl_xor(t, 1 << s, 64);
+    return t;
 }
 
 // out[q] = sum over the workgroup of v[q], q < nv (nv <= NV <= 64); out may be
-// LDS or global.  Ends with a barrier.
+// LDS or global.  Ends with a barrier.  v is destroyed.
 template <int NV>
-__device__ void block_sum(BatchSmem &sm, const double (&v)[NV], int nv, double *out) {
+__device__ void block_sum(BatchSmem &sm, double (&v)[NV], int nv, double *out) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = wsum(v[q]);
-        if (lane == 0) sm.red[w * 64 + q] = s;
-    }
+    const double t = wave_reduce<NV>(v);
+    if (lane < NV) sm.red[w * 64 + bitrev(lane, ilog2(NV))] = t;
     __syncthreads();
     if ((int)threadIdx.x < nv) {
         double s = 0.0;
@@ -91,11 +120,8 @@ __device__ void dots(BatchSmem &sm, int64_t n, const double *P, int64_t ldp, int
 #pragma unroll
                     for (int cc = 0; cc < DC; ++cc) acc[ii * DC + cc] = fma(p[ii], q[cc], acc[ii * DC + cc]);
             }
-#pragma unroll
-            for (int e = 0; e < DI * DC; ++e) {
-                double s = wsum(acc[e]);
-                if (lane == 0) sm.red[w * 64 + e] = s;
-            }
+            const double t = wave_reduce<DI * DC>(acc);
+            if (lane < DI * DC) sm.red[w * 64 + bitrev(lane, ilog2(DI * DC))] = t;
             __syncthreads();
             if ((int)threadIdx.x < DI * DC) {
                 const int ii = threadIdx.x / DC, cc = threadIdx.x % DC;
@@ -109,51 +135,8 @@ __device__ void dots(BatchSmem &sm, int64_t n, const double *P, int64_t ldp, int
         }
 }
 
-// out[:, c] = A V[:, c] for c < nc with A (n x n, column-major, symmetric) in
-// precision T: the vector rows are staged through LDS in precision T, the
-// products are accumulated in T, the result is stored as FP64.
-template <typename T, int CW>
-__device__ void matvec(BatchSmem &sm, int64_t n, const T *A, const double *V, int64_t ldv, int nc,
-                       double *out, int64_t ldo) {
-    static_assert(JB * CW * sizeof(T) <= sizeof(double) * MD * MD, "stage fits the LDS area");
-    T *stage = reinterpret_cast<T *>(sm.u);
-    for (int c0 = 0; c0 < nc; c0 += CW) {
-        const int cw = min(CW, nc - c0);
-        for (int64_t r0 = 0; r0 < n; r0 += BT) {
-            const int64_t r = r0 + threadIdx.x;
-            T acc[CW];
-#pragma unroll
-            for (int cc = 0; cc < CW; ++cc) acc[cc] = (T)0;
-            for (int64_t j0 = 0; j0 < n; j0 += JB) {
-                const int jb = (int)min((int64_t)JB, n - j0);
-                __syncthreads();
-                for (int e = threadIdx.x; e < JB * CW; e += BT) {
-                    const int jj = e % JB, cc = e / JB;
-                    stage[jj * CW + cc] =
-                        (jj < jb && cc < cw) ? (T)V[(j0 + jj) + ldv * (c0 + cc)] : (T)0;
-                }
-                __syncthreads();
-                if (r < n) {
-                    const T *Ar = A + r + n * j0;
-                    for (int jj = 0; jj < jb; ++jj) {
-                        const T a = Ar[n * jj];
-#pragma unroll
-                        for (int cc = 0; cc < CW; ++cc) acc[cc] = fma(a, stage[jj * CW + cc], acc[cc]);
-                    }
-                }
-            }
-            if (r < n) {
-#pragma unroll
-                for (int cc = 0; cc < CW; ++cc)
-                    if (cc < cw) out[r + ldo * (c0 + cc)] = (double)acc[cc];
-            }
-        }
-    }
-    __syncthreads();
-}
-
 // Eigen-decomposition of the symmetric m x m matrix in sm.u (leading dimension
-// MD), m <= MD: cyclic Jacobi (two-sided rotations, Golub & Van Loan,
+// LDU), m <= MD: cyclic Jacobi (two-sided rotations, Golub & Van Loan,
 // Matrix Computations, Alg. 8.5.2/8.5.3) with the round-robin schedule, the
 // m/2 disjoint rotations of a step applied in parallel.  A rotation is
 // skipped when |a_pq| < 1e-17 (|a_pp| + |a_qq|); the sweeps stop when a whole
@@ -161,7 +144,7 @@ __device__ void matvec(BatchSmem &sm, int64_t n, const T *A, const double *V, in
 // columns of sm.y, sm.perm = indices in ascending eigenvalue order.
 __device__ int jacobi(BatchSmem &sm, int m) {
     const int tid = threadIdx.x;
-    for (int e = tid; e < MD * MD; e += BT) sm.y[e] = (e % MD == e / MD) ? 1.0 : 0.0;
+    for (int e = tid; e < LDU * MD; e += BT) sm.y[e] = (e % LDU == e / LDU) ? 1.0 : 0.0;
     const int me = m + (m & 1), h = me / 2;
     __syncthreads();
     int sweep = 0;
@@ -179,13 +162,13 @@ __device__ int jacobi(BatchSmem &sm, int m) {
                     b = (r - tid + me - 1) % (me - 1);
                 }
                 if (a > b) { const int t = a; a = b; b = t; }
-                double c = 1.0, s = 0.0;
+                double c = 1.0, s = 0.0, t = 0.0;
                 if (b < m) {
-                    const double app = sm.u[a + MD * a], aqq = sm.u[b + MD * b];
-                    const double apq = sm.u[a + MD * b];
+                    const double app = sm.u[a + LDU * a], aqq = sm.u[b + LDU * b];
+                    const double apq = sm.u[a + LDU * b];
                     if (fabs(apq) > 1e-300 && fabs(apq) >= 1e-17 * (fabs(app) + fabs(aqq))) {
                         const double tau = (aqq - app) / (2.0 * apq);
-                        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
                         s = t * c;
                         sm.ctl[0] = 1;
@@ -195,66 +178,53 @@ __device__ int jacobi(BatchSmem &sm, int m) {
                 sm.pb[tid] = b;
                 sm.rc[tid] = c;
                 sm.rs[tid] = s;
+                sm.rt[tid] = t;
             }
             __syncthreads();
-            // rows a, b: J^T A.  Each thread owns JU items (h m <= JU BT); all
-            // loads are issued before the stores.
-            {
-                int ia[JU], ib[JU];
-                double cs[JU], sn[JU], xa[JU], xb[JU];
+            // J^T A J for all pairs at once, by 2x2 blocks (pair P rows, pair Q
+            // columns): A_PQ <- J_P^T A_PQ J_Q; diagonal blocks take the
+            // closed form a_pp - t a_pq, a_qq + t a_pq, a_pq = 0.  Then Y <- Y J.
 #pragma unroll
-                for (int u = 0; u < JU; ++u) {
-                    const int e = tid + u * BT;
-                    sn[u] = 0.0;
-                    if (e < h * m) {
-                        const int p = e / m, j = e - p * m;
-                        sn[u] = sm.rs[p];
-                        cs[u] = sm.rc[p];
-                        ia[u] = sm.pa[p] + MD * j;
-                        ib[u] = sm.pb[p] + MD * j;
-                    }
+            for (int u = 0; u < (MD / 2) * (MD / 2) / BT; ++u) {
+                const int e = tid + u * BT;
+                const int P = e / (MD / 2), Q = e % (MD / 2);
+                if (P >= h || Q >= h) continue;
+                const double sP = sm.rs[P], sQ = sm.rs[Q];
+                if (sP == 0.0 && sQ == 0.0) continue;
+                const int aP = sm.pa[P], bP = sm.pb[P], aQ = sm.pa[Q], bQ = sm.pb[Q];
+                if (P == Q) {
+                    const double tP = sm.rt[P], apq = sm.u[aP + LDU * bP];
+                    sm.u[aP + LDU * aP] -= tP * apq;
+                    sm.u[bP + LDU * bP] += tP * apq;
+                    sm.u[aP + LDU * bP] = 0.0;
+                    sm.u[bP + LDU * aP] = 0.0;
+                    continue;
                 }
-#pragma unroll
-                for (int u = 0; u < JU; ++u)
-                    if (sn[u] != 0.0) { xa[u] = sm.u[ia[u]]; xb[u] = sm.u[ib[u]]; }
-#pragma unroll
-                for (int u = 0; u < JU; ++u)
-                    if (sn[u] != 0.0) {
-                        sm.u[ia[u]] = cs[u] * xa[u] - sn[u] * xb[u];
-                        sm.u[ib[u]] = sn[u] * xa[u] + cs[u] * xb[u];
-                    }
+                const double cP = sm.rc[P], cQ = sm.rc[Q];
+                const bool vb = bP < m, vq = bQ < m;   // the odd-size dummy index is never stored
+                const double x00 = sm.u[aP + LDU * aQ];
+                const double x01 = vq ? sm.u[aP + LDU * bQ] : 0.0;
+                const double x10 = vb ? sm.u[bP + LDU * aQ] : 0.0;
+                const double x11 = (vb && vq) ? sm.u[bP + LDU * bQ] : 0.0;
+                const double r00 = cP * x00 - sP * x10, r01 = cP * x01 - sP * x11;
+                const double r10 = sP * x00 + cP * x10, r11 = sP * x01 + cP * x11;
+                sm.u[aP + LDU * aQ] = cQ * r00 - sQ * r01;
+                if (vq) sm.u[aP + LDU * bQ] = sQ * r00 + cQ * r01;
+                if (vb) sm.u[bP + LDU * aQ] = cQ * r10 - sQ * r11;
+                if (vb && vq) sm.u[bP + LDU * bQ] = sQ * r10 + cQ * r11;
             }
-            __syncthreads();
-            // columns a, b: (J^T A) J and Y J
-            {
-                int ia[JU], ib[JU];
-                double cs[JU], sn[JU], xa[JU], xb[JU], ya[JU], yb[JU];
 #pragma unroll
-                for (int u = 0; u < JU; ++u) {
-                    const int e = tid + u * BT;
-                    sn[u] = 0.0;
-                    if (e < h * m) {
-                        const int p = e / m, i = e - p * m;
-                        sn[u] = sm.rs[p];
-                        cs[u] = sm.rc[p];
-                        ia[u] = i + MD * sm.pa[p];
-                        ib[u] = i + MD * sm.pb[p];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < JU; ++u)
-                    if (sn[u] != 0.0) {
-                        xa[u] = sm.u[ia[u]]; xb[u] = sm.u[ib[u]];
-                        ya[u] = sm.y[ia[u]]; yb[u] = sm.y[ib[u]];
-                    }
-#pragma unroll
-                for (int u = 0; u < JU; ++u)
-                    if (sn[u] != 0.0) {
-                        sm.u[ia[u]] = cs[u] * xa[u] - sn[u] * xb[u];
-                        sm.u[ib[u]] = sn[u] * xa[u] + cs[u] * xb[u];
-                        sm.y[ia[u]] = cs[u] * ya[u] - sn[u] * yb[u];
-                        sm.y[ib[u]] = sn[u] * ya[u] + cs[u] * yb[u];
-                    }
+            for (int u = 0; u < MD * (MD / 2) / BT; ++u) {
+                const int e = tid + u * BT;
+                const int i = e % MD, Q = e / MD;
+                if (Q >= h || i >= m) continue;
+                const double sQ = sm.rs[Q];
+                if (sQ == 0.0) continue;
+                const int aQ = sm.pa[Q], bQ = sm.pb[Q];
+                const double cQ = sm.rc[Q];
+                const double ya = sm.y[i + LDU * aQ], yb = sm.y[i + LDU * bQ];
+                sm.y[i + LDU * aQ] = cQ * ya - sQ * yb;
+                sm.y[i + LDU * bQ] = sQ * ya + cQ * yb;
             }
             __syncthreads();
         }
@@ -262,7 +232,7 @@ __device__ int jacobi(BatchSmem &sm, int m) {
         __syncthreads();
         if (!rotated) break;
     }
-    if (tid < m) sm.theta[tid] = sm.u[tid + MD * tid];
+    if (tid < m) sm.theta[tid] = sm.u[tid + LDU * tid];
     __syncthreads();
     if (tid < m) {
         const double v = sm.theta[tid];
@@ -302,7 +272,7 @@ __device__ void ritz(BatchSmem &sm, int64_t n, int m, int kk, const double *V, c
                 const double v = V[r + n * j], av = AV[r + n * j];
 #pragma unroll
                 for (int cc = 0; cc < 8; ++cc) {
-                    const double yv = sm.y[j + MD * pc[cc]];
+                    const double yv = sm.y[j + LDU * pc[cc]];
                     x[cc] = fma(v, yv, x[cc]);
                     ax[cc] = fma(av, yv, ax[cc]);
                 }
@@ -343,7 +313,7 @@ __device__ void col_norms(BatchSmem &sm, int64_t n, const double *P, int64_t ld,
     __syncthreads();
 }
 
-// T[:, q] -= sum_j Q[:, j] C[j + MD q] for q < nt (C in LDS, sm.y)
+// T[:, q] -= sum_j Q[:, j] C[j + LDU q] for q < nt (C in LDS, sm.y)
 __device__ void subtract_proj(BatchSmem &sm, int64_t n, const double *Q, int mq, double *T, int nt) {
     for (int q0 = 0; q0 < nt; q0 += 8) {
         const int tq = min(8, nt - q0);
@@ -355,7 +325,7 @@ __device__ void subtract_proj(BatchSmem &sm, int64_t n, const double *Q, int mq,
                 const double v = Q[r + n * j];
 #pragma unroll
                 for (int qq = 0; qq < 8; ++qq)
-                    acc[qq] = fma(v, sm.y[j + MD * min(q0 + qq, nt - 1)], acc[qq]);
+                    acc[qq] = fma(v, sm.y[j + LDU * min(q0 + qq, nt - 1)], acc[qq]);
             }
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq)
@@ -365,37 +335,122 @@ __device__ void subtract_proj(BatchSmem &sm, int64_t n, const double *Q, int mq,
     __syncthreads();
 }
 
+// Orthonormalise the nt <= KM columns of T (n x nt) among themselves:
+// S = T^T T, Cholesky S = L L^T with pivots <= 1e-20 dropped, T <- T L^-T,
+// kept columns packed to the front.  Returns the number kept.  The
+// factorisation runs on wave 0 (lane i owns row i of L), the transform on all
+// threads (one row of T each).
+__device__ int chol_qr(BatchSmem &sm, int64_t n, double *T, int nt) {
+    const int tid = threadIdx.x;
+    dots(sm, n, T, n, nt, T, n, nt, sm.u, LDU);   // S in sm.u
+    double *L = sm.y;
+    if (tid < 64) {
+        const int i = tid;
+        for (int j = 0; j < nt; ++j) {
+            // pivot (every lane computes it from the same data)
+            double d = sm.u[j + LDU * j];
+            for (int p = 0; p < j; ++p) d -= L[j + LDU * p] * L[j + LDU * p];
+            const bool keep = d > 1.0e-20;
+            const double ljj = keep ? sqrt(d) : 0.0;
+            if (i > j && i < nt) {
+                double x = 0.0;
+                if (keep) {
+                    x = sm.u[i + LDU * j];
+                    for (int p = 0; p < j; ++p) x -= L[i + LDU * p] * L[j + LDU * p];
+                    x /= ljj;
+                }
+                L[i + LDU * j] = x;
+            }
+            if (i == j) {
+                L[j + LDU * j] = ljj;
+                sm.perm[j] = keep ? 1 : 0;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    __syncthreads();
+    int kept = 0;
+    for (int j = 0; j < nt; ++j) kept += sm.perm[j];
+    // T <- T L^-T: row t, forward substitution t'_j = (t_j - sum_p L_jp t'_p) / L_jj
+    for (int64_t r = tid; r < n; r += BT) {
+        double t[KM];
+#pragma unroll
+        for (int j = 0; j < KM; ++j) t[j] = j < nt ? T[r + n * j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) {
+            if (j < nt) {
+                if (sm.perm[j]) {
+                    double x = t[j];
+#pragma unroll
+                    for (int p = 0; p < j; ++p) x -= L[j + LDU * p] * t[p];
+                    t[j] = x / L[j + LDU * j];
+                } else {
+                    t[j] = 0.0;
+                }
+            }
+        }
+        int q = 0;
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < nt && sm.perm[j]) T[r + n * q++] = t[j];
+    }
+    __syncthreads();
+    return kept;
+}
+
 }  // namespace
 
-__global__ __launch_bounds__(BT) void davidson_batch_kernel(const BatchMol *__restrict__ mols) {
+// ---------------------------------------------------------------------------------------
+// Step kernel: one work-group per molecule advances its state machine by one
+// step.  The operator products come from batch_matvec_kernel, launched
+// before each step over all active molecules.
+//   B_INIT   start vectors (unit vectors on the lowest diagonal rows)
+//   B_ITER   AV[:, m:m+mnew] is ready: Gram block of H, Jacobi on H, Ritz
+//            vectors and residual norms, convergence test, restart,
+//            preconditioned residuals, block Gram-Schmidt + Cholesky QR
+//   B_REFINE AX = A64 X is ready: FP64 Rayleigh-Ritz in span(X), moments
+//   B_VEL    W = (0.5 B) V is ready: velocity-corrected moments
+__global__ __launch_bounds__(BT) void batch_step_kernel(const BatchMol *__restrict__ mols) {
     __shared__ BatchSmem sm;
     const BatchMol M = mols[blockIdx.x];
+    int *st = M.state;
+    const int phase = st[0];
+    if (phase == B_DONE) return;
     const int tid = threadIdx.x;
     const int64_t n = M.n;
     const int k = M.k, maxdim = M.maxdim, nguess = M.nguess;
     double *V = M.V, *AV = M.AV, *X = M.X, *AX = M.AX, *W = M.W, *H = M.H;
-
-    // start vectors: unit vectors on the rows with the lowest diagonal
-    for (int64_t e = tid; e < n * nguess; e += BT) V[e] = 0.0;
-    for (int e = tid; e < MD * MD; e += BT) H[e] = 0.0;
-    __syncthreads();
-    if (tid < nguess) V[M.guess[tid] + n * tid] = 1.0;
-    __syncthreads();
-
-    int m = 0, mnew = nguess, it = 0, kk = 0, conv = 0, sweeps = 0;
-    double rmax = 0.0;
-    // wall-clock ticks per phase (thread 0): matvec, Gram, Jacobi, Ritz, expansion, final
-    double tph[6] = {0, 0, 0, 0, 0, 0};
+    double *out = M.out;
+    double *info = out + 20 * KM;
+    // wall-clock ticks of the step's phases (thread 0): Gram, Jacobi, Ritz, expansion, final
     long long tk = wall_clock64();
     auto lap = [&](int ph) {
         const long long t = wall_clock64();
-        tph[ph] += (double)(t - tk);
+        if (tid == 0) info[4 + ph] += (double)(t - tk);
         tk = t;
     };
-    while (true) {
-        ++it;
-        matvec<float, CW32>(sm, n, M.A32, V + n * m, n, mnew, AV + n * m, n);
-        lap(0);
+
+    if (phase == B_INIT) {
+        for (int64_t e = tid; e < n * nguess; e += BT) V[e] = 0.0;
+        for (int e = tid; e < MD * MD; e += BT) H[e] = 0.0;
+        for (int e = tid; e < BATCH_OUT; e += BT) out[e] = 0.0;
+        __syncthreads();
+        if (tid < nguess) V[M.guess[tid] + n * tid] = 1.0;
+        if (tid == 0) {
+            st[1] = 0;        // m
+            st[2] = nguess;   // mnew
+            st[3] = 0;        // iterations
+            st[4] = 0;        // kk
+            st[0] = B_ITER;
+        }
+        return;
+    }
+
+    if (phase == B_ITER) {
+        int m = st[1], mnew = st[2];
+        const int it = st[3] + 1;
         const int mt = m + mnew;
         dots(sm, n, V, n, mt, AV + n * m, n, mnew, H + MD * m, MD);   // H[0:mt, m:mt]
         for (int e = tid; e < m * mnew; e += BT) {
@@ -406,19 +461,19 @@ __global__ __launch_bounds__(BT) void davidson_batch_kernel(const BatchMol *__re
         m = mt;
         for (int e = tid; e < m * m; e += BT) {
             const int i = e % m, j = e / m;
-            sm.u[i + MD * j] = 0.5 * (H[i + MD * j] + H[j + MD * i]);
+            sm.u[i + LDU * j] = 0.5 * (H[i + MD * j] + H[j + MD * i]);
         }
         __syncthreads();
+        lap(0);
+        const int sw = jacobi(sm, m);
         lap(1);
-        sweeps += jacobi(sm, m);
-        lap(2);
-        kk = min(k, m);
+        const int kk = min(k, m);
         if (tid < kk) sm.ths[tid] = sm.theta[sm.perm[tid]];
         __syncthreads();
         ritz(sm, n, m, kk, V, AV, X, AX, W);
-        lap(3);
+        lap(2);
         int nun = 0;
-        rmax = 0.0;
+        double rmax = 0.0;
         for (int c = 0; c < kk; ++c) {
             const double rn = sm.nrm[c];
             rmax = fmax(rmax, rn);
@@ -427,120 +482,176 @@ __global__ __launch_bounds__(BT) void davidson_batch_kernel(const BatchMol *__re
                 ++nun;
             }
         }
-        if ((nun == 0 && kk == k) || m == n) { conv = 1; break; }
-        if (it >= M.maxiter) break;
-        if (m + nun > maxdim) {   // restart with the Ritz vectors
-            for (int64_t e = tid; e < n * kk; e += BT) {
-                V[e] = X[e];
-                AV[e] = AX[e];
-            }
-            for (int e = tid; e < MD * MD; e += BT) {
-                const int i = e % MD, j = e / MD;
-                H[e] = (i == j && i < kk) ? sm.ths[i] : 0.0;
-            }
-            m = kk;
-        }
-        const int room = min(nun, maxdim - m);
-        __syncthreads();
-        // preconditioned residuals (theta - diag)^-1 r as new directions
-        for (int q = 0; q < room; ++q) {
-            const int col = sm.un[q];
-            const double th = sm.ths[col];
-            for (int64_t r = tid; r < n; r += BT) {
-                double d = th - M.diag[r];
-                if (fabs(d) < 1.0e-4) d = d < 0.0 ? -1.0e-4 : 1.0e-4;
-                V[r + n * (m + q)] = W[r + n * col] / d;
-            }
-        }
-        __syncthreads();
-        col_norms(sm, n, V + n * m, n, room);
-        for (int q = 0; q < room; ++q) {
-            const double nq = sm.nrm[q];
-            const double f = nq > 0.0 ? 1.0 / nq : 0.0;
-            for (int64_t r = tid; r < n; r += BT) V[r + n * (m + q)] *= f;
-        }
-        __syncthreads();
-        // classical Gram-Schmidt against the subspace, twice
-        for (int pass = 0; pass < 2 && m > 0; ++pass) {
-            dots(sm, n, V, n, m, V + n * m, n, room, sm.y, MD);
-            subtract_proj(sm, n, V, m, V + n * m, room);
-        }
-        // then among the new directions, dropping (near-)dependent ones
+        bool finish = (nun == 0 && kk == k) || m == n || it >= M.maxiter;
         int acc = 0;
-        for (int q = 0; q < room; ++q) {
-            double *t = V + n * (m + q);
-            for (int pass = 0; pass < 2 && acc > 0; ++pass) {
-                dots(sm, n, V + n * m, n, acc, t, n, 1, sm.y, MD);
-                subtract_proj(sm, n, V + n * m, acc, t, 1);
+        if (!finish) {
+            if (m + nun > maxdim) {   // restart with the Ritz vectors
+                for (int64_t e = tid; e < n * kk; e += BT) {
+                    V[e] = X[e];
+                    AV[e] = AX[e];
+                }
+                for (int e = tid; e < MD * MD; e += BT) {
+                    const int i = e % MD, j = e / MD;
+                    H[e] = (i == j && i < kk) ? sm.ths[i] : 0.0;
+                }
+                m = kk;
             }
-            col_norms(sm, n, t, n, 1);
-            const double nq = sm.nrm[0];
-            if (nq > 1.0e-10) {
-                double *dst = V + n * (m + acc);
-                const double f = 1.0 / nq;
-                for (int64_t r = tid; r < n; r += BT) dst[r] = t[r] * f;
-                ++acc;
+            const int room = min(nun, maxdim - m);
+            __syncthreads();
+            // preconditioned residuals (theta - diag)^-1 r as new directions
+            for (int q = 0; q < room; ++q) {
+                const int col = sm.un[q];
+                const double th = sm.ths[col];
+                for (int64_t r = tid; r < n; r += BT) {
+                    double d = th - M.diag[r];
+                    if (fabs(d) < 1.0e-4) d = d < 0.0 ? -1.0e-4 : 1.0e-4;
+                    V[r + n * (m + q)] = W[r + n * col] / d;
+                }
             }
             __syncthreads();
+            col_norms(sm, n, V + n * m, n, room);
+            for (int q = 0; q < room; ++q) {
+                const double nq = sm.nrm[q];
+                const double f = nq > 0.0 ? 1.0 / nq : 0.0;
+                for (int64_t r = tid; r < n; r += BT) V[r + n * (m + q)] *= f;
+            }
+            __syncthreads();
+            for (int pass = 0; pass < 2 && m > 0; ++pass) {
+                dots(sm, n, V, n, m, V + n * m, n, room, sm.y, LDU);
+                subtract_proj(sm, n, V, m, V + n * m, room);
+            }
+            acc = room;
+            for (int pass = 0; pass < 2 && acc > 0; ++pass) acc = chol_qr(sm, n, V + n * m, acc);
+            lap(3);
+            finish = acc == 0;   // no new direction left: converged to working precision
         }
-        mnew = acc;
-        lap(4);
-        if (mnew == 0) { conv = 1; break; }
+        if (tid == 0) {
+            info[0] = (double)it;
+            info[3] = rmax;
+            info[10] += (double)sw;
+            st[3] = it;
+            if (finish) {
+                info[1] = (nun == 0 || m == n || acc == 0) ? 1.0 : 0.0;
+                info[2] = (double)kk;
+                st[4] = kk;
+                st[0] = B_REFINE;   // X holds the Ritz vectors
+            } else {
+                st[1] = m;
+                st[2] = acc;
+            }
+        }
+        return;
     }
 
-    // FP64 Rayleigh-Ritz refinement in the span of the kk Ritz vectors
-    matvec<double, CW64>(sm, n, M.A64, X, n, kk, AX, n);
-    dots(sm, n, X, n, kk, AX, n, kk, sm.u, MD);
-    for (int e = tid; e < kk * kk; e += BT) {
-        const int i = e % kk, j = e / kk;
-        if (i < j) {
-            const double a = 0.5 * (sm.u[i + MD * j] + sm.u[j + MD * i]);
-            sm.u[i + MD * j] = a;
-            sm.u[j + MD * i] = a;
-        }
-    }
-    __syncthreads();
-    jacobi(sm, kk);
-    if (tid < kk) sm.ths[tid] = sm.theta[sm.perm[tid]];
-    __syncthreads();
-    ritz(sm, n, kk, kk, X, AX, V, AV, W);   // refined vectors -> V, A V -> AV
-    double *out = M.out;
-    if (tid < kk) {
-        out[tid] = sm.ths[tid];
-        out[19 * KM + tid] = sm.nrm[tid];
-    }
-    if (tid == 0) {
-        out[20 * KM + 0] = (double)it;
-        out[20 * KM + 1] = (double)conv;
-        out[20 * KM + 2] = (double)kk;
-        out[20 * KM + 3] = rmax;
-        out[20 * KM + 10] = (double)sweeps;
-    }
-    __syncthreads();
-    // transition moments of the roots and of the velocity-corrected vectors
-    // X_new = (0.5 B) X / (omega + 1e-8)  (apbtrafo.f:58-69)
-    for (int e = tid; e < 18 * KM; e += BT) out[KM + e] = 0.0;
-    __syncthreads();
-    if (M.mom) {
-        dots(sm, n, M.mom, n, 9, V, n, kk, out + KM, 9);
-        if (M.velcorr && M.B64) {
-            matvec<double, CW64>(sm, n, M.B64, V, n, kk, W, n);
-            for (int c = 0; c < kk; ++c) {
-                const double f = 1.0 / (sm.ths[c] + 1.0e-8);
-                for (int64_t r = tid; r < n; r += BT) W[r + n * c] *= f;
+    if (phase == B_REFINE) {   // AX = A64 X: FP64 Rayleigh-Ritz in span(X)
+        const int kk = st[4];
+        dots(sm, n, X, n, kk, AX, n, kk, sm.u, LDU);
+        for (int e = tid; e < kk * kk; e += BT) {
+            const int i = e % kk, j = e / kk;
+            if (i < j) {
+                const double a = 0.5 * (sm.u[i + LDU * j] + sm.u[j + LDU * i]);
+                sm.u[i + LDU * j] = a;
+                sm.u[j + LDU * i] = a;
             }
-            __syncthreads();
-            dots(sm, n, M.mom, n, 9, W, n, kk, out + 10 * KM, 9);
         }
+        __syncthreads();
+        jacobi(sm, kk);
+        if (tid < kk) sm.ths[tid] = sm.theta[sm.perm[tid]];
+        __syncthreads();
+        ritz(sm, n, kk, kk, X, AX, V, AV, W);   // refined vectors -> V, A V -> AV
+        if (tid < kk) {
+            out[tid] = sm.ths[tid];
+            out[19 * KM + tid] = sm.nrm[tid];
+        }
+        if (M.mom) dots(sm, n, M.mom, n, 9, V, n, kk, out + KM, 9);
+        lap(4);
+        if (tid == 0) st[0] = (M.velcorr && M.B64 && M.mom) ? B_VEL : B_DONE;
+        return;
     }
-    lap(5);
-    if (tid == 0)
-        for (int q = 0; q < 6; ++q) out[20 * KM + 4 + q] = tph[q];
+
+    if (phase == B_VEL) {   // W = (0.5 B) V: X_new = W / (omega + 1e-8) (apbtrafo.f:58-69)
+        const int kk = st[4];
+        for (int c = 0; c < kk; ++c) {
+            const double f = 1.0 / (out[c] + 1.0e-8);
+            for (int64_t r = tid; r < n; r += BT) W[r + n * c] *= f;
+        }
+        __syncthreads();
+        dots(sm, n, M.mom, n, 9, W, n, kk, out + 10 * KM, 9);
+        lap(4);
+        if (tid == 0) st[0] = B_DONE;
+    }
 }
 
-void launch_davidson_batch(hipStream_t s, int nmol, const BatchMol *mols) {
+// ---------------------------------------------------------------------------------------
+// Operator products of all active molecules: work-group (molecule, 256-row
+// block).  B_ITER: AV[:, m:m+mnew] = A32 V[:, m:m+mnew] (FP32 products and
+// accumulation); B_REFINE: AX = A64 X; B_VEL: W = B64 V[:, 0:kk] (FP64).
+// A is read down its columns (coalesced over the rows of the block); the
+// vector rows are staged through LDS in the product's precision.
+constexpr int MV_BT = 256;
+
+template <typename T, int CW>
+__device__ void matvec_rows(double *stage_mem, int64_t n, int64_t r0, const T *A, const double *Vin,
+                            int nc, double *out) {
+    T *stage = reinterpret_cast<T *>(stage_mem);
+    const int64_t r = r0 + threadIdx.x;
+    for (int c0 = 0; c0 < nc; c0 += CW) {
+        const int cw = min(CW, nc - c0);
+        T acc[CW];
+#pragma unroll
+        for (int cc = 0; cc < CW; ++cc) acc[cc] = (T)0;
+        for (int64_t j0 = 0; j0 < n; j0 += JB) {
+            const int jb = (int)min((int64_t)JB, n - j0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < JB * CW; e += MV_BT) {
+                const int jj = e % JB, cc = e / JB;
+                stage[jj * CW + cc] = (jj < jb && cc < cw) ? (T)Vin[(j0 + jj) + n * (c0 + cc)] : (T)0;
+            }
+            __syncthreads();
+            if (r < n) {
+                const T *Ar = A + r + n * j0;
+                for (int jj = 0; jj < jb; ++jj) {
+                    const T a = Ar[n * jj];
+#pragma unroll
+                    for (int cc = 0; cc < CW; ++cc) acc[cc] = fma(a, stage[jj * CW + cc], acc[cc]);
+                }
+            }
+        }
+        if (r < n) {
+#pragma unroll
+            for (int cc = 0; cc < CW; ++cc)
+                if (cc < cw) out[r + n * (c0 + cc)] = (double)acc[cc];
+        }
+    }
+}
+
+__global__ __launch_bounds__(MV_BT) void batch_matvec_kernel(const BatchMol *__restrict__ mols,
+                                                             const int2 *__restrict__ tiles) {
+    __shared__ double stage[JB * CW64];
+    static_assert(JB * CW32 * sizeof(float) <= sizeof(stage), "FP32 stage fits");
+    const int2 t = tiles[blockIdx.x];
+    const BatchMol &M = mols[t.x];
+    const int *st = M.state;
+    const int phase = st[0];
+    const int64_t n = M.n, r0 = (int64_t)t.y * MV_BT;
+    if (phase == B_ITER) {
+        const int m = st[1], mnew = st[2];
+        matvec_rows<float, CW32>(stage, n, r0, M.A32, M.V + n * m, mnew, M.AV + n * m);
+    } else if (phase == B_REFINE) {
+        matvec_rows<double, CW64>(stage, n, r0, M.A64, M.X, st[4], M.AX);
+    } else if (phase == B_VEL) {
+        matvec_rows<double, CW64>(stage, n, r0, M.B64, M.V, st[4], M.W);
+    }
+}
+
+void launch_batch_step(hipStream_t s, int nmol, const BatchMol *mols) {
     if (nmol <= 0) return;
-    hipLaunchKernelGGL(davidson_batch_kernel, dim3(nmol), dim3(BT), 0, s, mols);
+    hipLaunchKernelGGL(batch_step_kernel, dim3(nmol), dim3(BT), 0, s, mols);
+}
+
+void launch_batch_matvec(hipStream_t s, int ntiles, const BatchMol *mols, const int2 *tiles) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(batch_matvec_kernel, dim3(ntiles), dim3(MV_BT), 0, s, mols, tiles);
 }
 
 }  // namespace stda

@@ -74,7 +74,11 @@ void launch_residual(hipStream_t s, int64_t n, int k, const double *AX, const do
 void launch_col_norms(hipStream_t s, int64_t n, int k, const double *R, double *norms);
 void launch_precondition(hipStream_t s, int64_t n, int k, const double *R, const double *diag,
                          const double *theta, double *T);
+void launch_precondition_gather(hipStream_t s, int64_t n, int k, const double *R, const double *col,
+                                const double *diag, const double *theta, double *T);
 void launch_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *scale);
+// V(:, c) *= 1 / norm[c] (0 where norm[c] == 0)
+void launch_inv_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *norm);
 void launch_set_unit(hipStream_t s, int64_t n, int k, double *V, const int32_t *rows);
 // Y(:, c) = diag o X(:, c)
 void launch_axpby_diag(hipStream_t s, int64_t n, int k, const double *diag, const double *X,
@@ -150,14 +154,17 @@ void launch_gather_cols(hipStream_t s, int rows, int64_t n, const int32_t *idx, 
 // A(I, I) = d[I] for the n x n column-major A
 void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A);
 
-// --- batched Davidson (batch.hip): one workgroup per molecule -------------------------
+// --- batched Davidson (batch.hip): molecules in lock step ------------------------------
 constexpr int BATCH_MD = 64;                        // subspace limit
 constexpr int BATCH_KMAX = 24;                      // roots per molecule
 constexpr int BATCH_OUT = 20 * BATCH_KMAX + 12;     // doubles of output per molecule
+constexpr int BATCH_STATE = 8;                      // ints of state per molecule
 // Output of one molecule: omega[KMAX] | T[9 KMAX] = mom^T X | T2[9 KMAX] = mom^T Xnew
 // | FP64 residual norms[KMAX] | info[12]: iterations, converged, roots found, last
-// FP32 residual, wall-clock ticks of the phases (matvec, Gram, subspace eigensolve,
-// Ritz vectors, expansion, refinement + moments), Jacobi sweeps, unused.
+// FP32 residual, wall-clock ticks of the step phases (Gram, subspace eigensolve,
+// Ritz vectors, expansion, refinement + moments), unused, Jacobi sweeps, unused.
+// State: phase, m, mnew, iterations, roots kept, unused x3.
+enum BatchPhase { B_INIT = 0, B_ITER = 1, B_REFINE = 2, B_VEL = 3, B_DONE = 5 };
 struct BatchMol {
     int64_t n;                 // CSFs
     int k, nguess, maxdim, maxiter;
@@ -173,8 +180,12 @@ struct BatchMol {
     double *X, *AX, *W;        // n x BATCH_KMAX
     double *H;                 // BATCH_MD x BATCH_MD
     double *out;               // BATCH_OUT
+    int *state;                // BATCH_STATE
 };
-void launch_davidson_batch(hipStream_t s, int nmol, const BatchMol *mols);
+// one step of every molecule (one work-group each)
+void launch_batch_step(hipStream_t s, int nmol, const BatchMol *mols);
+// operator products of every active molecule; tiles = (molecule, 256-row block)
+void launch_batch_matvec(hipStream_t s, int ntiles, const BatchMol *mols, const int2 *tiles);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.
 void launch_build_a(hipStream_t s, TileSet tiles, int ncent, int no, int nv, const double *pK,

@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -281,12 +284,54 @@ struct stda_hip_ctx {
     // algorithmic work (flops or bytes) of the launches
     struct Stat { double ms = 0, launches = 0, work = 0; };
     std::vector<std::pair<std::string, Stat>> stats;
+    // the same summed over the molecules set up since the last reset (batches)
+    std::vector<std::pair<std::string, Stat>> stats_total;
     Stat &stat(const std::string &name) {
         for (auto &kv : stats)
             if (kv.first == name) return kv.second;
         stats.emplace_back(name, Stat{});
         return stats.back().second;
     }
+    void fold_stats() {
+        for (auto &kv : stats) {
+            Stat *t = nullptr;
+            for (auto &tv : stats_total)
+                if (tv.first == kv.first) t = &tv.second;
+            if (!t) {
+                stats_total.emplace_back(kv.first, Stat{});
+                t = &stats_total.back().second;
+            }
+            t->ms += kv.second.ms;
+            t->launches += kv.second.launches;
+            t->work += kv.second.work;
+        }
+        stats.clear();
+    }
+
+    // event pairs of timed launches, resolved lazily so that timing never
+    // synchronises the stream
+    struct PendingTimer {
+        std::string name;
+        double work;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingTimer> pending;
+    void flush_timers() {
+        for (auto &p : pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(p.b) == hipSuccess &&
+                hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                auto &st = stat(p.name);
+                st.ms += ms;
+                st.launches += 1;
+                st.work += p.work;
+            }
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending.clear();
+    }
+    ~stda_hip_ctx() { flush_timers(); }
 
     void sync() { STDA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
@@ -317,13 +362,9 @@ struct EventTimer {
     }
     void finish(const char *name, double work) {
         STDA_HIP_CHECK(hipEventRecord(b, c->stream));
-        STDA_HIP_CHECK(hipEventSynchronize(b));
-        float ms = 0.f;
-        STDA_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-        auto &st = c->stat(name);
-        st.ms += ms;
-        st.launches += 1;
-        st.work += work;
+        c->pending.push_back({name, work, a, b});
+        a = b = nullptr;
+        if (c->pending.size() > 512) c->flush_timers();
     }
     ~EventTimer() {
         if (a) (void)hipEventDestroy(a);
@@ -639,16 +680,10 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
         dgemm_tn(c, m, kept, n, V, n, T2, n, C);
         dgemm(c->blas, false, false, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
     }
-    std::vector<double> nrm(kept);
     DBuf<double> nd(kept);
     col_norms(c, n, kept, T2, nd.get());
-    nd.download(nrm.data(), kept, c->stream);
-    c->sync();
-    for (auto &x : nrm) x = x > 0 ? 1.0 / x : 0.0;
-    nd.upload(nrm.data(), kept, c->stream);
-    launch_scale_cols(c->stream, n, kept, T2, nd.get());
+    launch_inv_scale_cols(c->stream, n, kept, T2, nd.get());
     STDA_HIP_CHECK(hipMemcpyAsync(T, T2, (size_t)n * kept * 8, hipMemcpyDeviceToDevice, c->stream));
-    c->sync();
     return kept;
 }
 
@@ -694,13 +729,12 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     }
     std::vector<double> H((size_t)maxdim * maxdim, 0.0);
     const int maxiter = o.maxiter;
-    DBuf<double> Hc((size_t)maxdim * maxdim), thud(maxdim);
+    DBuf<double> Hc((size_t)maxdim * maxdim), thud(2 * (size_t)maxdim);
     Yd.resize((size_t)maxdim * k);
     for (int it = 0; it < maxiter; ++it) {
         {
             HostSpan hs(c, "dav_matvec");
             apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0, o.fp32);
-            c->sync();
         }
         HostSpan hs_it(c, "dav_iter_other");
         // H(0:m+mnew, m:m+mnew) = V^T AV
@@ -741,17 +775,8 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         if (un.empty() && kk == k) return;
         if (m == n) return;   // full space: exact
         if (it + 1 == maxiter && o.allow_unconverged) return;
-        // compact the unconverged residuals to the front of R, precondition
+        // preconditioned unconverged residuals become the new directions
         int nun = (int)un.size();
-        for (int q = 0; q < nun; ++q)
-            if (un[q] != q)
-                STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8,
-                                              hipMemcpyDeviceToDevice, s));
-        std::vector<double> thu(nun);
-        for (int q = 0; q < nun; ++q) thu[q] = theta[un[q]];
-        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu.data(), nun * 8, hipMemcpyHostToDevice, s));
-        launch_precondition(s, n, nun, R.get(), c->ed.get(), thud.get(), R.get());
-        c->sync();
         if (m + nun > maxdim) {   // restart with the Ritz vectors
             STDA_HIP_CHECK(hipMemcpyAsync(V.get(), X.get(), (size_t)n * kk * 8,
                                           hipMemcpyDeviceToDevice, s));
@@ -762,8 +787,15 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
             m = kk;
         }
         int room = std::min(nun, maxdim - m);
-        STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * room * 8,
-                                      hipMemcpyDeviceToDevice, s));
+        // theta of each new direction and the residual column it comes from
+        std::vector<double> thu(2 * (size_t)room);
+        for (int q = 0; q < room; ++q) {
+            thu[q] = theta[un[q]];
+            thu[room + q] = (double)un[q];
+        }
+        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu.data(), thu.size() * 8, hipMemcpyHostToDevice, s));
+        launch_precondition_gather(s, n, room, R.get(), thud.get() + room, c->ed.get(), thud.get(),
+                                   V.get() + n * m);
         {
             HostSpan hs(c, "dav_ortho");
             mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
@@ -1029,7 +1061,8 @@ static void do_select(stda_hip_ctx *c) {
     c->nb_max = 0;
     c->nb_max32 = 0;
     c->have32 = false;
-    c->stats.clear();
+    c->flush_timers();
+    c->fold_stats();
     const int na = c->ncent, nao = c->nao, no = c->no, nv = c->nv, moci = no + nv;
     const double alphak = c->alphak, betaj = c->betaj;
     Timer t0;
@@ -1474,6 +1507,7 @@ int stda_hip_setup(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_p
 int stda_hip_kernel_stats(const stda_hip_ctx *c, const char *name, double *ms, double *launches,
                           double *work) {
     if (!c || !name) return STDA_HIP_ERR_ARG;
+    const_cast<stda_hip_ctx *>(c)->flush_timers();
     for (const auto &kv : c->stats)
         if (kv.first == name) {
             if (ms) *ms = kv.second.ms;
@@ -1879,29 +1913,37 @@ int stda_hip_timings(const stda_hip_ctx *c, double *ms6) {
 // Batched solves of many independent molecules (C ABI: stda_hip_batch_*).
 //
 // `nworkers` contexts, each driven by its own host thread and HIP stream,
-// take the molecules in ascending size.  For each one a worker runs the setup
-// pipeline (load, gamma, charges, screening, PT2 selection).  A molecule
-// with at most nmax_batched CSFs and at most BATCH_KMAX roots then gets its
-// explicit FP64 A, the FP32 copy, 0.5 B (velocity correction), the diagonal
-// and the CSF moments built on the device and joins the worker's open group;
-// a full group (or the worker's last one) is solved by one launch of the
-// batched Davidson kernel on the worker's stream, which runs while the
-// worker goes on with the next molecules.  Larger molecules take the
-// single-molecule mixed path on their worker.  Allocations are stream-ordered
-// on each worker's stream, so the workers never synchronise the device.
+// take the molecules in ascending size and run the setup pipeline (load,
+// gamma, charges, screening, PT2 selection).  A molecule with at most
+// nmax_batched CSFs and at most BATCH_KMAX roots then gets its explicit FP64
+// A, the FP32 copy, 0.5 B (velocity correction), the diagonal and the CSF
+// moments built on the device and is handed to the driver (the calling
+// thread).  The driver keeps a set of active molecules in lock step on its
+// own stream: per iteration one launch of batch_matvec_kernel (operator
+// products of every active molecule) and one of batch_step_kernel (one
+// Davidson step per molecule).  Newly prepared molecules join the set at the
+// next iteration and finished ones leave it when the driver polls the state
+// table (every few iterations), so the set is continuously refilled while
+// the workers prepare the rest.  Larger molecules take the single-molecule
+// mixed path on their worker.  Allocations are stream-ordered on each
+// worker's stream, so the workers never synchronise the device.
 
 struct stda_hip_batch {
     int device = 0;
     std::vector<stda_hip_ctx *> workers;
+    hipStream_t dstream = nullptr;       // the driver's stream
     std::string last_error;
-    int64_t nmax_batched = 2048;         // CSF limit of the batched kernel
-    int group = 16;                      // molecules per batched launch
-    // ms: setup of all molecules incl. single-path solves (wall), 0, property
-    // tables, total, and the sum of the batched launches' HIP-event times
+    int64_t nmax_batched = 8192;         // CSF limit of the batched solver
+    int group = 4;                       // iterations between polls of the state table
+    // ms: worker phase (wall), driver loop (wall), property tables, total,
+    // summed HIP-event time of the batched launches (matvec + step)
     double timings[5] = {0, 0, 0, 0, 0};
-    int64_t counts[3] = {0, 0, 0};       // molecules: batched kernel, single path, failed
-    double kernel_bytes = 0.0;           // algorithmic bytes of the batched launches
-    double phase_us[7] = {0, 0, 0, 0, 0, 0, 0};   // summed per-molecule phase times, sweeps
+    int64_t counts[3] = {0, 0, 0};       // molecules: batched, single path, failed
+    double kernel_bytes = 0.0;           // algorithmic bytes of the batched operator products
+    double matvec_ms = 0.0;              // HIP-event time of the batched operator products
+    // summed step phases per molecule (us), 0, Jacobi sweeps, then the
+    // workers' summed host time (us) in setup, explicit matrices, single path
+    double phase_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double wall_tick_hz = 1.0e8;
 };
 
@@ -1915,27 +1957,6 @@ struct BatchItem {
     DBuf<float> A32;
     DBuf<double> A64, B64, diag, mom, work;
     DBuf<int32_t> guess;
-};
-
-// One batched launch of a worker: its molecules, descriptors and outputs.
-struct BatchGroup {
-    std::vector<int> ids;
-    DBuf<BatchMol> desc;
-    DBuf<double> out;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    BatchGroup() = default;
-    BatchGroup(BatchGroup &&o) noexcept
-        : ids(std::move(o.ids)), desc(std::move(o.desc)), out(std::move(o.out)), e0(o.e0),
-          e1(o.e1) {
-        o.e0 = o.e1 = nullptr;
-    }
-    BatchGroup(const BatchGroup &) = delete;
-    BatchGroup &operator=(const BatchGroup &) = delete;
-    BatchGroup &operator=(BatchGroup &&) = delete;
-    ~BatchGroup() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-    }
 };
 
 // 0.5 B = 0.5 ak (ia|jb)_K - 0.5 ax (ib|ja)_K for all CSF pairs (rtdacorr,
@@ -1953,16 +1974,24 @@ void build_bhalf_device(stda_hip_ctx *c, DBuf<double> &B) {
           B.get(), n);
 }
 
-// one molecule of the setup phase; returns true when it goes to the batched kernel
+// one molecule of the setup phase; returns true when it goes to the batched solver
 bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
                    int nroots, double tol, int velcorr, int64_t nmax, BatchItem &it,
-                   int32_t *nfound, double *omega, double *props) {
+                   int32_t *nfound, double *omega, double *props, double *us3) {
+    Timer t0;
     do_load(c, sys, par);
     do_select(c);
+    us3[0] += 1e3 * t0.ms();
+    Timer t1;
     const int64_t n = c->nci;
     const int k = (int)std::min<int64_t>(nroots, n);
     hipStream_t s = c->stream;
     if (n > nmax || k > BATCH_KMAX) {
+        struct Add {
+            double *d;
+            Timer t;
+            ~Add() { *d += 1e3 * t.ms(); }
+        } add{us3 + 2, Timer()};
         std::vector<double> w;
         DBuf<double> X;
         solve_tda_mixed(c, nroots, tol, w, X);
@@ -1996,51 +2025,37 @@ bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_p
     std::vector<int32_t> g = lowest_diag_rows(c->ed_h, it.nguess);
     it.guess.upload(g.data(), g.size(), s);
     it.work.resize((size_t)n * (2 * it.maxdim + 3 * BATCH_KMAX) + (size_t)BATCH_MD * BATCH_MD);
+    us3[1] += 1e3 * t1.ms();
     return true;
 }
 
-// launch the batched kernel for the molecules `ids` on the worker's stream
-void batch_launch(stda_hip_ctx *c, std::vector<BatchItem> &items, std::vector<int> ids,
-                  double tol32, BatchGroup &g) {
-    const int nb = (int)ids.size();
-    // largest first: the biggest molecules get their compute units first
-    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return items[a].n > items[b].n; });
-    std::vector<BatchMol> desc(nb);
-    g.out.resize((size_t)nb * BATCH_OUT);
-    for (int q = 0; q < nb; ++q) {
-        BatchItem &it = items[ids[q]];
-        BatchMol &d = desc[q];
-        const int64_t n = it.n;
-        d.n = n;
-        d.k = it.k;
-        d.nguess = it.nguess;
-        d.maxdim = it.maxdim;
-        d.maxiter = 300;
-        d.velcorr = it.velcorr ? 1 : 0;
-        d.pad = 0;
-        d.tol = tol32;
-        d.A32 = it.A32.get();
-        d.A64 = it.A64.get();
-        d.B64 = it.velcorr ? it.B64.get() : nullptr;
-        d.diag = it.diag.get();
-        d.mom = it.moments ? it.mom.get() : nullptr;
-        d.guess = it.guess.get();
-        double *w = it.work.get();
-        d.V = w;
-        d.AV = w + n * it.maxdim;
-        d.X = w + 2 * n * it.maxdim;
-        d.AX = d.X + n * BATCH_KMAX;
-        d.W = d.AX + n * BATCH_KMAX;
-        d.H = d.W + n * BATCH_KMAX;
-        d.out = g.out.get() + (size_t)q * BATCH_OUT;
-    }
-    g.ids = std::move(ids);
-    g.desc.upload(desc.data(), desc.size(), c->stream);
-    STDA_HIP_CHECK(hipEventCreate(&g.e0));
-    STDA_HIP_CHECK(hipEventCreate(&g.e1));
-    STDA_HIP_CHECK(hipEventRecord(g.e0, c->stream));
-    launch_davidson_batch(c->stream, nb, g.desc.get());
-    STDA_HIP_CHECK(hipEventRecord(g.e1, c->stream));
+BatchMol batch_desc(BatchItem &it, double tol32, double *out, int *state) {
+    BatchMol d;
+    const int64_t n = it.n;
+    d.n = n;
+    d.k = it.k;
+    d.nguess = it.nguess;
+    d.maxdim = it.maxdim;
+    d.maxiter = 300;
+    d.velcorr = it.velcorr ? 1 : 0;
+    d.pad = 0;
+    d.tol = tol32;
+    d.A32 = it.A32.get();
+    d.A64 = it.A64.get();
+    d.B64 = it.velcorr ? it.B64.get() : nullptr;
+    d.diag = it.diag.get();
+    d.mom = it.moments ? it.mom.get() : nullptr;
+    d.guess = it.guess.get();
+    double *w = it.work.get();
+    d.V = w;
+    d.AV = w + n * it.maxdim;
+    d.X = w + 2 * n * it.maxdim;
+    d.AX = d.X + n * BATCH_KMAX;
+    d.W = d.AX + n * BATCH_KMAX;
+    d.H = d.W + n * BATCH_KMAX;
+    d.out = out;
+    d.state = state;
+    return d;
 }
 
 }  // namespace
@@ -2061,6 +2076,11 @@ int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers) {
         b->workers.push_back(c);
     }
     b->device = b->workers[0]->device;
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->dstream, hipStreamNonBlocking) != hipSuccess) {
+        stda_hip_batch_destroy(b);
+        return STDA_HIP_ERR_DEVICE;
+    }
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess &&
         khz > 0)
@@ -2071,6 +2091,10 @@ int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers) {
 
 int stda_hip_batch_destroy(stda_hip_batch *b) {
     if (!b) return STDA_HIP_OK;
+    if (b->dstream) {
+        (void)hipStreamSynchronize(b->dstream);
+        (void)hipStreamDestroy(b->dstream);
+    }
     for (auto *c : b->workers) stda_hip_destroy(c);
     delete b;
     return STDA_HIP_OK;
@@ -2080,10 +2104,10 @@ const char *stda_hip_batch_last_error(const stda_hip_batch *b) {
     return b ? b->last_error.c_str() : "";
 }
 
-int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t group) {
-    if (!b || nmax_batched < 0 || group < 1) return STDA_HIP_ERR_ARG;
+int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t poll) {
+    if (!b || nmax_batched < 0 || poll < 1) return STDA_HIP_ERR_ARG;
     b->nmax_batched = nmax_batched;
-    b->group = group;
+    b->group = poll;
     return STDA_HIP_OK;
 }
 
@@ -2093,9 +2117,34 @@ int stda_hip_batch_kernel_bytes(const stda_hip_batch *b, double *bytes) {
     return STDA_HIP_OK;
 }
 
-int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us7) {
-    if (!b || !us7) return STDA_HIP_ERR_ARG;
-    std::copy(b->phase_us, b->phase_us + 7, us7);
+int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms,
+                                double *launches, double *work) {
+    if (!b || !name) return STDA_HIP_ERR_ARG;
+    double t[3] = {0, 0, 0};
+    if (std::string(name) == "batch_matvec") {
+        t[0] = b->matvec_ms;
+        t[1] = 1;
+        t[2] = b->kernel_bytes;
+    }
+    for (auto *c : b->workers) {
+        c->flush_timers();
+        for (const auto *v : {&c->stats, &c->stats_total})
+            for (const auto &kv : *v)
+                if (kv.first == name) {
+                    t[0] += kv.second.ms;
+                    t[1] += kv.second.launches;
+                    t[2] += kv.second.work;
+                }
+    }
+    if (ms) *ms = t[0];
+    if (launches) *launches = t[1];
+    if (work) *work = t[2];
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us10) {
+    if (!b || !us10) return STDA_HIP_ERR_ARG;
+    std::copy(b->phase_us, b->phase_us + 10, us10);
     return STDA_HIP_OK;
 }
 
@@ -2121,6 +2170,15 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         for (auto &x : b->timings) x = 0.0;
         for (auto &x : b->phase_us) x = 0.0;
         b->kernel_bytes = 0.0;
+        b->matvec_ms = 0.0;
+        for (auto *c : b->workers) {
+            c->flush_timers();
+            c->stats.clear();
+            c->stats_total.clear();
+        }
+        STDA_HIP_CHECK(hipSetDevice(b->device));
+        hipStream_t ds = b->dstream;
+        stda_hip_ctx *c0 = b->workers[0];
         std::vector<BatchItem> items(nmol);
         std::vector<char> batched(nmol, 0);
         std::vector<std::string> errs(nmol);
@@ -2132,23 +2190,36 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
             return (int64_t)sys[a].no * sys[a].nv < (int64_t)sys[c].no * sys[c].nv;
         });
+        // state table and outputs of every molecule, indexed by molecule
+        DBuf<int> stated;
+        DBuf<double> outd;
+        std::vector<int> stateh((size_t)std::max(nmol, 1) * BATCH_STATE, B_DONE);
+        {
+            AllocScope scope(ds);
+            stated.resize((size_t)std::max(nmol, 1) * BATCH_STATE);
+            outd.resize((size_t)std::max(nmol, 1) * BATCH_OUT);
+        }
+        // hand-over from the workers: prepared molecule and the event after its setup
+        std::mutex mu;
+        std::condition_variable cv;
+        std::vector<std::pair<int, hipEvent_t>> ready;
+        int workers_left = (int)b->workers.size();
         const int nw = (int)b->workers.size();
-        std::vector<std::vector<BatchGroup>> groups(nw);
         std::vector<std::string> werr(nw);
         std::atomic<int> next{0};
+        std::vector<std::array<double, 3>> wus(nw, std::array<double, 3>{0, 0, 0});
         auto work = [&](int wi) {
             stda_hip_ctx *c = b->workers[wi];
             AllocScope scope(c->stream);
             try {
                 STDA_HIP_CHECK(hipSetDevice(c->device));
-                std::vector<int> open;
                 for (int q = next.fetch_add(1); q < nmol; q = next.fetch_add(1)) {
                     const int i = order[q];
                     try {
                         batched[i] = batch_prepare(c, &sys[i], &par[i], nroots, tol, velcorr,
                                                    b->nmax_batched, items[i], &nfound[i],
                                                    omega + (size_t)i * nroots,
-                                                   props + (size_t)i * nroots * 5);
+                                                   props + (size_t)i * nroots * 5, wus[wi].data());
                         status[i] = STDA_HIP_OK;
                     } catch (const Error &e) {
                         if (e.status == STDA_HIP_ERR_DEVICE) throw;
@@ -2157,81 +2228,171 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                         continue;
                     }
                     if (batched[i]) {
-                        open.push_back(i);
-                        if ((int)open.size() >= b->group) {
-                            groups[wi].emplace_back();
-                            batch_launch(c, items, std::move(open), tol32, groups[wi].back());
-                            open.clear();
-                        }
+                        hipEvent_t ev;
+                        STDA_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                        STDA_HIP_CHECK(hipEventRecord(ev, c->stream));
+                        std::lock_guard<std::mutex> lk(mu);
+                        ready.emplace_back(i, ev);
+                        cv.notify_all();
                     }
                 }
-                if (!open.empty()) {
-                    groups[wi].emplace_back();
-                    batch_launch(c, items, std::move(open), tol32, groups[wi].back());
-                }
-                c->sync();
             } catch (const std::exception &e) {
                 werr[wi] = e.what();
             }
+            std::lock_guard<std::mutex> lk(mu);
+            --workers_left;
+            cv.notify_all();
         };
+        std::vector<std::thread> th;
+        // joins the workers on every way out of this scope (declared after
+        // everything they use, so it runs before those are destroyed)
+        struct JoinAll {
+            std::vector<std::thread> &t;
+            ~JoinAll() {
+                for (auto &x : t)
+                    if (x.joinable()) x.join();
+            }
+        } join_all{th};
+        for (int w = 0; w < nw; ++w) th.emplace_back(work, w);
+
+        // driver: lock-step iterations over the active set
+        Timer t_drv;
+        double drv_ms = 0.0;
         {
-            std::vector<std::thread> th;
-            for (int w = 0; w < nw; ++w) th.emplace_back(work, w);
-            for (auto &t : th) t.join();
+            AllocScope scope(ds);
+            std::vector<int> active;
+            DBuf<BatchMol> descd;
+            DBuf<int2> tilesd;
+            int ntiles = 0;
+            bool dirty = false;
+            std::vector<std::array<hipEvent_t, 3>> evs;
+            int iter = 0;
+            bool feeding = true;
+            while (true) {
+                std::vector<std::pair<int, hipEvent_t>> got;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    if (active.empty() && ready.empty() && workers_left > 0)
+                        cv.wait_for(lk, std::chrono::milliseconds(2));
+                    got.swap(ready);
+                    feeding = workers_left > 0;
+                }
+                for (auto &gi : got) {
+                    STDA_HIP_CHECK(hipStreamWaitEvent(ds, gi.second, 0));
+                    (void)hipEventDestroy(gi.second);
+                    active.push_back(gi.first);
+                    stateh[(size_t)gi.first * BATCH_STATE] = B_INIT;
+                    STDA_HIP_CHECK(hipMemcpyAsync(stated.get() + (size_t)gi.first * BATCH_STATE,
+                                                  &stateh[(size_t)gi.first * BATCH_STATE],
+                                                  sizeof(int), hipMemcpyHostToDevice, ds));
+                    dirty = true;
+                }
+                if (active.empty()) {
+                    if (!feeding) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (ready.empty() && workers_left == 0) break;
+                    }
+                    continue;
+                }
+                if (dirty) {
+                    // largest first: their work-groups start first
+                    std::stable_sort(active.begin(), active.end(),
+                                     [&](int a, int c) { return items[a].n > items[c].n; });
+                    std::vector<BatchMol> desc;
+                    std::vector<int2> tiles;
+                    for (int q = 0; q < (int)active.size(); ++q) {
+                        const int i = active[q];
+                        desc.push_back(batch_desc(items[i], tol32, outd.get() + (size_t)i * BATCH_OUT,
+                                                  stated.get() + (size_t)i * BATCH_STATE));
+                        const int nblk = (int)((items[i].n + 255) / 256);
+                        for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
+                    }
+                    descd.upload(desc.data(), desc.size(), ds);
+                    tilesd.upload(tiles.data(), tiles.size(), ds);
+                    ntiles = (int)tiles.size();
+                    dirty = false;
+                }
+                std::array<hipEvent_t, 3> e{};
+                for (auto &x : e) STDA_HIP_CHECK(hipEventCreate(&x));
+                STDA_HIP_CHECK(hipEventRecord(e[0], ds));
+                launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
+                STDA_HIP_CHECK(hipEventRecord(e[1], ds));
+                launch_batch_step(ds, (int)active.size(), descd.get());
+                STDA_HIP_CHECK(hipEventRecord(e[2], ds));
+                evs.push_back(e);
+                ++iter;
+                if (iter % b->group == 0) {
+                    // poll: drop the finished molecules from the active set
+                    STDA_HIP_CHECK(hipMemcpyAsync(stateh.data(), stated.get(),
+                                                  stateh.size() * sizeof(int),
+                                                  hipMemcpyDeviceToHost, ds));
+                    STDA_HIP_CHECK(hipStreamSynchronize(ds));
+                    const size_t before = active.size();
+                    active.erase(std::remove_if(active.begin(), active.end(),
+                                                [&](int i) {
+                                                    return stateh[(size_t)i * BATCH_STATE] == B_DONE;
+                                                }),
+                                 active.end());
+                    if (active.size() != before) dirty = true;
+                }
+                if (iter > 100000) throw Error(STDA_HIP_ERR_DEVICE, "batched driver did not finish");
+            }
+            STDA_HIP_CHECK(hipStreamSynchronize(ds));
+            for (auto &e : evs) {
+                float m0 = 0.f, m1 = 0.f;
+                STDA_HIP_CHECK(hipEventElapsedTime(&m0, e[0], e[1]));
+                STDA_HIP_CHECK(hipEventElapsedTime(&m1, e[1], e[2]));
+                b->matvec_ms += m0;
+                b->timings[4] += m0 + m1;
+                for (auto &x : e) (void)hipEventDestroy(x);
+            }
         }
-        for (int w = 0; w < nw; ++w)
+        drv_ms = t_drv.ms();
+        for (auto &t : th)
+            if (t.joinable()) t.join();
+        for (int w = 0; w < nw; ++w) {
             if (!werr[w].empty()) throw Error(STDA_HIP_ERR_DEVICE, werr[w]);
+            for (int q = 0; q < 3; ++q) b->phase_us[7 + q] += wus[w][q];
+        }
         b->timings[0] = t_all.ms();
+        b->timings[1] = drv_ms;
 
         // property tables of the batched molecules
         Timer t3;
+        std::vector<double> outh((size_t)std::max(nmol, 1) * BATCH_OUT);
+        outd.download(outh.data(), outh.size(), ds);
+        STDA_HIP_CHECK(hipStreamSynchronize(ds));
         const double us_per_tick = 1.0e6 / b->wall_tick_hz;
-        for (int w = 0; w < nw; ++w) {
-            stda_hip_ctx *c = b->workers[w];
-            AllocScope scope(c->stream);
-            STDA_HIP_CHECK(hipSetDevice(c->device));
-            for (BatchGroup &g : groups[w]) {
-                const int nb = (int)g.ids.size();
-                std::vector<double> outh((size_t)nb * BATCH_OUT);
-                g.out.download(outh.data(), outh.size(), c->stream);
-                c->sync();
-                float kms = 0.f;
-                STDA_HIP_CHECK(hipEventElapsedTime(&kms, g.e0, g.e1));
-                b->timings[4] += kms;
-                for (int q = 0; q < nb; ++q) {
-                    const int i = g.ids[q];
-                    const double *o = outh.data() + (size_t)q * BATCH_OUT;
-                    const double *info = o + 20 * BATCH_KMAX;
-                    const int kk = (int)info[2];
-                    const bool conv = info[1] != 0.0;
-                    // the FP32 A once per iteration, the FP64 A and 0.5 B once
-                    const double n2 = (double)items[i].n * (double)items[i].n;
-                    b->kernel_bytes += info[0] * 4.0 * n2 + 8.0 * n2 +
-                                       (items[i].velcorr ? 8.0 * n2 : 0.0);
-                    for (int ph = 0; ph < 6; ++ph) b->phase_us[ph] += info[4 + ph] * us_per_tick;
-                    b->phase_us[6] += info[10];
-                    std::vector<double> w(o, o + kk);
-                    std::vector<double> pr((size_t)5 * kk);
-                    tda_props_from_moments(items[i].ak, w, o + BATCH_KMAX,
-                                           items[i].velcorr ? o + 10 * BATCH_KMAX : nullptr,
-                                           pr.data(), nullptr);
-                    nfound[i] = kk;
-                    for (int j = 0; j < kk; ++j) {
-                        omega[(size_t)i * nroots + j] = w[j];
-                        for (int r = 0; r < 5; ++r)
-                            props[((size_t)i * nroots + j) * 5 + r] = pr[5 * j + r];
-                    }
-                    if (!conv) {
-                        status[i] = STDA_HIP_ERR_NOT_CONVERGED;
-                        errs[i] = "batched Davidson did not converge";
-                    }
-                    ++b->counts[0];
-                }
+        for (int i = 0; i < nmol; ++i) {
+            if (!batched[i] || status[i] != STDA_HIP_OK) continue;
+            const double *o = outh.data() + (size_t)i * BATCH_OUT;
+            const double *info = o + 20 * BATCH_KMAX;
+            const int kk = (int)info[2];
+            const bool conv = info[1] != 0.0;
+            // the FP32 A once per iteration, the FP64 A and 0.5 B once
+            const double n2 = (double)items[i].n * (double)items[i].n;
+            b->kernel_bytes += info[0] * 4.0 * n2 + 8.0 * n2 + (items[i].velcorr ? 8.0 * n2 : 0.0);
+            for (int ph = 0; ph < 5; ++ph) b->phase_us[ph] += info[4 + ph] * us_per_tick;
+            b->phase_us[6] += info[10];
+            std::vector<double> w(o, o + kk);
+            std::vector<double> pr((size_t)5 * kk);
+            tda_props_from_moments(items[i].ak, w, o + BATCH_KMAX,
+                                   items[i].velcorr ? o + 10 * BATCH_KMAX : nullptr, pr.data(),
+                                   nullptr);
+            nfound[i] = kk;
+            for (int j = 0; j < kk; ++j) {
+                omega[(size_t)i * nroots + j] = w[j];
+                for (int r = 0; r < 5; ++r) props[((size_t)i * nroots + j) * 5 + r] = pr[5 * j + r];
             }
-            groups[w].clear();
+            if (!conv) {
+                status[i] = STDA_HIP_ERR_NOT_CONVERGED;
+                errs[i] = "batched Davidson did not converge";
+            }
+            ++b->counts[0];
         }
-        items.clear();
         b->timings[2] = t3.ms();
+        items.clear();
+        (void)c0;
         b->timings[3] = t_all.ms();
         b->last_error.clear();
         for (int i = 0; i < nmol; ++i) {

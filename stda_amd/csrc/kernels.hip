@@ -418,6 +418,26 @@ __global__ void precondition_kernel(int64_t n, const double *R, const double *di
     T[i + n * c] = R[i + n * c] / d;
 }
 
+// T(:, c) = R(:, col[c]) / (theta[c] - diag), |denominator| >= 1e-4; col[c]
+// holds the column index as a double (packed with theta in one upload)
+__global__ void precondition_gather_kernel(int64_t n, const double *R, const double *col,
+                                           const double *diag, const double *theta, double *T) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= n) return;
+    const int64_t src = (int64_t)col[c];
+    double d = theta[c] - diag[i];
+    if (fabs(d) < 1.0e-4) d = d < 0.0 ? -1.0e-4 : 1.0e-4;
+    T[i + n * c] = R[i + n * src] / d;
+}
+
+void launch_precondition_gather(hipStream_t s, int64_t n, int k, const double *R, const double *col,
+                                const double *diag, const double *theta, double *T) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(precondition_gather_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256),
+                       0, s, n, R, col, diag, theta, T);
+}
+
 void launch_precondition(hipStream_t s, int64_t n, int k, const double *R, const double *diag,
                          const double *theta, double *T) {
     if (k == 0) return;
@@ -429,6 +449,19 @@ __global__ void scale_cols_kernel(int64_t n, double *V, const double *scale) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
     if (i < n) V[i + n * c] *= scale[c];
+}
+
+__global__ void inv_scale_cols_kernel(int64_t n, double *V, const double *norm) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    const double d = norm[c];
+    if (i < n) V[i + n * c] *= d > 0.0 ? 1.0 / d : 0.0;
+}
+
+void launch_inv_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *norm) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(inv_scale_cols_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s,
+                       n, V, norm);
 }
 
 void launch_scale_cols(hipStream_t s, int64_t n, int k, double *V, const double *scale) {

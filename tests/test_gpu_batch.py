@@ -59,6 +59,7 @@ def test_batch_matches_fp64(molecules, plain):
     from stda_amd.core import StdaHipBatch
     say(f"batch of {len(molecules)} molecules, 4 workers")
     with StdaHipBatch(0, workers=4) as b:
+        b.configure(nmax_batched=4096)     # the two ~80-atom molecules take the single path
         res = b.solve_tda(molecules, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
         t, cnt = b.timings()
     say(f"  paths {cnt}, timings {t}")
