@@ -133,7 +133,8 @@ int stda_hip_solve_rpa(stda_hip_ctx *ctx, int32_t nroots, int32_t capacity, doub
  * and the alpha part of lresp1): for each frequency w solve
  * ((A+B) - w^2 (A-B)^-1)(X+Y) = -2 mu for the 3 Cartesian right-hand sides
  * and return alpha(w)_rc = -2 mu_r^T (X+Y)_c as alpha(3,3,nfreq).
- * xpy (nci,3,nfreq) may be NULL.
+ * xpy (nci,3,nfreq) may be NULL.  tol: residual-norm tolerance of the
+ * response equations (<= 0: 1e-10; near a pole alpha amplifies the residual).
  */
 int stda_hip_lresp(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double tol,
                    double *alpha, double *xpy);

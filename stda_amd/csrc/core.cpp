@@ -388,6 +388,11 @@ enum Op { OP_A = 0, OP_APB = 1, OP_AMB = 2, OP_BHALF = 3 };
 void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A, int64_t lda,
               const double *B, int64_t ldb, double *C) {
     if (m == 0 || p == 0) return;
+    if (m <= TS_MMAX && p <= TS_PMAX) {   // the common shapes: hand-written tall-skinny kernel
+        c->tnbuf.ensure((size_t)tsgemm_tn_slices(n) * m * p);
+        launch_tsgemm_tn(c->stream, n, (int)m, (int)p, A, lda, B, ldb, c->tnbuf.get(), C);
+        return;
+    }
     const int64_t KC_MIN = 2048;
     if (n < 4 * KC_MIN) {
         dgemm(c->blas, true, false, m, p, n, 1.0, A, lda, B, ldb, 0.0, C, m);
@@ -403,6 +408,17 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
         dgemm(c->blas, true, false, m, p, rem, 1.0, A + nch * kc, lda, B + nch * kc, ldb, 0.0,
               c->tnbuf.get() + nch * m * p, m);
     launch_sum_slices(c->stream, m * p, (int)slots, c->tnbuf.get(), C);
+}
+
+// out (n x p) = alpha A C + beta out (A n x m, C m x p) for the tall-skinny
+// products of the solvers: hand-written kernel for p <= 32, rocBLAS beyond
+void gemm_nn(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, double alpha, const double *A,
+             int64_t lda, const double *C, int64_t ldc, double beta, double *out, int64_t ldo) {
+    if (n == 0 || p == 0) return;
+    if (p <= 32)
+        launch_tsgemm_nn(c->stream, n, (int)m, (int)p, alpha, A, lda, C, ldc, beta, out, ldo);
+    else
+        dgemm(c->blas, false, false, n, p, m, alpha, A, lda, C, ldc, beta, out, ldo);
 }
 
 // column 2-norms of R (n x k) into norms (device), deterministic
@@ -650,9 +666,16 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     if (nt == 0) return 0;
     tmp.ensure((size_t)std::max(m, 1) * nt + (size_t)nt * nt + (size_t)n * nt);
     double *C = tmp.get();
+    // unit columns first, so that the drop threshold below measures how much
+    // of each direction is new, not how small the residual it came from is
+    {
+        DBuf<double> nd(nt);
+        col_norms(c, n, nt, T, nd.get());
+        launch_inv_scale_cols(c->stream, n, nt, T, nd.get());
+    }
     for (int pass = 0; pass < 2 && m > 0; ++pass) {
         dgemm_tn(c, m, nt, n, V, n, T, n, C);
-        dgemm(c->blas, false, false, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
+        gemm_nn(c, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
     }
     // symmetric orthonormalisation within T: S = T^T T = U s U^T, T <- T U s^-1/2
     double *S = tmp.get() + (size_t)std::max(m, 1) * nt;
@@ -674,11 +697,11 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     if (kept == 0) return 0;
     DBuf<double> Zd;
     Zd.upload(Z.data(), Z.size(), c->stream);
-    dgemm(c->blas, false, false, n, kept, nt, 1.0, T, n, Zd.get(), nt, 0.0, T2, n);
+    gemm_nn(c, n, kept, nt, 1.0, T, n, Zd.get(), nt, 0.0, T2, n);
     // one more projection against V for safety, then renormalise
     if (m > 0) {
         dgemm_tn(c, m, kept, n, V, n, T2, n, C);
-        dgemm(c->blas, false, false, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
+        gemm_nn(c, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
     }
     DBuf<double> nd(kept);
     col_norms(c, n, kept, T2, nd.get());
@@ -762,8 +785,8 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         STDA_HIP_CHECK(hipMemcpyAsync(Yd.get(), Hs.data(), (size_t)m * kk * 8, hipMemcpyHostToDevice, s));
         theta.assign(w.begin(), w.begin() + kk);
         STDA_HIP_CHECK(hipMemcpyAsync(th.get(), theta.data(), kk * 8, hipMemcpyHostToDevice, s));
-        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
-        dgemm(c->blas, false, false, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
+        gemm_nn(c, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
+        gemm_nn(c, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
         launch_residual(s, n, kk, AX.get(), X.get(), th.get(), R.get());
         col_norms(c, n, kk, R.get(), nrm.get());
         std::vector<double> rn(kk);
@@ -881,11 +904,11 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
         std::copy(xr.begin(), xr.end(), both.begin());
         std::copy(yr.begin(), yr.end(), both.begin() + (size_t)m * kk);
         Yd.upload(both.data(), both.size(), s);
-        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XpY.get(), n);
-        dgemm(c->blas, false, false, n, kk, m, 1.0, V.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
+        gemm_nn(c, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XpY.get(), n);
+        gemm_nn(c, n, kk, m, 1.0, V.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
               XmY.get(), n);
-        dgemm(c->blas, false, false, n, kk, m, 1.0, P.get(), n, Yd.get(), m, 0.0, PX.get(), n);
-        dgemm(c->blas, false, false, n, kk, m, 1.0, M.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
+        gemm_nn(c, n, kk, m, 1.0, P.get(), n, Yd.get(), m, 0.0, PX.get(), n);
+        gemm_nn(c, n, kk, m, 1.0, M.get(), n, Yd.get() + (size_t)m * kk, m, 0.0,
               MX.get(), n);
         DBuf<double> om;
         om.upload(omega.data(), kk, s);
@@ -934,9 +957,9 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
             }
             DBuf<double> Zd, V2((size_t)n * kept), P2((size_t)n * kept), M2((size_t)n * kept);
             Zd.upload(Zc.data(), Zc.size(), s);
-            dgemm(c->blas, false, false, n, kept, m, 1.0, V.get(), n, Zd.get(), m, 0.0, V2.get(), n);
-            dgemm(c->blas, false, false, n, kept, m, 1.0, P.get(), n, Zd.get(), m, 0.0, P2.get(), n);
-            dgemm(c->blas, false, false, n, kept, m, 1.0, M.get(), n, Zd.get(), m, 0.0, M2.get(), n);
+            gemm_nn(c, n, kept, m, 1.0, V.get(), n, Zd.get(), m, 0.0, V2.get(), n);
+            gemm_nn(c, n, kept, m, 1.0, P.get(), n, Zd.get(), m, 0.0, P2.get(), n);
+            gemm_nn(c, n, kept, m, 1.0, M.get(), n, Zd.get(), m, 0.0, M2.get(), n);
             STDA_HIP_CHECK(hipMemcpyAsync(V.get(), V2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
             STDA_HIP_CHECK(hipMemcpyAsync(P.get(), P2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
             STDA_HIP_CHECK(hipMemcpyAsync(M.get(), M2.get(), (size_t)n * kept * 8, hipMemcpyDeviceToDevice, s));
@@ -1665,7 +1688,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
     STDA_TRY(c, {
         STDA_HIP_CHECK(hipSetDevice(c->device));
         Timer t3;
-        if (tol <= 0.0) tol = 1.0e-8;
+        if (tol <= 0.0) tol = 1.0e-10;
         if (c->precision == 1) tol = std::max(tol, 1.0e-6);   // FP32 operator noise floor
         const int64_t n = c->nci;
         hipStream_t s = c->stream;
@@ -1677,16 +1700,43 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         DBuf<double> m2d;
         m2d.upload(m2.data(), 3, s);
         launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
-        int maxdim = (int)std::min<int64_t>(n, std::max(40 * nrhs, 256));
+        // Near a pole, M(w) = (A+B) - w^2 (A-B)^-1 is nearly singular and
+        // indefinite; its generalized eigenvectors are the RPA X+Y, since
+        // M(w)(X+Y)_n = (w_n^2 - w^2)(A-B)^-1 (X+Y)_n.  The initial space
+        // therefore holds the RPA vectors of every root up to 1.25 max(w)
+        // (at least 8) besides the preconditioned right-hand sides.
+        double wmax = 0.0;
+        for (int f = 0; f < nfreq; ++f) wmax = std::max(wmax, std::fabs(omega[f]));
+        int nseed = 0;
+        DBuf<double> seedv;
+        if (wmax > 0.0) {
+            int kr = (int)std::min<int64_t>(n, 8);
+            std::vector<double> wr;
+            DBuf<double> XpY, XmY;
+            DavOpts o;
+            o.fp32 = c->precision == 1;
+            while (true) {
+                davidson_rpa(c, kr, 1.0e-6, wr, XpY, XmY, o);
+                if (wr.empty() || wr.back() > 1.25 * wmax || kr >= n || kr >= 96) break;
+                kr = (int)std::min<int64_t>(std::min<int64_t>(n, 96), 2 * (int64_t)kr);
+            }
+            nseed = (int)wr.size();
+            seedv = std::move(XpY);
+        }
+        int maxdim = (int)std::min<int64_t>(n, std::max(40 * nrhs, 256) + nseed);
         DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
         DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd, rhsV;
-        // initial space: preconditioned right-hand sides
+        // initial space: preconditioned right-hand sides (and the RPA vectors)
         STDA_HIP_CHECK(hipMemcpyAsync(V.get(), rhs.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
         std::vector<double> zero3(3, 0.0);
         DBuf<double> z3;
         z3.upload(zero3.data(), 3, s);
         launch_precondition(s, n, 3, V.get(), c->ed.get(), z3.get(), V.get());
-        int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), 3, tmp);
+        const int n0 = (int)std::min<int64_t>(maxdim, 3 + nseed);
+        if (n0 > 3)
+            STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * 3, seedv.get(), (size_t)n * (n0 - 3) * 8,
+                                          hipMemcpyDeviceToDevice, s));
+        int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), n0, tmp);
         std::vector<double> Ap((size_t)maxdim * maxdim), Am((size_t)maxdim * maxdim), Br((size_t)maxdim * 3);
         std::vector<double> rhsnorm(3);
         {
@@ -1763,10 +1813,10 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             // residuals R1 = P a - w V b - r ; R2 = M b - w V a
             Yd.upload(coef.data(), coef.size(), s);
             DBuf<double> VB((size_t)n * nrhs), VA((size_t)n * nrhs), MB((size_t)n * nrhs);
-            dgemm(c->blas, false, false, n, nrhs, m, 1.0, P.get(), n, Yd.get(), m, 0.0, R.get(), n);
-            dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0, VB.get(), n);
-            dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, VA.get(), n);
-            dgemm(c->blas, false, false, n, nrhs, m, 1.0, M.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0,
+            gemm_nn(c, n, nrhs, m, 1.0, P.get(), n, Yd.get(), m, 0.0, R.get(), n);
+            gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0, VB.get(), n);
+            gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, VA.get(), n);
+            gemm_nn(c, n, nrhs, m, 1.0, M.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0,
                   R.get() + n * nrhs, n);
             std::vector<double> wcol(nrhs);
             for (int f = 0; f < nfreq; ++f)
@@ -1805,7 +1855,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         if (!done) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response solver did not converge");
         // alpha_rc = -2 mu_r^T (X+Y)_c (linear_response.f:147-172)
         Yd.upload(sol.data(), sol.size(), s);
-        dgemm(c->blas, false, false, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
+        gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
         DBuf<double> Ad((size_t)3 * nrhs);
         dgemm_tn(c, 3, nrhs, n, c->mom.get(), n, XP.get(), n, Ad.get());
         std::vector<double> ah((size_t)3 * nrhs);

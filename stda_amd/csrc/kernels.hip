@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace stda {
 
 __device__ __forceinline__ int64_t lin0(int64_t x, int64_t y) {
@@ -1694,6 +1696,145 @@ __global__ void set_diag_kernel(int64_t n, const double *d, double *A) {
 void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A) {
     if (n == 0) return;
     hipLaunchKernelGGL(set_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, d, A);
+}
+
+}  // namespace stda
+
+namespace stda {
+
+// ----------------------------------------------------------------------------
+// Tall-skinny FP64 products of the solvers (n rows, a few dozen columns).
+// Memory-bound: each input element is read once per launch.
+//
+// C (m x p) = A^T B, A (n x m), B (n x p): work-group s reduces the rows of
+// slice s into part[s*m*p + i + m*j]; thread t owns up to two 4 x 4 tiles of
+// C; 32-row chunks of A and B are staged through LDS with coalesced loads.
+constexpr int TS_ROWS = 32;
+
+__global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
+                                                                const double *__restrict__ A,
+                                                                int64_t lda,
+                                                                const double *__restrict__ B,
+                                                                int64_t ldb, int64_t rows,
+                                                                double *__restrict__ part) {
+    __shared__ double As[TS_ROWS][TS_MMAX + 1];
+    __shared__ double Bs[TS_ROWS][TS_PMAX + 1];
+    const int tid = threadIdx.x;
+    const int64_t rb = (int64_t)blockIdx.x * rows, re = min(n, rb + rows);
+    const int nti = (m + 3) / 4, ntj = (p + 3) / 4, ntiles = nti * ntj;
+    double acc[2][16];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[t][e] = 0.0;
+    for (int64_t r0 = rb; r0 < re; r0 += TS_ROWS) {
+        const int nr = (int)min((int64_t)TS_ROWS, re - r0);
+        __syncthreads();
+        for (int e = tid; e < TS_ROWS * m; e += 256) {
+            const int rr = e % TS_ROWS, i = e / TS_ROWS;
+            As[rr][i] = rr < nr ? A[(r0 + rr) + lda * i] : 0.0;
+        }
+        for (int e = tid; e < TS_ROWS * p; e += 256) {
+            const int rr = e % TS_ROWS, j = e / TS_ROWS;
+            Bs[rr][j] = rr < nr ? B[(r0 + rr) + ldb * j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int tile = tid + 256 * t;
+            if (tile >= ntiles) continue;
+            const int i0 = 4 * (tile % nti), j0 = 4 * (tile / nti);
+            for (int rr = 0; rr < TS_ROWS; ++rr) {
+                double a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = i0 + u < m ? As[rr][i0 + u] : 0.0;
+                    b[u] = j0 + u < p ? Bs[rr][j0 + u] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[t][4 * u + v] = fma(a[u], b[v], acc[t][4 * u + v]);
+            }
+        }
+    }
+    double *dst = part + (int64_t)blockIdx.x * m * p;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int tile = tid + 256 * t;
+        if (tile >= ntiles) continue;
+        const int i0 = 4 * (tile % nti), j0 = 4 * (tile / nti);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (i0 + u < m && j0 + v < p) dst[(i0 + u) + (int64_t)m * (j0 + v)] = acc[t][4 * u + v];
+    }
+}
+
+int tsgemm_tn_slices(int64_t n) {
+    const int64_t rows = std::max<int64_t>(64, ((n + 479) / 480 + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
+    return (int)((n + rows - 1) / rows);
+}
+
+void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, int64_t lda,
+                      const double *B, int64_t ldb, double *part, double *C) {
+    if (m == 0 || p == 0) return;
+    const int64_t rows = std::max<int64_t>(64, ((n + 479) / 480 + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
+    const int ns = (int)((n + rows - 1) / rows);
+    if (ns == 0) return;
+    hipLaunchKernelGGL(tsgemm_tn_partial_kernel, dim3(ns), dim3(256), 0, s, n, m, p, A, lda, B, ldb,
+                       rows, part);
+    launch_sum_slices(s, (int64_t)m * p, ns, part, C);
+}
+
+// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32: one row
+// per thread, C staged through LDS in 128-row chunks.  beta == 0 never reads out.
+__global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
+                                                        const double *__restrict__ A, int64_t lda,
+                                                        const double *__restrict__ C, int64_t ldc,
+                                                        double beta, double *out, int64_t ldo) {
+    __shared__ double Cs[128][33];
+    const int tid = threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+    double acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) acc[j] = 0.0;
+    for (int i0 = 0; i0 < m; i0 += 128) {
+        const int mi = min(128, m - i0);
+        __syncthreads();
+        for (int e = tid; e < 128 * 32; e += 256) {
+            const int i = e % 128, j = e / 128;
+            Cs[i][j] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
+        }
+        __syncthreads();
+        if (r < n)
+            for (int i = 0; i < mi; ++i) {
+                const double a = A[r + lda * (i0 + i)];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) acc[j] = fma(a, Cs[i][j], acc[j]);
+            }
+    }
+    if (r < n) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j)
+            if (j < p) {
+                double v = alpha * acc[j];
+                if (beta != 0.0) v += beta * out[r + ldo * j];
+                out[r + ldo * j] = v;
+            }
+    }
+}
+
+void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
+                      int64_t lda, const double *C, int64_t ldc, double beta, double *out,
+                      int64_t ldo) {
+    if (n == 0 || p == 0) return;
+    for (int j0 = 0; j0 < p; j0 += 32) {
+        const int pj = std::min(32, p - j0);
+        hipLaunchKernelGGL(tsgemm_nn_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                           m, pj, alpha, A, lda, C + ldc * j0, ldc, beta, out + ldo * j0, ldo);
+    }
 }
 
 }  // namespace stda
