@@ -1772,15 +1772,22 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
     }
 }
 
+// at most 64 slices of at least 256 rows: enough work-groups to stream the
+// inputs, few enough partials for a cheap ordered sum
+static int64_t tsgemm_tn_rows(int64_t n) {
+    const int64_t ns = std::max<int64_t>(1, std::min<int64_t>(64, n / 256));
+    return std::max<int64_t>(TS_ROWS, (n + ns - 1) / ns);
+}
+
 int tsgemm_tn_slices(int64_t n) {
-    const int64_t rows = std::max<int64_t>(64, ((n + 479) / 480 + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
+    const int64_t rows = tsgemm_tn_rows(n);
     return (int)((n + rows - 1) / rows);
 }
 
 void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, int64_t lda,
                       const double *B, int64_t ldb, double *part, double *C) {
     if (m == 0 || p == 0) return;
-    const int64_t rows = std::max<int64_t>(64, ((n + 479) / 480 + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
+    const int64_t rows = tsgemm_tn_rows(n);
     const int ns = (int)((n + rows - 1) / rows);
     if (ns == 0) return;
     hipLaunchKernelGGL(tsgemm_tn_partial_kernel, dim3(ns), dim3(256), 0, s, n, m, p, A, lda, B, ldb,
