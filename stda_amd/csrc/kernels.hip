@@ -1707,8 +1707,10 @@ namespace stda {
 // Memory-bound: each input element is read once per launch.
 //
 // C (m x p) = A^T B, A (n x m), B (n x p): work-group s reduces the rows of
-// slice s into part[s*m*p + i + m*j]; thread t owns up to two 4 x 4 tiles of
-// C; 32-row chunks of A and B are staged through LDS with coalesced loads.
+// slice s (a few 32-row chunks) into part[s*m*p + i + m*j]; thread t owns up
+// to two 4 x 4 tiles of C; the chunks of A and B are staged through LDS with
+// coalesced loads.  Many small slices keep several work-groups per CU in
+// flight; tsgemm_reduce_kernel then sums the partials in slice order.
 constexpr int TS_ROWS = 32;
 
 __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
@@ -1744,7 +1746,7 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
             const int tile = tid + 256 * t;
             if (tile >= ntiles) continue;
             const int i0 = 4 * (tile % nti), j0 = 4 * (tile / nti);
-            for (int rr = 0; rr < TS_ROWS; ++rr) {
+            for (int rr = 0; rr < nr; ++rr) {
                 double a[4], b[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1772,11 +1774,27 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
     }
 }
 
-// at most 64 slices of at least 256 rows: enough work-groups to stream the
-// inputs, few enough partials for a cheap ordered sum
+// C[e] = sum_s part[s*len + e]: work-group of 4 x 64 threads per 64
+// elements; group g sums slices s = g, g+4, ... in order, then the four
+// partial sums are added in group order (deterministic).
+__global__ __launch_bounds__(256) void tsgemm_reduce_kernel(int64_t len, int ns,
+                                                            const double *__restrict__ part,
+                                                            double *__restrict__ C) {
+    __shared__ double red[4][64];
+    const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + l;
+    double s = 0.0;
+    if (e < len)
+        for (int q = g; q < ns; q += 4) s += part[(int64_t)q * len + e];
+    red[g][l] = s;
+    __syncthreads();
+    if (g == 0 && e < len) C[e] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+}
+
+// slices of whole 32-row chunks, about 256 of them (at least one chunk each)
 static int64_t tsgemm_tn_rows(int64_t n) {
-    const int64_t ns = std::max<int64_t>(1, std::min<int64_t>(64, n / 256));
-    return std::max<int64_t>(TS_ROWS, (n + ns - 1) / ns);
+    const int64_t r = (n + 255) / 256;
+    return std::max<int64_t>(TS_ROWS, (r + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
 }
 
 int tsgemm_tn_slices(int64_t n) {
@@ -1792,18 +1810,27 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
     if (ns == 0) return;
     hipLaunchKernelGGL(tsgemm_tn_partial_kernel, dim3(ns), dim3(256), 0, s, n, m, p, A, lda, B, ldb,
                        rows, part);
-    launch_sum_slices(s, (int64_t)m * p, ns, part, C);
+    const int64_t len = (int64_t)m * p;
+    hipLaunchKernelGGL(tsgemm_reduce_kernel, dim3((unsigned)((len + 63) / 64)), dim3(256), 0, s, len,
+                       ns, part, C);
 }
 
-// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32: one row
-// per thread, C staged through LDS in 128-row chunks.  beta == 0 never reads out.
+// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32.  A
+// work-group covers 64 rows; its four waves take the inner index in four
+// interleaved sets (i = g, g+4, ...) and their partial rows are added in
+// wave order through LDS.  C is staged through LDS in 128-row chunks.
+// beta == 0 never reads out.
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A, int64_t lda,
                                                         const double *__restrict__ C, int64_t ldc,
                                                         double beta, double *out, int64_t ldo) {
-    __shared__ double Cs[128][33];
-    const int tid = threadIdx.x;
-    const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+    // one LDS area: the staged C while accumulating, then the partial rows
+    __shared__ double lds[3 * 32 * 65];
+    double (*Cs)[33] = reinterpret_cast<double (*)[33]>(lds);
+    double (*red)[32][65] = reinterpret_cast<double (*)[32][65]>(lds);
+    static_assert(128 * 33 <= 3 * 32 * 65, "C chunk fits the LDS area");
+    const int tid = threadIdx.x, l = tid & 63, g = tid >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 64 + l;
     double acc[32];
 #pragma unroll
     for (int j = 0; j < 32; ++j) acc[j] = 0.0;
@@ -1816,17 +1843,24 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
         }
         __syncthreads();
         if (r < n)
-            for (int i = 0; i < mi; ++i) {
+            for (int i = g; i < mi; i += 4) {
                 const double a = A[r + lda * (i0 + i)];
 #pragma unroll
                 for (int j = 0; j < 32; ++j) acc[j] = fma(a, Cs[i][j], acc[j]);
             }
     }
-    if (r < n) {
+    __syncthreads();   // every wave is done with the staged C
+    if (g > 0) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) red[g - 1][j][l] = acc[j];
+    }
+    __syncthreads();
+    if (g == 0 && r < n) {
 #pragma unroll
         for (int j = 0; j < 32; ++j)
             if (j < p) {
-                double v = alpha * acc[j];
+                const double sum = ((acc[j] + red[0][j][l]) + red[1][j][l]) + red[2][j][l];
+                double v = alpha * sum;
                 if (beta != 0.0) v += beta * out[r + ldo * j];
                 out[r + ldo * j] = v;
             }
@@ -1839,8 +1873,8 @@ void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, cons
     if (n == 0 || p == 0) return;
     for (int j0 = 0; j0 < p; j0 += 32) {
         const int pj = std::min(32, p - j0);
-        hipLaunchKernelGGL(tsgemm_nn_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                           m, pj, alpha, A, lda, C + ldc * j0, ldc, beta, out + ldo * j0, ldo);
+        hipLaunchKernelGGL(tsgemm_nn_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, n, m,
+                           pj, alpha, A, lda, C + ldc * j0, ldc, beta, out + ldo * j0, ldo);
     }
 }
 
