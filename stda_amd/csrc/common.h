@@ -156,11 +156,16 @@ void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A);
 
 // Tall-skinny FP64 products (kernels.hip).  C (m x p, ld m) = A^T B with
 // A (n x m), B (n x p), m <= TS_MMAX, p <= TS_PMAX, via per-slice partial
-// sums in part (tsgemm_tn_slices(n) * m * p doubles) reduced in slice order.
+// sums in part (tsgemm_tn_slices(n, m, p) * m * p doubles) reduced in slice order.
 constexpr int TS_MMAX = 128, TS_PMAX = 64;
-int tsgemm_tn_slices(int64_t n);
+int tsgemm_tn_slices(int64_t n, int m, int p);
 void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, int64_t lda,
                       const double *B, int64_t ldb, double *part, double *C);
+// the two launches of launch_tsgemm_tn: slice partials, then their ordered sum
+void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
+                              int64_t lda, const double *B, int64_t ldb, double *part);
+void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
+                             double *C);
 // out (n x p) = alpha A C + beta out with A (n x m), C (m x p); any m, p
 // (32 columns per launch)
 void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,

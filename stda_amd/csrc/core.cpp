@@ -316,6 +316,17 @@ struct stda_hip_ctx {
         hipEvent_t a, b;
     };
     std::vector<PendingTimer> pending;
+    std::vector<hipEvent_t> event_pool;   // recycled timing events
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        STDA_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
     void flush_timers() {
         for (auto &p : pending) {
             float ms = 0.f;
@@ -326,12 +337,15 @@ struct stda_hip_ctx {
                 st.launches += 1;
                 st.work += p.work;
             }
-            (void)hipEventDestroy(p.a);
-            (void)hipEventDestroy(p.b);
+            event_pool.push_back(p.a);
+            event_pool.push_back(p.b);
         }
         pending.clear();
     }
-    ~stda_hip_ctx() { flush_timers(); }
+    ~stda_hip_ctx() {
+        flush_timers();
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    }
 
     void sync() { STDA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
@@ -356,8 +370,8 @@ struct EventTimer {
     stda_hip_ctx *c;
     hipEvent_t a = nullptr, b = nullptr;
     explicit EventTimer(stda_hip_ctx *ctx) : c(ctx) {
-        STDA_HIP_CHECK(hipEventCreate(&a));
-        STDA_HIP_CHECK(hipEventCreate(&b));
+        a = c->take_event();
+        b = c->take_event();
         STDA_HIP_CHECK(hipEventRecord(a, c->stream));
     }
     void finish(const char *name, double work) {
@@ -367,8 +381,8 @@ struct EventTimer {
         if (c->pending.size() > 512) c->flush_timers();
     }
     ~EventTimer() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
+        if (a) c->event_pool.push_back(a);
+        if (b) c->event_pool.push_back(b);
     }
 };
 }  // namespace
@@ -389,8 +403,11 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
               const double *B, int64_t ldb, double *C) {
     if (m == 0 || p == 0) return;
     if (m <= TS_MMAX && p <= TS_PMAX) {   // the common shapes: hand-written tall-skinny kernel
-        c->tnbuf.ensure((size_t)tsgemm_tn_slices(n) * m * p);
-        launch_tsgemm_tn(c->stream, n, (int)m, (int)p, A, lda, B, ldb, c->tnbuf.get(), C);
+        c->tnbuf.ensure((size_t)tsgemm_tn_slices(n, (int)m, (int)p) * m * p);
+        EventTimer ev(c);
+        launch_tsgemm_tn_partial(c->stream, n, (int)m, (int)p, A, lda, B, ldb, c->tnbuf.get());
+        ev.finish("tsgemm_tn", 8.0 * (double)n * (double)(m + p));   // each input element once
+        launch_tsgemm_tn_reduce(c->stream, n, (int)m, (int)p, c->tnbuf.get(), C);
         return;
     }
     const int64_t KC_MIN = 2048;
@@ -415,10 +432,15 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
 void gemm_nn(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, double alpha, const double *A,
              int64_t lda, const double *C, int64_t ldc, double beta, double *out, int64_t ldo) {
     if (n == 0 || p == 0) return;
-    if (p <= 32)
+    if (p <= 32) {
+        EventTimer ev(c);
         launch_tsgemm_nn(c->stream, n, (int)m, (int)p, alpha, A, lda, C, ldc, beta, out, ldo);
-    else
+        // A and out once each (out twice when accumulated), C once
+        ev.finish("tsgemm_nn", 8.0 * ((double)n * (double)(m + (beta != 0.0 ? 2 * p : p)) +
+                                      (double)m * (double)p));
+    } else {
         dgemm(c->blas, false, false, n, p, m, alpha, A, lda, C, ldc, beta, out, ldo);
+    }
 }
 
 // column 2-norms of R (n x k) into norms (device), deterministic
@@ -1991,6 +2013,7 @@ struct stda_hip_batch {
     int64_t counts[3] = {0, 0, 0};       // molecules: batched, single path, failed
     double kernel_bytes = 0.0;           // algorithmic bytes of the batched operator products
     double matvec_ms = 0.0;              // HIP-event time of the batched operator products
+    double matvec_launches = 0.0;        // and their number of launches
     // summed step phases per molecule (us), 0, Jacobi sweeps, then the
     // workers' summed host time (us) in setup, explicit matrices, single path
     double phase_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -2173,7 +2196,7 @@ int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms,
     double t[3] = {0, 0, 0};
     if (std::string(name) == "batch_matvec") {
         t[0] = b->matvec_ms;
-        t[1] = 1;
+        t[1] = b->matvec_launches;
         t[2] = b->kernel_bytes;
     }
     for (auto *c : b->workers) {
@@ -2221,6 +2244,7 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         for (auto &x : b->phase_us) x = 0.0;
         b->kernel_bytes = 0.0;
         b->matvec_ms = 0.0;
+        b->matvec_launches = 0.0;
         for (auto *c : b->workers) {
             c->flush_timers();
             c->stats.clear();
@@ -2393,6 +2417,7 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                 STDA_HIP_CHECK(hipEventElapsedTime(&m0, e[0], e[1]));
                 STDA_HIP_CHECK(hipEventElapsedTime(&m1, e[1], e[2]));
                 b->matvec_ms += m0;
+                b->matvec_launches += 1.0;
                 b->timings[4] += m0 + m1;
                 for (auto &x : e) (void)hipEventDestroy(x);
             }

@@ -1704,162 +1704,241 @@ namespace stda {
 
 // ----------------------------------------------------------------------------
 // Tall-skinny FP64 products of the solvers (n rows, a few dozen columns).
-// Memory-bound: each input element is read once per launch.
+// Memory-bound: each input element is read once per launch (the B columns of
+// C = A^T B once per 32 columns of A).
 //
-// C (m x p) = A^T B, A (n x m), B (n x p): work-group s reduces the rows of
-// slice s (a few 32-row chunks) into part[s*m*p + i + m*j]; thread t owns up
-// to two 4 x 4 tiles of C; the chunks of A and B are staged through LDS with
-// coalesced loads.  Many small slices keep several work-groups per CU in
-// flight; tsgemm_reduce_kernel then sums the partials in slice order.
-constexpr int TS_ROWS = 32;
+// C (m x p) = A^T B, A (n x m), B (n x p): work-group (s, ib) reduces the
+// rows of slice s for the 32 columns ib*32.. of A into part[s*m*p + i + m*j].
+// Rows are staged through LDS in 64-row chunks with coalesced loads, the next
+// chunk's loads in flight (registers) while the current one is multiplied;
+// the four waves take the chunk's rows in four interleaved sets and lane l
+// owns C(2*(l%16) + {0,1}, l/16 + 4u), u < NJ; the wave partials are added in
+// wave order at the end.  tsgemm_reduce_kernel then sums the slices in slice
+// order.
+constexpr int TS_ROWS = 32, TS_CH = 64;
 
+template <int NJ>
 __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
                                                                 const double *__restrict__ A,
                                                                 int64_t lda,
                                                                 const double *__restrict__ B,
                                                                 int64_t ldb, int64_t rows,
                                                                 double *__restrict__ part) {
-    __shared__ double As[TS_ROWS][TS_MMAX + 1];
-    __shared__ double Bs[TS_ROWS][TS_PMAX + 1];
-    const int tid = threadIdx.x;
+    constexpr int PJ = 4 * NJ, QA = TS_CH * 32 / 256, QB = TS_CH * PJ / 256;
+    // LDS: the staged chunk (As[64][33], Bs[64][PJ+1]), then the wave partials
+    constexpr int LDS_STAGE = TS_CH * 33 + TS_CH * (PJ + 1), LDS_RED = 3 * 64 * 2 * NJ;
+    __shared__ double lds[LDS_STAGE > LDS_RED ? LDS_STAGE : LDS_RED];
+    double *As = lds, *Bs = lds + TS_CH * 33;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i0 = 32 * blockIdx.y, mi = min(32, m - i0);
+    const int ii = 2 * (l & 15), jg = l >> 4;
     const int64_t rb = (int64_t)blockIdx.x * rows, re = min(n, rb + rows);
-    const int nti = (m + 3) / 4, ntj = (p + 3) / 4, ntiles = nti * ntj;
-    double acc[2][16];
+    double pa[QA], pb[QB];
+    auto fetch = [&](int64_t r0) {
+        const int nr = (int)min((int64_t)TS_CH, re - r0);
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[t][e] = 0.0;
-    for (int64_t r0 = rb; r0 < re; r0 += TS_ROWS) {
-        const int nr = (int)min((int64_t)TS_ROWS, re - r0);
-        __syncthreads();
-        for (int e = tid; e < TS_ROWS * m; e += 256) {
-            const int rr = e % TS_ROWS, i = e / TS_ROWS;
-            As[rr][i] = rr < nr ? A[(r0 + rr) + lda * i] : 0.0;
+        for (int q = 0; q < QA; ++q) {
+            const int e = tid + 256 * q, rr = e % TS_CH, i = e / TS_CH;
+            pa[q] = (rr < nr && i < mi) ? A[(r0 + rr) + lda * (i0 + i)] : 0.0;
         }
-        for (int e = tid; e < TS_ROWS * p; e += 256) {
-            const int rr = e % TS_ROWS, j = e / TS_ROWS;
-            Bs[rr][j] = rr < nr ? B[(r0 + rr) + ldb * j] : 0.0;
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int e = tid + 256 * q, rr = e % TS_CH, j = e / TS_CH;
+            pb[q] = (rr < nr && j < p) ? B[(r0 + rr) + ldb * j] : 0.0;
+        }
+    };
+    double acc[2][NJ];
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) acc[0][u] = acc[1][u] = 0.0;
+    if (rb < re) fetch(rb);
+    for (int64_t r0 = rb; r0 < re; r0 += TS_CH) {
+        const int nr = (int)min((int64_t)TS_CH, re - r0);
+        __syncthreads();   // the previous chunk is consumed
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int e = tid + 256 * q;
+            As[(e % TS_CH) * 33 + e / TS_CH] = pa[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int e = tid + 256 * q;
+            Bs[(e % TS_CH) * (PJ + 1) + e / TS_CH] = pb[q];
         }
         __syncthreads();
+        if (r0 + TS_CH < re) fetch(r0 + TS_CH);
+        for (int rr = w; rr < nr; rr += 4) {
+            const double a0 = As[rr * 33 + ii], a1 = As[rr * 33 + ii + 1];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int tile = tid + 256 * t;
-            if (tile >= ntiles) continue;
-            const int i0 = 4 * (tile % nti), j0 = 4 * (tile / nti);
-            for (int rr = 0; rr < nr; ++rr) {
-                double a[4], b[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    a[u] = i0 + u < m ? As[rr][i0 + u] : 0.0;
-                    b[u] = j0 + u < p ? Bs[rr][j0 + u] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) acc[t][4 * u + v] = fma(a[u], b[v], acc[t][4 * u + v]);
+            for (int u = 0; u < NJ; ++u) {
+                const double b = Bs[rr * (PJ + 1) + jg + 4 * u];
+                acc[0][u] = fma(a0, b, acc[0][u]);
+                acc[1][u] = fma(a1, b, acc[1][u]);
             }
         }
     }
-    double *dst = part + (int64_t)blockIdx.x * m * p;
+    __syncthreads();   // every wave is done with the staged chunk
+    double *red = lds;  // red[(w-1)][v][l], v < 2*NJ
+    if (w > 0) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int tile = tid + 256 * t;
-        if (tile >= ntiles) continue;
-        const int i0 = 4 * (tile % nti), j0 = 4 * (tile / nti);
+        for (int u = 0; u < NJ; ++u) {
+            red[((w - 1) * 2 * NJ + u) * 64 + l] = acc[0][u];
+            red[((w - 1) * 2 * NJ + NJ + u) * 64 + l] = acc[1][u];
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        double *dst = part + (int64_t)blockIdx.x * m * p;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                if (i0 + u < m && j0 + v < p) dst[(i0 + u) + (int64_t)m * (j0 + v)] = acc[t][4 * u + v];
+            for (int u = 0; u < NJ; ++u) {
+                const int i = ii + h, j = jg + 4 * u, v = h * NJ + u;
+                const double s = ((acc[h][u] + red[(0 * 2 * NJ + v) * 64 + l]) +
+                                  red[(1 * 2 * NJ + v) * 64 + l]) + red[(2 * 2 * NJ + v) * 64 + l];
+                if (i < mi && j < p) dst[(i0 + i) + (int64_t)m * j] = s;
+            }
     }
 }
 
-// C[e] = sum_s part[s*len + e]: work-group of 4 x 64 threads per 64
-// elements; group g sums slices s = g, g+4, ... in order, then the four
-// partial sums are added in group order (deterministic).
+// C[e] = sum_s part[s*len + e]: a work-group of 16 groups x 16 elements;
+// group g sums slices s = g, g+16, ... in order (loads issued eight at a time),
+// then the 16 group sums are added in group order (deterministic).
 __global__ __launch_bounds__(256) void tsgemm_reduce_kernel(int64_t len, int ns,
                                                             const double *__restrict__ part,
                                                             double *__restrict__ C) {
-    __shared__ double red[4][64];
-    const int l = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int64_t e = (int64_t)blockIdx.x * 64 + l;
+    __shared__ double red[16][17];
+    const int l = threadIdx.x & 15, g = threadIdx.x >> 4;
+    const int64_t e = (int64_t)blockIdx.x * 16 + l;
     double s = 0.0;
-    if (e < len)
-        for (int q = g; q < ns; q += 4) s += part[(int64_t)q * len + e];
+    if (e < len) {
+        int q = g;
+        for (; q + 7 * 16 < ns; q += 8 * 16) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)(q + 16 * u) * len + e];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += x[u];
+        }
+        for (; q < ns; q += 16) s += part[(int64_t)q * len + e];
+    }
     red[g][l] = s;
     __syncthreads();
-    if (g == 0 && e < len) C[e] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+    if (g == 0 && e < len) {
+        double t = red[0][l];
+#pragma unroll
+        for (int h = 1; h < 16; ++h) t += red[h][l];
+        C[e] = t;
+    }
 }
 
-// slices of whole 32-row chunks, about 256 of them (at least one chunk each)
-static int64_t tsgemm_tn_rows(int64_t n) {
-    const int64_t r = (n + 255) / 256;
+// Rows per slice: whole 32-row chunks, aiming at about 1024 work-groups in
+// all (slices x 32-column blocks of A), with the partial sums kept below a
+// quarter of the input bytes.
+static int64_t tsgemm_tn_rows(int64_t n, int m, int p) {
+    const int64_t cb = (m + 31) / 32;
+    int64_t ns = std::max<int64_t>(1, 1024 / cb);
+    const int64_t cap = (n * (m + p)) / (4 * (int64_t)m * p);
+    ns = std::max<int64_t>(1, std::min(ns, cap));
+    const int64_t r = (n + ns - 1) / ns;
     return std::max<int64_t>(TS_ROWS, (r + TS_ROWS - 1) / TS_ROWS * TS_ROWS);
 }
 
-int tsgemm_tn_slices(int64_t n) {
-    const int64_t rows = tsgemm_tn_rows(n);
+int tsgemm_tn_slices(int64_t n, int m, int p) {
+    if (n <= 0 || m <= 0 || p <= 0) return 0;
+    const int64_t rows = tsgemm_tn_rows(n, m, p);
     return (int)((n + rows - 1) / rows);
+}
+
+void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
+                              int64_t lda, const double *B, int64_t ldb, double *part) {
+    const int ns = tsgemm_tn_slices(n, m, p);
+    if (ns == 0) return;
+    const int64_t rows = tsgemm_tn_rows(n, m, p);
+    const dim3 grid(ns, (m + 31) / 32);
+    if (p <= 8)
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+    else if (p <= 16)
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+    else if (p <= 32)
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<8>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+    else
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<16>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+}
+
+void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
+                             double *C) {
+    const int ns = tsgemm_tn_slices(n, m, p);
+    if (ns == 0) return;
+    const int64_t len = (int64_t)m * p;
+    hipLaunchKernelGGL(tsgemm_reduce_kernel, dim3((unsigned)((len + 15) / 16)), dim3(256), 0, s, len,
+                       ns, part, C);
 }
 
 void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, int64_t lda,
                       const double *B, int64_t ldb, double *part, double *C) {
-    if (m == 0 || p == 0) return;
-    const int64_t rows = tsgemm_tn_rows(n);
-    const int ns = (int)((n + rows - 1) / rows);
-    if (ns == 0) return;
-    hipLaunchKernelGGL(tsgemm_tn_partial_kernel, dim3(ns), dim3(256), 0, s, n, m, p, A, lda, B, ldb,
-                       rows, part);
-    const int64_t len = (int64_t)m * p;
-    hipLaunchKernelGGL(tsgemm_reduce_kernel, dim3((unsigned)((len + 63) / 64)), dim3(256), 0, s, len,
-                       ns, part, C);
+    launch_tsgemm_tn_partial(s, n, m, p, A, lda, B, ldb, part);
+    launch_tsgemm_tn_reduce(s, n, m, p, part, C);
 }
 
-// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32.  A
+// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= PJ <= 32
+// (PJ a multiple of 4).  A
 // work-group covers 64 rows; its four waves take the inner index in four
-// interleaved sets (i = g, g+4, ...) and their partial rows are added in
-// wave order through LDS.  C is staged through LDS in 128-row chunks.
-// beta == 0 never reads out.
+// interleaved sets (i = g, g+4, ...), each lane loading all of its A values of
+// a 128-index chunk before the multiply-adds (32 loads in flight), and the
+// partial rows are added in wave order through LDS.  C is staged through LDS
+// in 128-row chunks.  beta == 0 never reads out.
+template <int PJ>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A, int64_t lda,
                                                         const double *__restrict__ C, int64_t ldc,
                                                         double beta, double *out, int64_t ldo) {
-    // one LDS area: the staged C while accumulating, then the partial rows
-    __shared__ double lds[3 * 32 * 65];
-    double (*Cs)[33] = reinterpret_cast<double (*)[33]>(lds);
-    double (*red)[32][65] = reinterpret_cast<double (*)[32][65]>(lds);
-    static_assert(128 * 33 <= 3 * 32 * 65, "C chunk fits the LDS area");
-    const int tid = threadIdx.x, l = tid & 63, g = tid >> 6;
+    constexpr int LDS_C = 128 * PJ, LDS_RED = 3 * PJ * 64;
+    __shared__ __align__(16) double lds[LDS_C > LDS_RED ? LDS_C : LDS_RED];
+    const int tid = threadIdx.x, l = tid & 63;
+    const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t r = (int64_t)blockIdx.x * 64 + l;
-    double acc[32];
+    const bool live = r < n;
+    double acc[PJ];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) acc[j] = 0.0;
+    for (int j = 0; j < PJ; ++j) acc[j] = 0.0;
     for (int i0 = 0; i0 < m; i0 += 128) {
         const int mi = min(128, m - i0);
         __syncthreads();
-        for (int e = tid; e < 128 * 32; e += 256) {
-            const int i = e % 128, j = e / 128;
-            Cs[i][j] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
+        for (int e = tid; e < 128 * PJ; e += 256) {
+            const int i = e / PJ, j = e % PJ;
+            lds[e] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
+        }
+        double a[32];
+        const double *ap = A + (live ? r : 0) + lda * (int64_t)(i0 + g);
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            a[u] = (live && g + 4 * u < mi) ? *ap : 0.0;
+            ap += 4 * lda;
         }
         __syncthreads();
-        if (r < n)
-            for (int i = g; i < mi; i += 4) {
-                const double a = A[r + lda * (i0 + i)];
 #pragma unroll
-                for (int j = 0; j < 32; ++j) acc[j] = fma(a, Cs[i][j], acc[j]);
+        for (int u = 0; u < 32; ++u) {
+            const int i = g + 4 * u;
+            if (i < mi) {
+                const double *ci = lds + i * PJ;
+#pragma unroll
+                for (int j = 0; j < PJ; ++j) acc[j] = fma(a[u], ci[j], acc[j]);
             }
+        }
     }
     __syncthreads();   // every wave is done with the staged C
     if (g > 0) {
 #pragma unroll
-        for (int j = 0; j < 32; ++j) red[g - 1][j][l] = acc[j];
+        for (int j = 0; j < PJ; ++j) lds[((g - 1) * PJ + j) * 64 + l] = acc[j];
     }
     __syncthreads();
-    if (g == 0 && r < n) {
+    if (g == 0 && live) {
 #pragma unroll
-        for (int j = 0; j < 32; ++j)
+        for (int j = 0; j < PJ; ++j)
             if (j < p) {
-                const double sum = ((acc[j] + red[0][j][l]) + red[1][j][l]) + red[2][j][l];
+                const double sum = ((acc[j] + lds[(0 * PJ + j) * 64 + l]) + lds[(1 * PJ + j) * 64 + l]) +
+                                   lds[(2 * PJ + j) * 64 + l];
                 double v = alpha * sum;
                 if (beta != 0.0) v += beta * out[r + ldo * j];
                 out[r + ldo * j] = v;
@@ -1871,10 +1950,21 @@ void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, cons
                       int64_t lda, const double *C, int64_t ldc, double beta, double *out,
                       int64_t ldo) {
     if (n == 0 || p == 0) return;
+    const dim3 grid((unsigned)((n + 63) / 64));
     for (int j0 = 0; j0 < p; j0 += 32) {
         const int pj = std::min(32, p - j0);
-        hipLaunchKernelGGL(tsgemm_nn_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, n, m,
-                           pj, alpha, A, lda, C + ldc * j0, ldc, beta, out + ldo * j0, ldo);
+        const double *Cj = C + ldc * j0;
+        double *oj = out + ldo * j0;
+#define STDA_NN_CASE(W)                                                                        \
+    case W / 4:                                                                                \
+        hipLaunchKernelGGL(tsgemm_nn_kernel<W>, grid, dim3(256), 0, s, n, m, pj, alpha, A, lda, Cj, \
+                           ldc, beta, oj, ldo);                                                \
+        break;
+        switch ((pj + 3) / 4) {
+            STDA_NN_CASE(4) STDA_NN_CASE(8) STDA_NN_CASE(12) STDA_NN_CASE(16)
+            STDA_NN_CASE(20) STDA_NN_CASE(24) STDA_NN_CASE(28) STDA_NN_CASE(32)
+        }
+#undef STDA_NN_CASE
     }
 }
 
