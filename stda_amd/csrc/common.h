@@ -133,6 +133,11 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
                             const double *de, const unsigned char *rowmask, unsigned char rowbit,
                             const unsigned char *colmask, unsigned char colbit, double ak,
                             int nsplit, double *part, double *colsum);
+// Cholesky QR factor of a block Gram matrix S (nt <= 64), dependent columns
+// dropped: LinvT = L^-T on the kept columns, alive[j] = column j kept
+// (pivot > thr * ref[j * ref_stride], or > thr when ref is null)
+void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
+                     double thr, double *LinvT, int *alive);
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);

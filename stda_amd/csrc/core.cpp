@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <optional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -314,7 +315,17 @@ struct stda_hip_ctx {
         std::string name;
         double work;
         hipEvent_t a, b;
+        double weight;   // launches this sample stands for
     };
+    // tall-skinny products issued; one in eight is timed, picked by a hash of
+    // the count so that no fixed position of a periodic call sequence is missed
+    unsigned long long ts_count = 0;
+    bool sample_ts() {
+        unsigned long long z = (ts_count++) + 0x9e3779b97f4a7c15ULL;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return ((z ^ (z >> 31)) & 7) == 0;
+    }
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;   // recycled timing events
     hipEvent_t take_event() {
@@ -333,18 +344,35 @@ struct stda_hip_ctx {
             if (hipEventSynchronize(p.b) == hipSuccess &&
                 hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
                 auto &st = stat(p.name);
-                st.ms += ms;
-                st.launches += 1;
-                st.work += p.work;
+                st.ms += p.weight * ms;
+                st.launches += p.weight;
+                st.work += p.weight * p.work;
             }
             event_pool.push_back(p.a);
             event_pool.push_back(p.b);
         }
         pending.clear();
     }
+    // pinned host staging of the solvers' small per-iteration transfers (grown
+    // on demand; every region is reused only after a stream synchronisation
+    // that follows its last transfer)
+    double *pin = nullptr;
+    size_t pin_n = 0;
+    double *pinned(size_t n) {
+        if (n > pin_n) {
+            STDA_HIP_CHECK(hipStreamSynchronize(stream));
+            if (pin) (void)hipHostFree(pin);
+            pin = nullptr;
+            pin_n = 0;
+            STDA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&pin), n * sizeof(double), 0));
+            pin_n = n;
+        }
+        return pin;
+    }
     ~stda_hip_ctx() {
         flush_timers();
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+        if (pin) (void)hipHostFree(pin);
     }
 
     void sync() { STDA_HIP_CHECK(hipStreamSynchronize(stream)); }
@@ -374,9 +402,10 @@ struct EventTimer {
         b = c->take_event();
         STDA_HIP_CHECK(hipEventRecord(a, c->stream));
     }
-    void finish(const char *name, double work) {
+    // weight: the number of launches this one stands for (sampled timing)
+    void finish(const char *name, double work, double weight = 1.0) {
         STDA_HIP_CHECK(hipEventRecord(b, c->stream));
-        c->pending.push_back({name, work, a, b});
+        c->pending.push_back({name, work, a, b, weight});
         a = b = nullptr;
         if (c->pending.size() > 512) c->flush_timers();
     }
@@ -404,9 +433,10 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
     if (m == 0 || p == 0) return;
     if (m <= TS_MMAX && p <= TS_PMAX) {   // the common shapes: hand-written tall-skinny kernel
         c->tnbuf.ensure((size_t)tsgemm_tn_slices(n, (int)m, (int)p) * m * p);
-        EventTimer ev(c);
+        std::optional<EventTimer> ev;
+        if (c->sample_ts()) ev.emplace(c);
         launch_tsgemm_tn_partial(c->stream, n, (int)m, (int)p, A, lda, B, ldb, c->tnbuf.get());
-        ev.finish("tsgemm_tn", 8.0 * (double)n * (double)(m + p));   // each input element once
+        if (ev) ev->finish("tsgemm_tn", 8.0 * (double)n * (double)(m + p), 8.0);   // inputs once
         launch_tsgemm_tn_reduce(c->stream, n, (int)m, (int)p, c->tnbuf.get(), C);
         return;
     }
@@ -433,11 +463,13 @@ void gemm_nn(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, double alpha, con
              int64_t lda, const double *C, int64_t ldc, double beta, double *out, int64_t ldo) {
     if (n == 0 || p == 0) return;
     if (p <= 32) {
-        EventTimer ev(c);
+        std::optional<EventTimer> ev;
+        if (c->sample_ts()) ev.emplace(c);
         launch_tsgemm_nn(c->stream, n, (int)m, (int)p, alpha, A, lda, C, ldc, beta, out, ldo);
         // A and out once each (out twice when accumulated), C once
-        ev.finish("tsgemm_nn", 8.0 * ((double)n * (double)(m + (beta != 0.0 ? 2 * p : p)) +
-                                      (double)m * (double)p));
+        if (ev)
+            ev->finish("tsgemm_nn", 8.0 * ((double)n * (double)(m + (beta != 0.0 ? 2 * p : p)) +
+                                           (double)m * (double)p), 8.0);
     } else {
         dgemm(c->blas, false, false, n, p, m, alpha, A, lda, C, ldc, beta, out, ldo);
     }
@@ -732,6 +764,39 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     return kept;
 }
 
+// The same for a block of nt <= 64 new directions T = V + n*m, without a
+// host round trip: classical Gram-Schmidt against V twice, Cholesky QR,
+// one more projection against V, Cholesky QR again.  Dependent directions
+// become zero columns marked dead in alive[0:nt] (device): a direction is
+// dropped when less than 1e-7 of its norm is new.  The first product
+// [V T]^T T also gives the squared norms that threshold is measured against.
+// T2: n x nt scratch; work: (m + nt) * nt + m * nt + 2 * nt * nt doubles.
+void orthonormalize_dev(stda_hip_ctx *c, const double *V, int m, double *T, int nt, double *T2,
+                        double *work, int *alive) {
+    const int64_t n = c->nci;
+    if (nt == 0) return;
+    double *G = work;                        // (m + nt) x nt
+    double *C = G + (size_t)(m + nt) * nt;   // m x nt
+    double *S = C + (size_t)m * nt;          // nt x nt
+    double *Li = S + (size_t)nt * nt;        // nt x nt
+    dgemm_tn(c, m + nt, nt, n, V, n, T, n, G);   // T follows V's m columns
+    if (m > 0) {
+        gemm_nn(c, n, nt, m, -1.0, V, n, G, m + nt, 1.0, T, n);
+        dgemm_tn(c, m, nt, n, V, n, T, n, C);
+        gemm_nn(c, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
+    }
+    dgemm_tn(c, nt, nt, n, T, n, T, n, S);
+    launch_chol_inv(c->stream, nt, S, G + m, m + nt + 1, 1.0e-14, Li, alive);
+    gemm_nn(c, n, nt, nt, 1.0, T, n, Li, nt, 0.0, T2, n);
+    if (m > 0) {
+        dgemm_tn(c, m, nt, n, V, n, T2, n, C);
+        gemm_nn(c, n, nt, m, -1.0, V, n, C, m, 1.0, T2, n);
+    }
+    dgemm_tn(c, nt, nt, n, T2, n, T2, n, S);
+    launch_chol_inv(c->stream, nt, S, nullptr, 0, 1.0e-14, Li, alive);
+    gemm_nn(c, n, nt, nt, 1.0, T2, n, Li, nt, 0.0, T, n);
+}
+
 std::vector<int32_t> lowest_diag_rows(const std::vector<double> &d, int k) {
     std::vector<int32_t> idx(d.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -758,67 +823,115 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     int nguess = (int)std::min<int64_t>(n, std::max(k + 8, k + k / 2));
     int maxdim = (int)std::min<int64_t>(n, std::max(6 * k, k + 64));
     if (maxdim < nguess) maxdim = nguess;
+    // new blocks of up to 64 directions are orthonormalised on the device
+    // (dependent ones stay as zero columns, marked dead; the host learns which
+    // with the next Gram block, in the same synchronisation)
+    const bool devortho = k <= 64;
     DBuf<double> V((size_t)n * maxdim), AV((size_t)n * maxdim), R((size_t)n * maxdim);
-    DBuf<double> AX((size_t)n * k), th(k), nrm(k), tmp, Yd;
+    DBuf<double> AX((size_t)n * k), nrm(k), tmp, T2;
+    DBuf<double> Hc((size_t)maxdim * maxdim), yt((size_t)maxdim * k + k), thud(2 * (size_t)k);
+    DBuf<double> ow;
+    DBuf<int> alive_d(maxdim);
+    if (devortho) {
+        T2.resize((size_t)n * k);
+        ow.resize((size_t)(maxdim + k) * k + (size_t)maxdim * k + 2 * (size_t)k * k);
+    }
     X.resize((size_t)n * k);
+    // pinned staging: Gram block | Y and theta | theta and source column of
+    // the new directions | residual norms | alive flags of the new block
+    const size_t o_hc = 0, o_yt = o_hc + (size_t)maxdim * maxdim, o_thu = o_yt + (size_t)maxdim * k + k,
+                 o_rn = o_thu + 2 * (size_t)k, o_al = o_rn + (size_t)k;
+    double *pin = c->pinned(o_al + (size_t)maxdim / 2 + 1);
+    int *pin_alive = reinterpret_cast<int *>(pin + o_al);
+    std::vector<char> alive(maxdim, 0);
     int m = 0, mnew = nguess;
+    bool flags_pending = false;   // alive flags of the new block still on the device
     if (o.V0) {
         int nv0 = std::min(o.nv0, maxdim);
         STDA_HIP_CHECK(hipMemcpyAsync(V.get(), o.V0, (size_t)n * nv0 * 8, hipMemcpyDeviceToDevice, s));
-        mnew = orthonormalize(c, V.get(), 0, V.get(), nv0, tmp);
+        if (devortho && nv0 <= k) {
+            orthonormalize_dev(c, V.get(), 0, V.get(), nv0, T2.get(), ow.get(), alive_d.get());
+            mnew = nv0;
+            flags_pending = true;
+        } else {
+            mnew = orthonormalize(c, V.get(), 0, V.get(), nv0, tmp);
+            std::fill(alive.begin(), alive.begin() + mnew, 1);
+        }
     } else {
         std::vector<int32_t> rows = lowest_diag_rows(c->ed_h, nguess);
         DBuf<int32_t> rowsd;
         rowsd.upload(rows.data(), rows.size(), s);
         launch_set_unit(s, n, nguess, V.get(), rowsd.get());
+        std::fill(alive.begin(), alive.begin() + nguess, 1);
     }
-    std::vector<double> H((size_t)maxdim * maxdim, 0.0);
+    std::vector<double> H((size_t)maxdim * maxdim, 0.0), Hs, w;
+    std::vector<int> idx;
     const int maxiter = o.maxiter;
-    DBuf<double> Hc((size_t)maxdim * maxdim), thud(2 * (size_t)maxdim);
-    Yd.resize((size_t)maxdim * k);
     for (int it = 0; it < maxiter; ++it) {
         {
             HostSpan hs(c, "dav_matvec");
             apply_ops(c, OP_A, -1, mnew, V.get() + n * m, n, AV.get() + n * m, n, nullptr, 0, o.fp32);
         }
         HostSpan hs_it(c, "dav_iter_other");
-        // H(0:m+mnew, m:m+mnew) = V^T AV
-        int mt = m + mnew;
+        // H(0:m+mnew, m:m+mnew) = V^T AV, with the alive flags of the new block
+        const int mt = m + mnew;
         dgemm_tn(c, mt, mnew, n, V.get(), n, AV.get() + n * m, n, Hc.get());
-        std::vector<double> hc((size_t)mt * mnew);
-        Hc.download(hc.data(), hc.size(), s);
+        STDA_HIP_CHECK(hipMemcpyAsync(pin + o_hc, Hc.get(), (size_t)mt * mnew * 8,
+                                      hipMemcpyDeviceToHost, s));
+        if (flags_pending)
+            STDA_HIP_CHECK(hipMemcpyAsync(pin_alive, alive_d.get() + m, (size_t)mnew * sizeof(int),
+                                          hipMemcpyDeviceToHost, s));
         c->sync();
+        if (flags_pending) {
+            int nnew = 0;
+            for (int j = 0; j < mnew; ++j) nnew += (alive[m + j] = pin_alive[j] ? 1 : 0);
+            flags_pending = false;
+            // no new direction survived: converged to working precision (X and
+            // theta are those of the previous iteration)
+            if (nnew == 0 && it > 0) return;
+        }
         for (int j = 0; j < mnew; ++j)
             for (int i = 0; i < mt; ++i) {
-                double v = hc[i + (size_t)mt * j];
+                double v = pin[o_hc + i + (size_t)mt * j];
                 H[i + (size_t)maxdim * (m + j)] = v;
                 H[(m + j) + (size_t)maxdim * i] = v;
             }
         m = mt;
-        // Rayleigh-Ritz on the host (FP64)
-        std::vector<double> Hs((size_t)m * m), w;
-        for (int j = 0; j < m; ++j)
-            for (int i = 0; i < m; ++i) Hs[i + (size_t)m * j] = 0.5 * (H[i + (size_t)maxdim * j] + H[j + (size_t)maxdim * i]);
+        // Rayleigh-Ritz on the host (FP64) over the live columns
+        idx.clear();
+        for (int i = 0; i < m; ++i)
+            if (alive[i]) idx.push_back(i);
+        const int na = (int)idx.size();
+        Hs.assign((size_t)na * na, 0.0);
+        for (int b = 0; b < na; ++b)
+            for (int a2 = 0; a2 < na; ++a2)
+                Hs[a2 + (size_t)na * b] = 0.5 * (H[idx[a2] + (size_t)maxdim * idx[b]] +
+                                                 H[idx[b] + (size_t)maxdim * idx[a2]]);
         {
             HostSpan hs(c, "dav_rr_eig");
-            host_syev(m, Hs, w);
+            host_syev(na, Hs, w);
         }
-        int kk = std::min(k, m);
-        STDA_HIP_CHECK(hipMemcpyAsync(Yd.get(), Hs.data(), (size_t)m * kk * 8, hipMemcpyHostToDevice, s));
+        const int kk = std::min(k, na);
+        double *Y = pin + o_yt;
+        std::fill(Y, Y + (size_t)m * kk, 0.0);
+        for (int j = 0; j < kk; ++j)
+            for (int a2 = 0; a2 < na; ++a2) Y[idx[a2] + (size_t)m * j] = Hs[a2 + (size_t)na * j];
+        for (int j = 0; j < kk; ++j) Y[(size_t)m * kk + j] = w[j];
+        STDA_HIP_CHECK(hipMemcpyAsync(yt.get(), Y, ((size_t)m * kk + kk) * 8, hipMemcpyHostToDevice, s));
         theta.assign(w.begin(), w.begin() + kk);
-        STDA_HIP_CHECK(hipMemcpyAsync(th.get(), theta.data(), kk * 8, hipMemcpyHostToDevice, s));
-        gemm_nn(c, n, kk, m, 1.0, V.get(), n, Yd.get(), m, 0.0, X.get(), n);
-        gemm_nn(c, n, kk, m, 1.0, AV.get(), n, Yd.get(), m, 0.0, AX.get(), n);
-        launch_residual(s, n, kk, AX.get(), X.get(), th.get(), R.get());
+        const double *th = yt.get() + (size_t)m * kk;
+        gemm_nn(c, n, kk, m, 1.0, V.get(), n, yt.get(), m, 0.0, X.get(), n);
+        gemm_nn(c, n, kk, m, 1.0, AV.get(), n, yt.get(), m, 0.0, AX.get(), n);
+        launch_residual(s, n, kk, AX.get(), X.get(), th, R.get());
         col_norms(c, n, kk, R.get(), nrm.get());
-        std::vector<double> rn(kk);
-        nrm.download(rn.data(), kk, s);
+        double *rn = pin + o_rn;
+        STDA_HIP_CHECK(hipMemcpyAsync(rn, nrm.get(), (size_t)kk * 8, hipMemcpyDeviceToHost, s));
         c->sync();
         std::vector<int> un;
         for (int j = 0; j < kk; ++j)
             if (!(rn[j] < tol)) un.push_back(j);
         if (un.empty() && kk == k) return;
-        if (m == n) return;   // full space: exact
+        if (na == n) return;   // full space: exact
         if (it + 1 == maxiter && o.allow_unconverged) return;
         // preconditioned unconverged residuals become the new directions
         int nun = (int)un.size();
@@ -829,23 +942,33 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
                                           hipMemcpyDeviceToDevice, s));
             std::fill(H.begin(), H.end(), 0.0);
             for (int j = 0; j < kk; ++j) H[j + (size_t)maxdim * j] = theta[j];
+            std::fill(alive.begin(), alive.end(), 0);
+            std::fill(alive.begin(), alive.begin() + kk, 1);
             m = kk;
         }
         int room = std::min(nun, maxdim - m);
         // theta of each new direction and the residual column it comes from
-        std::vector<double> thu(2 * (size_t)room);
+        double *thu = pin + o_thu;
         for (int q = 0; q < room; ++q) {
             thu[q] = theta[un[q]];
             thu[room + q] = (double)un[q];
         }
-        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu.data(), thu.size() * 8, hipMemcpyHostToDevice, s));
+        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu, 2 * (size_t)room * 8, hipMemcpyHostToDevice, s));
         launch_precondition_gather(s, n, room, R.get(), thud.get() + room, c->ed.get(), thud.get(),
                                    V.get() + n * m);
         {
             HostSpan hs(c, "dav_ortho");
-            mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+            if (devortho) {
+                orthonormalize_dev(c, V.get(), m, V.get() + n * m, room, T2.get(), ow.get(),
+                                   alive_d.get() + m);
+                mnew = room;
+                flags_pending = true;
+            } else {
+                mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
+                std::fill(alive.begin() + m, alive.begin() + m + mnew, 1);
+                if (mnew == 0) return;   // no new directions: converged to machine precision
+            }
         }
-        if (mnew == 0) return;   // no new directions: converged to machine precision
     }
     throw Error(STDA_HIP_ERR_NOT_CONVERGED, "Davidson did not converge");
 }

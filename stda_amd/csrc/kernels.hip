@@ -1545,6 +1545,70 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
     norms[c] = sqrt(s);
 }
 
+// Cholesky factor of the Gram matrix S = T^T T (nt x nt, ld nt, nt <= 64) of a
+// block of directions, dependent ones dropped, and the transposed inverse of
+// the factor on the kept columns: LinvT (nt x nt, ld nt), so that T LinvT is
+// orthonormal on the kept columns and zero on the dropped ones.  Column j is
+// dropped when its pivot (the squared norm of its part orthogonal to the kept
+// columns before it) is at most thr * ref[j * ref_stride] (thr when ref is
+// null); alive[j] = 1 for the kept columns.  One wave: lane i owns row i of
+// the factor, then column i of its inverse (forward substitution).
+__global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__restrict__ S,
+                                                      const double *__restrict__ ref,
+                                                      int ref_stride, double thr,
+                                                      double *__restrict__ LinvT,
+                                                      int *__restrict__ alive) {
+    __shared__ double L[64 * 65];
+    __shared__ double Z[64 * 65];
+    __shared__ int keep[64];
+    const int i = threadIdx.x;
+    for (int e = i; e < nt * nt; e += 64) L[(e % nt) * 65 + e / nt] = S[e];
+    __syncthreads();
+    for (int j = 0; j < nt; ++j) {
+        if (i == j) {
+            double d = L[j * 65 + j];
+            for (int p = 0; p < j; ++p) d -= L[j * 65 + p] * L[j * 65 + p];
+            const double lim = thr * (ref ? ref[(int64_t)j * ref_stride] : 1.0);
+            const bool kp = d > lim && d > 0.0;
+            keep[j] = kp ? 1 : 0;
+            L[j * 65 + j] = kp ? sqrt(d) : 0.0;
+        }
+        __syncthreads();
+        if (i > j && i < nt) {
+            double x = 0.0;
+            if (keep[j]) {
+                x = L[i * 65 + j];
+                for (int p = 0; p < j; ++p) x -= L[i * 65 + p] * L[j * 65 + p];
+                x /= L[j * 65 + j];
+            }
+            L[i * 65 + j] = x;
+        }
+        __syncthreads();
+    }
+    if (i < nt) {
+        const int q = i;
+        for (int r = 0; r < nt; ++r) {
+            double z = 0.0;
+            if (r >= q && keep[r] && keep[q]) {
+                z = r == q ? 1.0 : 0.0;
+                for (int p = q; p < r; ++p) z -= L[r * 65 + p] * Z[p * 65 + q];
+                z /= L[r * 65 + r];
+            }
+            Z[r * 65 + q] = z;
+        }
+    }
+    __syncthreads();
+    for (int e = i; e < nt * nt; e += 64) LinvT[e] = Z[(e / nt) * 65 + e % nt];
+    if (i < nt) alive[i] = keep[i];
+}
+
+void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
+                     double thr, double *LinvT, int *alive) {
+    if (nt <= 0) return;
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, nt, S, ref, ref_stride, thr, LinvT,
+                       alive);
+}
+
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms) {
     if (k == 0) return;
