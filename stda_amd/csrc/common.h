@@ -134,13 +134,20 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
                             const unsigned char *colmask, unsigned char colbit, double ak,
                             int nsplit, double *part, double *colsum);
 // Cholesky QR factor of a block Gram matrix S (nt <= 64), dependent columns
-// dropped: LinvT = L^-T on the kept columns, alive[j] = column j kept
+// dropped: LinvT = L^-T on the kept columns, alive[j] = 1.0 if column j is kept
 // (pivot > thr * ref[j * ref_stride], or > thr when ref is null)
 void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
-                     double thr, double *LinvT, int *alive);
+                     double thr, double *LinvT, double *alive);
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);
+// R = AX - X diag(theta) and the column norms of R (same slicing)
+void launch_residual_norms(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
+                           const double *theta, double *R, double *part, int nsl, double *norms);
+// T(:, q) = R(:, c) / (theta_c - diag) for the q-th column c with norms[c] >= tol, q < room
+void launch_precondition_compact(hipStream_t s, int64_t n, int k, const double *R,
+                                 const double *diag, const double *theta, const double *norms,
+                                 double tol, int room, double *T);
 // FP32 operator kernels (mixed precision)
 void launch_scatter_pairs_f32(hipStream_t s, const double *v, int64_t ldv, const int32_t *pair,
                               int64_t n, int nvec, float *full, int64_t ldfull);
@@ -176,6 +183,11 @@ void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const doubl
 void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
                       int64_t lda, const double *C, int64_t ldc, double beta, double *out,
                       int64_t ldo);
+// two such products with the same C in one launch: out0 = alpha A0 C + beta
+// out0, out1 = alpha A1 C + beta out1 (A1 null: the first only)
+void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A0,
+                       const double *A1, int64_t lda, const double *C, int64_t ldc, double beta,
+                       double *out0, double *out1, int64_t ldo);
 
 // --- batched Davidson (batch.hip): molecules in lock step ------------------------------
 constexpr int BATCH_MD = 64;                        // subspace limit

@@ -475,11 +475,36 @@ void gemm_nn(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, double alpha, con
     }
 }
 
+// out0 = A0 C and out1 = A1 C (beta 0) for two tall-skinny products sharing C
+void gemm_nn2(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, const double *A0, const double *A1,
+              int64_t lda, const double *C, int64_t ldc, double *out0, double *out1, int64_t ldo) {
+    if (n == 0 || p == 0) return;
+    if (p <= 32) {
+        std::optional<EventTimer> ev;
+        if (c->sample_ts()) ev.emplace(c);
+        launch_tsgemm_nn2(c->stream, n, (int)m, (int)p, 1.0, A0, A1, lda, C, ldc, 0.0, out0, out1, ldo);
+        if (ev)
+            ev->finish("tsgemm_nn", 2.0 * 8.0 * ((double)n * (double)(m + p) + (double)m * (double)p),
+                       8.0);
+    } else {
+        dgemm(c->blas, false, false, n, p, m, 1.0, A0, lda, C, ldc, 0.0, out0, ldo);
+        dgemm(c->blas, false, false, n, p, m, 1.0, A1, lda, C, ldc, 0.0, out1, ldo);
+    }
+}
+
 // column 2-norms of R (n x k) into norms (device), deterministic
 void col_norms(stda_hip_ctx *c, int64_t n, int k, const double *R, double *norms) {
     int nsl = (int)std::min<int64_t>(256, std::max<int64_t>(1, n / 8192));
     c->normbuf.ensure((size_t)nsl * std::max(k, 1));
     launch_col_norms_split(c->stream, n, k, R, c->normbuf.get(), nsl, norms);
+}
+
+// R = AX - X diag(theta) and its column norms (device), deterministic
+void residual_norms(stda_hip_ctx *c, int64_t n, int k, const double *AX, const double *X,
+                    const double *theta, double *R, double *norms) {
+    int nsl = (int)std::min<int64_t>(256, std::max<int64_t>(1, n / 8192));
+    c->normbuf.ensure((size_t)nsl * std::max(k, 1));
+    launch_residual_norms(c->stream, n, k, AX, X, theta, R, c->normbuf.get(), nsl, norms);
 }
 
 void ensure_work(stda_hip_ctx *c, int nb) {
@@ -767,12 +792,12 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
 // The same for a block of nt <= 64 new directions T = V + n*m, without a
 // host round trip: classical Gram-Schmidt against V twice, Cholesky QR,
 // one more projection against V, Cholesky QR again.  Dependent directions
-// become zero columns marked dead in alive[0:nt] (device): a direction is
+// become zero columns marked dead (alive[j] = 0.0, device): a direction is
 // dropped when less than 1e-7 of its norm is new.  The first product
 // [V T]^T T also gives the squared norms that threshold is measured against.
 // T2: n x nt scratch; work: (m + nt) * nt + m * nt + 2 * nt * nt doubles.
 void orthonormalize_dev(stda_hip_ctx *c, const double *V, int m, double *T, int nt, double *T2,
-                        double *work, int *alive) {
+                        double *work, double *alive) {
     const int64_t n = c->nci;
     if (nt == 0) return;
     double *G = work;                        // (m + nt) x nt
@@ -829,20 +854,18 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     const bool devortho = k <= 64;
     DBuf<double> V((size_t)n * maxdim), AV((size_t)n * maxdim), R((size_t)n * maxdim);
     DBuf<double> AX((size_t)n * k), nrm(k), tmp, T2;
-    DBuf<double> Hc((size_t)maxdim * maxdim), yt((size_t)maxdim * k + k), thud(2 * (size_t)k);
+    // Gram block, followed by the alive flags of the new block (one download)
+    DBuf<double> Hc((size_t)maxdim * maxdim + maxdim), yt((size_t)maxdim * k + k);
     DBuf<double> ow;
-    DBuf<int> alive_d(maxdim);
     if (devortho) {
         T2.resize((size_t)n * k);
         ow.resize((size_t)(maxdim + k) * k + (size_t)maxdim * k + 2 * (size_t)k * k);
     }
     X.resize((size_t)n * k);
-    // pinned staging: Gram block | Y and theta | theta and source column of
-    // the new directions | residual norms | alive flags of the new block
-    const size_t o_hc = 0, o_yt = o_hc + (size_t)maxdim * maxdim, o_thu = o_yt + (size_t)maxdim * k + k,
-                 o_rn = o_thu + 2 * (size_t)k, o_al = o_rn + (size_t)k;
-    double *pin = c->pinned(o_al + (size_t)maxdim / 2 + 1);
-    int *pin_alive = reinterpret_cast<int *>(pin + o_al);
+    // pinned staging: Gram block and alive flags | Y and theta | residual norms
+    const size_t o_hc = 0, o_yt = o_hc + (size_t)maxdim * maxdim + maxdim,
+                 o_rn = o_yt + (size_t)maxdim * k + k;
+    double *pin = c->pinned(o_rn + (size_t)k);
     std::vector<char> alive(maxdim, 0);
     int m = 0, mnew = nguess;
     bool flags_pending = false;   // alive flags of the new block still on the device
@@ -850,7 +873,8 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         int nv0 = std::min(o.nv0, maxdim);
         STDA_HIP_CHECK(hipMemcpyAsync(V.get(), o.V0, (size_t)n * nv0 * 8, hipMemcpyDeviceToDevice, s));
         if (devortho && nv0 <= k) {
-            orthonormalize_dev(c, V.get(), 0, V.get(), nv0, T2.get(), ow.get(), alive_d.get());
+            orthonormalize_dev(c, V.get(), 0, V.get(), nv0, T2.get(), ow.get(),
+                               Hc.get() + (size_t)nv0 * nv0);
             mnew = nv0;
             flags_pending = true;
         } else {
@@ -876,15 +900,14 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         // H(0:m+mnew, m:m+mnew) = V^T AV, with the alive flags of the new block
         const int mt = m + mnew;
         dgemm_tn(c, mt, mnew, n, V.get(), n, AV.get() + n * m, n, Hc.get());
-        STDA_HIP_CHECK(hipMemcpyAsync(pin + o_hc, Hc.get(), (size_t)mt * mnew * 8,
+        STDA_HIP_CHECK(hipMemcpyAsync(pin + o_hc, Hc.get(),
+                                      ((size_t)mt * mnew + (flags_pending ? mnew : 0)) * 8,
                                       hipMemcpyDeviceToHost, s));
-        if (flags_pending)
-            STDA_HIP_CHECK(hipMemcpyAsync(pin_alive, alive_d.get() + m, (size_t)mnew * sizeof(int),
-                                          hipMemcpyDeviceToHost, s));
         c->sync();
         if (flags_pending) {
             int nnew = 0;
-            for (int j = 0; j < mnew; ++j) nnew += (alive[m + j] = pin_alive[j] ? 1 : 0);
+            const double *fl = pin + o_hc + (size_t)mt * mnew;
+            for (int j = 0; j < mnew; ++j) nnew += (alive[m + j] = fl[j] != 0.0 ? 1 : 0);
             flags_pending = false;
             // no new direction survived: converged to working precision (X and
             // theta are those of the previous iteration)
@@ -920,10 +943,8 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
         STDA_HIP_CHECK(hipMemcpyAsync(yt.get(), Y, ((size_t)m * kk + kk) * 8, hipMemcpyHostToDevice, s));
         theta.assign(w.begin(), w.begin() + kk);
         const double *th = yt.get() + (size_t)m * kk;
-        gemm_nn(c, n, kk, m, 1.0, V.get(), n, yt.get(), m, 0.0, X.get(), n);
-        gemm_nn(c, n, kk, m, 1.0, AV.get(), n, yt.get(), m, 0.0, AX.get(), n);
-        launch_residual(s, n, kk, AX.get(), X.get(), th, R.get());
-        col_norms(c, n, kk, R.get(), nrm.get());
+        gemm_nn2(c, n, kk, m, V.get(), AV.get(), n, yt.get(), m, X.get(), AX.get(), n);
+        residual_norms(c, n, kk, AX.get(), X.get(), th, R.get(), nrm.get());
         double *rn = pin + o_rn;
         STDA_HIP_CHECK(hipMemcpyAsync(rn, nrm.get(), (size_t)kk * 8, hipMemcpyDeviceToHost, s));
         c->sync();
@@ -947,20 +968,15 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
             m = kk;
         }
         int room = std::min(nun, maxdim - m);
-        // theta of each new direction and the residual column it comes from
-        double *thu = pin + o_thu;
-        for (int q = 0; q < room; ++q) {
-            thu[q] = theta[un[q]];
-            thu[room + q] = (double)un[q];
-        }
-        STDA_HIP_CHECK(hipMemcpyAsync(thud.get(), thu, 2 * (size_t)room * 8, hipMemcpyHostToDevice, s));
-        launch_precondition_gather(s, n, room, R.get(), thud.get() + room, c->ed.get(), thud.get(),
-                                   V.get() + n * m);
+        // the first room unconverged residuals, preconditioned, in column
+        // order (the device applies the same test to the same norms)
+        launch_precondition_compact(s, n, kk, R.get(), c->ed.get(), th, nrm.get(), tol, room,
+                                    V.get() + n * m);
         {
             HostSpan hs(c, "dav_ortho");
             if (devortho) {
                 orthonormalize_dev(c, V.get(), m, V.get() + n * m, room, T2.get(), ow.get(),
-                                   alive_d.get() + m);
+                                   Hc.get() + (size_t)(m + room) * room);
                 mnew = room;
                 flags_pending = true;
             } else {

@@ -433,6 +433,31 @@ __global__ void precondition_gather_kernel(int64_t n, const double *R, const dou
     T[i + n * c] = R[i + n * src] / d;
 }
 
+// New directions from the unconverged residuals (norm not below tol), packed
+// in column order: T(:, q) = R(:, c) / (theta_c - diag) for the q-th
+// unconverged column c, q < room.
+__global__ void precondition_compact_kernel(int64_t n, const double *R, const double *diag,
+                                            const double *theta, const double *norms, double tol,
+                                            int room, double *T) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (norms[c] < tol) return;
+    int q = 0;
+    for (int c2 = 0; c2 < c; ++c2) q += !(norms[c2] < tol);
+    if (q >= room || i >= n) return;
+    double d = theta[c] - diag[i];
+    if (fabs(d) < 1.0e-4) d = d < 0.0 ? -1.0e-4 : 1.0e-4;
+    T[i + n * q] = R[i + n * c] / d;
+}
+
+void launch_precondition_compact(hipStream_t s, int64_t n, int k, const double *R,
+                                 const double *diag, const double *theta, const double *norms,
+                                 double tol, int room, double *T) {
+    if (k == 0 || room == 0) return;
+    hipLaunchKernelGGL(precondition_compact_kernel, dim3((unsigned)((n + 255) / 256), k), dim3(256),
+                       0, s, n, R, diag, theta, norms, tol, room, T);
+}
+
 void launch_precondition_gather(hipStream_t s, int64_t n, int k, const double *R, const double *col,
                                 const double *diag, const double *theta, double *T) {
     if (k == 0) return;
@@ -1537,6 +1562,27 @@ __global__ void col_sumsq_partial_kernel(int64_t n, int64_t slice, const double 
     if (threadIdx.x == 0) part[sl + (int64_t)gridDim.x * c] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
+// R = AX - X diag(theta) with the slice partial sums of squares of its
+// columns (col_norm_final_kernel completes the norms)
+__global__ void residual_sumsq_partial_kernel(int64_t n, int64_t slice, const double *AX,
+                                              const double *X, const double *theta, double *R,
+                                              double *part) {
+    __shared__ double ws[4];
+    const int c = blockIdx.y, sl = blockIdx.x;
+    const int64_t i0 = (int64_t)sl * slice, i1 = min(n, i0 + slice);
+    const double th = theta[c];
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const double v = AX[i + n * c] - th * X[i + n * c];
+        R[i + n * c] = v;
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[sl + (int64_t)gridDim.x * c] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
 __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double *norms) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= k) return;
@@ -1551,13 +1597,14 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
 // orthonormal on the kept columns and zero on the dropped ones.  Column j is
 // dropped when its pivot (the squared norm of its part orthogonal to the kept
 // columns before it) is at most thr * ref[j * ref_stride] (thr when ref is
-// null); alive[j] = 1 for the kept columns.  One wave: lane i owns row i of
+// null); alive[j] = 1.0 for the kept columns, 0.0 for the dropped ones.  One
+// wave: lane i owns row i of
 // the factor, then column i of its inverse (forward substitution).
 __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__restrict__ S,
                                                       const double *__restrict__ ref,
                                                       int ref_stride, double thr,
                                                       double *__restrict__ LinvT,
-                                                      int *__restrict__ alive) {
+                                                      double *__restrict__ alive) {
     __shared__ double L[64 * 65];
     __shared__ double Z[64 * 65];
     __shared__ int keep[64];
@@ -1599,11 +1646,11 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__re
     }
     __syncthreads();
     for (int e = i; e < nt * nt; e += 64) LinvT[e] = Z[(e / nt) * 65 + e % nt];
-    if (i < nt) alive[i] = keep[i];
+    if (i < nt) alive[i] = keep[i] ? 1.0 : 0.0;
 }
 
 void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
-                     double thr, double *LinvT, int *alive) {
+                     double thr, double *LinvT, double *alive) {
     if (nt <= 0) return;
     hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, nt, S, ref, ref_stride, thr, LinvT,
                        alive);
@@ -1614,6 +1661,15 @@ void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, do
     if (k == 0) return;
     int64_t slice = (n + nsl - 1) / nsl;
     hipLaunchKernelGGL(col_sumsq_partial_kernel, dim3(nsl, k), dim3(256), 0, s, n, slice, R, part);
+    hipLaunchKernelGGL(col_norm_final_kernel, dim3((k + 63) / 64), dim3(64), 0, s, nsl, k, part, norms);
+}
+
+void launch_residual_norms(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
+                           const double *theta, double *R, double *part, int nsl, double *norms) {
+    if (k == 0) return;
+    int64_t slice = (n + nsl - 1) / nsl;
+    hipLaunchKernelGGL(residual_sumsq_partial_kernel, dim3(nsl, k), dim3(256), 0, s, n, slice, AX, X,
+                       theta, R, part);
     hipLaunchKernelGGL(col_norm_final_kernel, dim3((k + 63) / 64), dim3(64), 0, s, nsl, k, part, norms);
 }
 
@@ -1954,9 +2010,14 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
 // in 128-row chunks.  beta == 0 never reads out.
 template <int PJ>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
-                                                        const double *__restrict__ A, int64_t lda,
+                                                        const double *__restrict__ A0,
+                                                        const double *__restrict__ A1, int64_t lda,
                                                         const double *__restrict__ C, int64_t ldc,
-                                                        double beta, double *out, int64_t ldo) {
+                                                        double beta, double *out0, double *out1,
+                                                        int64_t ldo) {
+    // blockIdx.y selects the product: (A0, out0) or (A1, out1), same C
+    const double *__restrict__ A = blockIdx.y ? A1 : A0;
+    double *out = blockIdx.y ? out1 : out0;
     constexpr int LDS_C = 128 * PJ, LDS_RED = 3 * PJ * 64;
     __shared__ __align__(16) double lds[LDS_C > LDS_RED ? LDS_C : LDS_RED];
     const int tid = threadIdx.x, l = tid & 63;
@@ -2010,19 +2071,20 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
     }
 }
 
-void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
-                      int64_t lda, const double *C, int64_t ldc, double beta, double *out,
-                      int64_t ldo) {
+void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A0,
+                       const double *A1, int64_t lda, const double *C, int64_t ldc, double beta,
+                       double *out0, double *out1, int64_t ldo) {
     if (n == 0 || p == 0) return;
-    const dim3 grid((unsigned)((n + 63) / 64));
+    const dim3 grid((unsigned)((n + 63) / 64), A1 ? 2 : 1);
     for (int j0 = 0; j0 < p; j0 += 32) {
         const int pj = std::min(32, p - j0);
         const double *Cj = C + ldc * j0;
-        double *oj = out + ldo * j0;
+        double *oj0 = out0 + ldo * j0;
+        double *oj1 = out1 ? out1 + ldo * j0 : oj0;
 #define STDA_NN_CASE(W)                                                                        \
     case W / 4:                                                                                \
-        hipLaunchKernelGGL(tsgemm_nn_kernel<W>, grid, dim3(256), 0, s, n, m, pj, alpha, A, lda, Cj, \
-                           ldc, beta, oj, ldo);                                                \
+        hipLaunchKernelGGL(tsgemm_nn_kernel<W>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,      \
+                           A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);                    \
         break;
         switch ((pj + 3) / 4) {
             STDA_NN_CASE(4) STDA_NN_CASE(8) STDA_NN_CASE(12) STDA_NN_CASE(16)
@@ -2030,6 +2092,12 @@ void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, cons
         }
 #undef STDA_NN_CASE
     }
+}
+
+void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
+                      int64_t lda, const double *C, int64_t ldc, double beta, double *out,
+                      int64_t ldo) {
+    launch_tsgemm_nn2(s, n, m, p, alpha, A, nullptr, lda, C, ldc, beta, out, nullptr, ldo);
 }
 
 }  // namespace stda
