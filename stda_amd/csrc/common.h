@@ -135,9 +135,10 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
                             int nsplit, double *part, double *colsum);
 // Cholesky QR factor of a block Gram matrix S (nt <= 64), dependent columns
 // dropped: LinvT = L^-T on the kept columns, alive[j] = 1.0 if column j is kept
-// (pivot > thr * ref[j * ref_stride], or > thr when ref is null)
-void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
-                     double thr, double *LinvT, double *alive);
+// (pivot > thr * ref[j * ref_stride], or > thr when ref is null).  S =
+// G[m:m+nt, :] - G[0:m, :]^T G[0:m, :] with G ((m + nt) x nt, ld ldg).
+void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
+                     int ref_stride, double thr, double *LinvT, double *alive);
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);
@@ -169,13 +170,17 @@ void launch_set_diag(hipStream_t s, int64_t n, const double *d, double *A);
 // Tall-skinny FP64 products (kernels.hip).  C (m x p, ld m) = A^T B with
 // A (n x m), B (n x p), m <= TS_MMAX, p <= TS_PMAX, via per-slice partial
 // sums in part (tsgemm_tn_slices(n, m, p) * m * p doubles) reduced in slice order.
-constexpr int TS_MMAX = 128, TS_PMAX = 64;
+constexpr int TS_MMAX = 256, TS_PMAX = 64;
 int tsgemm_tn_slices(int64_t n, int m, int p);
 void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, int64_t lda,
                       const double *B, int64_t ldb, double *part, double *C);
 // the two launches of launch_tsgemm_tn: slice partials, then their ordered sum
 void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
                               int64_t lda, const double *B, int64_t ldb, double *part);
+// slice partials of [A1 A2]^T B (m1 + m2 columns, same lda)
+void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A1, int m2,
+                               const double *A2, int p, int64_t lda, const double *B, int64_t ldb,
+                               double *part);
 void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
                              double *C);
 // out (n x p) = alpha A C + beta out with A (n x m), C (m x p); any m, p

@@ -457,6 +457,26 @@ void dgemm_tn(stda_hip_ctx *c, int64_t m, int64_t p, int64_t n, const double *A,
     launch_sum_slices(c->stream, m * p, (int)slots, c->tnbuf.get(), C);
 }
 
+// C ((m1 + m2) x p, ld m1 + m2) = [A1 A2]^T B for two column blocks with the
+// same leading dimension (one launch pair on the hand-written path)
+void dgemm_tn2(stda_hip_ctx *c, int64_t m1, const double *A1, int64_t m2, const double *A2,
+               int64_t p, int64_t n, int64_t lda, const double *B, int64_t ldb, double *C) {
+    const int64_t m = m1 + m2;
+    if (m == 0 || p == 0) return;
+    if (m <= TS_MMAX && p <= TS_PMAX) {
+        c->tnbuf.ensure((size_t)tsgemm_tn_slices(n, (int)m, (int)p) * m * p);
+        std::optional<EventTimer> ev;
+        if (c->sample_ts()) ev.emplace(c);
+        launch_tsgemm_tn2_partial(c->stream, n, (int)m1, A1, (int)m2, A2, (int)p, lda, B, ldb,
+                                  c->tnbuf.get());
+        if (ev) ev->finish("tsgemm_tn", 8.0 * (double)n * (double)(m + p), 8.0);
+        launch_tsgemm_tn_reduce(c->stream, n, (int)m, (int)p, c->tnbuf.get(), C);
+        return;
+    }
+    dgemm(c->blas, true, false, m1, p, n, 1.0, A1, lda, B, ldb, 0.0, C, m);
+    dgemm(c->blas, true, false, m2, p, n, 1.0, A2, lda, B, ldb, 0.0, C + m1, m);
+}
+
 // out (n x p) = alpha A C + beta out (A n x m, C m x p) for the tall-skinny
 // products of the solvers: hand-written kernel for p <= 32, rocBLAS beyond
 void gemm_nn(stda_hip_ctx *c, int64_t n, int64_t p, int64_t m, double alpha, const double *A,
@@ -789,36 +809,35 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     return kept;
 }
 
-// The same for a block of nt <= 64 new directions T = V + n*m, without a
-// host round trip: classical Gram-Schmidt against V twice, Cholesky QR,
-// one more projection against V, Cholesky QR again.  Dependent directions
+// The same for a block of nt <= 64 new directions T (n x nt), without a host
+// round trip: classical Gram-Schmidt against V twice, Cholesky QR, one more
+// projection against V, Cholesky QR again.  Each projection's product [V T]^T T
+// also carries the Gram matrix of the projected block (T^T T - C^T C), so the
+// Cholesky factors need no product of their own.  Dependent directions
 // become zero columns marked dead (alive[j] = 0.0, device): a direction is
-// dropped when less than 1e-7 of its norm is new.  The first product
-// [V T]^T T also gives the squared norms that threshold is measured against.
-// T2: n x nt scratch; work: (m + nt) * nt + m * nt + 2 * nt * nt doubles.
+// dropped when less than 1e-7 of its norm is new.  T2: n x nt scratch; work:
+// 3 (m + nt) nt + nt^2 doubles.
 void orthonormalize_dev(stda_hip_ctx *c, const double *V, int m, double *T, int nt, double *T2,
                         double *work, double *alive) {
     const int64_t n = c->nci;
     if (nt == 0) return;
-    double *G = work;                        // (m + nt) x nt
-    double *C = G + (size_t)(m + nt) * nt;   // m x nt
-    double *S = C + (size_t)m * nt;          // nt x nt
-    double *Li = S + (size_t)nt * nt;        // nt x nt
-    dgemm_tn(c, m + nt, nt, n, V, n, T, n, G);   // T follows V's m columns
+    const int ld = m + nt;
+    double *G1 = work, *G2 = G1 + (size_t)ld * nt, *G3 = G2 + (size_t)ld * nt;
+    double *Li = G3 + (size_t)ld * nt;
+    hipStream_t s = c->stream;
+    dgemm_tn2(c, m, V, nt, T, nt, n, n, T, n, G1);   // [V T]^T T: projections and squared norms
+    const double *Gs = G1;
     if (m > 0) {
-        gemm_nn(c, n, nt, m, -1.0, V, n, G, m + nt, 1.0, T, n);
-        dgemm_tn(c, m, nt, n, V, n, T, n, C);
-        gemm_nn(c, n, nt, m, -1.0, V, n, C, m, 1.0, T, n);
+        gemm_nn(c, n, nt, m, -1.0, V, n, G1, ld, 1.0, T, n);
+        dgemm_tn2(c, m, V, nt, T, nt, n, n, T, n, G2);
+        gemm_nn(c, n, nt, m, -1.0, V, n, G2, ld, 1.0, T, n);
+        Gs = G2;
     }
-    dgemm_tn(c, nt, nt, n, T, n, T, n, S);
-    launch_chol_inv(c->stream, nt, S, G + m, m + nt + 1, 1.0e-14, Li, alive);
+    launch_chol_inv(s, nt, Gs, m, ld, G1 + m, ld + 1, 1.0e-14, Li, alive);
     gemm_nn(c, n, nt, nt, 1.0, T, n, Li, nt, 0.0, T2, n);
-    if (m > 0) {
-        dgemm_tn(c, m, nt, n, V, n, T2, n, C);
-        gemm_nn(c, n, nt, m, -1.0, V, n, C, m, 1.0, T2, n);
-    }
-    dgemm_tn(c, nt, nt, n, T2, n, T2, n, S);
-    launch_chol_inv(c->stream, nt, S, nullptr, 0, 1.0e-14, Li, alive);
+    dgemm_tn2(c, m, V, nt, T2, nt, n, n, T2, n, G3);
+    if (m > 0) gemm_nn(c, n, nt, m, -1.0, V, n, G3, ld, 1.0, T2, n);
+    launch_chol_inv(s, nt, G3, m, ld, nullptr, 0, 1.0e-14, Li, alive);
     gemm_nn(c, n, nt, nt, 1.0, T2, n, Li, nt, 0.0, T, n);
 }
 
@@ -859,7 +878,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     DBuf<double> ow;
     if (devortho) {
         T2.resize((size_t)n * k);
-        ow.resize((size_t)(maxdim + k) * k + (size_t)maxdim * k + 2 * (size_t)k * k);
+        ow.resize(3 * (size_t)(maxdim + k) * k + (size_t)k * k);
     }
     X.resize((size_t)n * k);
     // pinned staging: Gram block and alive flags | Y and theta | residual norms

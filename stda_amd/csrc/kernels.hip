@@ -1591,7 +1591,7 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
     norms[c] = sqrt(s);
 }
 
-// Cholesky factor of the Gram matrix S = T^T T (nt x nt, ld nt, nt <= 64) of a
+// Cholesky factor of the Gram matrix S = T^T T (nt x nt, nt <= 64) of a
 // block of directions, dependent ones dropped, and the transposed inverse of
 // the factor on the kept columns: LinvT (nt x nt, ld nt), so that T LinvT is
 // orthonormal on the kept columns and zero on the dropped ones.  Column j is
@@ -1599,9 +1599,13 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
 // columns before it) is at most thr * ref[j * ref_stride] (thr when ref is
 // null); alive[j] = 1.0 for the kept columns, 0.0 for the dropped ones.  One
 // wave: lane i owns row i of
-// the factor, then column i of its inverse (forward substitution).
-__global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__restrict__ S,
-                                                      const double *__restrict__ ref,
+// the factor, then column i of its inverse (forward substitution).  S is
+// given through G ((m + nt) x nt, ld ldg) = [V T]^T T of directions T before
+// their projection T - V G[0:m, :] against orthonormal V: S = G[m:m+nt, :] -
+// G[0:m, :]^T G[0:m, :] is the Gram matrix of the projected block (m = 0:
+// G = T^T T).
+__global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__restrict__ G, int m,
+                                                      int ldg, const double *__restrict__ ref,
                                                       int ref_stride, double thr,
                                                       double *__restrict__ LinvT,
                                                       double *__restrict__ alive) {
@@ -1609,7 +1613,12 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__re
     __shared__ double Z[64 * 65];
     __shared__ int keep[64];
     const int i = threadIdx.x;
-    for (int e = i; e < nt * nt; e += 64) L[(e % nt) * 65 + e / nt] = S[e];
+    for (int e = i; e < nt * nt; e += 64) {
+        const int r = e % nt, c = e / nt;
+        double v = G[(m + r) + (int64_t)ldg * c];
+        for (int q = 0; q < m; ++q) v -= G[q + (int64_t)ldg * r] * G[q + (int64_t)ldg * c];
+        L[r * 65 + c] = v;
+    }
     __syncthreads();
     for (int j = 0; j < nt; ++j) {
         if (i == j) {
@@ -1649,11 +1658,11 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__re
     if (i < nt) alive[i] = keep[i] ? 1.0 : 0.0;
 }
 
-void launch_chol_inv(hipStream_t s, int nt, const double *S, const double *ref, int ref_stride,
-                     double thr, double *LinvT, double *alive) {
+void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
+                     int ref_stride, double thr, double *LinvT, double *alive) {
     if (nt <= 0) return;
-    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, nt, S, ref, ref_stride, thr, LinvT,
-                       alive);
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, nt, G, m, ldg, ref, ref_stride, thr,
+                       LinvT, alive);
 }
 
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
@@ -1840,10 +1849,13 @@ constexpr int TS_ROWS = 32, TS_CH = 64;
 template <int NJ>
 __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
                                                                 const double *__restrict__ A,
+                                                                int m1,
+                                                                const double *__restrict__ A2,
                                                                 int64_t lda,
                                                                 const double *__restrict__ B,
                                                                 int64_t ldb, int64_t rows,
                                                                 double *__restrict__ part) {
+    // the m columns of A: A's first m1 columns, then those of A2 (same lda)
     constexpr int PJ = 4 * NJ, QA = TS_CH * 32 / 256, QB = TS_CH * PJ / 256;
     // LDS: the staged chunk (As[64][33], Bs[64][PJ+1]), then the wave partials
     constexpr int LDS_STAGE = TS_CH * 33 + TS_CH * (PJ + 1), LDS_RED = 3 * 64 * 2 * NJ;
@@ -1859,8 +1871,9 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
         const int nr = (int)min((int64_t)TS_CH, re - r0);
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
-            const int e = tid + 256 * q, rr = e % TS_CH, i = e / TS_CH;
-            pa[q] = (rr < nr && i < mi) ? A[(r0 + rr) + lda * (i0 + i)] : 0.0;
+            const int e = tid + 256 * q, rr = e % TS_CH, i = e / TS_CH, col = i0 + i;
+            const double *ac = col < m1 ? A + lda * col : A2 + lda * (col - m1);
+            pa[q] = (rr < nr && i < mi) ? ac[r0 + rr] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
@@ -1970,20 +1983,28 @@ int tsgemm_tn_slices(int64_t n, int m, int p) {
     return (int)((n + rows - 1) / rows);
 }
 
-void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
-                              int64_t lda, const double *B, int64_t ldb, double *part) {
+void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A1, int m2,
+                               const double *A2, int p, int64_t lda, const double *B, int64_t ldb,
+                               double *part) {
+    const int m = m1 + m2;
     const int ns = tsgemm_tn_slices(n, m, p);
     if (ns == 0) return;
     const int64_t rows = tsgemm_tn_rows(n, m, p);
     const dim3 grid(ns, (m + 31) / 32);
+    if (!A2) A2 = A1;
     if (p <= 8)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
     else if (p <= 16)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
     else if (p <= 32)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<8>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<8>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
     else
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<16>, grid, dim3(256), 0, s, n, m, p, A, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<16>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+}
+
+void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
+                              int64_t lda, const double *B, int64_t ldb, double *part) {
+    launch_tsgemm_tn2_partial(s, n, m, A, 0, nullptr, p, lda, B, ldb, part);
 }
 
 void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
