@@ -36,30 +36,36 @@ class DBuf {
     ~DBuf() { release(); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
+    DBuf(DBuf &&o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_), s_(o.s_) {
+        o.p_ = nullptr;
+        o.n_ = o.cap_ = 0;
+    }
     DBuf &operator=(DBuf &&o) noexcept {
         if (this != &o) {
             release();
-            p_ = o.p_; n_ = o.n_; s_ = o.s_;
-            o.p_ = nullptr; o.n_ = 0;
+            p_ = o.p_; n_ = o.n_; cap_ = o.cap_; s_ = o.s_;
+            o.p_ = nullptr; o.n_ = o.cap_ = 0;
         }
         return *this;
     }
+    // n elements; the allocation is kept when it is large enough (buffers of
+    // a context are reused from one molecule to the next without allocating)
     void resize(size_t n) {
-        if (n == n_) return;
-        release();
-        if (n) {
-            s_ = alloc_stream();
-            STDA_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p_), n * sizeof(T), s_));
+        if (n <= cap_) {
+            n_ = n;
+            return;
         }
-        n_ = n;
+        release();
+        s_ = alloc_stream();
+        STDA_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p_), n * sizeof(T), s_));
+        n_ = cap_ = n;
     }
     void ensure(size_t n) { if (n > n_) resize(n); }
     // freed in the order of the stream it was allocated on
     void release() {
         if (p_) (void)hipFreeAsync(p_, s_);
         p_ = nullptr;
-        n_ = 0;
+        n_ = cap_ = 0;
     }
     T *get() const { return p_; }
     size_t size() const { return n_; }
@@ -76,7 +82,7 @@ class DBuf {
 
   private:
     T *p_ = nullptr;
-    size_t n_ = 0;
+    size_t n_ = 0, cap_ = 0;
     hipStream_t s_ = nullptr;
 };
 
