@@ -223,6 +223,7 @@ struct stda_hip_ctx {
         ~Owner() {
             if (blas) rocblas_destroy_handle(blas);
             if (stream) {
+                block_cache_drain(stream);
                 (void)hipStreamSynchronize(stream);
                 (void)hipStreamDestroy(stream);
             }
@@ -2323,6 +2324,7 @@ int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers) {
 int stda_hip_batch_destroy(stda_hip_batch *b) {
     if (!b) return STDA_HIP_OK;
     if (b->dstream) {
+        block_cache_drain(b->dstream);
         (void)hipStreamSynchronize(b->dstream);
         (void)hipStreamDestroy(b->dstream);
     }

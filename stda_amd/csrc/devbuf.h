@@ -4,11 +4,93 @@
 #include <rocblas/rocblas.h>
 
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "common.h"
 
 namespace stda {
+
+// Released device blocks kept per stream for reuse on the same stream
+// (stream order makes the reuse safe), so that solving molecule after
+// molecule does not go through the device allocator, whose calls serialise
+// the host threads of concurrent contexts.  At most kCacheBytes per stream
+// are kept (larger releases go back to the allocator); a stream's blocks
+// are returned with block_cache_drain before the stream is destroyed.
+class BlockCache {
+  public:
+    static constexpr size_t kCacheBytes = size_t(2) << 30;
+    // a kept block of at least `bytes` and at most 4x that (plus 1 MiB), or null
+    void *take(size_t bytes, hipStream_t s, size_t *got) {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.find(s);
+        if (it == free_.end()) return nullptr;
+        auto &v = it->second.blocks;
+        size_t best = v.size();
+        for (size_t q = 0; q < v.size(); ++q)
+            if (v[q].second >= bytes && v[q].second <= 4 * bytes + (size_t(1) << 20) &&
+                (best == v.size() || v[q].second < v[best].second))
+                best = q;
+        if (best == v.size()) return nullptr;
+        void *p = v[best].first;
+        *got = v[best].second;
+        it->second.bytes -= *got;
+        v[best] = v.back();
+        v.pop_back();
+        return p;
+    }
+    void give(void *p, size_t bytes, hipStream_t s) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto &e = free_[s];
+            if (e.bytes + bytes <= kCacheBytes) {
+                e.blocks.emplace_back(p, bytes);
+                e.bytes += bytes;
+                return;
+            }
+        }
+        (void)hipFreeAsync(p, s);
+    }
+    void drain(hipStream_t s) {
+        std::vector<std::pair<void *, size_t>> v;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.find(s);
+            if (it == free_.end()) return;
+            v.swap(it->second.blocks);
+            free_.erase(it);
+        }
+        for (auto &b : v) (void)hipFreeAsync(b.first, s);
+    }
+    // every stream's blocks (when the allocator runs out of memory)
+    void drain_all() {
+        std::unordered_map<hipStream_t, Entry> all;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            all.swap(free_);
+        }
+        for (auto &kv : all)
+            for (auto &b : kv.second.blocks) (void)hipFreeAsync(b.first, kv.first);
+    }
+
+  private:
+    struct Entry {
+        std::vector<std::pair<void *, size_t>> blocks;
+        size_t bytes = 0;
+    };
+    std::mutex mu_;
+    std::unordered_map<hipStream_t, Entry> free_;
+};
+
+// one per process, never destroyed (no device calls at static destruction)
+inline BlockCache &block_cache() {
+    static BlockCache *c = new BlockCache();
+    return *c;
+}
+
+inline void block_cache_drain(hipStream_t s) { block_cache().drain(s); }
 
 // Stream that orders the device allocations of the calling thread.  Every
 // entry point sets it to its context's stream (AllocScope), so that buffers
@@ -57,13 +139,24 @@ class DBuf {
         }
         release();
         s_ = alloc_stream();
-        STDA_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p_), n * sizeof(T), s_));
-        n_ = cap_ = n;
+        size_t got = 0;
+        void *q = block_cache().take(n * sizeof(T), s_, &got);
+        if (!q) {
+            got = n * sizeof(T);
+            if (hipMallocAsync(&q, got, s_) != hipSuccess) {
+                (void)hipGetLastError();
+                block_cache().drain_all();
+                STDA_HIP_CHECK(hipMallocAsync(&q, got, s_));
+            }
+        }
+        p_ = static_cast<T *>(q);
+        n_ = n;
+        cap_ = got / sizeof(T);
     }
     void ensure(size_t n) { if (n > n_) resize(n); }
-    // freed in the order of the stream it was allocated on
+    // kept for reuse on the stream it was allocated on (or freed in its order)
     void release() {
-        if (p_) (void)hipFreeAsync(p_, s_);
+        if (p_) block_cache().give(p_, cap_ * sizeof(T), s_);
         p_ = nullptr;
         n_ = cap_ = 0;
     }
