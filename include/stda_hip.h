@@ -184,7 +184,7 @@ int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
  * GPU_MAX_HW_QUEUES=32, so that the streams overlap), and a driver stream.
  * stda_hip_batch_solve_tda runs the setup pipeline of every molecule (as
  * stda_hip_setup) on the workers concurrently.  Molecules with at most
- * nmax_batched CSFs (default 8192) and nroots <= 24 are solved in lock step
+ * nmax_batched CSFs (default 1024) and nroots <= 24 are solved in lock step
  * by the calling thread in mixed precision (as stda_hip_set_precision(ctx,
  * 1): FP32 operator products, FP64 subspace numerics, residual norms 1e-6,
  * one FP64 Rayleigh-Ritz refinement): per iteration one launch for the

@@ -413,7 +413,7 @@ class StdaHipBatch:
     def last_error(self) -> str:
         return self._L.stda_hip_batch_last_error(self._b).decode()
 
-    def configure(self, nmax_batched: int = 8192, poll: int = 4):
+    def configure(self, nmax_batched: int = 1024, poll: int = 4):
         """CSF limit of the lock-step solver and iterations between polls."""
         st = self._L.stda_hip_batch_configure(self._b, nmax_batched, poll)
         if st != 0:

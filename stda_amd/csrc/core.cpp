@@ -2164,7 +2164,7 @@ struct stda_hip_batch {
     std::vector<stda_hip_ctx *> workers;
     hipStream_t dstream = nullptr;       // the driver's stream
     std::string last_error;
-    int64_t nmax_batched = 8192;         // CSF limit of the batched solver
+    int64_t nmax_batched = 1024;         // CSF limit of the batched solver
     int group = 4;                       // iterations between polls of the state table
     // ms: worker phase (wall), driver loop (wall), property tables, total,
     // summed HIP-event time of the batched launches (matvec + step)
