@@ -1597,29 +1597,57 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
 // orthonormal on the kept columns and zero on the dropped ones.  Column j is
 // dropped when its pivot (the squared norm of its part orthogonal to the kept
 // columns before it) is at most thr * ref[j * ref_stride] (thr when ref is
-// null); alive[j] = 1.0 for the kept columns, 0.0 for the dropped ones.  One
-// wave: lane i owns row i of
-// the factor, then column i of its inverse (forward substitution).  S is
+// null); alive[j] = 1.0 for the kept columns, 0.0 for the dropped ones.
+// Thread i owns row i of the factor, then column i of its inverse (forward
+// substitution).  S is
 // given through G ((m + nt) x nt, ld ldg) = [V T]^T T of directions T before
 // their projection T - V G[0:m, :] against orthonormal V: S = G[m:m+nt, :] -
 // G[0:m, :]^T G[0:m, :] is the Gram matrix of the projected block (m = 0:
 // G = T^T T).
-__global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__restrict__ G, int m,
-                                                      int ldg, const double *__restrict__ ref,
-                                                      int ref_stride, double thr,
-                                                      double *__restrict__ LinvT,
-                                                      double *__restrict__ alive) {
+__global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__restrict__ G, int m,
+                                                       int ldg, const double *__restrict__ ref,
+                                                       int ref_stride, double thr,
+                                                       double *__restrict__ LinvT,
+                                                       double *__restrict__ alive) {
     __shared__ double L[64 * 65];
-    __shared__ double Z[64 * 65];
+    __shared__ double Z[64 * 65];   // first the staged rows of G, then the inverse factor
     __shared__ int keep[64];
-    const int i = threadIdx.x;
-    for (int e = i; e < nt * nt; e += 64) {
-        const int r = e % nt, c = e / nt;
-        double v = G[(m + r) + (int64_t)ldg * c];
-        for (int q = 0; q < m; ++q) v -= G[q + (int64_t)ldg * r] * G[q + (int64_t)ldg * c];
-        L[r * 65 + c] = v;
+    const int tid = threadIdx.x;
+    // S = G[m:m+nt, :] - G[0:m, :]^T G[0:m, :]: the correction over 32-row
+    // chunks of G staged in LDS, each thread owning up to 16 elements of S
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    for (int q0 = 0; q0 < m; q0 += 32) {
+        const int mq = min(32, m - q0);
+        __syncthreads();
+        for (int e = tid; e < 32 * nt; e += 256) {
+            const int q = e % 32, c = e / 32;
+            Z[q * 65 + c] = q < mq ? G[(q0 + q) + (int64_t)ldg * c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = tid + 256 * u;
+            if (e < nt * nt) {
+                const int r = e % nt, c = e / nt;
+                double s = 0.0;
+                for (int q = 0; q < mq; ++q) s += Z[q * 65 + r] * Z[q * 65 + c];
+                acc[u] -= s;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * u;
+        if (e < nt * nt) {
+            const int r = e % nt, c = e / nt;
+            L[r * 65 + c] = G[(m + r) + (int64_t)ldg * c] + acc[u];
+        }
     }
     __syncthreads();
+    // factor: thread j finishes column j's pivot, threads i > j its column
+    const int i = tid;
     for (int j = 0; j < nt; ++j) {
         if (i == j) {
             double d = L[j * 65 + j];
@@ -1641,6 +1669,7 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__re
         }
         __syncthreads();
     }
+    // inverse on the kept indices: thread q solves column q
     if (i < nt) {
         const int q = i;
         for (int r = 0; r < nt; ++r) {
@@ -1654,15 +1683,15 @@ __global__ __launch_bounds__(64) void chol_inv_kernel(int nt, const double *__re
         }
     }
     __syncthreads();
-    for (int e = i; e < nt * nt; e += 64) LinvT[e] = Z[(e / nt) * 65 + e % nt];
-    if (i < nt) alive[i] = keep[i] ? 1.0 : 0.0;
+    for (int e = tid; e < nt * nt; e += 256) LinvT[e] = Z[(e / nt) * 65 + e % nt];
+    if (tid < nt) alive[tid] = keep[tid] ? 1.0 : 0.0;
 }
 
 void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
                      int ref_stride, double thr, double *LinvT, double *alive) {
     if (nt <= 0) return;
-    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, s, nt, G, m, ldg, ref, ref_stride, thr,
-                       LinvT, alive);
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(256), 0, s, nt, G, m, ldg, ref, ref_stride,
+                       thr, LinvT, alive);
 }
 
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
