@@ -173,8 +173,13 @@ def run_single(args):
 
 
 # --------------------------------------------------------------------------- batched
-def plain_fp64(cis, thr, ax, nroots):
-    """Sequential plain FP64 solves, one molecule per solve (the drop-in path)."""
+PLAIN_KERNELS = ("tsgemm_tn", "tsgemm_nn")
+
+
+def plain_fp64(cis, thr, ax, nroots, kst=None):
+    """Sequential plain FP64 solves, one molecule per solve (the drop-in path).
+    kst (dict): HIP-event statistics of the tall-skinny products summed over
+    the molecules (one stream, so an event pair brackets the kernel alone)."""
     from stda_amd.core import StdaHip, StdaHipError
     out, lat = [], []
     with StdaHip(0) as h:
@@ -186,11 +191,31 @@ def plain_fp64(cis, thr, ax, nroots):
                 h.setup(ci, thr, ax)
                 s = h.solve_tda(nroots, velcorr=True, tol=1e-8, vectors=False)
                 out.append((s.omega.copy(), s.props.copy()))
+                if kst is not None:
+                    for k in PLAIN_KERNELS:
+                        st = h.kernel_stats(k)
+                        acc = kst.setdefault(k, {"ms": 0.0, "launches": 0.0, "work": 0.0})
+                        for f in acc:
+                            acc[f] += st[f]
             except StdaHipError as e:
                 out.append(e.status)
             lat.append(time.perf_counter() - ts)
         wall = time.perf_counter() - t0
     return out, wall, np.array(lat)
+
+
+def plain_rates(kst):
+    """Per-launch HIP-event time and achieved HBM rate of the tall-skinny
+    products in the sequential plain pass (algorithmic bytes 8 n (m + p))."""
+    out = {}
+    for k, st in kst.items():
+        if st["ms"] <= 0:
+            continue
+        gbs = st["work"] / (st["ms"] * 1e-3) / 1e9
+        out[k] = {"ms": round(st["ms"], 2), "launches": int(st["launches"]),
+                  "avg_launch_ms": round(st["ms"] / max(st["launches"], 1.0), 4),
+                  "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def compare(res, ref):
@@ -241,7 +266,8 @@ def run_mixed(args):
                             workers=gen_workers())
     log(f"# workload mixed{n}: {n} molecules, natoms {min(c.ncent for c in cis)}-"
         f"{max(c.ncent for c in cis)} (inputs {time.perf_counter() - t0:.1f}s)")
-    ref, plain_wall, plain_lat = plain_fp64(cis, thr, ax, nroots)
+    plain_kst = {}
+    ref, plain_wall, plain_lat = plain_fp64(cis, thr, ax, nroots, plain_kst)
     nok = sum(1 for r in ref if not isinstance(r, int))
     log(f"# plain FP64 sequential: {plain_wall:.3f}s, {nok}/{n} molecules with CSFs")
     with StdaHipBatch(0, workers=args.workers) as b:
@@ -300,6 +326,7 @@ def run_mixed(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 operator products + f64 Davidson numerics and refinement",
         "plain_fp64": {"value": plain, "unit": "solves/s", "wall_s": plain_wall,
+                       "kernels": plain_rates(plain_kst),
                        "p50_ms": float(np.percentile(plain_lat * 1e3, 50)),
                        "p99_ms": float(np.percentile(plain_lat * 1e3, 99))},
         "speedup_vs_plain_fp64": value / plain,
