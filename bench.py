@@ -365,7 +365,8 @@ def run_serve(args):
             arr = poisson_arrivals(n, rate, seed=int(rate))
             jobs = [Job(jid=i, kind=str(kinds[i]), ci=cis[i], arrival=float(arr[i]),
                         omegas=(0.0, 0.05, 0.1, 0.15)) for i in range(n)]
-            st = srv.run(jobs, realtime=True)
+            st = srv.run(jobs, realtime=True,
+                         progress=lambda d, t: log(f"#   rate {rate}: {d}/{t} jobs done"))
             sweep.append({"rate": rate, "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,
                           "p99_ms": st.p99_ms, "ok": st.ok, "n": st.n, "batches": st.batches,
                           "mean_batch": st.mean_batch})

@@ -271,6 +271,7 @@ struct stda_hip_ctx {
     int nb_max32 = 0;
     bool have32 = false;
     DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
+    DBuf<double> tn2buf;  // the two blocks of dgemm_tn2 on its split-K path
     DBuf<double> normbuf; // slice partials of column norms
     int nb_max = 0;
     int nb_chunk = 0;     // vectors per operator block (from the free memory at first use)
@@ -474,8 +475,22 @@ void dgemm_tn2(stda_hip_ctx *c, int64_t m1, const double *A1, int64_t m2, const 
         launch_tsgemm_tn_reduce(c->stream, n, (int)m, (int)p, c->tnbuf.get(), C);
         return;
     }
-    dgemm(c->blas, true, false, m1, p, n, 1.0, A1, lda, B, ldb, 0.0, C, m);
-    dgemm(c->blas, true, false, m2, p, n, 1.0, A2, lda, B, ldb, 0.0, C + m1, m);
+    // larger shapes: the split-K path of dgemm_tn (a plain GEMM would run the
+    // whole reduction over n in a few work-groups)
+    if (A2 == A1 + lda * m1) {   // one contiguous block of columns
+        dgemm_tn(c, m, p, n, A1, lda, B, ldb, C);
+        return;
+    }
+    c->tn2buf.ensure((size_t)m * p);
+    double *t = c->tn2buf.get();
+    dgemm_tn(c, m1, p, n, A1, lda, B, ldb, t);
+    dgemm_tn(c, m2, p, n, A2, lda, B, ldb, t + m1 * p);
+    if (m1 > 0)
+        STDA_HIP_CHECK(hipMemcpy2DAsync(C, m * 8, t, m1 * 8, m1 * 8, p, hipMemcpyDeviceToDevice,
+                                        c->stream));
+    if (m2 > 0)
+        STDA_HIP_CHECK(hipMemcpy2DAsync(C + m1, m * 8, t + m1 * p, m2 * 8, m2 * 8, p,
+                                        hipMemcpyDeviceToDevice, c->stream));
 }
 
 // out (n x p) = alpha A C + beta out (A n x m, C m x p) for the tall-skinny

@@ -130,9 +130,10 @@ class Server:
     def _fits(self, fp: float) -> bool:
         return self._used + fp <= self.budget or self._used == 0.0
 
-    def run(self, jobs: list, realtime: bool = True) -> RunStats:
+    def run(self, jobs: list, realtime: bool = True, progress=None) -> RunStats:
         """Serve ``jobs``.  With ``realtime`` the arrival times are honoured
-        (open loop); otherwise every job is queued at t = 0 (closed batch)."""
+        (open loop); otherwise every job is queued at t = 0 (closed batch).
+        ``progress(done, total)`` is called about every 10 s while jobs run."""
         jobs = sorted(jobs, key=lambda j: j.arrival)
         if not realtime:
             for j in jobs:
@@ -237,10 +238,19 @@ class Server:
         done_feeding.set()
         with self._cv:
             self._cv.notify_all()
+        last = [now()]
+
+        def wait(t):
+            while t.is_alive():
+                t.join(timeout=1.0)
+                if progress is not None and now() - last[0] >= 10.0:
+                    last[0] = now()
+                    progress(sum(1 for j in jobs if not math.isnan(j.end)), len(jobs))
+
         for t in loops:
-            t.join()
-        for t in list(threads):
-            t.join()
+            wait(t)
+        for t in list(threads):     # every polarizability job has been started
+            wait(t)
         wall = now()
         lat = np.array([j.latency for j in jobs]) * 1e3 if jobs else np.zeros(1)
         ok = sum(1 for j in jobs if not j.error)
