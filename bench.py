@@ -367,9 +367,17 @@ def run_serve(args):
                         omegas=(0.0, 0.05, 0.1, 0.15)) for i in range(n)]
             st = srv.run(jobs, realtime=True,
                          progress=lambda d, t: log(f"#   rate {rate}: {d}/{t} jobs done"))
+            errs = {}
+            for j in jobs:
+                if j.error:
+                    key = f"{j.kind}: {j.error[:60]}"
+                    errs[key] = errs.get(key, 0) + 1
+            slow = sorted(jobs, key=lambda j: -j.latency)[:3]
             sweep.append({"rate": rate, "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,
                           "p99_ms": st.p99_ms, "ok": st.ok, "n": st.n, "batches": st.batches,
-                          "mean_batch": st.mean_batch})
+                          "mean_batch": st.mean_batch, "errors": errs,
+                          "slowest": [{"kind": j.kind, "natoms": int(j.ci.ncent),
+                                       "latency_s": round(j.latency, 2)} for j in slow]})
             log(f"# rate {rate}: {sweep[-1]}")
     best = max(sweep, key=lambda r: r["solves_per_s"])
     return {

@@ -1920,6 +1920,9 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             seedv = std::move(XpY);
         }
         int maxdim = (int)std::min<int64_t>(n, std::max(40 * nrhs, 256) + nseed);
+        // STDA_HIP_LRESP_MAXDIM caps the subspace (tests use it to force restarts)
+        if (const char *e = getenv("STDA_HIP_LRESP_MAXDIM"))
+            if (atoi(e) > 0) maxdim = std::min(maxdim, std::max(atoi(e), 3 + nseed + 2 * nrhs + 97));
         DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
         DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd, rhsV;
         // initial space: preconditioned right-hand sides (and the RPA vectors)
@@ -1933,6 +1936,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * 3, seedv.get(), (size_t)n * (n0 - 3) * 8,
                                           hipMemcpyDeviceToDevice, s));
         int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), n0, tmp);
+        const int m0 = mnew;   // the initial block, kept through restarts
         std::vector<double> Ap((size_t)maxdim * maxdim), Am((size_t)maxdim * maxdim), Br((size_t)maxdim * 3);
         std::vector<double> rhsnorm(3);
         {
@@ -2042,7 +2046,110 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             for (int q = 0; q < nun; ++q)
                 if (un[q] != q)
                     STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8, hipMemcpyDeviceToDevice, s));
-            if (m + nun > maxdim) nun = maxdim - m;
+            if (m + nun > maxdim) {
+                // restart: keep the initial block (right-hand sides and RPA
+                // seeds) and the current X+Y, X-Y of every right-hand side,
+                // orthonormalised in the reduced space (V is orthonormal, so
+                // V Q is for orthonormal Q); the operator images are the same
+                // combinations of P and M, the reduced matrices Q^T Ap Q etc.
+                std::vector<double> Q;
+                auto add_col = [&](std::vector<double> x) {
+                    double n0x = 0.0;
+                    for (double v : x) n0x += v * v;
+                    n0x = std::sqrt(n0x);
+                    const int r = (int)(Q.size() / m);
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int q = 0; q < r; ++q) {
+                            double d = 0.0;
+                            for (int i = 0; i < m; ++i) d += Q[i + (size_t)m * q] * x[i];
+                            for (int i = 0; i < m; ++i) x[i] -= d * Q[i + (size_t)m * q];
+                        }
+                    double nx = 0.0;
+                    for (double v : x) nx += v * v;
+                    nx = std::sqrt(nx);
+                    if (!(nx > 1.0e-10 * n0x) || nx == 0.0) return;
+                    for (double v : x) Q.push_back(v / nx);
+                };
+                for (int i = 0; i < m0; ++i) {
+                    std::vector<double> e(m, 0.0);
+                    e[i] = 1.0;
+                    add_col(e);
+                }
+                for (int col = 0; col < 2 * nrhs; ++col)
+                    add_col(std::vector<double>(coef.begin() + (size_t)m * col,
+                                                coef.begin() + (size_t)m * (col + 1)));
+                if (wmax > 0.0) {
+                    // and the space's RPA vectors (X+Y and X-Y) below 1.25
+                    // max(w): the poles the solutions have to resolve.  With
+                    // s = Am^1/2: s Ap s z = w^2 z, X+Y = s z, X-Y = w Am^-1 (X+Y)
+                    std::vector<double> sq((size_t)m * m, 0.0);
+                    for (int q = 0; q < m; ++q)
+                        for (int j = 0; j < m; ++j) {
+                            const double f = U[j + (size_t)m * q] * std::sqrt(ev[q]);
+                            for (int i = 0; i < m; ++i) sq[i + (size_t)m * j] += U[i + (size_t)m * q] * f;
+                        }
+                    std::vector<double> t((size_t)m * m, 0.0), S((size_t)m * m, 0.0), w2;
+                    for (int j = 0; j < m; ++j)
+                        for (int l = 0; l < m; ++l) {
+                            const double v = ap[l + (size_t)m * j];
+                            for (int i = 0; i < m; ++i) t[i + (size_t)m * j] += sq[i + (size_t)m * l] * v;
+                        }
+                    for (int j = 0; j < m; ++j)
+                        for (int l = 0; l < m; ++l) {
+                            const double v = sq[l + (size_t)m * j];
+                            for (int i = 0; i < m; ++i) S[i + (size_t)m * j] += t[i + (size_t)m * l] * v;
+                        }
+                    host_syev(m, S, w2);
+                    const double lim = 1.25 * wmax;
+                    for (int q = 0; q < m && q < 48; ++q) {
+                        if (!(w2[q] > 0.0) || std::sqrt(w2[q]) > lim) break;
+                        std::vector<double> a(m, 0.0), b(m, 0.0);
+                        for (int l = 0; l < m; ++l) {
+                            const double zl = S[l + (size_t)m * q];
+                            for (int i = 0; i < m; ++i) a[i] += sq[i + (size_t)m * l] * zl;
+                        }
+                        for (int j = 0; j < m; ++j)
+                            for (int i = 0; i < m; ++i) b[i] += aminv[i + (size_t)m * j] * a[j];
+                        add_col(a);
+                        add_col(b);
+                    }
+                }
+                const int r = (int)(Q.size() / m);
+                DBuf<double> Qd, rs((size_t)n * r);
+                Qd.upload(Q.data(), Q.size(), s);
+                for (DBuf<double> *B : {&V, &P, &M}) {
+                    gemm_nn(c, n, r, m, 1.0, B->get(), n, Qd.get(), m, 0.0, rs.get(), n);
+                    STDA_HIP_CHECK(hipMemcpyAsync(B->get(), rs.get(), (size_t)n * r * 8,
+                                                  hipMemcpyDeviceToDevice, s));
+                }
+                auto reduce = [&](const std::vector<double> &a, std::vector<double> &out) {
+                    std::vector<double> aq((size_t)m * r, 0.0);   // a Q
+                    for (int j = 0; j < r; ++j)
+                        for (int l = 0; l < m; ++l) {
+                            const double qlj = Q[l + (size_t)m * j];
+                            for (int i = 0; i < m; ++i) aq[i + (size_t)m * j] += a[i + (size_t)m * l] * qlj;
+                        }
+                    std::fill(out.begin(), out.end(), 0.0);
+                    for (int j = 0; j < r; ++j)
+                        for (int i = 0; i < r; ++i) {
+                            double v = 0.0;
+                            for (int l = 0; l < m; ++l) v += Q[l + (size_t)m * i] * aq[l + (size_t)m * j];
+                            out[i + (size_t)maxdim * j] = v;
+                        }
+                };
+                reduce(ap, Ap);
+                reduce(am, Am);
+                std::vector<double> br((size_t)maxdim * 3, 0.0);
+                for (int rr = 0; rr < 3; ++rr)
+                    for (int i = 0; i < r; ++i) {
+                        double v = 0.0;
+                        for (int l = 0; l < m; ++l) v += Q[l + (size_t)m * i] * Br[l + (size_t)maxdim * rr];
+                        br[i + (size_t)maxdim * rr] = v;
+                    }
+                Br.swap(br);
+                m = r;
+                if (m + nun > maxdim) nun = maxdim - m;
+            }
             if (nun <= 0) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
             STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * nun * 8, hipMemcpyDeviceToDevice, s));
             mnew = orthonormalize(c, V.get(), m, V.get() + n * m, nun, tmp);

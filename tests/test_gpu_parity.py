@@ -187,6 +187,23 @@ def test_polyene_lresp(polyene):
         np.testing.assert_allclose(a, ar, rtol=0, atol=1e-6 * np.abs(ar).max())
 
 
+def test_lresp_restarts(cluster30, monkeypatch):
+    """The response solver restarted (subspace capped far below its default
+    size; the initial block, the current solutions and the space's RPA
+    vectors below the frequencies are kept) reaches the same alpha(w) as the
+    uncapped solve, below and between poles."""
+    say("30-atom cluster lresp with forced restarts")
+    omegas = [0.0, 0.1, 0.15]
+    with _hip() as h:
+        h.setup(cluster30, 10.0, 0.25)
+        ref = h.lresp(omegas)
+        monkeypatch.setenv("STDA_HIP_LRESP_MAXDIM", "48")
+        res = h.lresp(omegas)
+    for (w, a), (wr, ar) in zip(res, ref):
+        assert w == wr
+        np.testing.assert_allclose(a, ar, rtol=0, atol=1e-6 * np.abs(ar).max())
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30
@@ -197,6 +214,27 @@ def test_cluster_selection_and_roots(cluster30):
         sol = h.solve_tda(12)
     np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
     assert_props(sol.props, ref.props)
+
+
+def test_many_roots_against_dense():
+    """50 roots: a Davidson subspace of up to 300 vectors, beyond the shapes
+    of the hand-written tall-skinny kernels (split-K Gram products, V C with
+    more than 32 columns, Cholesky QR of blocks of up to 50 directions).
+    Reference: the lowest eigenvalues of the explicit A of the same setup
+    (A itself is held to the oracle by test_polyene_explicit_matrices)."""
+    from stda_amd import synth
+    say("40-atom cluster TDA k=50 against the dense eigenvalues")
+    mol, ints = synth.cluster(40, seed=41, basis="minimal")
+    ci = prepare(mol, 7.0, 0.25, ints)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        say(f"  nci={h.nci}")
+        A = h.build_a()
+        sol = h.solve_tda(50, tol=1e-8)
+    w = np.linalg.eigvalsh(A)
+    k = len(sol.omega)
+    assert k == min(50, A.shape[0])
+    np.testing.assert_allclose(sol.omega, w[:k], atol=E_TOL)
 
 
 def test_triplet_and_ax0(polyene):
