@@ -153,6 +153,34 @@ int stda_hip_build_bhalf(stda_hip_ctx *ctx, double *bhalf);
  * which = 0 A, 1 A+B, 2 A-B, 3 0.5B.  v, av: (nci, nvec). */
 int stda_hip_matvec(stda_hip_ctx *ctx, int32_t which, int32_t nvec, const double *v, double *av);
 
+/* Gaussian broadening of line spectra, the curve loop of the reference's
+ * spectrum program g_spec (g_spec/g_spec.f:371-433; the program reads the
+ * tda.dat written by stda.f:2235-2265 / apbtrafo.f:115-288 and writes
+ * spec.dat).  Replaces that loop for nspec spectra at once; the caller does
+ * g_spec's parsing, sorting and grid choice (g_spec.f:60-260).
+ *   line_offsets[nspec+1]  lines of spectrum s are [line_offsets[s], line_offsets[s+1])
+ *   x, y, wei              line energies (eV), strengths, Gaussian widths (eV),
+ *                          in g_spec's summation order
+ *   emin, step [nspec]     grid e_j = emin + j*step, j = 1..npts
+ *   flags                  STDA_HIP_GSPEC_UV (absorption: f1 = 9.186e-3 and
+ *                          f -> dipole strength; else CD, f1 = 22.97),
+ *                          _NM (e -> 1e7/(8065.54093 e)), _CM (e -> 8065.54093 e,
+ *                          after _NM when both are given), _LOG (r -> max(log10 r, 0))
+ *   rfaktor                scale of the curve (RFAKTOR)
+ *   e, r [nspec*npts]      grid abscissae and curve, spectrum-major
+ * Lines with |x| <= 1e-6 are skipped as in the reference.  Arithmetic in
+ * FP64 with the reference's operation order and its single-precision
+ * constants; host arrays in and out. */
+enum {
+    STDA_HIP_GSPEC_UV = 1,
+    STDA_HIP_GSPEC_NM = 2,
+    STDA_HIP_GSPEC_CM = 4,
+    STDA_HIP_GSPEC_LOG = 8
+};
+int stda_hip_gspec(stda_hip_ctx *ctx, int32_t nspec, const int64_t *line_offsets, const double *x,
+                   const double *y, const double *wei, const double *emin, const double *step,
+                   int32_t npts, int32_t flags, double rfaktor, double *e, double *r);
+
 /* Arithmetic of the coupling operators in the solvers of this context:
  *   0 (default) FP64 throughout;
  *   1 mixed: the transition-density contractions (Q gamma Q^T products) in
@@ -170,7 +198,8 @@ int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
 
 /* Per-kernel statistics accumulated since the last select: HIP-event time
  * (ms) on the context's stream, launch count and algorithmic work (flops).
- * Names: "pt2_rows", "pt2_cols", "ao_matvec".  Unknown names give zeros. */
+ * Names: "pt2_rows", "pt2_cols", "ao_matvec", "gspec" (work: point-line
+ * terms).  Unknown names give zeros. */
 int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
                           double *launches, double *work);
 

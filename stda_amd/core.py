@@ -34,7 +34,7 @@ C_API = [
     "stda_hip_kernel_stats", "stda_hip_set_precision", "stda_hip_batch_create",
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
-    "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats",
+    "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
 ]
 
 
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_build_apb_amb.argtypes = [P, P, P]
     L.stda_hip_build_bhalf.argtypes = [P, P]
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
+    L.stda_hip_gspec.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -303,6 +304,19 @@ class StdaHip:
         out = np.zeros_like(V, order="F")
         self._check(self._L.stda_hip_matvec(self._h, which, V.shape[1], _ptr(V), _ptr(out)))
         return out
+
+    def gspec_curves(self, offsets, x, y, wei, emin, step, npts: int, flags: int,
+                     rfaktor: float):
+        """Broadened curves of len(offsets)-1 line spectra (stda_hip_gspec): (e, r), each
+        (nspec, npts)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, wei, emin, step)]
+        nspec = len(off) - 1
+        e = np.zeros((nspec, npts))
+        r = np.zeros((nspec, npts))
+        self._check(self._L.stda_hip_gspec(self._h, nspec, _ptr(off), *[_ptr(a) for a in arr],
+                                           npts, flags, rfaktor, _ptr(e), _ptr(r)))
+        return e, r
 
     def timings(self):
         t = np.zeros(6)

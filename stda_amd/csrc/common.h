@@ -146,6 +146,11 @@ void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, do
 void launch_residual_norms(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
                            const double *theta, double *R, double *part, int nsl, double *norms);
 // T(:, q) = R(:, c) / (theta_c - diag) for the q-th column c with norms[c] >= tol, q < room
+// spectrum broadening (spectrum.hip); flags as STDA_HIP_GSPEC_* of the public header
+enum { GSPEC_UV = 1, GSPEC_NM = 2, GSPEC_CM = 4, GSPEC_LOG = 8 };
+void launch_gspec(hipStream_t s, int nspec, const int64_t *off, const double *x, const double *y,
+                  const double *w, const double *emin, const double *step, int npts, int flags,
+                  double rfaktor, double *eout, double *rout);
 void launch_precondition_compact(hipStream_t s, int64_t n, int k, const double *R,
                                  const double *diag, const double *theta, const double *norms,
                                  double tol, int room, double *T);

@@ -2249,6 +2249,44 @@ int stda_hip_matvec(stda_hip_ctx *c, int32_t which, int32_t nvec, const double *
     })
 }
 
+static_assert(STDA_HIP_GSPEC_UV == GSPEC_UV && STDA_HIP_GSPEC_NM == GSPEC_NM &&
+                  STDA_HIP_GSPEC_CM == GSPEC_CM && STDA_HIP_GSPEC_LOG == GSPEC_LOG,
+              "spectrum flags of the header and the kernel agree");
+
+int stda_hip_gspec(stda_hip_ctx *c, int32_t nspec, const int64_t *line_offsets, const double *x,
+                   const double *y, const double *wei, const double *emin, const double *step,
+                   int32_t npts, int32_t flags, double rfaktor, double *e, double *r) {
+    if (!c || nspec < 0 || npts <= 0 || (flags & ~15) != 0) return STDA_HIP_ERR_ARG;
+    if (nspec == 0) return STDA_HIP_OK;
+    if (!line_offsets || !emin || !step || !e || !r) return STDA_HIP_ERR_ARG;
+    if (line_offsets[0] != 0) return STDA_HIP_ERR_ARG;
+    for (int32_t q = 0; q < nspec; ++q)
+        if (line_offsets[q + 1] < line_offsets[q]) return STDA_HIP_ERR_ARG;
+    const int64_t nl = line_offsets[nspec];
+    if (nl > 0 && (!x || !y || !wei)) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t st = c->stream;
+        const size_t np = (size_t)nspec * npts;
+        DBuf<int64_t> off;
+        DBuf<double> X, Y, W, E0, ST, EO(np), RO(np);
+        off.upload(line_offsets, (size_t)nspec + 1, st);
+        X.upload(x, (size_t)nl, st);
+        Y.upload(y, (size_t)nl, st);
+        W.upload(wei, (size_t)nl, st);
+        E0.upload(emin, (size_t)nspec, st);
+        ST.upload(step, (size_t)nspec, st);
+        EventTimer ev(c);
+        launch_gspec(st, nspec, off.get(), X.get(), Y.get(), W.get(), E0.get(), ST.get(), npts,
+                     flags, rfaktor, EO.get(), RO.get());
+        // work: the (point, line) terms
+        ev.finish("gspec", (double)nl * npts);
+        EO.download(e, np, st);
+        RO.download(r, np, st);
+        c->sync();
+    })
+}
+
 int stda_hip_set_precision(stda_hip_ctx *c, int32_t precision) {
     if (!c || precision < 0 || precision > 1) return STDA_HIP_ERR_ARG;
     c->precision = precision;

@@ -1,0 +1,432 @@
+"""Spectral output: the tda.dat file and the g_spec broadening program.
+
+* :func:`tda_dat` writes the file the reference's driver leaves for spectrum
+  plotting, in either of its two layouts: the velocity-corrected table of
+  ``apbtrafo`` (apbtrafo.f:115-128, 259-260; default sTDA) or
+  ``print_tdadat`` (stda.f:2235-2265; sTD-DFT and uncorrected sTDA).
+* :func:`gspec` / :func:`gspec_many` are the reference's spectrum program
+  ``g_spec`` (g_spec/g_spec.f): they read the text of a tda.dat-style input and
+  return the contents of ``rots.dat`` (line spectrum) and ``spec.dat``
+  (Gaussian-broadened curve).  The keyword parsing, sorting, grid and line
+  spectrum are host work done here; the broadening, nspec x npts x lines
+  exp terms, runs in the HIP kernel behind ``stda_hip_gspec``, all spectra
+  of a call in one launch.  ``python -m stda_amd.spectrum < tda.dat`` writes
+  rots.dat and spec.dat in the working directory like the reference program.
+
+Numbers are read with the reference's own reader (READL/READAA,
+g_spec.f:689-790: at most ten digits before and after the point, the
+fraction built by repeated division by ten) and written with Fortran's
+edit descriptors, so that outputs match the reference program's files.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_F = np.float32
+EV_PRINT = float(_F(27.21139))        # apbtrafo.f:259 (real*4 literal, widened)
+EV_CM = float(_F(8065.54093))         # g_spec.f:325
+MARGIN = float(_F(0.40))              # g_spec.f:258
+ETHR_MIN = float(_F(0.0001))          # g_spec.f:335
+FAU = 27.21139570                     # g_spec.f:76
+
+FLAG_UV, FLAG_NM, FLAG_CM, FLAG_LOG = 1, 2, 4, 8   # STDA_HIP_GSPEC_*
+
+
+# ---------------------------------------------------------------- Fortran edit descriptors
+def _nonfinite(v: float, w: int):
+    if v != v:
+        return "NaN".rjust(w)
+    if v in (math.inf, -math.inf):
+        return ("Inf" if v > 0 else "-Inf").rjust(w)
+    return None
+
+
+def fmt_f(v: float, w: int = 16, d: int = 6) -> str:
+    """Fw.d; asterisks when the value does not fit."""
+    t = _nonfinite(v, w)
+    if t is not None:
+        return t
+    s = "%.*f" % (d, v)
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def fmt_e(v: float, w: int = 14, d: int = 4) -> str:
+    """Ew.d: 0.d...d mantissa, E+xx exponent, or +xxx without the E from 100 on."""
+    t = _nonfinite(v, w)
+    if t is not None:
+        return t
+    sign = "-" if math.copysign(1.0, v) < 0 else ""
+    if v == 0.0:
+        mant, ex = "0" * d, 0
+    else:
+        m, e = ("%.*e" % (d - 1, abs(v))).split("e")
+        mant, ex = m.replace(".", ""), int(e) + 1
+    tail = ("E%+03d" % ex) if -99 <= ex <= 99 else ("%+04d" % ex)
+    return (sign + "0." + mant + tail).rjust(w)
+
+
+def fmt_list_real8(v: float) -> str:
+    """List-directed output of one real*8 as gfortran writes it (blank, then G25.17 field)."""
+    a = abs(v)
+    if a != 0.0 and (a < 0.1 or a >= 1e17):
+        m, e = ("%.16E" % v).split("E")
+        return " " + (m + "E%+04d" % int(e)).rjust(25)
+    k = 1 if a == 0.0 else len(str(int(a))) if a >= 1 else 0
+    return " " + ("%.*f" % (17 - k, v)).rjust(20) + " " * 5
+
+
+# ---------------------------------------------------------------- tda.dat
+_HEADER_TAIL = [" MMASS", None, " LFAKTOR", "  0.5", " RFAKTOR", "  1.0", " WIDTH", "  0.20",
+                " SHIFT", "  0.00", " DATXY"]
+
+
+def tda_dat(omega, props, xmolw: float, velcorr: bool = True, thr: float | None = None) -> str:
+    """Text of tda.dat for roots omega (Hartree) with props (k, >=4): fL, fV, RL, RV or Rv(corr).
+
+    velcorr=True: the apbtrafo layout ('VELO', every root, energies
+    omega * 27.21139 in double precision).  velcorr=False: the print_tdadat
+    layout ('UV', roots above thr (Hartree) left out, energies formed in
+    single precision as the reference's real*4 eigenvalues are).
+    Rows: (i4, F10.4, 4f13.6)."""
+    omega = np.asarray(omega, dtype=np.float64)
+    props = np.asarray(props, dtype=np.float64)
+    out = [" NM", " VELO" if velcorr else " UV"]
+    for h in _HEADER_TAIL:
+        out.append(fmt_list_real8(xmolw) if h is None else h)
+    for i in range(len(omega)):
+        if velcorr:
+            ev = float(omega[i]) * EV_PRINT
+        else:
+            e32 = _F(omega[i])
+            if thr is not None and float(e32) > thr:
+                continue
+            ev = float(_F(e32 * _F(27.21139)))
+        out.append("%4d" % (i + 1) + fmt_f(ev, 10, 4) +
+                   "".join(fmt_f(float(props[i, c]), 13, 6) for c in range(4)))
+    return "\n".join(out) + "\n"
+
+
+# ---------------------------------------------------------------- g_spec input
+def read_numbers(line: str) -> list:
+    """The numbers of one input line as READL reads them (g_spec.f:689-790)."""
+    a = line[:120].ljust(120)
+    nl = len(a)
+    vals = []
+    p = 0                                   # 0-based scan start
+    while True:
+        q = p
+        while q < nl - 1:
+            ch, nx = a[q], a[q + 1]
+            if ch.isdigit() or ch == "." or (ch == "-" and (nx.isdigit() or nx == ".")):
+                break
+            q += 1
+        else:
+            return vals
+        start = q
+        ip = fp = 0.0
+        sign = 1.0
+        scale = 1.0
+        nd = 0
+        u = q
+        if a[u] == "-":
+            sign = -1.0
+            u += 1
+        while u < nl and a[u].isdigit() and nd < 10:
+            ip = ip * 10 + int(a[u])
+            nd += 1
+            u += 1
+        if u < nl and a[u] == ".":
+            nd = 0
+            u += 1
+            while u < nl and a[u].isdigit() and nd < 10:
+                fp = fp * 10 + int(a[u])
+                scale = scale / 10
+                nd += 1
+                u += 1
+        val = sign * (ip + fp * scale)
+        # the token runs to the next blank; a D or E on the way starts an exponent
+        u = start
+        end = nl
+        while u < nl:
+            if a[u] == " ":
+                end = u
+                break
+            if a[u] in "DE":
+                ex, es = 0.0, 1.0
+                v = u + 1
+                end = nl
+                while v < nl:
+                    if a[v] == " ":
+                        end = v
+                        break
+                    if a[v].isdigit():
+                        ex = ex * 10.0 + int(a[v])
+                    if a[v] == "-":
+                        es = -1.0
+                    v += 1
+                val = val * 10.0 ** (es * ex)
+                break
+            u += 1
+        vals.append(val)
+        if end >= nl:
+            return vals
+        p = end
+
+
+@dataclass
+class GSpecInput:
+    """One g_spec input after parsing, sorting and choosing the grid (g_spec.f:60-270)."""
+    x: np.ndarray                      # line energies (eV), summation order
+    y: np.ndarray
+    wei: np.ndarray
+    npts: int
+    emin: float
+    step: float
+    flags: int
+    rfaktor: float
+    rots: str = ""
+    opts: dict = field(default_factory=dict)
+
+
+def _selection_order(key: np.ndarray) -> np.ndarray:
+    """Permutation of dsort (g_spec.f:571-609): selection sort ascending with swaps, the last
+    of equal minima taken first.  Distinct keys reduce to an argsort."""
+    n = len(key)
+    if n == 0 or len(np.unique(key)) == n:
+        return np.argsort(key, kind="stable")
+    ew = key.copy()
+    perm = np.arange(n)
+    for i in range(n - 1):
+        seg = ew[i:]
+        k = i + (len(seg) - 1 - int(np.argmin(seg[::-1])))
+        if k != i:
+            ew[i], ew[k] = ew[k], ew[i]
+            perm[i], perm[k] = perm[k], perm[i]
+    return perm
+
+
+def parse_gspec(text: str) -> GSpecInput:
+    """Keyword input of g_spec (g_spec.f:46-218), sort (:222-247), grid (:249-262) and the
+    line spectrum written to rots.dat (:296-350)."""
+    lines = text.split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    imax = len(lines)
+    o = dict(itype=2, uv=0, cm=0, nm=0, log=0, inv=0, fau=1.0, rfaktor=1.0, fktl=1.0, sh=0.0,
+             ethr=0.0, forb=0.0, nol=False)
+    xs = np.zeros(imax + 1)
+    ys = np.zeros(imax + 1)
+    ws = np.zeros(imax + 1)
+    cols = [0.0] * 52                     # READL's array keeps values of earlier lines
+    it = iter(lines)
+
+    def line():
+        return next(it)[:120].ljust(120)
+
+    def nums(s):
+        v = read_numbers(s)
+        for q, val in enumerate(v):
+            cols[q + 1] = val
+        cols[len(v) + 1] = 0.0
+        return len(v)
+
+    k = 0
+    try:
+        while True:
+            s = line()
+            if "AU " in s:
+                o["fau"] = FAU
+            if "ETHR " in s:
+                s = line()
+                nums(s)
+                o["ethr"] = cols[1]
+            if "FORB " in s:
+                s = line()
+                nums(s)
+                o["forb"] = cols[1]
+            for key, name in (("UV", "uv"), ("NM", "nm"), ("CM", "cm"), ("LOG", "log"),
+                              ("INV", "inv")):
+                if key in s:
+                    o[name] = 1
+            if "VELO" in s:
+                o["itype"] = 3
+            if "MIX" in s:
+                o["itype"] = 4
+            for key, name in (("MMASS", None), ("RFAKTOR", "rfaktor"), ("LFAKTOR", "fktl"),
+                              ("SHIFT", "sh"), ("WIDTH", "width")):
+                if key in s:
+                    s = line()
+                    nums(s)
+                    if name == "width":
+                        ws[:] = cols[1]
+                    elif name:
+                        o[name] = cols[1]
+                    if name == "fktl" and cols[1] < 1.0e-5:
+                        o["nol"] = True
+            if "DATAX" in s:
+                k = 0
+                while True:
+                    s = line()
+                    if "DATAY" in s:
+                        break
+                    n = nums(s)
+                    if n > 0:
+                        k += 1
+                        kk = int(cols[1]) if n > 1 else k
+                        xs[kk] = cols[2] * o["fau"] + o["sh"]
+            if "DATAY" in s:
+                k = 0
+                while True:
+                    s = line()
+                    if "DATAW" in s or "END" in s:
+                        break
+                    n = nums(s)
+                    if n > 0:
+                        k += 1
+                        kk = int(cols[1]) if n > 1 else k
+                        ys[kk] = _strength(cols[o["itype"]], o)
+            if "DATAW" in s:
+                while True:
+                    s = line()
+                    if nums(s) == 2:
+                        ws[int(cols[1])] = cols[2]
+            if "DATXY" in s:
+                k = 0
+                col = (5 if o["uv"] == 0 else 3) + o["itype"] - 2
+                while True:
+                    s = line()
+                    if nums(s) > 0:
+                        k += 1
+                        xs[k] = cols[2] * o["fau"] + o["sh"]
+                        ys[k] = _strength(cols[col], o)
+    except StopIteration:
+        pass
+    x, y, w = xs[1:k + 1].copy(), ys[1:k + 1].copy(), ws[1:k + 1].copy()
+    au = o["fau"] >= 1.10
+    perm = _selection_order(x if au else -1 * x)
+    x, y, w = x[perm], y[perm], w[perm]
+    if au and k:
+        rel = np.abs(x[1:]) > 1.0e-6
+        x[1:][rel] = x[1:][rel] - x[0]
+        x[0] = 0.0
+    if o["uv"] == 1 and o["forb"] > 1.0e-8:
+        y[y < 1.0e-5] = o["forb"]
+    xma = max(-1.0e10, float(x.max())) if k else -1.0e10
+    xmi = min(1.0e10, float(x.min())) if k else 1.0e10
+    dx = xma - xmi
+    xma = xma + MARGIN * dx
+    xmi = xmi - MARGIN * dx
+    if xmi < 0:
+        xmi = 0.5
+    npts = max(2000, math.ceil(imax / 50.0))
+    flags = ((FLAG_UV if o["uv"] else 0) | (FLAG_NM if o["nm"] else 0) |
+             (FLAG_CM if o["cm"] else 0) | (FLAG_LOG if o["log"] else 0))
+    return GSpecInput(x=x, y=y, wei=w, npts=npts, emin=xmi, step=(xma - xmi) / npts, flags=flags,
+                      rfaktor=o["rfaktor"], rots=_rots(x, y, o), opts=o)
+
+
+def _strength(v: float, o: dict) -> float:
+    if o["inv"] == 1:
+        v = -1.0 * v
+    if o["log"] == 1 and v < 1.0e-6:
+        v = 1.0e-4
+    return v
+
+
+def _abscissae(v: float, o: dict) -> list:
+    out = []
+    if o["nm"] == 0 and o["cm"] == 0:
+        out.append(v)
+    if o["nm"] == 1:
+        out.append(1.0e7 / (EV_CM * v))
+    if o["cm"] == 1:
+        out.append(EV_CM * v)
+    return out
+
+
+def _rots(x, y, o) -> str:
+    """rots.dat (g_spec.f:296-350): scaled lines, or lines merged within ETHR."""
+    out = []
+    rf = o["rfaktor"]
+    if not (o["ethr"] > ETHR_MIN or o["nol"]):
+        for xi, yi in zip(x.tolist(), y.tolist()):
+            if xi == 0.0:
+                continue
+            yy = yi * o["fktl"]
+            if o["uv"] == 1:
+                yy = yy * 1.0e5
+            if o["log"] == 1:
+                if yy < 0:
+                    yy = 1.0e-6
+                yy = math.log10(yy) if yy > 0 else -math.inf
+            out += [fmt_f(a) + fmt_e(yy * rf) for a in _abscissae(xi, o)]
+    if o["ethr"] >= ETHR_MIN:
+        n = len(x)
+        alive = np.ones(n, dtype=bool)
+        for i in range(n):
+            if not alive[i]:
+                continue
+            tot = float(y[i])
+            for j in range(i + 1, n):
+                if abs(x[i] - x[j]) < o["ethr"]:
+                    alive[j] = False
+                    tot = tot + float(y[j])
+            out += [fmt_f(a) + fmt_e(tot) for a in _abscissae(float(x[i]), o)]
+    return "".join(s + "\n" for s in out)
+
+
+def _spec_text(e: np.ndarray, r: np.ndarray) -> str:
+    return "".join(fmt_f(a) + fmt_e(b) + "\n" for a, b in zip(e.tolist(), r.tolist()))
+
+
+# ---------------------------------------------------------------- broadening on the device
+def broaden(h, inputs: list) -> list:
+    """Curves (e, r) of parsed inputs; one kernel launch per distinct grid size."""
+    out = [None] * len(inputs)
+    groups: dict = {}
+    for q, g in enumerate(inputs):
+        groups.setdefault((g.npts, g.flags, g.rfaktor), []).append(q)
+    for (npts, flags, rf), idx in groups.items():
+        off = np.zeros(len(idx) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(inputs[q].x) for q in idx])
+        cat = lambda attr: (np.concatenate([getattr(inputs[q], attr) for q in idx])
+                            if off[-1] else np.zeros(0))
+        e, r = h.gspec_curves(off, cat("x"), cat("y"), cat("wei"),
+                              np.array([inputs[q].emin for q in idx]),
+                              np.array([inputs[q].step for q in idx]), npts, flags, rf)
+        for t, q in enumerate(idx):
+            out[q] = (e[t], r[t])
+    return out
+
+
+def gspec_many(h, texts: list) -> list:
+    """g_spec on many inputs: [(rots.dat text, spec.dat text)], broadening in one launch."""
+    inputs = [parse_gspec(t) for t in texts]
+    curves = broaden(h, inputs)
+    return [(g.rots, _spec_text(*c)) for g, c in zip(inputs, curves)]
+
+
+def gspec(h, text: str):
+    """g_spec on one input: (rots.dat text, spec.dat text)."""
+    return gspec_many(h, [text])[0]
+
+
+def main(argv=None) -> int:
+    from .core import StdaHip
+    text = sys.stdin.read()
+    with StdaHip(0) as h:
+        rots, spec = gspec(h, text)
+    with open("rots.dat", "w") as f:
+        f.write(rots)
+    with open("spec.dat", "w") as f:
+        f.write(spec)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
