@@ -287,6 +287,7 @@ def run_mixed(args):
         kst = {k: b.kernel_stats(k) for k in SINGLE_KERNELS + COMPOSITE_SPANS}
         phase_us = b.phase_times()
     err = compare(res, ref)
+    spectra = spectra_leg(cis, res)
     t, cnt = phases[-1]
     # dominant single kernel of the timed steps (HIP-event time on the
     # launching stream, summed over the workers; with concurrent workers a
@@ -340,8 +341,64 @@ def run_mixed(args):
                    "phase_ms_summed": {k: round(v / 1e3, 2) for k, v in phase_us.items()
                                        if k not in ("sweeps", "unused")}},
         "roofline": roofline,
+        "spectra": spectra,
         "cpu_baseline": cpu,
     }
+
+
+def spectra_leg(cis, res, reps: int = 20):
+    """The batch's roots through tda.dat and g_spec: every molecule's spectrum (the CD curve
+    g_spec draws from a velocity-corrected tda.dat) broadened in one launch of its kernel.  Kernel time from HIP events on the context's stream;
+    host parsing/formatting timed separately (outside the kernel figure)."""
+    from stda_amd import spectrum as S
+    from stda_amd.core import StdaHip
+    t0 = time.perf_counter()
+    texts = [S.tda_dat(om, pr, ci.xmolw, velcorr=True)
+             for ci, (om, pr, st) in zip(cis, res) if st == 0 and len(om)]
+    inputs = [S.parse_gspec(t) for t in texts]
+    parse_ms = (time.perf_counter() - t0) * 1e3
+    lines = sum(len(g.x) for g in inputs)
+    with StdaHip(0) as h:
+        S.broaden(h, inputs)                      # warm-up
+        k0 = h.kernel_stats("gspec")
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            curves = S.broaden(h, inputs)
+        call_ms = (time.perf_counter() - t1) * 1e3 / reps
+        k1 = h.kernel_stats("gspec")
+    t2 = time.perf_counter()
+    for g, c in zip(inputs[:16], curves[:16]):
+        S._spec_text(*c)
+    fmt_ms = (time.perf_counter() - t2) * 1e3 / 16 * len(inputs)
+    kms = (k1["ms"] - k0["ms"]) / max(k1["launches"] - k0["launches"], 1.0)
+    # the reference program itself (built from its source into oracle/_ref) on the same files
+    ref_prog = os.path.join(ROOT, "oracle", "_ref", "g_spec")
+    ref = None
+    if os.path.exists(ref_prog):
+        import subprocess
+        import tempfile
+        tr = time.perf_counter()
+        done = 0
+        with tempfile.TemporaryDirectory() as d:
+            for text in texts:
+                subprocess.run([ref_prog], input=text.encode(), stdout=subprocess.DEVNULL,
+                               cwd=d, check=True)
+                done += 1
+                if time.perf_counter() - tr > 5.0:
+                    break
+        el = time.perf_counter() - tr
+        ref = {"value": done / el, "unit": "spectra/s", "cores": 1, "kind": "reference",
+               "sample": f"reference g_spec (flang build of g_spec/g_spec.f) run once per "
+                         f"tda.dat on the first {done} files, one process each"}
+    terms = (k1["work"] - k0["work"]) / max(k1["launches"] - k0["launches"], 1.0)
+    log(f"# spectra: {len(inputs)} spectra, {lines} lines, kernel {kms:.4f} ms, call "
+        f"{call_ms:.3f} ms")
+    return {"spectra": len(inputs), "lines": lines, "points": inputs[0].npts if inputs else 0,
+            "kernel_ms": round(kms, 4), "terms_per_launch": terms,
+            "terms_per_s": terms / (kms * 1e-3) if kms > 0 else None,
+            "call_ms": round(call_ms, 3), "spectra_per_s_call": len(inputs) / (call_ms * 1e-3),
+            "host_parse_ms": round(parse_ms, 1), "host_format_ms_est": round(fmt_ms, 1),
+            "cpu_baseline": ref}
 
 
 # --------------------------------------------------------------------------- serving

@@ -40,10 +40,18 @@ ROT = 235.7220
 _lib = None
 
 
+REFERENCE = "/root/reference"
+
+
 def build(force: bool = False) -> str:
+    """liboracle.so; with force, also the reference's g_spec (_ref/g_spec) where the
+    reference tree is present."""
     so = os.path.join(HERE, "liboracle.so")
     if force or not os.path.exists(so):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    if force and os.path.exists(os.path.join(REFERENCE, "g_spec", "g_spec.f")):
+        subprocess.run(["make", "-s", "-C", HERE, "-B", "ref", f"REFERENCE={REFERENCE}"],
+                       check=True)
     return so
 
 
