@@ -139,6 +139,21 @@ int stda_hip_solve_rpa(stda_hip_ctx *ctx, int32_t nroots, int32_t capacity, doub
 int stda_hip_lresp(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double tol,
                    double *alpha, double *xpy);
 
+/*
+ * SHG first hyperpolarizability beta(-2w; w, w) and its hyper-Rayleigh
+ * scattering invariants (replaces the beta part of lresp1,
+ * linear_response.f:373-699 with beta_resp_fast/A_beta_fast/B_beta_fast/
+ * List_A/List_B :911-1226 and HRS :1489-1662; std2 -resp N).  For each
+ * frequency w (Hartree; w = 0 is the static case) the linear responses at w
+ * and 2w are solved as in stda_hip_lresp, X-Y follows from the response
+ * equation, and beta_xyz = A - B over the reference's six orderings.
+ * Out: beta(3,3,3,nfreq) (Fortran order, beta[i + 3j + 9k + 27f]) and, if
+ * hrs is not NULL, hrs(4,nfreq) = beta_HRS, DR, <beta^2_ZZZ>, <beta^2_XZZ>
+ * (the reference's beta_HRS file holds w and beta_HRS).  Needs moments.
+ */
+int stda_hip_beta_shg(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double tol,
+                      double *beta, double *hrs);
+
 /* Per-routine FP64 equivalents of the reference's matrix builders, for
  * parity checks and small systems (explicit matrices, O(nci^2) memory):
  *   rtdamat  (stda.f:2010-2056):     a(nci,nci) full;
@@ -199,7 +214,7 @@ int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
 /* Per-kernel statistics accumulated since the last select: HIP-event time
  * (ms) on the context's stream, launch count and algorithmic work (flops).
  * Names: "pt2_rows", "pt2_cols", "ao_matvec", "gspec" (work: point-line
- * terms).  Unknown names give zeros. */
+ * terms), "beta" (the GEMMs of one frequency).  Unknown names give zeros. */
 int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
                           double *launches, double *work);
 

@@ -370,6 +370,128 @@ def lresp(ci, iconf, apb, amb, omegas):
     return out
 
 
+def _f32(v):
+    return float(np.float32(v))
+
+
+def _hrs_coef(num, den):
+    return float(np.float32(np.float32(num) / np.float32(den)))
+
+
+def hrs(beta):
+    """HRS (linear_response.f:1489-1662): <beta^2_ZZZ>, <beta^2_XZZ>, beta_HRS, DR of a
+    3x3x3 tensor, with the same sums over distinct indices."""
+    b = beta
+    R = range(3)
+    ne = lambda *t: len(set(t)) == len(t)
+    s = {}
+    s["iii"] = sum(b[i, i, i] ** 2 for i in R)
+    s["iij"] = sum(b[i, i, j] ** 2 for i in R for j in R if i != j)
+    s["jii"] = sum(b[j, i, i] ** 2 for i in R for j in R if i != j)
+    s["ijk"] = sum(b[i, j, k] ** 2 for i in R for j in R for k in R if ne(i, j, k))
+    s["iii_ijj"] = sum(b[i, i, i] * b[i, j, j] for i in R for j in R if i != j)
+    s["jii_iij"] = sum(b[j, i, i] * b[i, i, j] for i in R for j in R if i != j)
+    s["iii_jji"] = sum(b[i, i, i] * b[j, j, i] for i in R for j in R if i != j)
+    s["iij_jkk"] = sum(b[i, i, j] * b[j, k, k] for i in R for j in R for k in R if ne(i, j, k))
+    s["jii_jkk"] = sum(b[j, i, i] * b[j, k, k] for i in R for j in R for k in R if ne(i, j, k))
+    s["iij_kkj"] = sum(b[i, i, j] * b[k, k, j] for i in R for j in R for k in R if ne(i, j, k))
+    s["ijk_jik"] = sum(b[i, j, k] * b[j, i, k] for i in R for j in R for k in R if ne(i, j, k))
+    s["ijj"] = sum(b[i, j, j] ** 2 for i in R for j in R if i != j)
+    s["iij_jii"] = sum(b[i, i, j] * b[j, i, i] for i in R for j in R if i != j)
+    s["ijj_ikk"] = sum(b[i, j, j] * b[i, k, k] for i in R for j in R for k in R if ne(i, j, k))
+    s["iik_jjk"] = sum(b[i, i, k] * b[j, j, k] for i in R for j in R for k in R if ne(i, j, k))
+    c = _hrs_coef
+    zzz = (c(1, 7) * s["iii"] + c(4, 35) * s["iij"] + c(1, 35) * s["jii"]
+           + c(2, 105) * s["ijk"] + c(2, 35) * s["iii_ijj"] + c(4, 35) * s["jii_iij"]
+           + c(4, 35) * s["iii_jji"] + c(4, 105) * s["iij_jkk"] + c(1, 105) * s["jii_jkk"]
+           + c(4, 105) * s["iij_kkj"] + c(4, 105) * s["ijk_jik"])
+    xzz = (c(1, 35) * s["iii"] + c(4, 105) * s["iii_ijj"] - c(2, 35) * s["iii_jji"]
+           + c(8, 105) * s["iij"] - c(2, 105) * s["iij_jkk"] + c(2, 35) * s["ijk"]
+           - c(2, 105) * s["ijk_jik"] + c(3, 35) * s["ijj"] - c(2, 35) * s["iij_jii"]
+           + c(1, 35) * s["ijj_ikk"] - c(2, 105) * s["iik_jjk"])
+    return zzz, xzz, np.sqrt(zzz + xzz), zzz / xzz
+
+
+def beta_shg(ci, iconf, apb, amb, omegas):
+    """SHG first hyperpolarizability beta(-2w; w, w) and beta_HRS (lresp1 and its helpers,
+    linear_response.f:196-699, 911-1226, 1489-1662).
+
+    For each w: the responses (X+Y) at w and -2w of ((A+B) - w^2 (A-B)^-1)(X+Y) = -2 mu,
+    X-Y = w (A-B)^-1 (X+Y), X = ((X+Y)+(X-Y))/2, Y = (X+Y) - X; then
+    beta_xyz = A - B summed over the six orderings of beta_resp_fast, with
+      A(x,y,z) = sum over CSF pairs (i,a),(j,a): X_x[ia] (-mu_y[ij]) Y_z[ja]   (List_A)
+      B(x,y,z) = sum over CSF pairs (i,a),(i,b): X_x[ia] (-mu_y[ab]) Y_z[ib]   (List_B)
+    computed for the components the reference computes and completed by its symmetry
+    copies (Kleinman at w = 0, ijk = ikj otherwise).  Returns [(w, beta 3x3x3, hrs tuple)]."""
+    mom = csf_moments(ci, iconf)
+    rhs = -2.0 * mom[0:3].T
+    inv_amb = np.linalg.inv(amb)
+    npk = ci.moments.shape[1]
+    no = ci.no
+    io = iconf[:, 0].astype(np.int64)
+    iv = iconf[:, 1].astype(np.int64)
+    nci = len(io)
+
+    def lin(a, b):                     # 1-based MO indices -> 0-based packed index
+        hi, lo = max(a, b), min(a, b)
+        return lo + hi * (hi - 1) // 2 - 1
+
+    # List_A / List_B (linear_response.f:1089-1167), in the same order: for every CSF i, the
+    # CSFs kk sharing its virtual (by occupied index j) or its occupied (by virtual index)
+    A_list, B_list = [], []
+    for i in range(nci):
+        kk = np.nonzero(iv == iv[i])[0]
+        for q in kk[np.argsort(io[kk], kind="stable")]:
+            A_list.append((i, lin(io[i], io[q]), q))
+        kk = np.nonzero(io == io[i])[0]
+        for q in kk[np.argsort(iv[kk], kind="stable")]:
+            B_list.append((i, lin(iv[i], iv[q]), q))
+    A_list = np.array(A_list, dtype=np.int64).reshape(-1, 3)
+    B_list = np.array(B_list, dtype=np.int64).reshape(-1, 3)
+    assert npk == ci.moments.shape[1]
+    mu = ci.moments[0:3]               # xl, yl, zl packed
+    out = []
+    for w in omegas:
+        X = np.zeros((nci, 2, 3))
+        Y = np.zeros((nci, 2, 3))
+        for jj in (0, 1):
+            om = w * (jj + 1) * (-1.0 if jj == 1 else 1.0)
+            xpy = np.linalg.solve(apb - om * om * inv_amb, rhs)
+            xmy = om * (inv_amb @ xpy)
+            X[:, jj, :] = 0.5 * (xpy + xmy)
+            Y[:, jj, :] = xpy - X[:, jj, :]
+
+        def term(lst, x, y, z, wx, wz):
+            a, m, b = lst[:, 0], lst[:, 1], lst[:, 2]
+            return float(np.sum(X[a, wx, x] * (-mu[y, m]) * Y[b, wz, z]))
+
+        def comp(ix, iy, iz):
+            perms = [(ix, iy, iz, 1, 0), (ix, iz, iy, 1, 0), (iy, iz, ix, 0, 1),
+                     (iy, ix, iz, 0, 0), (iz, ix, iy, 0, 0), (iz, iy, ix, 0, 1)]
+            A = sum(term(A_list, *p) for p in perms)
+            B = sum(term(B_list, *p) for p in perms)
+            return A - B
+
+        beta = np.zeros((3, 3, 3))
+        if w == 0.0:                   # Kleinman symmetry (linear_response.f:471-538)
+            for i in range(3):
+                beta[i, i, i] = comp(i, i, i)
+            for (i, j) in [(0, 1), (0, 2), (1, 0), (1, 2), (2, 1), (2, 0)]:
+                v = comp(i, j, j)
+                beta[i, j, j] = beta[j, i, j] = beta[j, j, i] = v
+            v = comp(0, 1, 2)
+            for t in [(0, 1, 2), (0, 2, 1), (2, 0, 1), (2, 1, 0), (1, 2, 0), (1, 0, 2)]:
+                beta[t] = v
+        else:                          # ijk = ikj (linear_response.f:540-653)
+            for t in [(0, 0, 0), (1, 1, 1), (2, 2, 2), (0, 1, 1), (1, 0, 1), (0, 2, 2),
+                      (2, 0, 2), (1, 0, 0), (0, 1, 0), (1, 2, 2), (2, 1, 2), (2, 1, 1),
+                      (1, 2, 1), (2, 0, 0), (0, 2, 0), (0, 1, 2), (2, 0, 1), (1, 2, 0)]:
+                v = comp(*t)
+                beta[t] = beta[t[0], t[2], t[1]] = v
+        out.append((w, beta, hrs(beta)))
+    return out
+
+
 def tda_dat_rows(res, thr_ev):
     """DATXY rows as the reference writes them (F10.4, 4f13.6 with 27.21139)."""
     rows = []

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstda_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["kernels.hip", "batch.hip", "spectrum.hip", "core.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "spectrum.hip", "hyper.hip", "core.cpp"]
 HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
             # packed FP32 instructions stay off (hipcc's host pass warns about the feature)
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -35,6 +35,7 @@ C_API = [
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
+    "stda_hip_beta_shg",
 ]
 
 
@@ -90,6 +91,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_build_bhalf.argtypes = [P, P]
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
     L.stda_hip_gspec.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
+    L.stda_hip_beta_shg.argtypes = [P, I32, P, F64, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -303,6 +305,21 @@ class StdaHip:
         V = np.asfortranarray(V, dtype=np.float64).reshape(self.nci, -1, order="F")
         out = np.zeros_like(V, order="F")
         self._check(self._L.stda_hip_matvec(self._h, which, V.shape[1], _ptr(V), _ptr(out)))
+        return out
+
+    def beta_shg(self, omegas, tol: float = 1e-10):
+        """SHG beta(-2w; w, w) per frequency (stda_hip_beta_shg): [(w, beta (3,3,3), hrs)] with
+        hrs = {"beta_hrs", "dr", "b2_zzz", "b2_xzz"}; beta[i, j, k] as the reference's
+        beta(i+1, j+1, k+1)."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        b = np.zeros(27 * len(om))
+        h = np.zeros(4 * len(om))
+        self._check(self._L.stda_hip_beta_shg(self._h, len(om), _ptr(om), tol, _ptr(b), _ptr(h)))
+        out = []
+        for f, w in enumerate(om.tolist()):
+            t = b[27 * f:27 * (f + 1)].reshape((3, 3, 3), order="F")
+            hv = h[4 * f:4 * (f + 1)]
+            out.append((w, t, {"beta_hrs": hv[0], "dr": hv[1], "b2_zzz": hv[2], "b2_xzz": hv[3]}))
         return out
 
     def gspec_curves(self, offsets, x, y, wei, emin, step, npts: int, flags: int,

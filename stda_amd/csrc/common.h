@@ -146,6 +146,17 @@ void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, do
 void launch_residual_norms(hipStream_t s, int64_t n, int k, const double *AX, const double *X,
                            const double *theta, double *R, double *part, int nsl, double *norms);
 // T(:, q) = R(:, c) / (theta_c - diag) for the q-th column c with norms[c] >= tol, q < room
+// quadratic response (hyper.hip): output column c of the dense X/Y stacks reads column
+// src[c] of the (X+Y) and (A+B)(X+Y) arrays; inv_w[c] = 1/w of its frequency (0 for w = 0)
+struct BetaCols {
+    int64_t src[6];
+    double inv_w[6];
+};
+void launch_beta_xy(hipStream_t s, int64_t n, const int32_t *pair, int no, int nv,
+                    const double *XP, const double *AX, const double *rhs, const BetaCols &cols,
+                    double *Xd, double *Yd);
+void launch_mu_blocks(hipStream_t s, const double *moments, int64_t npk, int no, int nv,
+                      double *muoo, double *muvv);
 // spectrum broadening (spectrum.hip); flags as STDA_HIP_GSPEC_* of the public header
 enum { GSPEC_UV = 1, GSPEC_NM = 2, GSPEC_CM = 4, GSPEC_LOG = 8 };
 void launch_gspec(hipStream_t s, int nspec, const int64_t *off, const double *x, const double *y,

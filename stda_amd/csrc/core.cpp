@@ -813,16 +813,36 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
     DBuf<double> Zd;
     Zd.upload(Z.data(), Z.size(), c->stream);
     gemm_nn(c, n, kept, nt, 1.0, T, n, Zd.get(), nt, 0.0, T2, n);
-    // one more projection against V for safety, then renormalise
-    if (m > 0) {
+    // A direction kept above with little new norm (Gram eigenvalue near the
+    // threshold) is mostly the rounding error of the projections, scaled up:
+    // it is far from orthogonal to V.  Project twice more and orthonormalise
+    // the block again, dropping the directions that lost more than ~30 % of
+    // their norm to V (Gram eigenvalue below 0.5).  With one projection and
+    // a plain renormalisation instead, V drifted from orthonormality
+    // (off-diagonal 1e-4 after 20 iterations) when the new directions were
+    // nearly dependent, e.g. response residuals at close frequencies, and
+    // the response solver stalled near 1e-6.
+    for (int pass = 0; pass < 2 && m > 0; ++pass) {
         dgemm_tn(c, m, kept, n, V, n, T2, n, C);
         gemm_nn(c, n, kept, m, -1.0, V, n, C, m, 1.0, T2, n);
     }
-    DBuf<double> nd(kept);
-    col_norms(c, n, kept, T2, nd.get());
-    launch_inv_scale_cols(c->stream, n, kept, T2, nd.get());
-    STDA_HIP_CHECK(hipMemcpyAsync(T, T2, (size_t)n * kept * 8, hipMemcpyDeviceToDevice, c->stream));
-    return kept;
+    dgemm_tn(c, kept, kept, n, T2, n, T2, n, S);
+    std::vector<double> S2((size_t)kept * kept), w2;
+    STDA_HIP_CHECK(hipMemcpyAsync(S2.data(), S, S2.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    host_syev(kept, S2, w2);
+    std::vector<double> Z2;
+    int kept2 = 0;
+    for (int k = kept - 1; k >= 0; --k) {
+        if (!(w2[k] > 0.5)) continue;
+        const double f = 1.0 / std::sqrt(w2[k]);
+        for (int i = 0; i < kept; ++i) Z2.push_back(S2[i + (size_t)kept * k] * f);
+        ++kept2;
+    }
+    if (kept2 == 0) return 0;
+    Zd.upload(Z2.data(), Z2.size(), c->stream);
+    gemm_nn(c, n, kept2, kept, 1.0, T2, n, Zd.get(), kept, 0.0, T, n);
+    return kept2;
 }
 
 // The same for a block of nt <= 64 new directions T (n x nt), without a host
@@ -2246,6 +2266,135 @@ int stda_hip_matvec(stda_hip_ctx *c, int32_t which, int32_t nvec, const double *
         apply_ops(c, which, -1, nvec, V.get(), n, AV.get(), n, nullptr, 0);
         AV.download(av, (size_t)n * nvec, c->stream);
         c->sync();
+    })
+}
+
+// <beta^2_ZZZ>, <beta^2_XZZ>, beta_HRS and DR of a 3x3x3 tensor b[i + 3j + 9k]
+// (HRS, linear_response.f:1489-1662); the coefficients are quotients of
+// default-real literals in the reference and keep their single-precision values.
+static void beta_hrs(const double *b, double out[4]) {
+    auto B = [&](int i, int j, int k) { return b[i + 3 * j + 9 * k]; };
+    auto cf = [](int num, int den) { return (double)((float)num / (float)den); };
+    double iii = 0, iij = 0, jii = 0, ijk = 0, iii_ijj = 0, jii_iij = 0, iii_jji = 0,
+           iij_jkk = 0, jii_jkk = 0, iij_kkj = 0, ijk_jik = 0, ijj = 0, iij_jii = 0,
+           ijj_ikk = 0, iik_jjk = 0;
+    for (int i = 0; i < 3; ++i) iii += B(i, i, i) * B(i, i, i);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            if (i == j) continue;
+            iij += B(i, i, j) * B(i, i, j);
+            jii += B(j, i, i) * B(j, i, i);
+            iii_ijj += B(i, i, i) * B(i, j, j);
+            jii_iij += B(j, i, i) * B(i, i, j);
+            iii_jji += B(i, i, i) * B(j, j, i);
+            ijj += B(i, j, j) * B(i, j, j);
+            iij_jii += B(i, i, j) * B(j, i, i);
+            for (int k = 0; k < 3; ++k) {
+                if (k == i || k == j) continue;
+                ijk += B(i, j, k) * B(i, j, k);
+                iij_jkk += B(i, i, j) * B(j, k, k);
+                jii_jkk += B(j, i, i) * B(j, k, k);
+                iij_kkj += B(i, i, j) * B(k, k, j);
+                ijk_jik += B(i, j, k) * B(j, i, k);
+                ijj_ikk += B(i, j, j) * B(i, k, k);
+                iik_jjk += B(i, i, k) * B(j, j, k);
+            }
+        }
+    const double zzz = cf(1, 7) * iii + cf(4, 35) * iij + cf(1, 35) * jii + cf(2, 105) * ijk +
+                       cf(2, 35) * iii_ijj + cf(4, 35) * jii_iij + cf(4, 35) * iii_jji +
+                       cf(4, 105) * iij_jkk + cf(1, 105) * jii_jkk + cf(4, 105) * iij_kkj +
+                       cf(4, 105) * ijk_jik;
+    const double xzz = cf(1, 35) * iii + cf(4, 105) * iii_ijj - cf(2, 35) * iii_jji +
+                       cf(8, 105) * iij - cf(2, 105) * iij_jkk + cf(2, 35) * ijk -
+                       cf(2, 105) * ijk_jik + cf(3, 35) * ijj - cf(2, 35) * iij_jii +
+                       cf(1, 35) * ijj_ikk - cf(2, 105) * iik_jjk;
+    out[0] = std::sqrt(zzz + xzz);
+    out[1] = zzz / xzz;
+    out[2] = zzz;
+    out[3] = xzz;
+}
+
+int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, double tol,
+                      double *beta, double *hrs) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    if (nfreq < 1 || !omega || !beta || !c->have_moments) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci, L = (int64_t)c->no * c->nv;
+        const int no = c->no, nv = c->nv;
+        hipStream_t s = c->stream;
+        // linear responses at w and 2w of every frequency (linear_response.f:306-323;
+        // -2w there: the sign enters only X-Y)
+        std::vector<double> wl;
+        std::vector<int> at(2 * (size_t)nfreq);
+        for (int f = 0; f < nfreq; ++f)
+            for (int jj = 0; jj < 2; ++jj) {
+                const double w = std::fabs(omega[f]) * (jj + 1);
+                size_t q = 0;
+                while (q < wl.size() && wl[q] != w) ++q;
+                if (q == wl.size()) wl.push_back(w);
+                at[2 * f + jj] = (int)q;
+            }
+        const int nw = (int)wl.size();
+        std::vector<double> al((size_t)9 * nw), xpy((size_t)n * 3 * nw);
+        const int st = stda_hip_lresp(c, nw, wl.data(), tol, al.data(), xpy.data());
+        if (st != STDA_HIP_OK) throw Error(st, c->last_error);
+        DBuf<double> XP, AX((size_t)n * 3 * nw), rhs((size_t)n * 3);
+        XP.upload(xpy.data(), (size_t)n * 3 * nw, s);
+        apply_ops(c, OP_APB, -1, 3 * nw, XP.get(), n, AX.get(), n, nullptr, 0);
+        STDA_HIP_CHECK(hipMemcpyAsync(rhs.get(), c->mom.get(), (size_t)n * 3 * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        std::vector<double> m2(3, -2.0);
+        DBuf<double> m2d;
+        m2d.upload(m2.data(), 3, s);
+        launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
+        DBuf<double> muoo((size_t)3 * no * no), muvv((size_t)3 * nv * nv);
+        launch_mu_blocks(s, c->moments.get(), c->npk, no, nv, muoo.get(), muvv.get());
+        DBuf<double> Xd((size_t)6 * L), Yd((size_t)6 * L), M((size_t)18 * L), T(6 * 18);
+        std::vector<double> Th(6 * 18);
+        for (int f = 0; f < nfreq; ++f) {
+            EventTimer ev(c);
+            BetaCols cols;
+            for (int jj = 0; jj < 2; ++jj)
+                for (int r = 0; r < 3; ++r) {
+                    // signed frequency of column (jj, r): w, then -2w (linear_response.f:308)
+                    const double ws = jj == 0 ? omega[f] : -2.0 * omega[f];
+                    cols.src[3 * jj + r] = 3 * (int64_t)at[2 * f + jj] + r;
+                    cols.inv_w[3 * jj + r] = ws == 0.0 ? 0.0 : 1.0 / ws;
+                }
+            Xd.zero(s);
+            Yd.zero(s);
+            launch_beta_xy(s, n, c->pair.get(), no, nv, XP.get(), AX.get(), rhs.get(), cols,
+                           Xd.get(), Yd.get());
+            // M(q; jj, r) = mu_q^oo Y(jj, r) - Y(jj, r) mu_q^vv, column q*6 + 3*jj + r
+            for (int q = 0; q < 3; ++q) {
+                double *Mq = M.get() + (size_t)q * 6 * L;
+                dgemm_batched(c->blas, false, false, no, nv, no, 1.0, muoo.get() + (size_t)q * no * no,
+                              no, 0, Yd.get(), no, L, 0.0, Mq, no, L, 6);
+                dgemm_batched(c->blas, false, false, no, nv, nv, -1.0, Yd.get(), no, L,
+                              muvv.get() + (size_t)q * nv * nv, nv, 0, 1.0, Mq, no, L, 6);
+            }
+            // T = Xd^T M: <X(jj', x), M(q; jj, r)> for all 6 x 18 pairs
+            dgemm(c->blas, true, false, 6, 18, L, 1.0, Xd.get(), L, M.get(), L, 0.0, T.get(), 6);
+            ev.finish("beta", 18.0 * 2.0 * ((double)no * no * nv + (double)no * nv * nv) +
+                                  2.0 * 6 * 18 * (double)L);
+            T.download(Th.data(), 6 * 18, s);
+            c->sync();
+            // beta(ix,iy,iz) = A - B over the six orderings of beta_resp_fast
+            // (linear_response.f:911-1087): term (x, y, z; w_x, w_z) = -<X_x(w_x), M(y; w_z, z)>
+            auto term = [&](int x, int y, int z, int wx, int wz) {
+                return -Th[(3 * wx + x) + 6 * (y * 6 + 3 * wz + z)];
+            };
+            double *bf = beta + (size_t)27 * f;
+            for (int iz = 0; iz < 3; ++iz)
+                for (int iy = 0; iy < 3; ++iy)
+                    for (int ix = 0; ix < 3; ++ix)
+                        bf[ix + 3 * iy + 9 * iz] = term(ix, iy, iz, 1, 0) + term(ix, iz, iy, 1, 0) +
+                                                   term(iy, iz, ix, 0, 1) + term(iy, ix, iz, 0, 0) +
+                                                   term(iz, ix, iy, 0, 0) + term(iz, iy, ix, 0, 1);
+            if (hrs) beta_hrs(bf, hrs + 4 * (size_t)f);
+        }
     })
 }
 

@@ -35,3 +35,12 @@ def read_datxy(path):
 def read_table(path):
     with open(path) as fh:
         return [[float(x) for x in ln.split()] for ln in fh if ln.strip()]
+
+
+def beta_test_omegas():
+    """Frequencies of the reference's BETA tests (-resp 2): 0 and the wavelength file's
+    1907 and 1064 nm, converted as lresp1 does in single precision (45.56335/lambda,
+    linear_response.f:265-268)."""
+    import numpy as np
+    lam = [float(x) for x in open(os.path.join(GOLDEN, "water", "wavelength")).read().split()]
+    return [0.0] + [float(np.float32(np.float32(45.56335) / np.float32(v))) for v in lam]

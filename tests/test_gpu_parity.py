@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, read_datxy
+from conftest import GOLDEN, beta_test_omegas, read_datxy, read_table
 import oracle as O
 from stda_amd.prep import prepare
 
@@ -202,6 +202,48 @@ def test_lresp_restarts(cluster30, monkeypatch):
     for (w, a), (wr, ar) in zip(res, ref):
         assert w == wr
         np.testing.assert_allclose(a, ar, rtol=0, atol=1e-6 * np.abs(ar).max())
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_beta(water, case):
+    """SHG beta and beta_HRS (std2 -resp 2) against the reference's BETA_TEST files (its
+    FP32 noise: 1e-6 relative) and the full tensor against the oracle (1e-8 of its largest
+    component)."""
+    ax, thr, alk, bej = WATER[case]
+    say(f"water beta case {case}")
+    ci = prepare(water, thr, ax)
+    om = beta_test_omegas()
+    r = O.run(ci, thr, ax, alphak=alk, betaj=bej, rpa=True, resp_omegas=[0.0], keep_matrices=True)
+    ref = O.beta_shg(ci, r.iconf, r.apb, r.amb, om)
+    with _hip() as h:
+        h.setup(ci, thr, ax, alphak=alk, betaj=bej)
+        res = h.beta_shg(om)
+    exp = read_table(os.path.join(GOLDEN, "water", f"BETA_TEST{case}.txt"))
+    for (w, b, hr), (wr, br, hrr), row in zip(res, ref, exp):
+        assert w == wr
+        np.testing.assert_allclose(b, br, rtol=0, atol=1e-8 * np.abs(br).max())
+        assert abs(hr["beta_hrs"] - hrr[2]) <= 1e-8 * hrr[2]
+        assert abs(hr["dr"] - hrr[3]) <= 1e-7 * abs(hrr[3])
+        assert abs(hr["beta_hrs"] - row[1]) <= 1e-6 * row[1]
+
+
+@pytest.mark.parametrize("which", ["polyene", "cluster30"])
+def test_beta_against_oracle(which, polyene, cluster30):
+    """Larger systems: the C14H16 polyene (1034 of its 1040 occupied-virtual pairs selected:
+    the pairs left out are zero-filled in the dense form and must drop out of both sums)
+    and a 30-atom cluster (2301 CSFs)."""
+    ci, thr = (polyene, 7.0) if which == "polyene" else (cluster30, 10.0)
+    om = [0.0, 0.02, 0.04]
+    say(f"{which} beta at {om}")
+    r = O.run(ci, thr, 0.25, rpa=True, resp_omegas=[0.0], keep_matrices=True)
+    ref = O.beta_shg(ci, r.iconf, r.apb, r.amb, om)
+    with _hip() as h:
+        h.setup(ci, thr, 0.25)
+        assert h.nci == len(r.iconf)
+        res = h.beta_shg(om)
+    for (w, b, hr), (wr, br, hrr) in zip(res, ref):
+        np.testing.assert_allclose(b, br, rtol=0, atol=1e-7 * np.abs(br).max())
+        assert abs(hr["beta_hrs"] - hrr[2]) <= 1e-7 * hrr[2]
 
 
 def test_cluster_selection_and_roots(cluster30):

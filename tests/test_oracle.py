@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, read_datxy, read_table
+from conftest import GOLDEN, beta_test_omegas, read_datxy, read_table
 import oracle as O
 from stda_amd.prep import prepare
 
@@ -183,3 +183,36 @@ def test_rpa_normalisation(polyene_inputs):
     r = O.run(ci, 7.0, 0.25, rpa=True, nroots=5)
     G = (r.X + r.Y).T @ (r.X - r.Y)
     np.testing.assert_allclose(G, np.eye(5), atol=1e-10)
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_beta_fixture(water, case):
+    """BETA_TEST*: beta_HRS of -resp 2 (static, 1907 nm, 1064 nm).  The FP64 oracle
+    agrees with the reference's FP32 binary to 1e-6 relative (2.5e-7 observed); this pins
+    the X/Y extraction, both CSF-pair sums, the six orderings and the HRS invariants."""
+    p = WATER_CASES[case]
+    ci = prepare(water, p["thr"], p["ax"])
+    r = O.run(ci, p["thr"], p["ax"], alphak=p["alphak"], betaj=p["betaj"], rpa=True,
+              resp_omegas=[0.0], keep_matrices=True)
+    om = beta_test_omegas()
+    res = O.beta_shg(ci, r.iconf, r.apb, r.amb, om)
+    exp = read_table(os.path.join(GOLDEN, "water", f"BETA_TEST{case}.txt"))
+    assert len(exp) == len(res) == 3
+    for (w, b, h), row in zip(res, exp):
+        assert abs(w * O.HARTREE_EV_PRINT - row[0]) < 1e-6
+        assert abs(h[2] - row[1]) <= 1e-6 * row[1]
+
+
+def test_beta_small_frequency_limit(polyene_inputs):
+    """At w -> 0 the SHG tensor (18 components computed on their own, ijk = ikj copies)
+    approaches the static one (10 components, Kleinman copies) to O(w^2): a swapped
+    frequency label or operand breaks the permutation symmetry and fails here."""
+    ci = polyene_inputs
+    r = O.run(ci, 7.0, 0.25, rpa=True, resp_omegas=[0.0], keep_matrices=True)
+    (_, b0, h0), (_, b1, h1) = O.beta_shg(ci, r.iconf, r.apb, r.amb, [0.0, 1e-4])
+    scale = np.abs(b0).max()
+    assert scale > 1.0
+    np.testing.assert_allclose(b1, b0, rtol=0, atol=1e-6 * scale)
+    for t in [(0, 1, 0), (1, 0, 1), (2, 0, 1)]:        # Kleinman: all index orders equal
+        assert abs(b1[t] - b1[t[1], t[0], t[2]]) <= 1e-6 * scale
+    assert abs(h1[2] - h0[2]) <= 1e-6 * h0[2]
