@@ -154,6 +154,25 @@ int stda_hip_lresp(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double
 int stda_hip_beta_shg(stda_hip_ctx *ctx, int32_t nfreq, const double *omega, double tol,
                       double *beta, double *hrs);
 
+/*
+ * Two-photon absorption of RPA states, both photons at half the excitation
+ * energy (replaces lresp_2PA_SP and TPA_resp_fast_SP with its A/B_2PA_*_fast_SP
+ * sums, linear_response.f:4604-5153; std2 -2PA N).  Inputs are what srpapack
+ * hands that routine: excitation energies eci (Hartree, > 0) and the states'
+ * X and Y (nci x nstates, leading dimension ldx; stda.f:998-1042).  For each
+ * state the response at w = -eci/2 with right-hand side -mu is solved, and
+ * sigma(x,y) = (-A + B)/2 is formed from the CSF-pair sums.
+ * Out: sigma(3,3,nstates); if delta is not NULL, delta(4,nstates) =
+ * delta_2PA parallel, perpendicular, circular and rho = parallel/perpendicular
+ * (the columns of the reference's 2PA-abs file).  In -2PA runs the
+ * reference's srpapack forms X and Y with the diagonals of (A-B)^1/2 and
+ * A+B counted twice (the halving at srpapack.f:98-105 is skipped in that
+ * mode); passing its uci/hci reproduces its numbers, passing the X, Y of
+ * stda_hip_solve_rpa gives the 2PA of the properly normalised states.
+ */
+int stda_hip_tpa(stda_hip_ctx *ctx, int32_t nstates, const double *eci, const double *xci,
+                 const double *yci, int64_t ldx, double tol, double *sigma, double *delta);
+
 /* Per-routine FP64 equivalents of the reference's matrix builders, for
  * parity checks and small systems (explicit matrices, O(nci^2) memory):
  *   rtdamat  (stda.f:2010-2056):     a(nci,nci) full;
@@ -214,7 +233,8 @@ int stda_hip_timings(const stda_hip_ctx *ctx, double *ms6);
 /* Per-kernel statistics accumulated since the last select: HIP-event time
  * (ms) on the context's stream, launch count and algorithmic work (flops).
  * Names: "pt2_rows", "pt2_cols", "ao_matvec", "gspec" (work: point-line
- * terms), "beta" (the GEMMs of one frequency).  Unknown names give zeros. */
+ * terms), "beta" (the GEMMs of one frequency), "tpa" (those of one state).
+ * Unknown names give zeros. */
 int stda_hip_kernel_stats(const stda_hip_ctx *ctx, const char *name, double *ms,
                           double *launches, double *work);
 

@@ -492,6 +492,93 @@ def beta_shg(ci, iconf, apb, amb, omegas):
     return out
 
 
+def _pair_lists(ci, iconf):
+    """List_A / List_B of linear_response.f:1089-1167 (0-based CSF and packed indices)."""
+    no = ci.no
+    io = iconf[:, 0].astype(np.int64)
+    iv = iconf[:, 1].astype(np.int64)
+
+    def lin(a, b):
+        hi, lo = max(a, b), min(a, b)
+        return lo + hi * (hi - 1) // 2 - 1
+
+    A_list, B_list = [], []
+    for i in range(len(io)):
+        kk = np.nonzero(iv == iv[i])[0]
+        for q in kk[np.argsort(io[kk], kind="stable")]:
+            A_list.append((i, lin(io[i], io[q]), q))
+        kk = np.nonzero(io == io[i])[0]
+        for q in kk[np.argsort(iv[kk], kind="stable")]:
+            B_list.append((i, lin(iv[i], iv[q]), q))
+    assert no > 0
+    return (np.array(A_list, dtype=np.int64).reshape(-1, 3),
+            np.array(B_list, dtype=np.int64).reshape(-1, 3))
+
+
+def rpa_states_tpa_mode(apb, amb, nroots):
+    """The RPA states srpapack hands the 2PA routine (srpapack.f:56-153 with TPA on): the
+    eigenpairs of (A-B)^1/2 (A+B) (A-B)^1/2, then X+Y and X-Y from packed loops that visit
+    each diagonal element twice -- the halving of the diagonals (srpapack.f:98-105) is
+    skipped in this mode, so both diagonals count twice.  Returns (eci, X, Y)."""
+    e, V = np.linalg.eigh(amb)
+    sq = (V * np.sqrt(e)) @ V.T
+    w2, Z = np.linalg.eigh(sq @ apb @ sq)
+    w = np.sqrt(w2[:nroots])
+    Z = Z[:, :nroots]
+    sqd = sq + np.diag(np.diag(sq))
+    apbd = apb + np.diag(np.diag(apb))
+    xpy = (sqd @ Z) / np.sqrt(w)[None, :]
+    xmy = (apbd @ xpy) / w[None, :]
+    X = 0.5 * (xpy + xmy)
+    return w, X, xpy - X
+
+
+def tpa(ci, iconf, apb, amb, eci, Xci, Yci, ntrans):
+    """Two-photon absorption of the lowest ntrans RPA states, both photons at half the
+    excitation energy (lresp_2PA_SP and TPA_resp_fast_SP, linear_response.f:4604-5153).
+
+    For state n: (X+Y) of ((A+B) - w^2 (A-B)^-1)(X+Y) = -mu at w = -E_n/2,
+    X-Y = w (A-B)^-1 (X+Y), X = ((X+Y)+(X-Y))/2, Y = (X+Y) - X; then
+    sigma(x,y) = (-A + B)/2 with A = A1+A2+A3+A4 over the CSF pairs of List_A,
+      A1 = sum X_x[p] (-mu_y[m]) Yn[q],   A2 = the same with x and y exchanged,
+      A3 = sum Xn[p] (-mu_x[m]) Y_y[q],   A4 = sum Xn[p] (-mu_y[m]) Y_x[q],
+    B likewise over List_B; F, G, H = sums of sigma products / 30 and
+    delta_par = 2F+2G+2H, delta_perp = -F+4G-H, delta_circ = -2F+3G+3H, rho = par/perp.
+    Returns [(E_n, sigma 3x3, (par, perp, circ, rho))]."""
+    mom = csf_moments(ci, iconf)
+    rhs = -1.0 * mom[0:3].T
+    inv_amb = np.linalg.inv(amb)
+    A_list, B_list = _pair_lists(ci, iconf)
+    mu = ci.moments[0:3]
+    out = []
+    for n in range(ntrans):
+        w = -eci[n] / 2.0
+        xpy = np.linalg.solve(apb - w * w * inv_amb, rhs)
+        xmy = w * (inv_amb @ xpy)
+        X = 0.5 * (xpy + xmy)
+        Y = xpy - X
+        xn, yn = Xci[:, n], Yci[:, n]
+
+        def s3(lst, u, q, v):
+            a, m, b = lst[:, 0], lst[:, 1], lst[:, 2]
+            return float(np.sum(u[a] * (-mu[q, m]) * v[b]))
+
+        sig = np.zeros((3, 3))
+        for ix in range(3):
+            for iy in range(ix + 1):
+                terms = []
+                for lst in (A_list, B_list):
+                    terms.append(s3(lst, X[:, ix], iy, yn) + s3(lst, X[:, iy], ix, yn)
+                                 + s3(lst, xn, ix, Y[:, iy]) + s3(lst, xn, iy, Y[:, ix]))
+                sig[ix, iy] = sig[iy, ix] = (-terms[0] + terms[1]) / 2.0
+        F = sum(sig[i, i] * sig[j, j] for i in range(3) for j in range(3)) / 30.0
+        G = sum(sig[i, j] * sig[i, j] for i in range(3) for j in range(3)) / 30.0
+        H = sum(sig[i, j] * sig[j, i] for i in range(3) for j in range(3)) / 30.0
+        par, perp, circ = 2 * F + 2 * G + 2 * H, -F + 4 * G - H, -2 * F + 3 * G + 3 * H
+        out.append((eci[n], sig, (par, perp, circ, par / perp)))
+    return out
+
+
 def tda_dat_rows(res, thr_ev):
     """DATXY rows as the reference writes them (F10.4, 4f13.6 with 27.21139)."""
     rows = []

@@ -35,7 +35,7 @@ C_API = [
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
-    "stda_hip_beta_shg",
+    "stda_hip_beta_shg", "stda_hip_tpa",
 ]
 
 
@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
     L.stda_hip_gspec.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
     L.stda_hip_beta_shg.argtypes = [P, I32, P, F64, P, P]
+    L.stda_hip_tpa.argtypes = [P, I32, P, P, P, I64, F64, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -321,6 +322,21 @@ class StdaHip:
             hv = h[4 * f:4 * (f + 1)]
             out.append((w, t, {"beta_hrs": hv[0], "dr": hv[1], "b2_zzz": hv[2], "b2_xzz": hv[3]}))
         return out
+
+    def tpa(self, eci, X, Y, tol: float = 1e-10):
+        """Two-photon absorption of the given RPA states (stda_hip_tpa): [(E, sigma (3,3),
+        delta)] with delta = {"par", "perp", "circ", "rho"}."""
+        e = np.ascontiguousarray(eci, dtype=np.float64)
+        Xf = np.asfortranarray(X, dtype=np.float64).reshape(self.nci, -1, order="F")
+        Yf = np.asfortranarray(Y, dtype=np.float64).reshape(self.nci, -1, order="F")
+        k = len(e)
+        sig = np.zeros(9 * k)
+        d = np.zeros(4 * k)
+        self._check(self._L.stda_hip_tpa(self._h, k, _ptr(e), _ptr(Xf), _ptr(Yf), self.nci, tol,
+                                         _ptr(sig), _ptr(d)))
+        return [(float(e[q]), sig[9 * q:9 * q + 9].reshape((3, 3), order="F"),
+                 dict(zip(["par", "perp", "circ", "rho"], d[4 * q:4 * q + 4].tolist())))
+                for q in range(k)]
 
     def gspec_curves(self, offsets, x, y, wei, emin, step, npts: int, flags: int,
                      rfaktor: float):

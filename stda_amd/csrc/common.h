@@ -151,10 +151,14 @@ void launch_residual_norms(hipStream_t s, int64_t n, int k, const double *AX, co
 struct BetaCols {
     int64_t src[6];
     double inv_w[6];
+    double scale;   // applied to X and Y (right-hand sides -mu instead of -2 mu: 0.5)
 };
 void launch_beta_xy(hipStream_t s, int64_t n, const int32_t *pair, int no, int nv,
                     const double *XP, const double *AX, const double *rhs, const BetaCols &cols,
-                    double *Xd, double *Yd);
+                    int ncols, double *Xd, double *Yd);
+// columns of V (n x k, leading dimension ldv) into dense occupied x virtual matrices
+void launch_scatter_pairs(hipStream_t s, int64_t n, const int32_t *pair, int no, int nv, int k,
+                          const double *V, int64_t ldv, double *D);
 void launch_mu_blocks(hipStream_t s, const double *moments, int64_t npk, int no, int nv,
                       double *muoo, double *muvv);
 // spectrum broadening (spectrum.hip); flags as STDA_HIP_GSPEC_* of the public header

@@ -2356,6 +2356,7 @@ int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, doubl
         for (int f = 0; f < nfreq; ++f) {
             EventTimer ev(c);
             BetaCols cols;
+            cols.scale = 1.0;
             for (int jj = 0; jj < 2; ++jj)
                 for (int r = 0; r < 3; ++r) {
                     // signed frequency of column (jj, r): w, then -2w (linear_response.f:308)
@@ -2365,7 +2366,7 @@ int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, doubl
                 }
             Xd.zero(s);
             Yd.zero(s);
-            launch_beta_xy(s, n, c->pair.get(), no, nv, XP.get(), AX.get(), rhs.get(), cols,
+            launch_beta_xy(s, n, c->pair.get(), no, nv, XP.get(), AX.get(), rhs.get(), cols, 6,
                            Xd.get(), Yd.get());
             // M(q; jj, r) = mu_q^oo Y(jj, r) - Y(jj, r) mu_q^vv, column q*6 + 3*jj + r
             for (int q = 0; q < 3; ++q) {
@@ -2394,6 +2395,113 @@ int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, doubl
                                                    term(iy, iz, ix, 0, 1) + term(iy, ix, iz, 0, 0) +
                                                    term(iz, ix, iy, 0, 0) + term(iz, iy, ix, 0, 1);
             if (hrs) beta_hrs(bf, hrs + 4 * (size_t)f);
+        }
+    })
+}
+
+int stda_hip_tpa(stda_hip_ctx *c, int32_t nstates, const double *eci, const double *xci,
+                 const double *yci, int64_t ldx, double tol, double *sigma, double *delta) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    if (nstates < 1 || !eci || !xci || !yci || !sigma || !c->have_moments ||
+        ldx < c->nci)
+        return STDA_HIP_ERR_ARG;
+    for (int32_t q = 0; q < nstates; ++q)
+        if (!(eci[q] > 0.0)) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci, L = (int64_t)c->no * c->nv;
+        const int no = c->no, nv = c->nv;
+        hipStream_t s = c->stream;
+        // responses at half of every excitation energy (lresp_2PA_SP,
+        // linear_response.f:4704-4725; right-hand side -mu, here -2 mu and X, Y halved)
+        std::vector<double> wl(nstates);
+        for (int q = 0; q < nstates; ++q) wl[q] = eci[q] / 2.0;
+        std::vector<double> al((size_t)9 * nstates), xpy((size_t)n * 3 * nstates);
+        const int st = stda_hip_lresp(c, nstates, wl.data(), tol, al.data(), xpy.data());
+        if (st != STDA_HIP_OK) throw Error(st, c->last_error);
+        DBuf<double> XP, AX((size_t)n * 3 * nstates), rhs((size_t)n * 3), XN, YN;
+        XP.upload(xpy.data(), (size_t)n * 3 * nstates, s);
+        apply_ops(c, OP_APB, -1, 3 * nstates, XP.get(), n, AX.get(), n, nullptr, 0);
+        STDA_HIP_CHECK(hipMemcpyAsync(rhs.get(), c->mom.get(), (size_t)n * 3 * 8,
+                                      hipMemcpyDeviceToDevice, s));
+        std::vector<double> m2(3, -2.0);
+        DBuf<double> m2d;
+        m2d.upload(m2.data(), 3, s);
+        launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
+        XN.resize((size_t)n * nstates);
+        YN.resize((size_t)n * nstates);
+        for (int q = 0; q < nstates; ++q) {
+            STDA_HIP_CHECK(hipMemcpyAsync(XN.get() + n * q, xci + ldx * q, (size_t)n * 8,
+                                          hipMemcpyHostToDevice, s));
+            STDA_HIP_CHECK(hipMemcpyAsync(YN.get() + n * q, yci + ldx * q, (size_t)n * 8,
+                                          hipMemcpyHostToDevice, s));
+        }
+        DBuf<double> muoo((size_t)3 * no * no), muvv((size_t)3 * nv * nv);
+        launch_mu_blocks(s, c->moments.get(), c->npk, no, nv, muoo.get(), muvv.get());
+        // columns 0-2: X, Y of the responses (x, y, z); column 3: the state's X, Y
+        DBuf<double> Xd((size_t)4 * L), Yd((size_t)4 * L), M((size_t)12 * L), T(4 * 12);
+        std::vector<double> Th(4 * 12);
+        for (int q = 0; q < nstates; ++q) {
+            EventTimer ev(c);
+            const double w = -eci[q] / 2.0;   // linear_response.f:4703
+            BetaCols cols;
+            cols.scale = 0.5;
+            for (int r = 0; r < 3; ++r) {
+                cols.src[r] = 3 * (int64_t)q + r;
+                cols.inv_w[r] = 1.0 / w;
+            }
+            Xd.zero(s);
+            Yd.zero(s);
+            launch_beta_xy(s, n, c->pair.get(), no, nv, XP.get(), AX.get(), rhs.get(), cols, 3,
+                           Xd.get(), Yd.get());
+            launch_scatter_pairs(s, n, c->pair.get(), no, nv, 1, XN.get() + n * q, n,
+                                 Xd.get() + 3 * L);
+            launch_scatter_pairs(s, n, c->pair.get(), no, nv, 1, YN.get() + n * q, n,
+                                 Yd.get() + 3 * L);
+            // M(qd; col) = mu_qd^oo Y_col - Y_col mu_qd^vv, column 4 qd + col
+            for (int qd = 0; qd < 3; ++qd) {
+                double *Mq = M.get() + (size_t)qd * 4 * L;
+                dgemm_batched(c->blas, false, false, no, nv, no, 1.0,
+                              muoo.get() + (size_t)qd * no * no, no, 0, Yd.get(), no, L, 0.0, Mq,
+                              no, L, 4);
+                dgemm_batched(c->blas, false, false, no, nv, nv, -1.0, Yd.get(), no, L,
+                              muvv.get() + (size_t)qd * nv * nv, nv, 0, 1.0, Mq, no, L, 4);
+            }
+            dgemm(c->blas, true, false, 4, 12, L, 1.0, Xd.get(), L, M.get(), L, 0.0, T.get(), 4);
+            ev.finish("tpa", 12.0 * 2.0 * ((double)no * no * nv + (double)no * nv * nv) +
+                                 2.0 * 4 * 12 * (double)L);
+            T.download(Th.data(), 4 * 12, s);
+            c->sync();
+            // sigma(x,y) = (-A + B)/2 (TPA_resp_fast_SP, linear_response.f:4960-5022):
+            // -A1+B1 = <X_x, M(y; n)>, -A2+B2 = <X_y, M(x; n)>,
+            // -A3+B3 = <X_n, M(x; y)>, -A4+B4 = <X_n, M(y; x)>
+            auto t = [&](int xc, int qd, int col) { return Th[xc + 4 * (qd * 4 + col)]; };
+            double sg[9];
+            for (int ix = 0; ix < 3; ++ix)
+                for (int iy = 0; iy <= ix; ++iy) {
+                    const double v =
+                        0.5 * (t(ix, iy, 3) + t(iy, ix, 3) + t(3, ix, iy) + t(3, iy, ix));
+                    sg[ix + 3 * iy] = sg[iy + 3 * ix] = v;
+                }
+            for (int e = 0; e < 9; ++e) sigma[9 * (size_t)q + e] = sg[e];
+            if (delta) {
+                double F = 0, G = 0, H = 0;
+                for (int ix = 0; ix < 3; ++ix)
+                    for (int iy = 0; iy < 3; ++iy) {
+                        F += sg[ix + 3 * ix] * sg[iy + 3 * iy];
+                        G += sg[ix + 3 * iy] * sg[ix + 3 * iy];
+                        H += sg[ix + 3 * iy] * sg[iy + 3 * ix];
+                    }
+                F /= 30.0;
+                G /= 30.0;
+                H /= 30.0;
+                double *d = delta + 4 * (size_t)q;
+                d[0] = 2.0 * F + 2.0 * G + 2.0 * H;
+                d[1] = -F + 4.0 * G - H;
+                d[2] = -2.0 * F + 3.0 * G + 3.0 * H;
+                d[3] = d[0] / d[1];
+            }
         }
     })
 }

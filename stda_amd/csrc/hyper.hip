@@ -24,7 +24,8 @@ __device__ inline int64_t lin0h(int64_t a, int64_t b) {   // 0-based packed inde
 
 // X = ((X+Y) + (X-Y))/2, Y = (X+Y) - X with X-Y = ((A+B)(X+Y) - rhs) / w (linear_response.f:375-400:
 // X-Y = w (A-B)^-1 (X+Y), and (A-B)^-1 (X+Y) = ((A+B)(X+Y) - rhs) / w^2 by the response equation);
-// column c of the output reads column src[c] of XP/AX, component c % 3 of rhs.
+// column c of the output reads column src[c] of XP/AX, component c % 3 of rhs; X and Y
+// are multiplied by cols.scale.
 __global__ void beta_xy_kernel(int64_t n, const int32_t *__restrict__ pair, int no, int nv,
                                const double *__restrict__ XP, const double *__restrict__ AX,
                                const double *__restrict__ rhs, BetaCols cols, double *Xd,
@@ -40,8 +41,19 @@ __global__ void beta_xy_kernel(int64_t n, const int32_t *__restrict__ pair, int 
         const double x = 0.5 * (xpy + xmy);
         const int64_t p = pair[I];
         const int64_t idx = (p / nv) + (int64_t)no * (p % nv);
-        Xd[idx + L * c] = x;
-        Yd[idx + L * c] = xpy - x;
+        Xd[idx + L * c] = cols.scale * x;
+        Yd[idx + L * c] = cols.scale * (xpy - x);
+    }
+}
+
+__global__ void scatter_pairs_kernel(int64_t n, const int32_t *__restrict__ pair, int no, int nv,
+                                     const double *__restrict__ V, int64_t ldv, double *D) {
+    const int64_t L = (int64_t)no * nv;
+    const int c = blockIdx.y;
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n;
+         I += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = pair[I];
+        D[(p / nv) + (int64_t)no * (p % nv) + L * c] = V[I + ldv * c];
     }
 }
 
@@ -71,10 +83,19 @@ unsigned blocks_for(int64_t n) {
 
 void launch_beta_xy(hipStream_t s, int64_t n, const int32_t *pair, int no, int nv,
                     const double *XP, const double *AX, const double *rhs, const BetaCols &cols,
-                    double *Xd, double *Yd) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(beta_xy_kernel, dim3(blocks_for(n), 6), dim3(256), 0, s, n, pair, no, nv,
-                       XP, AX, rhs, cols, Xd, Yd);
+                    int ncols, double *Xd, double *Yd) {
+    if (n == 0 || ncols <= 0) return;
+    if (ncols > 6) throw Error(1, "launch_beta_xy: at most 6 columns");   // STDA_HIP_ERR_ARG
+    hipLaunchKernelGGL(beta_xy_kernel, dim3(blocks_for(n), ncols), dim3(256), 0, s, n, pair, no,
+                       nv, XP, AX, rhs, cols, Xd, Yd);
+    STDA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scatter_pairs(hipStream_t s, int64_t n, const int32_t *pair, int no, int nv, int k,
+                          const double *V, int64_t ldv, double *D) {
+    if (n == 0 || k <= 0) return;
+    hipLaunchKernelGGL(scatter_pairs_kernel, dim3(blocks_for(n), k), dim3(256), 0, s, n, pair, no,
+                       nv, V, ldv, D);
     STDA_HIP_CHECK(hipGetLastError());
 }
 

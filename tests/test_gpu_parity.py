@@ -246,6 +246,44 @@ def test_beta_against_oracle(which, polyene, cluster30):
         assert abs(hr["beta_hrs"] - hrr[2]) <= 1e-7 * hrr[2]
 
 
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_water_2pa(water, case):
+    """2PA (std2 -2PA 3) with the states srpapack hands the routine in that mode: the
+    deltas against the reference's 2PA_TEST files (printed F20.3, its FP32 noise) and sigma
+    against the oracle (1e-8 of its largest element)."""
+    ax, thr, alk, bej = WATER[case]
+    say(f"water 2PA case {case}")
+    ci = prepare(water, thr, ax)
+    r = O.run(ci, thr, ax, alphak=alk, betaj=bej, rpa=True, keep_matrices=True)
+    eci, X, Y = O.rpa_states_tpa_mode(r.apb, r.amb, 3)
+    ref = O.tpa(ci, r.iconf, r.apb, r.amb, eci, X, Y, 3)
+    with _hip() as h:
+        h.setup(ci, thr, ax, alphak=alk, betaj=bej)
+        res = h.tpa(eci, X, Y)
+    exp = read_table(os.path.join(GOLDEN, "water", f"2PA_TEST{case}.txt"))
+    for (e, sg, d), (er, sgr, dr), row in zip(res, ref, exp):
+        np.testing.assert_allclose(sg, sgr, rtol=0, atol=1e-8 * np.abs(sgr).max())
+        got = [d["par"], d["perp"], d["circ"], d["rho"]]
+        np.testing.assert_allclose(got, dr, rtol=1e-7, atol=1e-12)
+        for v, rv in zip(got, row[2:6]):
+            assert abs(v - rv) <= 0.5e-3 + 1e-4 + 2e-4 * abs(rv)
+
+
+def test_polyene_2pa_of_solved_states(polyene):
+    """2PA of the polyene's four lowest states from the GPU RPA solve, against the oracle
+    on the same states (sigma within 1e-7 of its largest element)."""
+    say("polyene 2PA")
+    ci = polyene
+    r = O.run(ci, 7.0, 0.25, rpa=True, nroots=4, keep_matrices=True)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        sol = h.solve_rpa(4)
+        res = h.tpa(sol.omega, sol.X, sol.Y)
+    ref = O.tpa(ci, r.iconf, r.apb, r.amb, sol.omega, sol.X, sol.Y, 4)
+    for (e, sg, d), (er, sgr, dr) in zip(res, ref):
+        np.testing.assert_allclose(sg, sgr, rtol=0, atol=1e-7 * np.abs(sgr).max())
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30
