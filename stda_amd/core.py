@@ -361,18 +361,29 @@ class StdaHip:
 def stda(ci: CoreInputs, thr: float, thrp: float, ax: float, alphak: float = -100.0,
          betaj: float = -100.0, fthr: float | None = None, nroots: int | None = None,
          rpa: bool = False, triplet: bool = False, velcorr: bool | None = None,
-         resp_omegas=None, tol: float = 1e-8, device: int = 0):
+         resp_omegas=None, tol: float = 1e-8, device: int = 0, beta_omegas=None,
+         tpa_states: int | None = None):
     """Drop-in for the reference's ``stda`` hot path (stda.f:454-1471).
 
-    Returns ``(Solution | list-of-alpha, csf)`` where ``csf`` = (iconf, ed).
-    ``resp_omegas`` (Hartree) selects the linear-response branch of
-    ``-resp``/``-aresp`` (stda.f:878-903) instead of the eigenvalue problem.
+    Returns ``(Solution | list-of-alpha | list-of-beta | list-of-2PA, csf)`` where
+    ``csf`` = (iconf, ed).  ``resp_omegas`` (Hartree) selects the linear-response
+    branch of ``-aresp`` (stda.f:905-920) instead of the eigenvalue problem;
+    ``beta_omegas`` the SHG hyperpolarizability of ``-resp`` (lresp1, stda.f:878-903);
+    ``tpa_states`` = N the two-photon absorption of the N lowest RPA states of
+    ``-2PA N`` (stda.f:1034-1048), here of the properly normalised states of the RPA
+    solve (the reference's srpapack forms them with doubled diagonals in that mode;
+    :meth:`StdaHip.tpa` takes any states).
     """
     if velcorr is None:
         velcorr = not rpa                 # main.f:102, 662
     with StdaHip(device) as h:
         h.setup(ci, thr, ax, thrp=thrp, alphak=alphak, betaj=betaj, fthr=fthr, triplet=triplet)
         csf = h.csf()
+        if beta_omegas is not None:
+            return h.beta_shg(beta_omegas), csf
+        if tpa_states is not None:
+            sol = h.solve_rpa(tpa_states, tol=tol)
+            return h.tpa(sol.omega, sol.X, sol.Y), csf
         if resp_omegas is not None:
             return h.lresp(resp_omegas, tol=tol), csf
         if rpa:
