@@ -21,6 +21,7 @@
 #include "../../include/stda_hip.h"
 #include "common.h"
 #include "devbuf.h"
+#include "selsort.h"
 
 #include <dlfcn.h>
 
@@ -1359,8 +1360,12 @@ static void do_select(stda_hip_ctx *c) {
     c->nprim = (int64_t)prim.size();
     c->nsec = (int64_t)sec.size();
     if (c->nprim < 1) throw Error(STDA_HIP_ERR_NO_CSF, "no CSF, increase energy threshold (-e option)");
-    // energy order of the primaries (stda.f:651-668; ties keep pair order)
-    std::stable_sort(prim.begin(), prim.end(), [&](int32_t a, int32_t b) { return deh[a] < deh[b]; });
+    // energy order of the primaries: the reference's selection sort, ties included
+    {
+        std::vector<double> keys(prim.size());
+        for (size_t q = 0; q < prim.size(); ++q) keys[q] = deh[prim[q]];
+        reference_selection_order(std::move(keys), prim);
+    }
     c->timings[1] = t1.ms();
 
     // PT2 selection (stda.f:1773-1863)

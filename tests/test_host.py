@@ -65,3 +65,54 @@ def test_poisson_arrivals():
     assert np.all(np.diff(t) > 0)
     rate = len(t) / t[-1]
     assert abs(rate - 500.0) / 500.0 < 0.03
+
+
+def test_primary_order_is_the_reference_selection_sort(tmp_path):
+    """stda_amd/csrc/selsort.h reproduces the reference's selection sort of the primary CSFs
+    (stda.f:651-668: first strict minimum of the rest, then a swap) on keys with many ties,
+    where it differs from a stable sort."""
+    import ctypes
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "w.cpp"
+    src.write_text('#include "selsort.h"\n'
+                   'extern "C" void order(int n, const double *k, int *out) {\n'
+                   '  std::vector<double> key(k, k + n); std::vector<int32_t> it(n);\n'
+                   '  for (int i = 0; i < n; ++i) it[i] = i;\n'
+                   '  stda::reference_selection_order(key, it);\n'
+                   '  for (int i = 0; i < n; ++i) out[i] = it[i]; }\n')
+    so = tmp_path / "w.so"
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "stda_amd",
+                        "csrc")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-I", csrc, str(src), "-o",
+                    str(so)], check=True)
+    lib = ctypes.CDLL(str(so))
+
+    def reference(keys):
+        ed = list(keys)
+        items = list(range(len(ed)))
+        for i in range(len(ed) - 1):
+            k, pp = i, ed[i]
+            for j in range(i + 1, len(ed)):
+                if ed[j] >= pp:
+                    continue
+                k, pp = j, ed[j]
+            if k == i:
+                continue
+            ed[k], ed[i] = ed[i], pp
+            items[i], items[k] = items[k], items[i]
+        return items
+
+    rng = np.random.default_rng(3)
+    differs = 0
+    for n in [1, 2, 3, 7, 64, 257, 1000]:
+        for _ in range(5):
+            keys = rng.integers(0, max(2, n // 8), n).astype(np.float64)
+            out = np.zeros(n, dtype=np.int32)
+            lib.order(n, keys.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+            ref = reference(keys)
+            assert out.tolist() == ref
+            differs += ref != np.argsort(keys, kind="stable").tolist()
+    assert differs > 0
