@@ -95,6 +95,18 @@ int stda_hip_setup(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip
 int stda_hip_load(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip_params *par);
 int stda_hip_select(stda_hip_ctx *ctx, int64_t *nci, int64_t *nprim, int64_t *nsec);
 
+/* Selection variants, applied by the next stda_hip_select / stda_hip_setup:
+ *   natoms > 0: dual thresholds of std2 -dual (stda_rw_dual, stda-rw_dual.f:658-853;
+ *     the 'dual' file holds thr2_ev and the atom list).  Occupied MOs whose
+ *     squared Loewdin coefficients on the listed atoms (1-based) sum above 0.1
+ *     keep Ethr and the PT2 window; the others use thr2_ev and the window
+ *     2 (1 + 0.8 a_x) thr2.  CSFs of the chromophore MOs come first before the
+ *     energy sort, as in the reference.  natoms = 0: one threshold.
+ *   pt2_off != 0: no PT2 selection, the primaries only (-dual 1, -rw 1).
+ * No counterpart in the reference's RKS driver `stda` itself. */
+int stda_hip_set_selection(stda_hip_ctx *ctx, double thr2_ev, int32_t natoms,
+                           const int32_t *atoms, int32_t pt2_off);
+
 /* CSF list after selection: iconf(nci,2) 1-based (occupied MO, absolute
  * virtual MO index no+1..moci) and ed(nci), the PT2-corrected diagonal
  * (same order as the reference: primaries sorted by energy, then the

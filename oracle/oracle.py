@@ -68,6 +68,8 @@ def lib():
         L.orc_charges_packed.argtypes = [i32, i32, i32, i32, P, P, P, P]
         L.orc_screen.argtypes = [i32, i32, i32, P, P, P, P, f64, f64, f64, i32, f64, f64, f64,
                                  i64, P, P, P, P, P, P]
+        L.orc_screen_dual.argtypes = [i32, i32, i32, P, P, P, P, f64, P, P, P, i32, f64, f64,
+                                      f64, i64, P, P, P, P, P, P]
         L.orc_ptselect.argtypes = [i64, i32, i32, i32, i64, i64, P, P, f64, P, P, P, P, P, P,
                                    f64, P]
         L.orc_ptselect.restype = i64
@@ -159,8 +161,39 @@ def intermediates(ci, ax, alphak=-100.0, betaj=-100.0):
                 qij=qij, qab=qab, pij=pij, alphak=alphak, betaj=betaj)
 
 
-def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=None):
-    """Screening (stda.f:593-668) then PT2 selection (stda.f:1773-1863)."""
+def dual_groups(ci, atoms, thr, thr2_ev, fthr, ax):
+    """Chromophore classification and thresholds of std2 -dual (stda-rw_dual.f:658-780).
+
+    An occupied MO belongs to the chromophore when the squared Loewdin coefficients of the
+    AOs on the listed atoms (1-based; an atom listed twice counts twice) sum above 0.1.
+    Chromophore MOs keep the -e threshold (converted to eV and back) and the PT2 window;
+    the others use thr2 and the window 2 (1 + 0.8 ax) thr2.  Returns (occ_order, thr_i,
+    fthr_i) with the chromophore MOs first."""
+    no = ci.no
+    count = np.zeros(ci.ncent + 1)
+    for a in atoms:
+        count[a] += 1.0
+    w = np.zeros(no)
+    for mu in range(ci.nao):
+        c = count[ci.iaoat[mu]]
+        if c:
+            w += c * ci.clow[mu, :no] ** 2
+    chrom = w > float(np.float32(0.1))
+    t1 = (thr * HARTREE_EV) / HARTREE_EV
+    t2 = thr2_ev / HARTREE_EV
+    f2 = ((1.0 + float(np.float32(0.8)) * ax) * t2) * 2.0
+    thr_i = np.where(chrom, t1, t2)
+    fthr_i = np.where(chrom, fthr, f2)
+    order = np.r_[np.nonzero(chrom)[0], np.nonzero(~chrom)[0]].astype(np.int32)
+    return order, thr_i, fthr_i
+
+
+def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=None, dual=None,
+           pt2_off=False):
+    """Screening (stda.f:593-668) then PT2 selection (stda.f:1773-1863).
+
+    dual = (thr2_ev, atoms, ax): the dual-threshold screening of std2 -dual
+    (stda-rw_dual.f:658-853); pt2_off: no PT2 selection (-dual 1 / -rw 1)."""
     L = lib()
     no, nv, n = ci.no, ci.nv, ci.ncent
     ak = 0.0 if triplet else 2.0
@@ -174,17 +207,26 @@ def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=Non
     nex = ctypes.c_int64()
     nexpt = ctypes.c_int64()
     ks = kshift or (0.5, 0.1, 4.0)
-    L.orc_screen(n, no, nv, _p(np.ascontiguousarray(ci.epsi)), _p(inter["jdiag"]),
-                 _p(inter["pia"]), _p(inter["qia"]), ak, thr, fthr, 1 if kshift else 0,
-                 ks[0], ks[1], ks[2], mx, _p(iconf), _p(ed), _p(kconf), _p(edpt),
-                 ctypes.byref(nex), ctypes.byref(nexpt))
+    if dual is None:
+        L.orc_screen(n, no, nv, _p(np.ascontiguousarray(ci.epsi)), _p(inter["jdiag"]),
+                     _p(inter["pia"]), _p(inter["qia"]), ak, thr, fthr, 1 if kshift else 0,
+                     ks[0], ks[1], ks[2], mx, _p(iconf), _p(ed), _p(kconf), _p(edpt),
+                     ctypes.byref(nex), ctypes.byref(nexpt))
+    else:
+        order, thr_i, fthr_i = dual_groups(ci, dual[1], thr, dual[0], fthr, dual[2])
+        L.orc_screen_dual(n, no, nv, _p(np.ascontiguousarray(ci.epsi)), _p(inter["jdiag"]),
+                          _p(inter["pia"]), _p(inter["qia"]), ak, _p(order), _p(thr_i),
+                          _p(fthr_i), 1 if kshift else 0, ks[0], ks[1], ks[2], mx, _p(iconf),
+                          _p(ed), _p(kconf), _p(edpt), ctypes.byref(nex), ctypes.byref(nexpt))
     nex, nexpt = nex.value, nexpt.value
     if nex < 1:
         raise RuntimeError("no CSF, increase energy threshold (-e option)")
+    if pt2_off:
+        nexpt = 0
     incl = np.zeros(max(nexpt, 1), dtype=np.uint8)
-    new = L.orc_ptselect(nex, n, no, nv, nexpt, mx, _p(iconf), _p(kconf), ak, _p(ed),
-                         _p(edpt), _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]),
-                         _p(inter["qab"]), thrp, _p(incl))
+    new = 0 if nexpt == 0 else L.orc_ptselect(
+        nex, n, no, nv, nexpt, mx, _p(iconf), _p(kconf), ak, _p(ed), _p(edpt),
+        _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]), _p(inter["qab"]), thrp, _p(incl))
     nci = nex + new
     ic = np.stack([iconf[:nci], iconf[mx:mx + nci]], axis=1)
     return dict(nex=nex, npt=new, nci=nci, nexpt=nexpt, iconf_raw=iconf, iconf=ic,
@@ -288,7 +330,7 @@ def window_eigh(M, lo, hi):
 
 def run(ci, thr_ev, ax, thrp=1e-4, alphak=-100.0, betaj=-100.0, fthr_ev=None,
         triplet=False, rpa=False, velcorr=None, nroots=None, kshift=None,
-        resp_omegas=None, keep_matrices=False):
+        resp_omegas=None, keep_matrices=False, dual=None, pt2_off=False):
     """Whole sTDA / sTD-DFT core in FP64 (stda.f:454-1471, srpapack.f, linear_response.f).
 
     ``nroots=None`` reproduces the reference's window (all roots in
@@ -302,7 +344,8 @@ def run(ci, thr_ev, ax, thrp=1e-4, alphak=-100.0, betaj=-100.0, fthr_ev=None,
         velcorr = not rpa                        # main.f:102, 662
     inter = intermediates(ci, ax, alphak, betaj)
     t1 = time.perf_counter()
-    sel = select(ci, inter, thr_ev, thrp, fthr_ev, triplet, kshift)
+    sel = select(ci, inter, thr_ev, thrp, fthr_ev, triplet, kshift,
+                 dual=None if dual is None else (dual[0], dual[1], ax), pt2_off=pt2_off)
     t2 = time.perf_counter()
     nci, ak, thr = sel["nci"], sel["ak"], sel["thr"]
     res = OracleResult(nex=sel["nex"], npt=sel["npt"], nci=nci, iconf=sel["iconf"],

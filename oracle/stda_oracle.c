@@ -212,6 +212,60 @@ void orc_screen(int ncent, int no, int nv, const double *epsi, const double *jdi
 }
 
 /*
+ * Dual-threshold screening of std2 -dual, stda-rw_dual.f:712-853: the occupied
+ * MOs are visited in occ_order (the chromophore MOs first, then the rest, each
+ * ascending), each with its own primary threshold thr_i and PT2 window fthr_i;
+ * then the same selection sort.
+ */
+void orc_screen_dual(int ncent, int no, int nv, const double *epsi, const double *jdiag,
+                     const double *pia, const double *qia, double ak, const int *occ_order,
+                     const double *thr_i, const double *fthr_i, int dokshift, double smax,
+                     double width, double steep, int64_t mx, int *iconf, double *ed, int *kconf,
+                     double *edpt, int64_t *nex_out, int64_t *nexpt_out) {
+    int64_t k = 0, j = 0;
+    for (int q = 0; q < no; ++q) {
+        const int io = occ_order[q];
+        for (int l = 0; l < nv; ++l) {
+            double de = epsi[no + l] - epsi[io];
+            de -= jdiag[l + (int64_t)nv * io];
+            int64_t i = (int64_t)nv * io + l;
+            double ek = dot(ncent, pia + (int64_t)ncent * i, qia + (int64_t)ncent * i);
+            de += ak * ek;
+            if (dokshift) de = kshift(de, ek, smax, width, steep);
+            if (de <= thr_i[io]) {
+                iconf[k] = io + 1;
+                iconf[k + mx] = no + l + 1;
+                ed[k] = de;
+                ++k;
+            }
+            if (de > thr_i[io] && de < fthr_i[io]) {
+                kconf[j] = io + 1;
+                kconf[j + mx] = no + l + 1;
+                edpt[j] = de;
+                ++j;
+            }
+        }
+    }
+    int64_t nex = k;
+    for (int64_t ii = 1; ii < nex; ++ii) {   /* stda-rw_dual.f:836-853 */
+        int64_t i = ii - 1, kk = i;
+        double pp = ed[i];
+        for (int64_t jj = ii; jj < nex; ++jj)
+            if (ed[jj] < pp) { kk = jj; pp = ed[jj]; }
+        if (kk == i) continue;
+        ed[kk] = ed[i];
+        ed[i] = pp;
+        for (int m = 0; m < 2; ++m) {
+            int h = iconf[i + m * mx];
+            iconf[i + m * mx] = iconf[kk + m * mx];
+            iconf[kk + m * mx] = h;
+        }
+    }
+    *nex_out = nex;
+    *nexpt_out = j;
+}
+
+/*
  * PT2 selection of secondary CSFs, stda.f:1773-1863.  Appends the selected
  * secondaries to iconf/ed (in kconf order), lowers ed of the primaries by the
  * PT2 sum of the rejected secondaries.  Returns the number added.

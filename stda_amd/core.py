@@ -35,7 +35,7 @@ C_API = [
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
-    "stda_hip_beta_shg", "stda_hip_tpa",
+    "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection",
 ]
 
 
@@ -93,6 +93,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_gspec.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
     L.stda_hip_beta_shg.argtypes = [P, I32, P, F64, P, P]
     L.stda_hip_tpa.argtypes = [P, I32, P, P, P, I64, F64, P, P]
+    L.stda_hip_set_selection.argtypes = [P, F64, I32, P, I32]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -173,6 +174,15 @@ class StdaHip:
             raise StdaHipError(st, f"{msg}: {detail}" if detail else msg)
 
     # ---------------------------------------------------------------- setup
+    def set_selection(self, dual=None, pt2_off: bool = False):
+        """Selection variants for the next setup (stda_hip_set_selection): dual = (thr2_ev,
+        atoms 1-based) as std2 -dual reads them from its 'dual' file; pt2_off: primaries only."""
+        atoms = np.ascontiguousarray([] if dual is None else dual[1], dtype=np.int32)
+        thr2 = 0.0 if dual is None else float(dual[0])
+        self._check(self._L.stda_hip_set_selection(self._h, thr2, len(atoms),
+                                                   _ptr(atoms) if len(atoms) else None,
+                                                   1 if pt2_off else 0))
+
     def setup(self, ci: CoreInputs, thr: float, ax: float, thrp: float = 1e-4,
               alphak: float = -100.0, betaj: float = -100.0, fthr: float | None = None,
               triplet: bool = False, kshift=None):

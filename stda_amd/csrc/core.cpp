@@ -283,6 +283,11 @@ struct stda_hip_ctx {
     double alphak = 0.0, betaj = 0.0;
     int kshift = 0;
     double ks_max = 0.5, ks_width = 0.1, ks_steep = 4.0;
+    // selection variants (stda_hip_set_selection): dual thresholds, no PT2
+    double dual_thr2_ev = 0.0;
+    std::vector<int32_t> dual_atoms;   // 1-based, as listed
+    bool pt2_off = false;
+    std::vector<int> aptr_h;           // AO ranges per atom (AOs sorted by atom)
 
     // per-kernel statistics: HIP-event time on the context's stream and the
     // algorithmic work (flops or bytes) of the launches
@@ -1279,6 +1284,7 @@ static void do_load(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_
         for (int m = 0; m < nao; ++m) cth[m + (size_t)nao * p] = sys->clow[order[m] + (size_t)nao * p];
     c->ct.upload(cth.data(), cth.size(), s);
     c->atom_ptr.upload(aptr.data(), aptr.size(), s);
+    c->aptr_h = aptr;
     c->atom_of.upload(aof.data(), aof.size(), s);
     c->eta.upload(kEta, 94, s);
     c->xyz.upload(sys->xyz, (size_t)4 * na, s);
@@ -1351,12 +1357,62 @@ static void do_select(stda_hip_ctx *c) {
     c->flags.download(fl.data(), fl.size(), s);
     c->de.download(deh.data(), deh.size(), s);
     c->sync();
-    // order-preserving lists (the pair loop order of stda.f:600-631)
-    std::vector<int32_t> prim, sec;
-    for (int64_t p = 0; p < c->npair; ++p) {
-        if (fl[p] & 1) prim.push_back((int32_t)p);
-        if (fl[p] & 2) sec.push_back((int32_t)p);
+    // occupied MOs in the order the screening loop visits them (stda.f:600-631), or for
+    // dual thresholds the chromophore MOs first (stda-rw_dual.f:712-826)
+    std::vector<int> occ_order;
+    if (!c->dual_atoms.empty()) {
+        // chromophore MOs: squared Loewdin coefficients on the listed atoms sum above 0.1
+        // (stda-rw_dual.f:676-698; 0.1 is a default-real literal there)
+        std::vector<double> cth((size_t)nao * no), w(no, 0.0);
+        STDA_HIP_CHECK(hipMemcpyAsync(cth.data(), c->ct.get(), cth.size() * 8,
+                                      hipMemcpyDeviceToHost, s));
+        c->sync();
+        std::vector<double> cnt(na, 0.0);
+        for (int32_t a : c->dual_atoms) {
+            if (a < 1 || a > na) throw Error(STDA_HIP_ERR_ARG, "dual atom list: atom out of range");
+            cnt[a - 1] += 1.0;
+        }
+        for (int a = 0; a < na; ++a) {
+            if (cnt[a] == 0.0) continue;
+            for (int i = 0; i < no; ++i)
+                for (int m = c->aptr_h[a]; m < c->aptr_h[a + 1]; ++m) {
+                    const double v = cth[m + (size_t)nao * i];
+                    w[i] += cnt[a] * (v * v);
+                }
+        }
+        const double t1 = (c->thr * kHartreeEv) / kHartreeEv;   // thresh(1), eV and back
+        const double t2 = c->dual_thr2_ev / kHartreeEv;
+        const double f2 = ((1.0 + (double)0.8f * c->ax) * t2) * 2.0;   // stda-rw_dual.f:776
+        std::vector<double> ti(no), fi(no);
+        std::vector<char> chrom(no);
+        for (int i = 0; i < no; ++i) {
+            chrom[i] = w[i] > (double)0.1f;
+            ti[i] = chrom[i] ? t1 : t2;
+            fi[i] = chrom[i] ? c->fthr : f2;
+        }
+        for (int g = 1; g >= 0; --g)
+            for (int i = 0; i < no; ++i)
+                if (chrom[i] == g) occ_order.push_back(i);
+        for (int64_t p = 0; p < c->npair; ++p) {
+            const int i = (int)(p / nv);
+            unsigned char f = 0;
+            if (deh[p] <= ti[i]) f |= 1;
+            if (deh[p] > ti[i] && deh[p] < fi[i]) f |= 2;
+            fl[p] = f;
+        }
+    } else {
+        for (int i = 0; i < no; ++i) occ_order.push_back(i);
     }
+    if (c->pt2_off)
+        for (auto &f : fl) f &= (unsigned char)~2u;
+    if (!c->dual_atoms.empty() || c->pt2_off) c->flags.upload(fl.data(), fl.size(), s);
+    std::vector<int32_t> prim, sec;
+    for (int i : occ_order)
+        for (int a = 0; a < nv; ++a) {
+            const int64_t p = (int64_t)i * nv + a;
+            if (fl[p] & 1) prim.push_back((int32_t)p);
+            if (fl[p] & 2) sec.push_back((int32_t)p);
+        }
     c->nprim = (int64_t)prim.size();
     c->nsec = (int64_t)sec.size();
     if (c->nprim < 1) throw Error(STDA_HIP_ERR_NO_CSF, "no CSF, increase energy threshold (-e option)");
@@ -2402,6 +2458,16 @@ int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, doubl
             if (hrs) beta_hrs(bf, hrs + 4 * (size_t)f);
         }
     })
+}
+
+int stda_hip_set_selection(stda_hip_ctx *c, double thr2_ev, int32_t natoms, const int32_t *atoms,
+                           int32_t pt2_off) {
+    if (!c || natoms < 0 || (natoms > 0 && (!atoms || !(thr2_ev > 0.0)))) return STDA_HIP_ERR_ARG;
+    c->dual_atoms.assign(atoms, atoms + natoms);
+    c->dual_thr2_ev = natoms > 0 ? thr2_ev : 0.0;
+    c->pt2_off = pt2_off != 0;
+    c->ready = false;   // takes effect at the next select
+    return STDA_HIP_OK;
 }
 
 int stda_hip_tpa(stda_hip_ctx *c, int32_t nstates, const double *eci, const double *xci,

@@ -284,6 +284,31 @@ def test_polyene_2pa_of_solved_states(polyene):
         np.testing.assert_allclose(sg, sgr, rtol=0, atol=1e-7 * np.abs(sgr).max())
 
 
+@pytest.mark.parametrize("variant", ["dual", "dual_all_atoms", "pt2_off", "dual_pt2_off"])
+def test_selection_variants(variant, polyene, cluster30):
+    """-dual thresholds and PT2 off (stda-rw_dual.f:658-853): the same CSF list as the
+    oracle (identical iconf) and the same lowest roots (1e-6 eV).  ed to 1e-10 Hartree plus
+    1e-9 relative: with two thresholds, secondaries of the non-chromophore MOs can lie below
+    chromophore primaries, so PT2 denominators (de_s - de_p) pass near zero and the sums
+    carry large terms whose rounding depends on the summation order."""
+    say(f"selection variant {variant}")
+    cases = [(polyene, 7.0, [1, 2], 5.0), (cluster30, 10.0, [1, 2, 3, 4], 6.0)]
+    for ci, thr, atoms, thr2 in cases:
+        if variant == "dual_all_atoms":
+            atoms = list(range(1, ci.ncent + 1))
+        dual = None if variant == "pt2_off" else (thr2, atoms)
+        off = variant.endswith("pt2_off")
+        ref = O.run(ci, thr, 0.25, nroots=5, dual=dual, pt2_off=off)
+        with _hip() as h:
+            h.set_selection(dual=dual, pt2_off=off)
+            h.setup(ci, thr, 0.25)
+            iconf, ed = h.csf()
+            sol = h.solve_tda(5)
+        np.testing.assert_array_equal(iconf, ref.iconf)
+        np.testing.assert_allclose(ed, ref.ed, rtol=1e-9, atol=1e-10)
+        np.testing.assert_allclose(sol.omega, ref.omega, rtol=0, atol=E_TOL)
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30

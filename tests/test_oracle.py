@@ -251,3 +251,38 @@ def test_2pa_is_quadratic_in_the_state(polyene_inputs):
     for (_, sa, da), (_, sb, db) in zip(a, b):
         np.testing.assert_allclose(sb, -2.0 * sa, rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(db[:3], 4.0 * np.array(da[:3]), rtol=1e-12, atol=1e-14)
+
+
+def test_dual_selection(polyene_inputs):
+    """-dual thresholds (stda-rw_dual.f:658-853).  With every atom in the chromophore the
+    selection is the standard one; with two chromophore atoms and 5 eV for the rest, the
+    primaries are exactly the pairs under their MO's threshold (checked against screening
+    energies from the standard path with an unbounded window), chromophore MOs first before
+    the sort; pt2_off keeps the primaries only, unlowered.  Chromophore: the two terminal
+    carbons (10 of 26 occupied MOs)."""
+    ci = polyene_inputs
+    inter = O.intermediates(ci, 0.25)
+    std = O.select(ci, inter, 7.0)
+    alla = O.select(ci, inter, 7.0, dual=(3.0, list(range(1, ci.ncent + 1)), 0.25))
+    np.testing.assert_array_equal(std["iconf"], alla["iconf"])
+    np.testing.assert_array_equal(std["ed"], alla["ed"])
+    # screening energies of every pair: primaries + secondaries of a window without bound
+    every = O.select(ci, inter, 7.0, pt2_off=True)
+    de = {}
+    for (io, iv), e in zip(every["iconf"], every["ed"]):
+        de[(io, iv)] = e
+    for (io, iv), e in zip(every["kconf"], every["edpt"]):
+        de[(io, iv)] = e
+    order, thr_i, fthr_i = O.dual_groups(ci, [1, 2], 7.0 / O.HARTREE_EV, 5.0,
+                                         1.7976931348623157e308, 0.25)
+    chrom = set(order[:int(np.sum(thr_i > 6.0 / O.HARTREE_EV))].tolist())
+    assert 0 < len(chrom) < ci.no
+    d = O.select(ci, inter, 7.0, dual=(5.0, [1, 2], 0.25))
+    prim = {tuple(x) for x in d["iconf"][:d["nex"]].tolist()}
+    want = {k for k, e in de.items() if e <= thr_i[k[0] - 1]}
+    assert prim == want
+    assert len({k for k in want if k[0] - 1 not in chrom}) > 0
+    assert np.all(np.diff(d["ed_raw"][:d["nex"]]) >= -1e-15) or d["npt"] > 0
+    off = O.select(ci, inter, 7.0, pt2_off=True)
+    assert off["nci"] == off["nex"] and off["npt"] == 0
+    np.testing.assert_array_equal(off["iconf"], std["iconf"][:std["nex"]])
