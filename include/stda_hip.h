@@ -95,6 +95,15 @@ int stda_hip_setup(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip
 int stda_hip_load(stda_hip_ctx *ctx, const stda_hip_system *sys, const stda_hip_params *par);
 int stda_hip_select(stda_hip_ctx *ctx, int64_t *nci, int64_t *nprim, int64_t *nsec);
 
+/* Axis-resolved transition properties of TDA states (std2 -aniso: tdax.dat,
+ * tday.dat, tdaz.dat; stda.f:1360-1376 and, velocity-corrected,
+ * apbtrafo.f:245-262).  w (k) excitation energies, x (nci, k; leading
+ * dimension ldx) their vectors.  out(4,3,k): for axis q = x, y, z the
+ * fL, fV, RL and RV (Rv(corr) when velcorr) of that axis, scaled as the
+ * reference prints them: f = (f_x + f_y + f_z)/3 and R = R_x + R_y + R_z. */
+int stda_hip_props_xyz(stda_hip_ctx *ctx, int32_t k, const double *w, const double *x,
+                       int64_t ldx, int32_t velcorr, double *out);
+
 /* Selection variants, applied by the next stda_hip_select / stda_hip_setup:
  *   natoms > 0: dual thresholds of std2 -dual (stda_rw_dual, stda-rw_dual.f:658-853;
  *     the 'dual' file holds thr2_ev and the atom list).  Occupied MOs whose

@@ -321,6 +321,28 @@ def properties_velcorr(ci, iconf, omega, X, Bhalf, ak):
     return np.stack([fl, fv, rl, rv], axis=1)
 
 
+def properties_xyz(ci, iconf, omega, X, ak, Bhalf=None):
+    """Axis-resolved properties of -aniso, (k, 3, 4): per axis f_L, f_V, R_L and R_V
+    (stda.f:1360-1376) or, with Bhalf, the velocity-corrected ones of apbtrafo
+    (apbtrafo.f:245-262), so that f = (f_x + f_y + f_z)/3 and R = R_x + R_y + R_z."""
+    mom = csf_moments(ci, iconf)
+    ef = 1.0 / (omega + 1.0e-8)
+    l, m, v = mom[0:3] @ X, mom[3:6] @ X, mom[6:9] @ X       # (3, k), without sqrt(ak)
+    out = np.zeros((len(omega), 3, 4))
+    out[:, :, 0] = (l * l * (omega * 2.0 * ak)).T
+    out[:, :, 1] = (v * v * (ef * 2.0 * ak)).T
+    out[:, :, 2] = (l * m * (-ROT * ak)).T
+    if Bhalf is None:
+        out[:, :, 3] = (v * m * (-ROT * ak) * ef).T
+    else:
+        Xn = (Bhalf @ X) * ef[None, :]
+        v2, m2 = mom[6:9] @ Xn, mom[3:6] @ Xn
+        hilf = 1.0 - np.exp(-150.0 * omega * omega)
+        efmod = 1.0 / (omega ** hilf + 1.0e-8)
+        out[:, :, 3] = ((v * m + v2 * m + v * m2) * (-ROT * ak) * efmod).T
+    return out
+
+
 def window_eigh(M, lo, hi):
     """ssyevr RANGE='V': eigenpairs with lo < e <= hi, ascending."""
     e, V = np.linalg.eigh(M)

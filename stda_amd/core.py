@@ -35,7 +35,7 @@ C_API = [
     "stda_hip_batch_destroy", "stda_hip_batch_last_error", "stda_hip_batch_solve_tda",
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
-    "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection",
+    "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
 ]
 
 
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_beta_shg.argtypes = [P, I32, P, F64, P, P]
     L.stda_hip_tpa.argtypes = [P, I32, P, P, P, I64, F64, P, P]
     L.stda_hip_set_selection.argtypes = [P, F64, I32, P, I32]
+    L.stda_hip_props_xyz.argtypes = [P, I32, P, P, I64, I32, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -332,6 +333,15 @@ class StdaHip:
             hv = h[4 * f:4 * (f + 1)]
             out.append((w, t, {"beta_hrs": hv[0], "dr": hv[1], "b2_zzz": hv[2], "b2_xzz": hv[3]}))
         return out
+
+    def props_xyz(self, omega, X, velcorr: bool = True):
+        """Axis-resolved fL, fV, RL, RV of TDA states (stda_hip_props_xyz): (k, 3, 4)."""
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        Xf = np.asfortranarray(X, dtype=np.float64).reshape(self.nci, -1, order="F")
+        out = np.zeros(12 * len(w))
+        self._check(self._L.stda_hip_props_xyz(self._h, len(w), _ptr(w), _ptr(Xf), self.nci,
+                                               1 if velcorr else 0, _ptr(out)))
+        return out.reshape(len(w), 3, 4)
 
     def tpa(self, eci, X, Y, tol: float = 1e-10):
         """Two-photon absorption of the given RPA states (stda_hip_tpa): [(E, sigma (3,3),

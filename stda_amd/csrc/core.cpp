@@ -2460,6 +2460,57 @@ int stda_hip_beta_shg(stda_hip_ctx *c, int32_t nfreq, const double *omega, doubl
     })
 }
 
+int stda_hip_props_xyz(stda_hip_ctx *c, int32_t k, const double *w, const double *x, int64_t ldx,
+                       int32_t velcorr, double *out) {
+    if (!c) return STDA_HIP_ERR_ARG;
+    if (!c->ready) return STDA_HIP_ERR_STATE;
+    if (k < 1 || !w || !x || !out || ldx < c->nci || !c->have_moments) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        const int64_t n = c->nci;
+        hipStream_t s = c->stream;
+        DBuf<double> X((size_t)n * k);
+        for (int j = 0; j < k; ++j)
+            STDA_HIP_CHECK(hipMemcpyAsync(X.get() + n * j, x + ldx * j, (size_t)n * 8,
+                                          hipMemcpyHostToDevice, s));
+        std::vector<double> wv(w, w + k);
+        std::vector<double> T = moment_products(c, X.get(), k), T2;
+        if (velcorr) {   // the B-corrected vectors of apbtrafo (apbtrafo.f:23-69)
+            DBuf<double> Xn((size_t)n * k);
+            apply_ops(c, OP_BHALF, -1, k, X.get(), n, Xn.get(), n, nullptr, 0);
+            std::vector<double> sc(k);
+            for (int j = 0; j < k; ++j) sc[j] = 1.0 / (wv[j] + 1.0e-8);
+            DBuf<double> scd;
+            scd.upload(sc.data(), k, s);
+            launch_scale_cols(s, n, k, Xn.get(), scd.get());
+            T2 = moment_products(c, Xn.get(), k);
+        }
+        // per axis q: f_L, f_V, R_L, R_V or Rv(corr), scaled so that f = (f_x + f_y + f_z)/3
+        // and R = R_x + R_y + R_z (stda.f:1360-1376; apbtrafo.f:245-262)
+        const double ak = c->ak, sak = std::sqrt(ak);
+        for (int j = 0; j < k; ++j) {
+            const double *t = &T[(size_t)9 * j];
+            const double de = wv[j], ef = 1.0 / (de + 1.0e-8);
+            const double hilf = 1.0 - std::exp(-150.0 * de * de);
+            const double efmod = 1.0 / (std::pow(de, hilf) + 1.0e-8);
+            for (int q = 0; q < 3; ++q) {
+                const double l = t[q] * sak, m = t[3 + q] * sak, v = t[6 + q] * sak;
+                double *o = out + 12 * (size_t)j + 4 * q;
+                o[0] = l * l * (de * 2.0);
+                o[1] = v * v * (ef * 2.0);
+                o[2] = -kRot * l * m;
+                if (velcorr) {
+                    const double *u = &T2[(size_t)9 * j];
+                    o[3] = -kRot * (t[6 + q] * t[3 + q] + u[6 + q] * t[3 + q] + t[6 + q] * u[3 + q]) *
+                           ak * efmod;
+                } else {
+                    o[3] = -kRot * v * m * ef;
+                }
+            }
+        }
+    })
+}
+
 int stda_hip_set_selection(stda_hip_ctx *c, double thr2_ev, int32_t natoms, const int32_t *atoms,
                            int32_t pt2_off) {
     if (!c || natoms < 0 || (natoms > 0 && (!atoms || !(thr2_ev > 0.0)))) return STDA_HIP_ERR_ARG;

@@ -4,6 +4,7 @@
   plotting, in either of its two layouts: the velocity-corrected table of
   ``apbtrafo`` (apbtrafo.f:115-128, 259-260; default sTDA) or
   ``print_tdadat`` (stda.f:2235-2265; sTD-DFT and uncorrected sTDA).
+* :func:`tda_dat_xyz` writes the axis-resolved files of ``-aniso`` (tdax/y/z.dat).
 * :func:`gspec` / :func:`gspec_many` are the reference's spectrum program
   ``g_spec`` (g_spec/g_spec.f): they read the text of a tda.dat-style input and
   return the contents of ``rots.dat`` (line spectrum) and ``spec.dat``
@@ -109,6 +110,15 @@ def tda_dat(omega, props, xmolw: float, velcorr: bool = True, thr: float | None 
         out.append("%4d" % (i + 1) + fmt_f(ev, 10, 4) +
                    "".join(fmt_f(float(props[i, c]), 13, 6) for c in range(4)))
     return "\n".join(out) + "\n"
+
+
+def tda_dat_xyz(omega, props_xyz, xmolw: float, velcorr: bool = True,
+                thr: float | None = None):
+    """Texts of tdax.dat, tday.dat and tdaz.dat (-aniso; apbtrafo.f:130-174, stda.f:1414-1425)
+    from the axis-resolved properties (k, 3, 4) of ``StdaHip.props_xyz``, in the layout of
+    :func:`tda_dat`."""
+    p = np.asarray(props_xyz, dtype=np.float64)
+    return tuple(tda_dat(omega, p[:, q, :], xmolw, velcorr, thr) for q in range(3))
 
 
 # ---------------------------------------------------------------- g_spec input

@@ -309,6 +309,31 @@ def test_selection_variants(variant, polyene, cluster30):
         np.testing.assert_allclose(sol.omega, ref.omega, rtol=0, atol=E_TOL)
 
 
+@pytest.mark.parametrize("velcorr", [False, True])
+def test_axis_resolved_properties(polyene, velcorr):
+    """-aniso: per-axis fL, fV, RL, RV (Rv(corr) with velcorr) against the oracle, and the
+    reference's stated sums f = (f_x + f_y + f_z)/3, R = R_x + R_y + R_z against the solve's
+    own totals."""
+    say(f"axis-resolved properties velcorr={velcorr}")
+    ci = polyene
+    ref = O.run(ci, 7.0, 0.25, nroots=6, velcorr=velcorr, keep_matrices=True)
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        sol = h.solve_tda(6, velcorr=velcorr)
+        pxyz = h.props_xyz(sol.omega, sol.X, velcorr=velcorr)
+    np.testing.assert_allclose(pxyz[:, :, :2].sum(1) / 3.0, sol.props[:, :2], rtol=1e-10,
+                               atol=1e-14)
+    np.testing.assert_allclose(pxyz[:, :, 2:].sum(1), sol.props[:, 2:], rtol=1e-10, atol=1e-12)
+    X = sol.X * np.sign((sol.X * ref.X).sum(0))[None, :]
+    oref = O.properties_xyz(ci, ref.iconf, sol.omega, X, 2.0,
+                            ref.bhalf if velcorr else None)
+    np.testing.assert_allclose(pxyz[:, :, :2], oref[:, :, :2], atol=F_TOL, rtol=0)
+    np.testing.assert_allclose(pxyz[:, :, 2:], oref[:, :, 2:], atol=F_TOL, rtol=R_RTOL)
+    from stda_amd import spectrum as S
+    tx, ty, tz = S.tda_dat_xyz(sol.omega, pxyz, ci.xmolw, velcorr=velcorr, thr=0.3)
+    assert tx.splitlines()[1] == (" VELO" if velcorr else " UV")
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30
