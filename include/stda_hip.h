@@ -236,6 +236,12 @@ int stda_hip_gspec(stda_hip_ctx *ctx, int32_t nspec, const int64_t *line_offsets
                    const double *y, const double *wei, const double *emin, const double *step,
                    int32_t npts, int32_t flags, double rfaktor, double *e, double *r);
 
+/* The lines of spec.dat (g_spec.f format 1000, (f16.6, e14.4)) for n grid
+ * points: out receives 31 n bytes, each line 30 characters and a newline, no
+ * terminating zero.  Host only (no device work, no context).  No counterpart in
+ * the reference, whose program writes the file with Fortran I/O. */
+int stda_hip_format_curve(int64_t n, const double *e, const double *r, char *out);
+
 /* Arithmetic of the coupling operators in the solvers of this context:
  *   0 (default) FP64 throughout;
  *   1 mixed: the transition-density contractions (Q gamma Q^T products) in

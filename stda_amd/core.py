@@ -36,6 +36,7 @@ C_API = [
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
+    "stda_hip_format_curve",
 ]
 
 
@@ -95,6 +96,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_tpa.argtypes = [P, I32, P, P, P, I64, F64, P, P]
     L.stda_hip_set_selection.argtypes = [P, F64, I32, P, I32]
     L.stda_hip_props_xyz.argtypes = [P, I32, P, P, I64, I32, P]
+    L.stda_hip_format_curve.argtypes = [I64, P, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]

@@ -2628,6 +2628,79 @@ int stda_hip_tpa(stda_hip_ctx *c, int32_t nstates, const double *eci, const doub
     })
 }
 
+// Fortran Fw.d / Ew.d output of the spectrum files (host only; no device calls).
+// Non-finite numbers as the flang runtime writes them ("Inf", "-Inf", "NaN").
+static int fort_special(double v, int w, char *out) {
+    const char *t = std::isnan(v) ? "NaN" : (std::isinf(v) ? (v > 0 ? "Inf" : "-Inf") : nullptr);
+    if (!t) return 0;
+    std::snprintf(out, (size_t)w + 1, "%*s", w, t);
+    return 1;
+}
+
+static void fort_f(double v, int w, int d, char *out) {   // Fw.d, asterisks on overflow
+    if (fort_special(v, w, out)) return;
+    char b[400];
+    const int len = std::snprintf(b, sizeof b, "%.*f", d, v);
+    if (len > w) {
+        std::memset(out, '*', (size_t)w);
+        out[w] = 0;
+    } else {
+        std::snprintf(out, (size_t)w + 1, "%*s", w, b);
+    }
+}
+
+static void fort_e(double v, int w, int d, char *out) {   // Ew.d: 0.dddd, E+xx or +xxx
+    if (fort_special(v, w, out)) return;
+    char mant[32], b[64];
+    int ex = 0;
+    if (v == 0.0) {
+        std::memset(mant, '0', (size_t)d);
+        mant[d] = 0;
+    } else {
+        char t[64];
+        std::snprintf(t, sizeof t, "%.*e", d - 1, std::fabs(v));   // d.ddde+XX
+        int q = 0;
+        for (const char *p = t; *p && *p != 'e'; ++p)
+            if (*p != '.') mant[q++] = *p;
+        mant[q] = 0;
+        ex = std::atoi(std::strchr(t, 'e') + 1) + 1;
+    }
+    const char *sg = std::signbit(v) ? "-" : "";
+    if (ex >= -99 && ex <= 99)
+        std::snprintf(b, sizeof b, "%s0.%sE%c%02d", sg, mant, ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    else
+        std::snprintf(b, sizeof b, "%s0.%s%c%03d", sg, mant, ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    std::snprintf(out, (size_t)w + 1, "%*s", w, b);
+}
+
+int stda_hip_format_curve(int64_t n, const double *e, const double *r, char *out) {
+    if (n < 0 || (n > 0 && (!e || !r || !out))) return STDA_HIP_ERR_ARG;
+    // lines are independent: split them over up to 16 host threads for long inputs
+    const int64_t per = 4096;
+    const int nth = (int)std::min<int64_t>(
+        std::max<int64_t>(1, (n + per - 1) / per),
+        std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    auto work = [&](int t) {
+        for (int64_t i = t; i < n; i += nth) {
+            char *o = out + 31 * i;
+            char line[48];
+            fort_f(e[i], 16, 6, line);
+            fort_e(r[i], 14, 4, line + 16);
+            std::memcpy(o, line, 30);
+            o[30] = '\n';
+        }
+    };
+    if (nth == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th) x.join();
+    }
+    return STDA_HIP_OK;
+}
+
 static_assert(STDA_HIP_GSPEC_UV == GSPEC_UV && STDA_HIP_GSPEC_NM == GSPEC_NM &&
                   STDA_HIP_GSPEC_CM == GSPEC_CM && STDA_HIP_GSPEC_LOG == GSPEC_LOG,
               "spectrum flags of the header and the kernel agree");

@@ -391,7 +391,15 @@ def _rots(x, y, o) -> str:
 
 
 def _spec_text(e: np.ndarray, r: np.ndarray) -> str:
-    return "".join(fmt_f(a) + fmt_e(b) + "\n" for a, b in zip(e.tolist(), r.tolist()))
+    """spec.dat lines, formatted by the library (stda_hip_format_curve)."""
+    from .core import _ptr, load_library
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    buf = np.zeros(31 * len(e), dtype=np.uint8)
+    st = load_library().stda_hip_format_curve(len(e), _ptr(e), _ptr(r), _ptr(buf))
+    if st != 0:
+        raise ValueError("stda_hip_format_curve failed")
+    return buf.tobytes().decode("ascii")
 
 
 # ---------------------------------------------------------------- broadening on the device
@@ -415,10 +423,20 @@ def broaden(h, inputs: list) -> list:
 
 
 def gspec_many(h, texts: list) -> list:
-    """g_spec on many inputs: [(rots.dat text, spec.dat text)], broadening in one launch."""
+    """g_spec on many inputs: [(rots.dat text, spec.dat text)], broadening in one launch and
+    the spec.dat lines of all inputs formatted in one library call."""
     inputs = [parse_gspec(t) for t in texts]
     curves = broaden(h, inputs)
-    return [(g.rots, _spec_text(*c)) for g, c in zip(inputs, curves)]
+    if not curves:
+        return []
+    text = _spec_text(np.concatenate([c[0] for c in curves]),
+                      np.concatenate([c[1] for c in curves]))
+    out, at = [], 0
+    for g, c in zip(inputs, curves):
+        n = 31 * len(c[0])
+        out.append((g.rots, text[at:at + n]))
+        at += n
+    return out
 
 
 def gspec(h, text: str):

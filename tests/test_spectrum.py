@@ -172,3 +172,17 @@ def test_tda_dat_print_layout():
     # g_spec reads the strengths back from the columns the header selects (UV: fL)
     g = S.parse_gspec(text)
     assert np.allclose(sorted(g.y.tolist()), [0.0, 0.1], rtol=1e-15, atol=0)
+
+
+def test_native_formatter_matches_the_oracle():
+    """spec.dat lines from the library (stda_hip_format_curve, host only) equal the oracle's
+    Fortran formats on values that reach every branch: subnormals and three-digit exponents,
+    signed zeros, rounding carries, F overflow, non-finite numbers, and a batch long enough
+    to be split over threads."""
+    rng = np.random.default_rng(7)
+    r = np.concatenate([rng.normal(size=9000) * 10.0 ** rng.integers(-320, 300, 9000),
+                        [0.0, -0.0, 1e-320, 5e-324, 0.99996, -0.099995, 9.99949e10, np.inf,
+                         -np.inf, np.nan]])
+    e = rng.normal(size=len(r)) * 10.0 ** rng.integers(-3, 12, len(r))
+    got = S._spec_text(e, r)
+    assert got == "".join(OG.fmt_line(a, b) + "\n" for a, b in zip(e.tolist(), r.tolist()))
