@@ -23,6 +23,7 @@ edit descriptors, so that outputs match the reference program's files.
 from __future__ import annotations
 
 import math
+import re
 import sys
 from dataclasses import dataclass, field
 
@@ -122,8 +123,36 @@ def tda_dat_xyz(omega, props_xyz, xmolw: float, velcorr: bool = True,
 
 
 # ---------------------------------------------------------------- g_spec input
+_SIMPLE = re.compile(r"-?(?:\d{1,10}(?:\.\d{0,10})?|\.\d{0,10})\Z")
+_SCALE = [1.0]
+for _ in range(10):
+    _SCALE.append(_SCALE[-1] / 10)       # READAA's X = X/10, repeated
+
+
 def read_numbers(line: str) -> list:
-    """The numbers of one input line as READL reads them (g_spec.f:689-790)."""
+    """The numbers of one input line as READL reads them (g_spec.f:689-790).
+
+    Lines of printable characters whose blank-separated tokens are all plain decimals (at
+    most ten digits on either side of the point) take a direct path with READAA's
+    arithmetic; anything else goes through the character scan."""
+    a = line[:120].rstrip(" ")
+    if len(a) < 120 and a.isprintable():
+        vals = []
+        for t in a.split():
+            if not _SIMPLE.match(t):
+                break
+            neg = t[0] == "-"
+            ip, _, fp = (t[1:] if neg else t).partition(".")
+            v = float(int(ip)) if ip else 0.0
+            if fp:
+                v = v + float(int(fp)) * _SCALE[len(fp)]
+            vals.append(-1.0 * v if neg else v)
+        else:
+            return vals
+    return _read_numbers_scan(line)
+
+
+def _read_numbers_scan(line: str) -> list:
     a = line[:120].ljust(120)
     nl = len(a)
     vals = []

@@ -93,7 +93,9 @@ def test_number_reader_matches_oracle():
     rng = np.random.default_rng(5)
     lines = ["   1   14.0839     0.003718     0.024292    -0.000001    -0.000002",
              "   18.015200000000000     ", "  0.5", " -.25 7 1.5E-3 2D+02 -3.0E-5x 12345678901234.5",
-             "abc -def - 4", " 0.2046E-63 0.4200-315 1.23456789012345", "x" * 130 + " 5"]
+             "abc -def - 4", " 0.2046E-63 0.4200-315 1.23456789012345", "x" * 130 + " 5",
+             " . -. 1. .5 -0.000000 0012 -3 1234567890.1234567890", " 7" + " " * 117 + "8",
+             "1\t2 3", "   1   14.0839" + " " * 110 + "9"]
     for _ in range(200):
         toks = []
         for _ in range(rng.integers(1, 7)):
