@@ -4,7 +4,9 @@
   plotting, in either of its two layouts: the velocity-corrected table of
   ``apbtrafo`` (apbtrafo.f:115-128, 259-260; default sTDA) or
   ``print_tdadat`` (stda.f:2235-2265; sTD-DFT and uncorrected sTDA).
-* :func:`tda_dat_xyz` writes the axis-resolved files of ``-aniso`` (tdax/y/z.dat).
+* :func:`tda_dat_xyz` writes the axis-resolved files of ``-aniso`` (tdax/y/z.dat);
+  :func:`beta_hrs_file` and :func:`tpa_abs_file` the beta_HRS and 2PA-abs files of
+  ``-resp`` and ``-2PA``.
 * :func:`gspec` / :func:`gspec_many` are the reference's spectrum program
   ``g_spec`` (g_spec/g_spec.f): they read the text of a tda.dat-style input and
   return the contents of ``rots.dat`` (line spectrum) and ``spec.dat``
@@ -80,6 +82,39 @@ def fmt_list_real8(v: float) -> str:
         return " " + (m + "E%+04d" % int(e)).rjust(25)
     k = 1 if a == 0.0 else len(str(int(a))) if a >= 1 else 0
     return " " + ("%.*f" % (17 - k, v)).rjust(20) + " " * 5
+
+
+def fmt_list_real4(v: float) -> str:
+    """List-directed output of one real*4 as gfortran writes it (blank, then a G16.9 field:
+    9 significant digits, F form for 0.1 <= |v| < 1e9, zero as 0.00000000)."""
+    x = float(np.float32(v))
+    a = abs(x)
+    if a != 0.0 and (a < 0.1 or a >= 1e9):
+        m, e = ("%.8E" % x).split("E")
+        return " " + (m + "E%+03d" % int(e)).rjust(16)
+    k = 1 if a == 0.0 else (len(str(int(a))) if a >= 1 else 0)
+    return " " + ("%.*f" % (9 - k, x)).rjust(12) + " " * 4
+
+
+def beta_hrs_file(omegas, beta_hrs) -> str:
+    """The reference's beta_HRS file (linear_response.f:695, list-directed
+    ``freq*27.21139, beta_HRS``: the frequency in eV formed in single precision)."""
+    out = []
+    for w, b in zip(omegas, beta_hrs):
+        ev = np.float32(np.float32(w) * np.float32(27.21139))
+        out.append(fmt_list_real4(float(ev)) + fmt_list_real8(float(b)))
+    return "\n".join(out) + "\n"
+
+
+def tpa_abs_file(eci, deltas) -> str:
+    """The reference's 2PA-abs file (linear_response.f:4905-4909, format (I3,F7.3,4F20.3)):
+    state, energy in eV (single precision) and delta parallel, perpendicular, circular, rho."""
+    out = []
+    for i, (e, d) in enumerate(zip(eci, deltas)):
+        ev = float(np.float32(np.float32(e) * np.float32(27.21139)))
+        out.append(("%3d" % (i + 1))[-3:] + fmt_f(ev, 7, 3) +
+                   "".join(fmt_f(float(np.float32(x)), 20, 3) for x in d))
+    return "\n".join(out) + "\n"
 
 
 # ---------------------------------------------------------------- tda.dat

@@ -188,3 +188,27 @@ def test_native_formatter_matches_the_oracle():
     e = rng.normal(size=len(r)) * 10.0 ** rng.integers(-3, 12, len(r))
     got = S._spec_text(e, r)
     assert got == "".join(OG.fmt_line(a, b) + "\n" for a, b in zip(e.tolist(), r.tolist()))
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_beta_hrs_file_reproduces_reference_file(case):
+    """beta_HRS (linear_response.f:695: list-directed real*4 energy in eV and real*8 beta)
+    rebuilt from the frequencies of the wavelength file and the file's own beta values is
+    the reference's BETA_TEST file byte for byte."""
+    sys.path.insert(0, HERE)
+    from conftest import beta_test_omegas
+    with open(os.path.join(WATER, f"BETA_TEST{case}.txt")) as f:
+        ref = f.read()
+    vals = [float(l.split()[1]) for l in ref.splitlines()]
+    assert S.beta_hrs_file(beta_test_omegas(), vals) == ref
+
+
+@pytest.mark.parametrize("case", [1, 2, 3])
+def test_tpa_abs_file_reproduces_reference_file(case):
+    """2PA-abs (format (I3,F7.3,4F20.3)) rebuilt from the rows of the reference's 2PA_TEST
+    file (energies converted back to Hartree) is that file byte for byte."""
+    with open(os.path.join(WATER, f"2PA_TEST{case}.txt")) as f:
+        ref = f.read()
+    rows = [[float(t) for t in l.split()] for l in ref.splitlines()]
+    eci = [r[1] / 27.21139 for r in rows]
+    assert S.tpa_abs_file(eci, [r[2:6] for r in rows]) == ref
