@@ -104,6 +104,20 @@ int stda_hip_select(stda_hip_ctx *ctx, int64_t *nci, int64_t *nprim, int64_t *ns
 int stda_hip_props_xyz(stda_hip_ctx *ctx, int32_t k, const double *w, const double *x,
                        int64_t ldx, int32_t velcorr, double *out);
 
+/* The upstream transforms before the boundary, on the device: the nine MO
+ * moment arrays moments(npk, 9) = shrink(ca^T A_q ca) (onetri.f:20-55,
+ * stda.f:351-412, shrink stda.f:1695-1706; A_q = ao9[q]: dipole x, y, z,
+ * angular momentum x, y, z, velocity x, y, z; packed as the reference's
+ * lin(i,j), npk = moci (moci+1)/2) and the Loewdin orbitals clow = S^1/2 ca
+ * (makel, stda.f:1615-1691, 430-452).  ca (nao, moci), S (nao, nao),
+ * ao9 (nao, nao, 9), clow (nao, moci), all column-major host arrays.
+ * A negative eigenvalue of S stops as the reference does unless bfw (-BFW:
+ * set to zero); |<c'|c'> - 1| > 1e-2 is the reference's MO norm error.
+ * Needs a context (its stream and BLAS handle), not a loaded system. */
+int stda_hip_transform(stda_hip_ctx *ctx, int32_t nao, int32_t moci, const double *ca,
+                       const double *S, const double *ao9, int32_t bfw, double *clow,
+                       double *moments);
+
 /* Selection variants, applied by the next stda_hip_select / stda_hip_setup:
  *   natoms > 0: dual thresholds of std2 -dual (stda_rw_dual, stda-rw_dual.f:658-853;
  *     the 'dual' file holds thr2_ev and the atom list).  Occupied MOs whose

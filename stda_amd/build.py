@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstda_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["kernels.hip", "batch.hip", "spectrum.hip", "hyper.hip", "core.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "spectrum.hip", "hyper.hip", "transform.hip", "core.cpp"]
 HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
             # packed FP32 instructions stay off (hipcc's host pass warns about the feature)
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,7 +40,7 @@ def build(verbose: bool = False) -> str:
         objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC",
-           *objs, "-o", tmp, "-L", os.path.join(ROCM, "lib"), "-lrocblas",
+           *objs, "-o", tmp, "-L", os.path.join(ROCM, "lib"), "-lrocblas", "-lrocsolver",
            "-ldl", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -36,7 +36,7 @@ C_API = [
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
-    "stda_hip_format_curve",
+    "stda_hip_format_curve", "stda_hip_transform",
 ]
 
 
@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_set_selection.argtypes = [P, F64, I32, P, I32]
     L.stda_hip_props_xyz.argtypes = [P, I32, P, P, I64, I32, P]
     L.stda_hip_format_curve.argtypes = [I64, P, P, P]
+    L.stda_hip_transform.argtypes = [P, I32, I32, P, P, P, I32, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -335,6 +336,20 @@ class StdaHip:
             hv = h[4 * f:4 * (f + 1)]
             out.append((w, t, {"beta_hrs": hv[0], "dr": hv[1], "b2_zzz": hv[2], "b2_xzz": hv[3]}))
         return out
+
+    def transform(self, ca, S, ao9, bfw: bool = False):
+        """Device upstream transforms (stda_hip_transform): (clow (nao, moci), moments (9, npk))
+        from the window's MO coefficients ca, the overlap S and the nine AO moment matrices."""
+        ca = np.asfortranarray(ca, dtype=np.float64)
+        nao, moci = ca.shape
+        Sf = np.asfortranarray(S, dtype=np.float64)
+        a9 = np.ascontiguousarray(np.stack([np.asfortranarray(a, dtype=np.float64).ravel(order="F")
+                                            for a in ao9]))
+        clow = np.zeros((nao, moci), order="F")
+        mom = np.zeros((9, moci * (moci + 1) // 2))
+        self._check(self._L.stda_hip_transform(self._h, nao, moci, _ptr(ca), _ptr(Sf), _ptr(a9),
+                                               1 if bfw else 0, _ptr(clow), _ptr(mom)))
+        return clow, mom
 
     def props_xyz(self, omega, X, velcorr: bool = True):
         """Axis-resolved fL, fV, RL, RV of TDA states (stda_hip_props_xyz): (k, 3, 4)."""

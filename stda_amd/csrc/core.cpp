@@ -20,6 +20,8 @@
 
 #include "../../include/stda_hip.h"
 #include "common.h"
+#include <rocsolver/rocsolver.h>
+
 #include "devbuf.h"
 #include "selsort.h"
 
@@ -2507,6 +2509,64 @@ int stda_hip_props_xyz(stda_hip_ctx *c, int32_t k, const double *w, const double
                     o[3] = -kRot * v * m * ef;
                 }
             }
+        }
+    })
+}
+
+int stda_hip_transform(stda_hip_ctx *c, int32_t nao, int32_t moci, const double *ca,
+                       const double *S, const double *ao9, int32_t bfw, double *clow,
+                       double *moments) {
+    if (!c || nao < 1 || moci < 1 || moci > nao || !ca || !S || !ao9 || !clow || !moments)
+        return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        const int64_t na = nao, nm = moci, npk = nm * (nm + 1) / 2;
+        DBuf<double> CA, M((size_t)na * na), T((size_t)na * nm), P((size_t)nm * nm);
+        DBuf<double> OUT((size_t)9 * npk);
+        CA.upload(ca, (size_t)na * nm, s);
+        // moments: shrink(ca^T M_q ca) for the 9 AO matrices (stda.f:351-412, onetri.f:20-55)
+        for (int q = 0; q < 9; ++q) {
+            STDA_HIP_CHECK(hipMemcpyAsync(M.get(), ao9 + (size_t)q * na * na, (size_t)na * na * 8,
+                                          hipMemcpyHostToDevice, s));
+            dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
+            dgemm(c->blas, true, false, nm, nm, na, 1.0, CA.get(), na, T.get(), na, 0.0, P.get(), nm);
+            launch_pack_upper(s, nm, P.get(), OUT.get() + (size_t)q * npk);
+        }
+        OUT.download(moments, (size_t)9 * npk, s);
+        // Loewdin: clow = S^1/2 ca = U (sqrt(e) o (U^T ca)) (makel, stda.f:1615-1691)
+        STDA_HIP_CHECK(hipMemcpyAsync(M.get(), S, (size_t)na * na * 8, hipMemcpyHostToDevice, s));
+        DBuf<double> D(na), E(na);
+        DBuf<int> info(1);
+        STDA_BLAS_CHECK(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                         (rocblas_int)na, M.get(), (rocblas_int)na, D.get(),
+                                         E.get(), info.get()));
+        std::vector<double> e(na);
+        int infoh = 0;
+        D.download(e.data(), na, s);
+        STDA_HIP_CHECK(hipMemcpyAsync(&infoh, info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        c->sync();
+        if (infoh != 0) throw Error(STDA_HIP_ERR_DEVICE, "eigensolve of S did not converge");
+        for (auto &v : e) {
+            if (v < 0.0) {
+                if (!bfw) throw Error(STDA_HIP_ERR_ARG, "negative eigenvalue of S: must stop in S^1/2");
+                v = 0.0;   // -BFW (stda.f:1660-1670)
+            }
+            v = std::sqrt(v);
+        }
+        D.upload(e.data(), na, s);
+        dgemm(c->blas, true, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
+        launch_scale_rows(s, na, nm, D.get(), T.get());
+        dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, T.get(), na, 0.0, CA.get(), na);
+        CA.download(clow, (size_t)na * nm, s);
+        c->sync();
+        // |<c'|c'> - 1| > 1e-2 stops the reference (stda.f:445-450)
+        for (int64_t j = 0; j < nm; ++j) {
+            double nr = 0.0;
+            for (int64_t i = 0; i < na; ++i) nr += clow[i + na * j] * clow[i + na * j];
+            if (std::fabs(nr - 1.0) > 1.0e-2)
+                throw Error(STDA_HIP_ERR_ARG, "internal MO norm error (MO " + std::to_string(j + 1) +
+                                                  ", norm " + std::to_string(nr) + ")");
         }
     })
 }

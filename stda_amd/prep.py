@@ -89,8 +89,12 @@ def _selection_sort_quirk(eps, C_full, moci):
 
 
 def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
-            bfw: bool = False) -> CoreInputs:
-    """Build the core's FP64 inputs for molecule ``mol`` (see module docstring)."""
+            bfw: bool = False, device=None) -> CoreInputs:
+    """Build the core's FP64 inputs for molecule ``mol`` (see module docstring).
+
+    ``device``: a :class:`stda_amd.core.StdaHip` context; the moment transforms and the
+    Loewdin orthogonalisation (steps 3-4) then run on it (``stda_hip_transform``),
+    otherwise with numpy on the host."""
     eps, occ, C = mol.eps, mol.occ, mol.C
     thr = thr_ev / HARTREE_EV
     deps = 2.0 * (1.0 + 0.8 * ax) * thr
@@ -117,6 +121,15 @@ def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
     if ints is None:
         ints = one_electron(mol)
     S = ints["S"]
+    if device is not None:
+        from .core import StdaHipError
+        ao9 = [ints["R"][k] for k in range(3)] + [ints["M"][k] for k in range(3)] + \
+              [ints["V"][k] for k in range(3)]
+        try:
+            clow, mom = device.transform(ca, S, ao9, bfw=bfw)
+        except StdaHipError as err:
+            raise StdaInputError(str(err)) from err
+        return _core_inputs(mol, no, nv, clow, epsi, mom, othr, vthr)
     mom = np.empty((9, len(sel) * (len(sel) + 1) // 2))
     for k in range(3):
         mom[k] = shrink(ca.T @ ints["R"][k] @ ca)
@@ -133,6 +146,10 @@ def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
     bad = np.nonzero(np.abs(nrm - 1.0) > 1e-2)[0]
     if len(bad):
         raise StdaInputError(f"internal MO norm error (MO {bad[0]+1}, norm {nrm[bad[0]]})")
+    return _core_inputs(mol, no, nv, clow, epsi, mom, othr, vthr)
+
+
+def _core_inputs(mol, no, nv, clow, epsi, mom, othr, vthr) -> CoreInputs:
     xyz = np.zeros((4, mol.natoms))
     xyz[:3] = mol.xyz.T
     xyz[3] = mol.Z

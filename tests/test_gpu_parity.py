@@ -334,6 +334,46 @@ def test_axis_resolved_properties(polyene, velcorr):
     assert tx.splitlines()[1] == (" VELO" if velcorr else " UV")
 
 
+@pytest.mark.parametrize("which", ["water", "polyene", "cluster30"])
+def test_device_transforms(which, water):
+    """stda_hip_transform (moments and Loewdin on the device) against the host numpy
+    preparation: Loewdin orbitals and the nine packed moment arrays to 1e-12 (of the
+    largest element); for water the GPU-prepared inputs also reproduce EXCI_TEST1."""
+    from stda_amd import synth
+    from stda_amd.aoints import one_electron
+    say(f"device transforms {which}")
+    if which == "water":
+        mol, thr, ax = water, 20.0, 1.0
+        ints = one_electron(mol)
+    elif which == "polyene":
+        (mol, ints), thr, ax = synth.polyene(), 7.0, 0.25
+    else:
+        (mol, ints), thr, ax = synth.cluster(30, seed=3, basis="minimal"), 10.0, 0.25
+    host = prepare(mol, thr, ax, ints)
+    with _hip() as h:
+        dev = prepare(mol, thr, ax, ints, device=h)
+    np.testing.assert_allclose(dev.clow, host.clow, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dev.moments, host.moments, rtol=0,
+                               atol=1e-12 * np.abs(host.moments).max())
+    if which == "water":
+        r = O.run(dev, thr, ax)
+        exp = read_datxy(os.path.join(GOLDEN, "water", "EXCI_TEST1.dat"))
+        for k, row in enumerate(exp):
+            assert abs(r.omega[k] * O.HARTREE_EV_PRINT - row[1]) <= 0.5e-4 + 1e-6
+
+
+def test_device_transform_negative_overlap(water):
+    """A negative eigenvalue of S stops as the reference does ('must stop in S^1/2')."""
+    from stda_amd.aoints import one_electron
+    from stda_amd.prep import StdaInputError
+    ints = dict(one_electron(water))
+    e, U = np.linalg.eigh(ints["S"])
+    ints["S"] = ints["S"] - 2.0 * e[0] * np.outer(U[:, 0], U[:, 0])
+    with _hip() as h:
+        with pytest.raises(StdaInputError, match="negative eigenvalue"):
+            prepare(water, 20.0, 1.0, ints, device=h)
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30
