@@ -104,6 +104,20 @@ int stda_hip_select(stda_hip_ctx *ctx, int64_t *nci, int64_t *nprim, int64_t *ns
 int stda_hip_props_xyz(stda_hip_ctx *ctx, int32_t k, const double *w, const double *x,
                        int64_t ldx, int32_t velcorr, double *out);
 
+/* One-electron AO integrals over contracted Cartesian Gaussians, one thread
+ * per AO pair (replaces the libcint calls of the reference's integral driver:
+ * overlap; dipole length -<mu|r|nu>, libcint.f:252-352; velocity
+ * -<mu|d/dr|nu>, libcint.f:653-756; angular momentum <mu|r x d/dr|nu>,
+ * intpack.f90:1208-1283; origin 0).  Per AO m: primitives
+ * [prim_ptr[m], prim_ptr[m+1]) with exponents alpha and contraction
+ * coefficients coef (normalisation included), centre center[3m..3m+2] (Bohr),
+ * Cartesian exponents lmn[3m..3m+2] (each <= 4).  Out (column-major, nao x
+ * nao): S and ao9 = dipole x, y, z, angular momentum x, y, z, velocity
+ * x, y, z (the order stda_hip_transform takes). */
+int stda_hip_ao_ints(stda_hip_ctx *ctx, int32_t nao, const int64_t *prim_ptr, const double *alpha,
+                     const double *coef, const double *center, const int32_t *lmn, double *S,
+                     double *ao9);
+
 /* The upstream transforms before the boundary, on the device: the nine MO
  * moment arrays moments(npk, 9) = shrink(ca^T A_q ca) (onetri.f:20-55,
  * stda.f:351-412, shrink stda.f:1695-1706; A_q = ao9[q]: dipole x, y, z,

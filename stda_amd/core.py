@@ -36,7 +36,7 @@ C_API = [
     "stda_hip_batch_timings", "stda_hip_batch_kernel_bytes", "stda_hip_batch_configure",
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
-    "stda_hip_format_curve", "stda_hip_transform",
+    "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
 ]
 
 
@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_props_xyz.argtypes = [P, I32, P, P, I64, I32, P]
     L.stda_hip_format_curve.argtypes = [I64, P, P, P]
     L.stda_hip_transform.argtypes = [P, I32, I32, P, P, P, I32, P, P]
+    L.stda_hip_ao_ints.argtypes = [P, I32, P, P, P, P, P, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -336,6 +337,23 @@ class StdaHip:
             hv = h[4 * f:4 * (f + 1)]
             out.append((w, t, {"beta_hrs": hv[0], "dr": hv[1], "b2_zzz": hv[2], "b2_xzz": hv[3]}))
         return out
+
+    def ao_ints(self, mol):
+        """One-electron AO integrals on the device (stda_hip_ao_ints), in the layout of
+        :func:`stda_amd.aoints.one_electron`: {"S": (nao, nao), "R", "M", "V": (3, nao, nao)}."""
+        nao = mol.nao
+        ptr = np.ascontiguousarray(mol.ao_prim_ptr, dtype=np.int64)
+        al = np.ascontiguousarray(mol.prim_alpha, dtype=np.float64)
+        co = np.ascontiguousarray(mol.prim_coef, dtype=np.float64)
+        cen = np.ascontiguousarray(mol.xyz[mol.ao_atom], dtype=np.float64)
+        lmn = np.ascontiguousarray(mol.ao_lmn, dtype=np.int32)
+        S = np.zeros((nao, nao), order="F")
+        ao9 = np.zeros((9, nao, nao))
+        self._check(self._L.stda_hip_ao_ints(self._h, nao, _ptr(ptr), _ptr(al), _ptr(co),
+                                             _ptr(cen), _ptr(lmn), _ptr(S), _ptr(ao9)))
+        # ao9[q] holds a column-major matrix: transpose the C-order view
+        m = [ao9[q].T for q in range(9)]
+        return {"S": S, "R": np.stack(m[0:3]), "M": np.stack(m[3:6]), "V": np.stack(m[6:9])}
 
     def transform(self, ca, S, ao9, bfw: bool = False):
         """Device upstream transforms (stda_hip_transform): (clow (nao, moci), moments (9, npk))

@@ -164,6 +164,9 @@ void launch_mu_blocks(hipStream_t s, const double *moments, int64_t npk, int no,
 // upstream transforms (transform.hip)
 void launch_pack_upper(hipStream_t s, int64_t n, const double *P, double *out);
 void launch_scale_rows(hipStream_t s, int64_t m, int64_t k, const double *f, double *T);
+void launch_ao_ints(hipStream_t s, int nao, const int64_t *ptr, const double *alpha,
+                    const double *coef, const double *cen, const int32_t *lmn, double *S,
+                    double *ao9);
 // spectrum broadening (spectrum.hip); flags as STDA_HIP_GSPEC_* of the public header
 enum { GSPEC_UV = 1, GSPEC_NM = 2, GSPEC_CM = 4, GSPEC_LOG = 8 };
 void launch_gspec(hipStream_t s, int nspec, const int64_t *off, const double *x, const double *y,

@@ -2513,6 +2513,39 @@ int stda_hip_props_xyz(stda_hip_ctx *c, int32_t k, const double *w, const double
     })
 }
 
+int stda_hip_ao_ints(stda_hip_ctx *c, int32_t nao, const int64_t *prim_ptr, const double *alpha,
+                     const double *coef, const double *center, const int32_t *lmn, double *S,
+                     double *ao9) {
+    if (!c || nao < 1 || !prim_ptr || !alpha || !coef || !center || !lmn || !S || !ao9)
+        return STDA_HIP_ERR_ARG;
+    if (prim_ptr[0] != 0) return STDA_HIP_ERR_ARG;
+    for (int32_t m = 0; m < nao; ++m) {
+        if (prim_ptr[m + 1] <= prim_ptr[m]) return STDA_HIP_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (lmn[3 * m + d] < 0 || lmn[3 * m + d] > 4) return STDA_HIP_ERR_ARG;   // AO_LMAX
+    }
+    const int64_t nprim = prim_ptr[nao];
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        const size_t nn = (size_t)nao * nao;
+        DBuf<int64_t> P;
+        DBuf<double> AL, CO, CE, SO(nn), AO((size_t)9 * nn);
+        DBuf<int32_t> LM;
+        P.upload(prim_ptr, (size_t)nao + 1, s);
+        AL.upload(alpha, (size_t)nprim, s);
+        CO.upload(coef, (size_t)nprim, s);
+        CE.upload(center, (size_t)3 * nao, s);
+        LM.upload(lmn, (size_t)3 * nao, s);
+        EventTimer ev(c);
+        launch_ao_ints(s, nao, P.get(), AL.get(), CO.get(), CE.get(), LM.get(), SO.get(), AO.get());
+        ev.finish("ao_ints", (double)nn);
+        SO.download(S, nn, s);
+        AO.download(ao9, 9 * nn, s);
+        c->sync();
+    })
+}
+
 int stda_hip_transform(stda_hip_ctx *c, int32_t nao, int32_t moci, const double *ca,
                        const double *S, const double *ao9, int32_t bfw, double *clow,
                        double *moments) {

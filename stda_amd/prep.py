@@ -92,9 +92,10 @@ def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
             bfw: bool = False, device=None) -> CoreInputs:
     """Build the core's FP64 inputs for molecule ``mol`` (see module docstring).
 
-    ``device``: a :class:`stda_amd.core.StdaHip` context; the moment transforms and the
-    Loewdin orthogonalisation (steps 3-4) then run on it (``stda_hip_transform``),
-    otherwise with numpy on the host."""
+    ``device``: a :class:`stda_amd.core.StdaHip` context; the AO integrals (when ``ints``
+    is not given: ``stda_hip_ao_ints``), the moment transforms and the Loewdin
+    orthogonalisation (steps 3-4: ``stda_hip_transform``) then run on it, otherwise with
+    numpy on the host."""
     eps, occ, C = mol.eps, mol.occ, mol.C
     thr = thr_ev / HARTREE_EV
     deps = 2.0 * (1.0 + 0.8 * ax) * thr
@@ -119,7 +120,7 @@ def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
                       "sort_vec pairs energies and orbitals inconsistently here, and "
                       "so does this restatement")
     if ints is None:
-        ints = one_electron(mol)
+        ints = device.ao_ints(mol) if device is not None else one_electron(mol)
     S = ints["S"]
     if device is not None:
         from .core import StdaHipError

@@ -362,6 +362,33 @@ def test_device_transforms(which, water):
             assert abs(r.omega[k] * O.HARTREE_EV_PRINT - row[1]) <= 0.5e-4 + 1e-6
 
 
+@pytest.mark.parametrize("which", ["water", "polyene", "cluster30"])
+def test_device_ao_integrals(which, water):
+    """stda_hip_ao_ints against the host Obara-Saika integrals (aoints.one_electron): S, the
+    dipole, angular-momentum and velocity matrices to 1e-13 of their largest element; and
+    the whole preparation on the device (integrals, transforms, Loewdin) against the host's."""
+    from stda_amd import synth
+    from stda_amd.aoints import one_electron
+    say(f"device AO integrals {which}")
+    if which == "water":
+        mol, thr, ax = water, 20.0, 1.0
+    elif which == "polyene":
+        mol, thr, ax = synth.polyene()[0], 7.0, 0.25
+    else:
+        mol, thr, ax = synth.cluster(30, seed=3, basis="minimal")[0], 10.0, 0.25
+    ref = one_electron(mol)
+    with _hip() as h:
+        got = h.ao_ints(mol)
+        dev = prepare(mol, thr, ax, device=h)
+    for key in ("S", "R", "M", "V"):
+        np.testing.assert_allclose(got[key], ref[key], rtol=0,
+                                   atol=1e-13 * max(1.0, np.abs(ref[key]).max()), err_msg=key)
+    host = prepare(mol, thr, ax, ref)
+    np.testing.assert_allclose(dev.clow, host.clow, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dev.moments, host.moments, rtol=0,
+                               atol=1e-12 * np.abs(host.moments).max())
+
+
 def test_device_transform_negative_overlap(water):
     """A negative eigenvalue of S stops as the reference does ('must stop in S^1/2')."""
     from stda_amd.aoints import one_electron
