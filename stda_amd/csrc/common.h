@@ -161,7 +161,9 @@ void launch_scatter_pairs(hipStream_t s, int64_t n, const int32_t *pair, int no,
                           const double *V, int64_t ldv, double *D);
 void launch_mu_blocks(hipStream_t s, const double *moments, int64_t npk, int no, int nv,
                       double *muoo, double *muvv);
-// upstream transforms (transform.hip)
+// upstream transforms (transform.hip).  AO_LMAX: the largest Cartesian exponent per
+// direction the integral kernel's recurrence tables are sized for (checked on the host).
+constexpr int AO_LMAX = 4;
 void launch_pack_upper(hipStream_t s, int64_t n, const double *P, double *out);
 void launch_scale_rows(hipStream_t s, int64_t m, int64_t k, const double *f, double *T);
 void launch_ao_ints(hipStream_t s, int nao, const int64_t *ptr, const double *alpha,

@@ -2522,7 +2522,7 @@ int stda_hip_ao_ints(stda_hip_ctx *c, int32_t nao, const int64_t *prim_ptr, cons
     for (int32_t m = 0; m < nao; ++m) {
         if (prim_ptr[m + 1] <= prim_ptr[m]) return STDA_HIP_ERR_ARG;
         for (int d = 0; d < 3; ++d)
-            if (lmn[3 * m + d] < 0 || lmn[3 * m + d] > 4) return STDA_HIP_ERR_ARG;   // AO_LMAX
+            if (lmn[3 * m + d] < 0 || lmn[3 * m + d] > AO_LMAX) return STDA_HIP_ERR_ARG;
     }
     const int64_t nprim = prim_ptr[nao];
     STDA_TRY(c, {

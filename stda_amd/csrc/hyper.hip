@@ -11,6 +11,7 @@
 // (x, frequency) pairs (rocBLAS, host side).  The kernels below build the dense
 // operands; both are HBM-bound scatters/gathers.
 
+#include "../../include/stda_hip.h"
 #include "common.h"
 
 namespace stda {
@@ -85,7 +86,7 @@ void launch_beta_xy(hipStream_t s, int64_t n, const int32_t *pair, int no, int n
                     const double *XP, const double *AX, const double *rhs, const BetaCols &cols,
                     int ncols, double *Xd, double *Yd) {
     if (n == 0 || ncols <= 0) return;
-    if (ncols > 6) throw Error(1, "launch_beta_xy: at most 6 columns");   // STDA_HIP_ERR_ARG
+    if (ncols > 6) throw Error(STDA_HIP_ERR_ARG, "launch_beta_xy: at most 6 columns");
     hipLaunchKernelGGL(beta_xy_kernel, dim3(blocks_for(n), ncols), dim3(256), 0, s, n, pair, no,
                        nv, XP, AX, rhs, cols, Xd, Yd);
     STDA_HIP_CHECK(hipGetLastError());

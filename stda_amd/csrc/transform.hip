@@ -44,8 +44,6 @@ __global__ void scale_rows_kernel(int64_t m, int64_t k, const double *__restrict
 // and S = sx sy sz, R_k = -(x_k s s), V_k = -(g_k s s), M = s (x g - g x) (cyclic),
 // contracted with the primitive coefficients, ket primitives summed first.
 // Output column-major: out[q][mu + nao nu] for q = S, R x y z, M x y z, V x y z.
-constexpr int AO_LMAX = 4;
-
 __global__ void __launch_bounds__(256)
 ao_ints_kernel(int nao, const int64_t *__restrict__ ptr, const double *__restrict__ alpha,
                const double *__restrict__ coef, const double *__restrict__ cen,
