@@ -132,6 +132,14 @@ int stda_hip_transform(stda_hip_ctx *ctx, int32_t nao, int32_t moci, const doubl
                        const double *S, const double *ao9, int32_t bfw, double *clow,
                        double *moments);
 
+/* stda_hip_ao_ints followed by stda_hip_transform with the AO matrices kept on
+ * the device: the basis arrays as for stda_hip_ao_ints, then moci, ca, bfw,
+ * clow and moments as for stda_hip_transform. */
+int stda_hip_prepare_ao(stda_hip_ctx *ctx, int32_t nao, const int64_t *prim_ptr,
+                        const double *alpha, const double *coef, const double *center,
+                        const int32_t *lmn, int32_t moci, const double *ca, int32_t bfw,
+                        double *clow, double *moments);
+
 /* Selection variants, applied by the next stda_hip_select / stda_hip_setup:
  *   natoms > 0: dual thresholds of std2 -dual (stda_rw_dual, stda-rw_dual.f:658-853;
  *     the 'dual' file holds thr2_ev and the atom list).  Occupied MOs whose

@@ -37,6 +37,7 @@ C_API = [
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
+    "stda_hip_prepare_ao",
 ]
 
 
@@ -99,6 +100,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_format_curve.argtypes = [I64, P, P, P]
     L.stda_hip_transform.argtypes = [P, I32, I32, P, P, P, I32, P, P]
     L.stda_hip_ao_ints.argtypes = [P, I32, P, P, P, P, P, P, P]
+    L.stda_hip_prepare_ao.argtypes = [P, I32, P, P, P, P, P, I32, P, I32, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
@@ -354,6 +356,23 @@ class StdaHip:
         # ao9[q] holds a column-major matrix: transpose the C-order view
         m = [ao9[q].T for q in range(9)]
         return {"S": S, "R": np.stack(m[0:3]), "M": np.stack(m[3:6]), "V": np.stack(m[6:9])}
+
+    def prepare_ao(self, mol, ca, bfw: bool = False):
+        """AO integrals, moment transforms and Loewdin on the device without a host round
+        trip of the AO matrices (stda_hip_prepare_ao): (clow, moments)."""
+        ca = np.asfortranarray(ca, dtype=np.float64)
+        nao, moci = ca.shape
+        ptr = np.ascontiguousarray(mol.ao_prim_ptr, dtype=np.int64)
+        al = np.ascontiguousarray(mol.prim_alpha, dtype=np.float64)
+        co = np.ascontiguousarray(mol.prim_coef, dtype=np.float64)
+        cen = np.ascontiguousarray(mol.xyz[mol.ao_atom], dtype=np.float64)
+        lmn = np.ascontiguousarray(mol.ao_lmn, dtype=np.int32)
+        clow = np.zeros((nao, moci), order="F")
+        mom = np.zeros((9, moci * (moci + 1) // 2))
+        self._check(self._L.stda_hip_prepare_ao(self._h, nao, _ptr(ptr), _ptr(al), _ptr(co),
+                                                _ptr(cen), _ptr(lmn), moci, _ptr(ca),
+                                                1 if bfw else 0, _ptr(clow), _ptr(mom)))
+        return clow, mom
 
     def transform(self, ca, S, ao9, bfw: bool = False):
         """Device upstream transforms (stda_hip_transform): (clow (nao, moci), moments (9, npk))

@@ -2546,6 +2546,61 @@ int stda_hip_ao_ints(stda_hip_ctx *c, int32_t nao, const int64_t *prim_ptr, cons
     })
 }
 
+// Moments and Loewdin orbitals from S and the nine AO matrices, which are host
+// arrays (kind = hipMemcpyHostToDevice) or device buffers (hipMemcpyDeviceToDevice).
+static void transform_impl(stda_hip_ctx *c, int32_t nao, int32_t moci, const double *ca,
+                           const double *S, const double *ao9, hipMemcpyKind kind, int32_t bfw,
+                           double *clow, double *moments) {
+    hipStream_t s = c->stream;
+    const int64_t na = nao, nm = moci, npk = nm * (nm + 1) / 2;
+    DBuf<double> CA, M((size_t)na * na), T((size_t)na * nm), P((size_t)nm * nm);
+    DBuf<double> OUT((size_t)9 * npk);
+    CA.upload(ca, (size_t)na * nm, s);
+    // moments: shrink(ca^T M_q ca) for the 9 AO matrices (stda.f:351-412, onetri.f:20-55)
+    for (int q = 0; q < 9; ++q) {
+        STDA_HIP_CHECK(hipMemcpyAsync(M.get(), ao9 + (size_t)q * na * na, (size_t)na * na * 8,
+                                      kind, s));
+        dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
+        dgemm(c->blas, true, false, nm, nm, na, 1.0, CA.get(), na, T.get(), na, 0.0, P.get(), nm);
+        launch_pack_upper(s, nm, P.get(), OUT.get() + (size_t)q * npk);
+    }
+    OUT.download(moments, (size_t)9 * npk, s);
+    // Loewdin: clow = S^1/2 ca = U (sqrt(e) o (U^T ca)) (makel, stda.f:1615-1691)
+    STDA_HIP_CHECK(hipMemcpyAsync(M.get(), S, (size_t)na * na * 8, kind, s));
+    DBuf<double> D(na), E(na);
+    DBuf<int> info(1);
+    STDA_BLAS_CHECK(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                     (rocblas_int)na, M.get(), (rocblas_int)na, D.get(),
+                                     E.get(), info.get()));
+    std::vector<double> e(na);
+    int infoh = 0;
+    D.download(e.data(), na, s);
+    STDA_HIP_CHECK(hipMemcpyAsync(&infoh, info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    if (infoh != 0) throw Error(STDA_HIP_ERR_DEVICE, "eigensolve of S did not converge");
+    for (auto &v : e) {
+        if (v < 0.0) {
+            if (!bfw) throw Error(STDA_HIP_ERR_ARG, "negative eigenvalue of S: must stop in S^1/2");
+            v = 0.0;   // -BFW (stda.f:1660-1670)
+        }
+        v = std::sqrt(v);
+    }
+    D.upload(e.data(), na, s);
+    dgemm(c->blas, true, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
+    launch_scale_rows(s, na, nm, D.get(), T.get());
+    dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, T.get(), na, 0.0, CA.get(), na);
+    CA.download(clow, (size_t)na * nm, s);
+    c->sync();
+    // |<c'|c'> - 1| > 1e-2 stops the reference (stda.f:445-450)
+    for (int64_t j = 0; j < nm; ++j) {
+        double nr = 0.0;
+        for (int64_t i = 0; i < na; ++i) nr += clow[i + na * j] * clow[i + na * j];
+        if (std::fabs(nr - 1.0) > 1.0e-2)
+            throw Error(STDA_HIP_ERR_ARG, "internal MO norm error (MO " + std::to_string(j + 1) +
+                                              ", norm " + std::to_string(nr) + ")");
+    }
+}
+
 int stda_hip_transform(stda_hip_ctx *c, int32_t nao, int32_t moci, const double *ca,
                        const double *S, const double *ao9, int32_t bfw, double *clow,
                        double *moments) {
@@ -2553,54 +2608,41 @@ int stda_hip_transform(stda_hip_ctx *c, int32_t nao, int32_t moci, const double 
         return STDA_HIP_ERR_ARG;
     STDA_TRY(c, {
         STDA_HIP_CHECK(hipSetDevice(c->device));
+        transform_impl(c, nao, moci, ca, S, ao9, hipMemcpyHostToDevice, bfw, clow, moments);
+    })
+}
+
+int stda_hip_prepare_ao(stda_hip_ctx *c, int32_t nao, const int64_t *prim_ptr, const double *alpha,
+                        const double *coef, const double *center, const int32_t *lmn,
+                        int32_t moci, const double *ca, int32_t bfw, double *clow,
+                        double *moments) {
+    if (!c || nao < 1 || moci < 1 || moci > nao || !prim_ptr || !alpha || !coef || !center ||
+        !lmn || !ca || !clow || !moments)
+        return STDA_HIP_ERR_ARG;
+    if (prim_ptr[0] != 0) return STDA_HIP_ERR_ARG;
+    for (int32_t m = 0; m < nao; ++m) {
+        if (prim_ptr[m + 1] <= prim_ptr[m]) return STDA_HIP_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (lmn[3 * m + d] < 0 || lmn[3 * m + d] > AO_LMAX) return STDA_HIP_ERR_ARG;
+    }
+    const int64_t nprim = prim_ptr[nao];
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
         hipStream_t s = c->stream;
-        const int64_t na = nao, nm = moci, npk = nm * (nm + 1) / 2;
-        DBuf<double> CA, M((size_t)na * na), T((size_t)na * nm), P((size_t)nm * nm);
-        DBuf<double> OUT((size_t)9 * npk);
-        CA.upload(ca, (size_t)na * nm, s);
-        // moments: shrink(ca^T M_q ca) for the 9 AO matrices (stda.f:351-412, onetri.f:20-55)
-        for (int q = 0; q < 9; ++q) {
-            STDA_HIP_CHECK(hipMemcpyAsync(M.get(), ao9 + (size_t)q * na * na, (size_t)na * na * 8,
-                                          hipMemcpyHostToDevice, s));
-            dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
-            dgemm(c->blas, true, false, nm, nm, na, 1.0, CA.get(), na, T.get(), na, 0.0, P.get(), nm);
-            launch_pack_upper(s, nm, P.get(), OUT.get() + (size_t)q * npk);
-        }
-        OUT.download(moments, (size_t)9 * npk, s);
-        // Loewdin: clow = S^1/2 ca = U (sqrt(e) o (U^T ca)) (makel, stda.f:1615-1691)
-        STDA_HIP_CHECK(hipMemcpyAsync(M.get(), S, (size_t)na * na * 8, hipMemcpyHostToDevice, s));
-        DBuf<double> D(na), E(na);
-        DBuf<int> info(1);
-        STDA_BLAS_CHECK(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper,
-                                         (rocblas_int)na, M.get(), (rocblas_int)na, D.get(),
-                                         E.get(), info.get()));
-        std::vector<double> e(na);
-        int infoh = 0;
-        D.download(e.data(), na, s);
-        STDA_HIP_CHECK(hipMemcpyAsync(&infoh, info.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        c->sync();
-        if (infoh != 0) throw Error(STDA_HIP_ERR_DEVICE, "eigensolve of S did not converge");
-        for (auto &v : e) {
-            if (v < 0.0) {
-                if (!bfw) throw Error(STDA_HIP_ERR_ARG, "negative eigenvalue of S: must stop in S^1/2");
-                v = 0.0;   // -BFW (stda.f:1660-1670)
-            }
-            v = std::sqrt(v);
-        }
-        D.upload(e.data(), na, s);
-        dgemm(c->blas, true, false, na, nm, na, 1.0, M.get(), na, CA.get(), na, 0.0, T.get(), na);
-        launch_scale_rows(s, na, nm, D.get(), T.get());
-        dgemm(c->blas, false, false, na, nm, na, 1.0, M.get(), na, T.get(), na, 0.0, CA.get(), na);
-        CA.download(clow, (size_t)na * nm, s);
-        c->sync();
-        // |<c'|c'> - 1| > 1e-2 stops the reference (stda.f:445-450)
-        for (int64_t j = 0; j < nm; ++j) {
-            double nr = 0.0;
-            for (int64_t i = 0; i < na; ++i) nr += clow[i + na * j] * clow[i + na * j];
-            if (std::fabs(nr - 1.0) > 1.0e-2)
-                throw Error(STDA_HIP_ERR_ARG, "internal MO norm error (MO " + std::to_string(j + 1) +
-                                                  ", norm " + std::to_string(nr) + ")");
-        }
+        const size_t nn = (size_t)nao * nao;
+        DBuf<int64_t> P;
+        DBuf<double> AL, CO, CE, SO(nn), AO((size_t)9 * nn);
+        DBuf<int32_t> LM;
+        P.upload(prim_ptr, (size_t)nao + 1, s);
+        AL.upload(alpha, (size_t)nprim, s);
+        CO.upload(coef, (size_t)nprim, s);
+        CE.upload(center, (size_t)3 * nao, s);
+        LM.upload(lmn, (size_t)3 * nao, s);
+        EventTimer ev(c);
+        launch_ao_ints(s, nao, P.get(), AL.get(), CO.get(), CE.get(), LM.get(), SO.get(), AO.get());
+        ev.finish("ao_ints", (double)nn);
+        transform_impl(c, nao, moci, ca, SO.get(), AO.get(), hipMemcpyDeviceToDevice, bfw, clow,
+                       moments);
     })
 }
 

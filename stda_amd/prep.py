@@ -119,8 +119,15 @@ def prepare(mol: Molecule, thr_ev: float, ax: float, ints: dict | None = None,
         warnings.warn("MO energies of the window are not ascending: the reference's "
                       "sort_vec pairs energies and orbitals inconsistently here, and "
                       "so does this restatement")
+    if device is not None and ints is None:
+        from .core import StdaHipError
+        try:
+            clow, mom = device.prepare_ao(mol, ca, bfw=bfw)
+        except StdaHipError as err:
+            raise StdaInputError(str(err)) from err
+        return _core_inputs(mol, no, nv, clow, epsi, mom, othr, vthr)
     if ints is None:
-        ints = device.ao_ints(mol) if device is not None else one_electron(mol)
+        ints = one_electron(mol)
     S = ints["S"]
     if device is not None:
         from .core import StdaHipError
