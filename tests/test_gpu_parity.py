@@ -401,6 +401,25 @@ def test_device_transform_negative_overlap(water):
             prepare(water, 20.0, 1.0, ints, device=h)
 
 
+def test_stda_dropin_branches(water):
+    """The stda() drop-in's -resp (beta), -2PA and -dual/-rw 1 branches return what the
+    entry points return."""
+    from stda_amd.core import stda
+    say("stda() drop-in branches")
+    ci = prepare(water, 20.0, 1.0)
+    om = beta_test_omegas()
+    betas, csf = stda(ci, 20.0, 1e-4, 1.0, beta_omegas=om)
+    ref = O.run(ci, 20.0, 1.0, rpa=True, resp_omegas=[0.0], keep_matrices=True)
+    for (w, b, hr), (wr, br, hrr) in zip(betas, O.beta_shg(ci, ref.iconf, ref.apb, ref.amb, om)):
+        assert abs(hr["beta_hrs"] - hrr[2]) <= 1e-8 * hrr[2]
+    tp, _ = stda(ci, 20.0, 1e-4, 1.0, rpa=True, tpa_states=2)
+    assert len(tp) == 2 and all(d["par"] >= 0.0 for _, _, d in tp)
+    sol, (iconf, ed) = stda(ci, 20.0, 1e-4, 1.0, nroots=3, pt2_off=True)
+    r2 = O.run(ci, 20.0, 1.0, nroots=3, pt2_off=True)
+    np.testing.assert_array_equal(iconf, r2.iconf)
+    np.testing.assert_allclose(sol.omega, r2.omega, rtol=0, atol=E_TOL)
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30
