@@ -303,7 +303,7 @@ def parse_gspec(text: str) -> GSpecInput:
         return next(it)[:120].ljust(120)
 
     def nums(s):
-        v = read_numbers(s)
+        v = read_numbers(s)[:50]          # xx(50) in g_spec.f:14
         for q, val in enumerate(v):
             cols[q + 1] = val
         cols[len(v) + 1] = 0.0
