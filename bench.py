@@ -84,6 +84,24 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_probe_bytes": n * 8}), flush=True)
 
 
+def host_info():
+    """CPU model and the CPUs this process may use (the batched line depends on them)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return {"cpu_model": model, "usable_cpus": ncpu}
+
+
 def gen_workers():
     return max(1, min(16, os.cpu_count() or 1))
 
@@ -343,7 +361,7 @@ def run_mixed(args):
         "config": {"workload": f"mixed{n}: {n} molecules, natoms log-uniform [10,200], minimal "
                                f"basis, Ethr 7 eV, ax 0.25, 20 roots each, batched mixed precision",
                    "molecules": n, "molecules_with_csf": nok, "workers": args.workers,
-                   "nmax_batched": args.nmax_batched, "poll": args.poll,
+                   "nmax_batched": args.nmax_batched, "poll": args.poll, "host": host_info(),
                    "paths": cnt, "phases_ms": t,
                    "phase_ms_summed": {k: round(v / 1e3, 2) for k, v in phase_us.items()
                                        if k not in ("sweeps", "unused")}},
