@@ -19,8 +19,18 @@ ssptrf/ssptri/ssptrs       numpy.linalg.inv / solve (linear_response.f)
 =========================  ==============================================
 
 Every function cites the reference lines it follows.  Pinning: the water
-fixtures of the reference's tests (EXCI_TEST*, 2PA_TEST*) and the
-invariants listed in tests/test_oracle.py.
+fixtures of the reference's tests (EXCI_TEST*, 2PA_TEST*, BETA_TEST*) and the
+invariants listed in tests/test_oracle.py:
+
+* beta_shg / hrs: the nine BETA_TEST beta_HRS values (2.5e-7 relative);
+* tpa with rpa_states_tpa_mode: the 36 printed 2PA_TEST values;
+* dual_groups / orc_screen_dual (-dual): no fixture exists; pinned by the
+  reduction "every atom listed = the standard selection" and an independent
+  per-MO threshold check;
+* properties_xyz (-aniso): its axis sums equal the fixture-pinned totals
+  (checked against the GPU path, whose totals are held to properties_tda);
+* the g_spec restatement lives in gspec.py, pinned by the reference program's
+  own outputs.
 """
 
 from __future__ import annotations
