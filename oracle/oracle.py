@@ -27,8 +27,8 @@ invariants listed in tests/test_oracle.py:
 * dual_groups / orc_screen_dual (-dual): no fixture exists; pinned by the
   reduction "every atom listed = the standard selection" and an independent
   per-MO threshold check;
-* properties_xyz (-aniso): its axis sums equal the fixture-pinned totals
-  (checked against the GPU path, whose totals are held to properties_tda);
+* properties_xyz (-aniso): its axis sums equal the totals of properties_tda
+  and properties_velcorr, which the EXCI fixtures pin;
 * the g_spec restatement lives in gspec.py, pinned by the reference program's
   own outputs.
 """

@@ -286,3 +286,14 @@ def test_dual_selection(polyene_inputs):
     off = O.select(ci, inter, 7.0, pt2_off=True)
     assert off["nci"] == off["nex"] and off["npt"] == 0
     np.testing.assert_array_equal(off["iconf"], std["iconf"][:std["nex"]])
+
+
+@pytest.mark.parametrize("velcorr", [False, True])
+def test_axis_resolved_properties_sum_to_totals(polyene_inputs, velcorr):
+    """-aniso (stda.f:1360-1376, apbtrafo.f:245-262): f = (f_x + f_y + f_z)/3 and
+    R = R_x + R_y + R_z reproduce the totals of the fixture-pinned property routines."""
+    ci = polyene_inputs
+    r = O.run(ci, 7.0, 0.25, nroots=6, velcorr=velcorr, keep_matrices=True)
+    p = O.properties_xyz(ci, r.iconf, r.omega, r.X, 2.0, r.bhalf if velcorr else None)
+    np.testing.assert_allclose(p[:, :, :2].sum(1) / 3.0, r.props[:, :2], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(p[:, :, 2:].sum(1), r.props[:, 2:], rtol=1e-12, atol=1e-12)
