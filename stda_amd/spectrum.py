@@ -6,7 +6,8 @@
   ``print_tdadat`` (stda.f:2235-2265; sTD-DFT and uncorrected sTDA).
 * :func:`tda_dat_xyz` writes the axis-resolved files of ``-aniso`` (tdax/y/z.dat);
   :func:`beta_hrs_file` and :func:`tpa_abs_file` the beta_HRS and 2PA-abs files of
-  ``-resp`` and ``-2PA``.
+  ``-resp`` and ``-2PA``; :func:`sos_rotation` is the sum-over-states optical rotation
+  table of every sTDA run.
 * :func:`gspec` / :func:`gspec_many` are the reference's spectrum program
   ``g_spec`` (g_spec/g_spec.f): they read the text of a tda.dat-style input and
   return the contents of ``rots.dat`` (line spectrum) and ``spec.dat``
@@ -155,6 +156,42 @@ def tda_dat_xyz(omega, props_xyz, xmolw: float, velcorr: bool = True,
     :func:`tda_dat`."""
     p = np.asarray(props_xyz, dtype=np.float64)
     return tuple(tda_dat(omega, p[:, q, :], xmolw, velcorr, thr) for q in range(3))
+
+
+# ---------------------------------------------------------------- SOS optical rotation
+_SOS_LAMBDA = [float(_F(v)) for v in (632.8, 589.3, 579.0, 546.0, 436.0, 365.0)]   # sosor.f:58-63
+
+
+def sos_rotation(omega, rv, xmolw: float, refindex: float = 1.4):
+    """Sum-over-states specific optical rotation at six wavelengths, as ``sosor`` prints it
+    after every sTDA run (sosor.f:19-112, called at stda.f:1471 with the roots and the
+    velocity-corrected rotatory strengths): [(lambda nm, eV, [alpha] in
+    grad cm^3 g^-1 dm^-1)], with the Lorentz factor of a solvent of refractive index 1.4.
+    The roots enter in single precision, as the reference's real*4 eigenvalues do."""
+    c_beta = float(_F(_F(2.0) * _F(137.036)) / _F(3.0))      # (2.*137.036/3.), real*4
+    r_unit = float(_F(64604.8))
+    reffaktor = (refindex ** 2 + 2.0) / 3.0
+    x = [float(_F(w)) for w in np.asarray(omega, dtype=np.float64)]
+    y = [float(v) for v in np.asarray(rv, dtype=np.float64)]
+    out = []
+    for lam in _SOS_LAMBDA:
+        eau = 1.0e7 / lam / 2.19474625e5
+        r1 = 0.0
+        for xau, yi in zip(x, y):
+            r1 = r1 + c_beta * (yi / r_unit) / (xau ** 2 - eau ** 2)
+        vorfaktor = (38652.0 / xmolw) * (_SOS_LAMBDA[1] / lam) ** 2
+        out.append((lam, 1.0e7 / (EV_CM * lam), r1 * vorfaktor * reffaktor))
+    return out
+
+
+def sos_rotation_text(rows, refval: float = 0.0) -> str:
+    """The table sosor writes (formats 142/143: f6.1, f6.2, f12.2; the 589.3 nm line also
+    carries the reference value read from a '.ref' file, 0 without one)."""
+    lines = []
+    for j, (lam, ev, a) in enumerate(rows):
+        t = fmt_f(lam, 6, 1) + fmt_f(ev, 6, 2) + fmt_f(a, 12, 2)
+        lines.append(t + fmt_f(refval, 12, 2) + " ##" if j == 1 else t)
+    return "\n".join(lines) + "\n"
 
 
 # ---------------------------------------------------------------- g_spec input

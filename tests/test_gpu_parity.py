@@ -420,6 +420,32 @@ def test_stda_dropin_branches(water):
     np.testing.assert_allclose(sol.omega, r2.omega, rtol=0, atol=E_TOL)
 
 
+def test_sos_rotation_mirror_image():
+    """The SOS optical rotation table (sosor.f) from GPU solves on device-prepared inputs:
+    equal to the oracle's, and the negative for the mirror image (x -> -x)."""
+    import dataclasses
+    from stda_amd import spectrum as S
+    from stda_amd import synth
+    say("SOS rotation, mirror image")
+    mol, _ = synth.cluster(12, seed=5, basis="minimal")
+    sign = np.where(mol.ao_lmn[:, 0] % 2 == 1, -1.0, 1.0)
+    xyz = mol.xyz.copy()
+    xyz[:, 0] *= -1.0
+    mirror = dataclasses.replace(mol, xyz=xyz, C=mol.C * sign[:, None])
+    vals = []
+    with _hip() as h:
+        for m in (mol, mirror):
+            ci = prepare(m, 7.0, 0.25, device=h)
+            h.setup(ci, 7.0, 0.25)
+            sol = h.solve_tda(10)
+            got = np.array([x[2] for x in S.sos_rotation(sol.omega, sol.props[:, 3], ci.xmolw)])
+            ref = O.run(ci, 7.0, 0.25, nroots=10)
+            want = np.array([x[2] for x in S.sos_rotation(ref.omega, ref.props[:, 3], ci.xmolw)])
+            np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
+            vals.append(got)
+    np.testing.assert_allclose(vals[0], -vals[1], rtol=1e-5, atol=1e-4)
+
+
 def test_cluster_selection_and_roots(cluster30):
     say("30-atom cluster TDA k=12")
     ci = cluster30

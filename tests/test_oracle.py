@@ -297,3 +297,35 @@ def test_axis_resolved_properties_sum_to_totals(polyene_inputs, velcorr):
     p = O.properties_xyz(ci, r.iconf, r.omega, r.X, 2.0, r.bhalf if velcorr else None)
     np.testing.assert_allclose(p[:, :, :2].sum(1) / 3.0, r.props[:, :2], rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(p[:, :, 2:].sum(1), r.props[:, 2:], rtol=1e-12, atol=1e-12)
+
+
+def _mirror(mol):
+    """The mirror image through the yz plane: x -> -x, and every AO odd in x changes sign."""
+    import dataclasses
+    sign = np.where(mol.ao_lmn[:, 0] % 2 == 1, -1.0, 1.0)
+    xyz = mol.xyz.copy()
+    xyz[:, 0] *= -1.0
+    return dataclasses.replace(mol, xyz=xyz, C=mol.C * sign[:, None])
+
+
+def test_sos_rotation_mirror_image():
+    """The SOS optical rotation (sosor.f) of the mirror image is the negative of the
+    molecule's: the sign conventions of the r x nabla and nabla integrals, the velocity
+    correction and the rotation formula hold together.  Energies and oscillator strengths
+    are mirror-invariant; water (achiral) rotates nothing."""
+    from stda_amd import spectrum as S
+    from stda_amd import synth
+    mol, _ = synth.cluster(12, seed=5, basis="minimal")
+    out = []
+    for m in (mol, _mirror(mol)):
+        ci = prepare(m, 7.0, 0.25)
+        r = O.run(ci, 7.0, 0.25, nroots=10)
+        out.append((r, S.sos_rotation(r.omega, r.props[:, 3], ci.xmolw)))
+    (ra, a), (rb, b) = out
+    np.testing.assert_allclose(ra.omega, rb.omega, rtol=1e-12)
+    np.testing.assert_allclose(ra.props[:, :2], rb.props[:, :2], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ra.props[:, 2:], -rb.props[:, 2:], rtol=1e-7, atol=1e-9)
+    va = np.array([x[2] for x in a])
+    vb = np.array([x[2] for x in b])
+    assert np.abs(va).max() > 1.0
+    np.testing.assert_allclose(va, -vb, rtol=1e-6, atol=1e-6)
