@@ -512,7 +512,7 @@ def main():
     ap.add_argument("--workload", default="mixed512",
                     help="mixed512 (default), cluster400, cluster100, polyene, serve")
     ap.add_argument("--nmol", type=int, default=512)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--nmax-batched", type=int, default=1024,
                     help="CSF limit of the lock-step solver (0: every molecule on the worker path)")
     ap.add_argument("--poll", type=int, default=4,
