@@ -579,10 +579,11 @@ class StdaHipBatch:
 
     def timings(self):
         """Host times (ms) of the last call's phases, the batched kernel's
-        HIP-event time, and how many molecules took which path."""
+        HIP-event time, and how many molecules took which path
+        ("nonzero_status": no CSF in the window, or not converged)."""
         t = np.zeros(5)
         cnt = np.zeros(3, dtype=np.int64)
         self._L.stda_hip_batch_timings(self._b, _ptr(t), _ptr(cnt))
         return ({"solve": t[0], "driver": t[1], "properties": t[2], "total": t[3],
                  "batch_kernels": t[4]},
-                {"batched": int(cnt[0]), "single": int(cnt[1]), "failed": int(cnt[2])})
+                {"batched": int(cnt[0]), "single": int(cnt[1]), "nonzero_status": int(cnt[2])})
