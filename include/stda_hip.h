@@ -338,9 +338,10 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
 int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t poll);
 /* ms5: host time of [0] the setup and solve phase (wall), [1] the driver
  * loop (wall), [2] property tables, [3] total, and [4] the summed HIP-event
- * times of the batched launches; counts3: molecules solved by the batched
- * solver, by the single path, and returned with a nonzero status (no CSF in
- * the energy window counts here, as does a solve that did not converge). */
+ * times of the batched launches; counts3 (disjoint, summing to nmol):
+ * molecules solved with status OK by the batched solver, with status OK by
+ * the single path, and returned with a nonzero status (no CSF in the energy
+ * window counts here, as does a solve that did not converge on either path). */
 int stda_hip_batch_timings(const stda_hip_batch *b, double *ms5, int64_t *counts3);
 /* Per-molecule phase times of the batched step kernel summed over the
  * molecules of the last call (microseconds): Gram products, subspace

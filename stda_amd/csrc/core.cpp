@@ -57,8 +57,21 @@ constexpr double kRot = 235.7220;             // stda.f:1300
 // Wilkinson & Reinsch, Handbook for Automatic Computation II) is used.
 using dsyevd_t = void (*)(const char *, const char *, const int *, double *, const int *, double *,
                           double *, const int *, int *, const int *, int *);
-static dsyevd_t find_dsyevd() {
-    dsyevd_t fn = nullptr;
+using dgemm_t = void (*)(const char *, const char *, const int *, const int *, const int *,
+                         const double *, const double *, const int *, const double *, const int *,
+                         const double *, double *, const int *);
+using dpotrf_t = void (*)(const char *, const int *, double *, const int *, int *);
+using dtrsm_t = void (*)(const char *, const char *, const char *, const char *, const int *,
+                         const int *, const double *, const double *, const int *, double *,
+                         const int *);
+struct HostLapack {
+    dsyevd_t syevd = nullptr;
+    dgemm_t gemm = nullptr;
+    dpotrf_t potrf = nullptr;
+    dtrsm_t trsm = nullptr;
+};
+static HostLapack find_lapack() {
+    HostLapack L;
     const char *env = getenv("STDA_HIP_LAPACK");
     const char *cands[] = {env ? env : "", "/opt/conda/lib/libmkl_rt.so.1",
                            "/opt/conda/lib/libmkl_rt.so"};
@@ -68,17 +81,22 @@ static dsyevd_t find_dsyevd() {
         if (!h) continue;
         using set_t = int (*)(int);
         if (auto st = (set_t)dlsym(h, "MKL_Set_Threading_Layer")) st(1);   // sequential
-        fn = (dsyevd_t)dlsym(h, "dsyevd_");
-        if (fn) break;
+        L.syevd = (dsyevd_t)dlsym(h, "dsyevd_");
+        if (!L.syevd) continue;
+        L.gemm = (dgemm_t)dlsym(h, "dgemm_");
+        L.potrf = (dpotrf_t)dlsym(h, "dpotrf_");
+        L.trsm = (dtrsm_t)dlsym(h, "dtrsm_");
+        break;
     }
-    return fn;
+    return L;
 }
 
 // Looked up once per process; contexts driven from several host threads share it.
-static dsyevd_t load_dsyevd() {
-    static const dsyevd_t fn = find_dsyevd();
-    return fn;
+static const HostLapack &host_lapack() {
+    static const HostLapack L = find_lapack();
+    return L;
 }
+static dsyevd_t load_dsyevd() { return host_lapack().syevd; }
 
 static void tred2_tql2(int n, std::vector<double> &a, std::vector<double> &d) {
     // a: column-major n x n symmetric on input, eigenvectors on output
@@ -202,6 +220,68 @@ void host_syev(int n, std::vector<double> &a, std::vector<double> &w) {
     std::vector<int> iwork(liwork);
     fn("V", "U", &n, a.data(), &n, w.data(), work.data(), &lwork, iwork.data(), &liwork, &info);
     if (info != 0) throw Error(STDA_HIP_ERR_DEVICE, "dsyevd failed, info=" + std::to_string(info));
+}
+
+// C(m x n) = alpha op(A) op(B) + beta C, column-major (op = transpose when t*).
+void host_gemm(bool ta, bool tb, int m, int n, int k, double alpha, const double *A, int lda,
+               const double *B, int ldb, double beta, double *C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    if (auto fn = host_lapack().gemm) {
+        fn(ta ? "T" : "N", tb ? "T" : "N", &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc);
+        return;
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (int l = 0; l < k; ++l)
+                s += (ta ? A[l + (size_t)lda * i] : A[i + (size_t)lda * l]) *
+                     (tb ? B[j + (size_t)ldb * l] : B[l + (size_t)ldb * j]);
+            C[i + (size_t)ldc * j] = alpha * s + (beta == 0.0 ? 0.0 : beta * C[i + (size_t)ldc * j]);
+        }
+}
+
+// Lower Cholesky factor in place (the strict upper triangle is zeroed); false
+// when the matrix is not positive definite.
+bool host_potrf_lower(int n, std::vector<double> &a) {
+    int info = 0;
+    if (auto fn = host_lapack().potrf) {
+        fn("L", &n, a.data(), &n, &info);
+    } else {
+        for (int j = 0; j < n && info == 0; ++j) {
+            double d = a[j + (size_t)n * j];
+            for (int l = 0; l < j; ++l) d -= a[j + (size_t)n * l] * a[j + (size_t)n * l];
+            if (!(d > 0.0)) { info = j + 1; break; }
+            d = std::sqrt(d);
+            a[j + (size_t)n * j] = d;
+            for (int i = j + 1; i < n; ++i) {
+                double s = a[i + (size_t)n * j];
+                for (int l = 0; l < j; ++l) s -= a[i + (size_t)n * l] * a[j + (size_t)n * l];
+                a[i + (size_t)n * j] = s / d;
+            }
+        }
+    }
+    if (info != 0) return false;
+    for (int j = 1; j < n; ++j)
+        for (int i = 0; i < j; ++i) a[i + (size_t)n * j] = 0.0;
+    return true;
+}
+
+// B(n x k) <- L^-T B for the lower triangular L(n x n).
+void host_solve_lt(int n, int k, const std::vector<double> &L, double *B, int ldb) {
+    if (n <= 0 || k <= 0) return;
+    if (auto fn = host_lapack().trsm) {
+        const double one = 1.0;
+        fn("L", "L", "T", "N", &n, &k, &one, L.data(), &n, B, &ldb);
+        return;
+    }
+    for (int c = 0; c < k; ++c) {
+        double *b = B + (size_t)ldb * c;
+        for (int i = n - 1; i >= 0; --i) {
+            double s = b[i];
+            for (int l = i + 1; l < n; ++l) s -= L[l + (size_t)n * i] * b[l];
+            b[i] = s / L[i + (size_t)n * i];
+        }
+    }
 }
 
 struct Timer {
@@ -1867,10 +1947,11 @@ int stda_hip_solve_tda(stda_hip_ctx *c, int32_t nroots, int32_t capacity, int32_
             for (double e : w) keep += (e > 0.0 && e <= c->thr);
             int first = 0;
             while (first < (int)w.size() && w[first] <= 0.0) ++first;
-            if (first > 0) {
-                STDA_HIP_CHECK(hipMemcpyAsync(X.get(), X.get() + n * first, (size_t)n * keep * 8,
-                                              hipMemcpyDeviceToDevice, c->stream));
-            }
+            // column by column in ascending order: source and destination of
+            // one column are first >= 1 columns apart, so no copy overlaps
+            for (int j = 0; first > 0 && j < keep; ++j)
+                STDA_HIP_CHECK(hipMemcpyAsync(X.get() + n * j, X.get() + n * (first + j),
+                                              (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
             w = std::vector<double>(w.begin() + first, w.begin() + first + keep);
         }
         int k = (int)w.size();
@@ -1972,6 +2053,8 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         const int64_t n = c->nci;
         hipStream_t s = c->stream;
         const int nrhs = 3 * nfreq;
+        double wmax = 0.0;
+        for (int f = 0; f < nfreq; ++f) wmax = std::max(wmax, std::fabs(omega[f]));
         // right-hand sides -2 mu (linear_response.f:98-116): mu = xl of each CSF
         DBuf<double> rhs((size_t)n * 3);
         STDA_HIP_CHECK(hipMemcpyAsync(rhs.get(), c->mom.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
@@ -1979,48 +2062,32 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         DBuf<double> m2d;
         m2d.upload(m2.data(), 3, s);
         launch_scale_cols(s, n, 3, rhs.get(), m2d.get());
-        // Near a pole, M(w) = (A+B) - w^2 (A-B)^-1 is nearly singular and
-        // indefinite; its generalized eigenvectors are the RPA X+Y, since
-        // M(w)(X+Y)_n = (w_n^2 - w^2)(A-B)^-1 (X+Y)_n.  The initial space
-        // therefore holds the RPA vectors of every root up to 1.25 max(w)
-        // (at least 8) besides the preconditioned right-hand sides.
-        double wmax = 0.0;
-        for (int f = 0; f < nfreq; ++f) wmax = std::max(wmax, std::fabs(omega[f]));
-        int nseed = 0;
-        DBuf<double> seedv;
-        if (wmax > 0.0) {
-            int kr = (int)std::min<int64_t>(n, 8);
-            std::vector<double> wr;
-            DBuf<double> XpY, XmY;
-            DavOpts o;
-            o.fp32 = c->precision == 1;
-            while (true) {
-                davidson_rpa(c, kr, 1.0e-6, wr, XpY, XmY, o);
-                if (wr.empty() || wr.back() > 1.25 * wmax || kr >= n || kr >= 96) break;
-                kr = (int)std::min<int64_t>(std::min<int64_t>(n, 96), 2 * (int64_t)kr);
-            }
-            nseed = (int)wr.size();
-            seedv = std::move(XpY);
-        }
-        int maxdim = (int)std::min<int64_t>(n, std::max(40 * nrhs, 256) + nseed);
+        // Subspace size.  Above many RPA poles M(w) = (A+B) - w^2 (A-B)^-1 is
+        // indefinite and the Galerkin residuals oscillate before they fall;
+        // the space that resolves the poles near w must be kept whole, so the
+        // space is sized not to restart in practice (a 60 000-CSF molecule
+        // with 200 poles below w converges at ~600 vectors): V, P = (A+B)V and
+        // M = (A-B)V take 24 n bytes per vector, given up to an eighth of the
+        // free HBM (at most 32 GiB) and 2400 vectors.
+        size_t freeb = 0, totalb = 0;
+        (void)hipMemGetInfo(&freeb, &totalb);
+        const size_t budget = std::min<size_t>(freeb / 8, (size_t)32 << 30);
+        const int need = 3 + 4 * nrhs;   // initial block, kept solutions, one expansion
+        int64_t md = std::min<int64_t>((int64_t)(budget / (24.0 * (double)n)), 2400);
+        int maxdim = (int)std::min<int64_t>(n, std::max<int64_t>(md, need));
         // STDA_HIP_LRESP_MAXDIM caps the subspace (tests use it to force restarts)
         if (const char *e = getenv("STDA_HIP_LRESP_MAXDIM"))
-            if (atoi(e) > 0) maxdim = std::min(maxdim, std::max(atoi(e), 3 + nseed + 2 * nrhs + 97));
+            if (atoi(e) > 0) maxdim = (int)std::min<int64_t>(maxdim, std::max(atoi(e), need));
         DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
-        DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd, rhsV;
-        // initial space: preconditioned right-hand sides (and the RPA vectors)
+        DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd;
+        // initial space: the right-hand sides preconditioned by the diagonal
         STDA_HIP_CHECK(hipMemcpyAsync(V.get(), rhs.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
         std::vector<double> zero3(3, 0.0);
         DBuf<double> z3;
         z3.upload(zero3.data(), 3, s);
         launch_precondition(s, n, 3, V.get(), c->ed.get(), z3.get(), V.get());
-        const int n0 = (int)std::min<int64_t>(maxdim, 3 + nseed);
-        if (n0 > 3)
-            STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * 3, seedv.get(), (size_t)n * (n0 - 3) * 8,
-                                          hipMemcpyDeviceToDevice, s));
-        int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), n0, tmp);
-        const int m0 = mnew;   // the initial block, kept through restarts
-        std::vector<double> Ap((size_t)maxdim * maxdim), Am((size_t)maxdim * maxdim), Br((size_t)maxdim * 3);
+        int m = 0, mnew = orthonormalize(c, V.get(), 0, V.get(), 3, tmp);
+        int m0 = mnew;   // the initial block, kept through restarts
         std::vector<double> rhsnorm(3);
         {
             DBuf<double> rn(3);
@@ -2028,12 +2095,22 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             rn.download(rhsnorm.data(), 3, s);
             c->sync();
         }
-        std::vector<double> sol;   // (m x nrhs) reduced X+Y
-        bool done = false;
-        for (int it = 0; it < 300 && !done; ++it) {
+        std::vector<double> coef;   // (m x 2 nrhs): X+Y coefficients a | X-Y coefficients b
+        bool done = mnew == 0;      // no dipole moments at all: alpha = 0
+        std::vector<double> Ap((size_t)maxdim * maxdim), Am((size_t)maxdim * maxdim),
+            Br((size_t)maxdim * 3);
+        std::vector<double> wcol(nrhs);
+        for (int f = 0; f < nfreq; ++f)
+            for (int r = 0; r < 3; ++r) wcol[f * 3 + r] = omega[f];
+        DBuf<double> wd;
+        wd.upload(wcol.data(), nrhs, s);
+        std::vector<double> onev(3, 1.0);
+        DBuf<double> oned;
+        oned.upload(onev.data(), 3, s);
+        for (int it = 0; it < 500 && !done; ++it) {
             apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
                       M.get() + n * m, n, c->precision == 1);
-            int mt = m + mnew;
+            const int mt = m + mnew;
             DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
             dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
             dgemm_tn(c, mt, mnew, n, V.get(), n, M.get() + n * m, n, Hc.get() + (size_t)mt * mnew);
@@ -2050,67 +2127,57 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             for (int r = 0; r < 3; ++r)
                 for (int j = 0; j < mnew; ++j) Br[(m + j) + (size_t)maxdim * r] = hc[(size_t)2 * mt * mnew + j + (size_t)mnew * r];
             m = mt;
+            // Reduced problem, one factorisation for every frequency: with
+            // Am = L L^T and L^T Ap L = Z diag(lam) Z^T (lam: the space's RPA
+            // w_n^2), (Ap - w^2 Am^-1) a = V^T r gives a = L Z c with
+            // c = (lam - w^2)^-1 Z^T L^T V^T r, and b = w Am^-1 a = w L^-T Z c.
             std::vector<double> ap((size_t)m * m), am((size_t)m * m);
             for (int j = 0; j < m; ++j)
                 for (int i = 0; i < m; ++i) {
                     ap[i + (size_t)m * j] = 0.5 * (Ap[i + (size_t)maxdim * j] + Ap[j + (size_t)maxdim * i]);
                     am[i + (size_t)m * j] = 0.5 * (Am[i + (size_t)maxdim * j] + Am[j + (size_t)maxdim * i]);
                 }
-            // Am^-1 via its eigen-decomposition (A-B positive definite)
-            std::vector<double> U(am), ev;
-            host_syev(m, U, ev);
-            if (ev[0] <= 0.0) throw Error(STDA_HIP_ERR_AMB_NOT_PD, "A-B not positive definite");
-            std::vector<double> aminv((size_t)m * m, 0.0);
-            for (int q = 0; q < m; ++q)
-                for (int j = 0; j < m; ++j) {
-                    double f = U[j + (size_t)m * q] / ev[q];
-                    for (int i = 0; i < m; ++i) aminv[i + (size_t)m * j] += U[i + (size_t)m * q] * f;
+            std::vector<double> L(am);
+            if (!host_potrf_lower(m, L)) throw Error(STDA_HIP_ERR_AMB_NOT_PD, "A-B not positive definite");
+            std::vector<double> t1((size_t)m * m), Z((size_t)m * m), lam;
+            host_gemm(false, false, m, m, m, 1.0, ap.data(), m, L.data(), m, 0.0, t1.data(), m);
+            host_gemm(true, false, m, m, m, 1.0, L.data(), m, t1.data(), m, 0.0, Z.data(), m);
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < j; ++i) {
+                    double v = 0.5 * (Z[i + (size_t)m * j] + Z[j + (size_t)m * i]);
+                    Z[i + (size_t)m * j] = v; Z[j + (size_t)m * i] = v;
                 }
-            // reduced solves: (Ap - w^2 Am^-1) a = V^T r ; b = w Am^-1 a
-            std::vector<double> coef((size_t)m * 2 * nrhs);   // [a | b] per rhs
+            host_syev(m, Z, lam);
+            std::vector<double> W((size_t)m * m), Yz(Z), G((size_t)m * 3);
+            host_gemm(false, false, m, m, m, 1.0, L.data(), m, Z.data(), m, 0.0, W.data(), m);
+            host_solve_lt(m, m, L, Yz.data(), m);
+            host_gemm(true, false, m, 3, m, 1.0, W.data(), m, Br.data(), maxdim, 0.0, G.data(), m);
+            const double lamscale = std::max(1.0, std::fabs(lam[m - 1]));
+            coef.assign((size_t)m * 2 * nrhs, 0.0);
+            std::vector<double> cv((size_t)m * 3);
             for (int f = 0; f < nfreq; ++f) {
-                double w = omega[f];
-                std::vector<double> Mr((size_t)m * m);
-                for (size_t q = 0; q < Mr.size(); ++q) Mr[q] = ap[q] - w * w * aminv[q];
-                // symmetric solve through eigen-decomposition
-                std::vector<double> Uq(Mr), eq;
-                host_syev(m, Uq, eq);
-                for (int r = 0; r < 3; ++r) {
-                    std::vector<double> a(m, 0.0);
+                const double w = omega[f];
+                for (int r = 0; r < 3; ++r)
                     for (int q = 0; q < m; ++q) {
-                        double proj = 0.0;
-                        for (int i = 0; i < m; ++i) proj += Uq[i + (size_t)m * q] * Br[i + (size_t)maxdim * r];
-                        proj /= eq[q];
-                        for (int i = 0; i < m; ++i) a[i] += Uq[i + (size_t)m * q] * proj;
+                        double den = lam[q] - w * w;
+                        if (std::fabs(den) < 1.0e-300 * lamscale) den = den < 0.0 ? -1.0e-300 : 1.0e-300;
+                        cv[q + (size_t)m * r] = G[q + (size_t)m * r] / den;
                     }
-                    int col = f * 3 + r;
-                    for (int i = 0; i < m; ++i) coef[i + (size_t)m * col] = a[i];
-                    for (int i = 0; i < m; ++i) {
-                        double bsum = 0.0;
-                        for (int j = 0; j < m; ++j) bsum += aminv[i + (size_t)m * j] * a[j];
-                        coef[i + (size_t)m * (nrhs + col)] = w * bsum;
-                    }
-                }
+                host_gemm(false, false, m, 3, m, 1.0, W.data(), m, cv.data(), m, 0.0,
+                          coef.data() + (size_t)m * 3 * f, m);
+                host_gemm(false, false, m, 3, m, w, Yz.data(), m, cv.data(), m, 0.0,
+                          coef.data() + (size_t)m * (nrhs + 3 * f), m);
             }
-            sol.assign(coef.begin(), coef.begin() + (size_t)m * nrhs);
             // residuals R1 = P a - w V b - r ; R2 = M b - w V a
             Yd.upload(coef.data(), coef.size(), s);
-            DBuf<double> VB((size_t)n * nrhs), VA((size_t)n * nrhs), MB((size_t)n * nrhs);
+            DBuf<double> VB((size_t)n * nrhs), VA((size_t)n * nrhs);
             gemm_nn(c, n, nrhs, m, 1.0, P.get(), n, Yd.get(), m, 0.0, R.get(), n);
             gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0, VB.get(), n);
             gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, VA.get(), n);
             gemm_nn(c, n, nrhs, m, 1.0, M.get(), n, Yd.get() + (size_t)m * nrhs, m, 0.0,
-                  R.get() + n * nrhs, n);
-            std::vector<double> wcol(nrhs);
-            for (int f = 0; f < nfreq; ++f)
-                for (int r = 0; r < 3; ++r) wcol[f * 3 + r] = omega[f];
-            DBuf<double> wd;
-            wd.upload(wcol.data(), nrhs, s);
+                    R.get() + n * nrhs, n);
             launch_residual(s, n, nrhs, R.get(), VB.get(), wd.get(), R.get());
             launch_residual(s, n, nrhs, R.get() + n * nrhs, VA.get(), wd.get(), R.get() + n * nrhs);
-            std::vector<double> onev(3, 1.0);
-            DBuf<double> oned;
-            oned.upload(onev.data(), 3, s);
             for (int f = 0; f < nfreq; ++f)   // subtract the right-hand sides: R1 -= r
                 launch_residual(s, n, 3, R.get() + n * 3 * f, rhs.get(), oned.get(),
                                 R.get() + n * 3 * f);
@@ -2130,16 +2197,20 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                 if (un[q] != q)
                     STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8, hipMemcpyDeviceToDevice, s));
             if (m + nun > maxdim) {
-                // restart: keep the initial block (right-hand sides and RPA
-                // seeds) and the current X+Y, X-Y of every right-hand side,
-                // orthonormalised in the reduced space (V is orthonormal, so
-                // V Q is for orthonormal Q); the operator images are the same
-                // combinations of P and M, the reduced matrices Q^T Ap Q etc.
+                // Restart within half the space: the initial block, the current
+                // X+Y and X-Y of every right-hand side, then the space's RPA
+                // pairs (X+Y = L z, X-Y ~ L^-T z) from the lowest up to 1.25
+                // max(w), orthonormalised in the reduced space.  V is orthonormal,
+                // so V Q is; the operator images are P Q and M Q and the reduced
+                // matrices Q^T Ap Q and Q^T Am Q.
                 std::vector<double> Q;
-                auto add_col = [&](std::vector<double> x) {
+                auto add_col = [&](const double *x0) {
+                    std::vector<double> x(x0, x0 + m);
                     double n0x = 0.0;
                     for (double v : x) n0x += v * v;
                     n0x = std::sqrt(n0x);
+                    if (!(n0x > 0.0)) return;
+                    for (double &v : x) v /= n0x;
                     const int r = (int)(Q.size() / m);
                     for (int pass = 0; pass < 2; ++pass)
                         for (int q = 0; q < r; ++q) {
@@ -2150,52 +2221,22 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                     double nx = 0.0;
                     for (double v : x) nx += v * v;
                     nx = std::sqrt(nx);
-                    if (!(nx > 1.0e-10 * n0x) || nx == 0.0) return;
+                    if (!(nx > 1.0e-8)) return;
                     for (double v : x) Q.push_back(v / nx);
                 };
+                std::vector<double> e(m, 0.0);
                 for (int i = 0; i < m0; ++i) {
-                    std::vector<double> e(m, 0.0);
                     e[i] = 1.0;
-                    add_col(e);
+                    add_col(e.data());
+                    e[i] = 0.0;
                 }
-                for (int col = 0; col < 2 * nrhs; ++col)
-                    add_col(std::vector<double>(coef.begin() + (size_t)m * col,
-                                                coef.begin() + (size_t)m * (col + 1)));
-                if (wmax > 0.0) {
-                    // and the space's RPA vectors (X+Y and X-Y) below 1.25
-                    // max(w): the poles the solutions have to resolve.  With
-                    // s = Am^1/2: s Ap s z = w^2 z, X+Y = s z, X-Y = w Am^-1 (X+Y)
-                    std::vector<double> sq((size_t)m * m, 0.0);
-                    for (int q = 0; q < m; ++q)
-                        for (int j = 0; j < m; ++j) {
-                            const double f = U[j + (size_t)m * q] * std::sqrt(ev[q]);
-                            for (int i = 0; i < m; ++i) sq[i + (size_t)m * j] += U[i + (size_t)m * q] * f;
-                        }
-                    std::vector<double> t((size_t)m * m, 0.0), S((size_t)m * m, 0.0), w2;
-                    for (int j = 0; j < m; ++j)
-                        for (int l = 0; l < m; ++l) {
-                            const double v = ap[l + (size_t)m * j];
-                            for (int i = 0; i < m; ++i) t[i + (size_t)m * j] += sq[i + (size_t)m * l] * v;
-                        }
-                    for (int j = 0; j < m; ++j)
-                        for (int l = 0; l < m; ++l) {
-                            const double v = sq[l + (size_t)m * j];
-                            for (int i = 0; i < m; ++i) S[i + (size_t)m * j] += t[i + (size_t)m * l] * v;
-                        }
-                    host_syev(m, S, w2);
-                    const double lim = 1.25 * wmax;
-                    for (int q = 0; q < m && q < 48; ++q) {
-                        if (!(w2[q] > 0.0) || std::sqrt(w2[q]) > lim) break;
-                        std::vector<double> a(m, 0.0), b(m, 0.0);
-                        for (int l = 0; l < m; ++l) {
-                            const double zl = S[l + (size_t)m * q];
-                            for (int i = 0; i < m; ++i) a[i] += sq[i + (size_t)m * l] * zl;
-                        }
-                        for (int j = 0; j < m; ++j)
-                            for (int i = 0; i < m; ++i) b[i] += aminv[i + (size_t)m * j] * a[j];
-                        add_col(a);
-                        add_col(b);
-                    }
+                for (int col = 0; col < 2 * nrhs; ++col) add_col(coef.data() + (size_t)m * col);
+                const int room = std::max(0, (maxdim / 2 - m0 - 2 * nrhs) / 2);
+                for (int q = 0, added = 0; q < m && added < room && wmax > 0.0; ++q) {
+                    if (!(lam[q] > 0.0) || lam[q] > 1.5625 * wmax * wmax) break;
+                    add_col(W.data() + (size_t)m * q);
+                    add_col(Yz.data() + (size_t)m * q);
+                    ++added;
                 }
                 const int r = (int)(Q.size() / m);
                 DBuf<double> Qd, rs((size_t)n * r);
@@ -2206,42 +2247,38 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                                                   hipMemcpyDeviceToDevice, s));
                 }
                 auto reduce = [&](const std::vector<double> &a, std::vector<double> &out) {
-                    std::vector<double> aq((size_t)m * r, 0.0);   // a Q
+                    std::vector<double> aq((size_t)m * r), qaq((size_t)r * r);
+                    host_gemm(false, false, m, r, m, 1.0, a.data(), m, Q.data(), m, 0.0, aq.data(), m);
+                    host_gemm(true, false, r, r, m, 1.0, Q.data(), m, aq.data(), m, 0.0, qaq.data(), r);
                     for (int j = 0; j < r; ++j)
-                        for (int l = 0; l < m; ++l) {
-                            const double qlj = Q[l + (size_t)m * j];
-                            for (int i = 0; i < m; ++i) aq[i + (size_t)m * j] += a[i + (size_t)m * l] * qlj;
-                        }
-                    std::fill(out.begin(), out.end(), 0.0);
-                    for (int j = 0; j < r; ++j)
-                        for (int i = 0; i < r; ++i) {
-                            double v = 0.0;
-                            for (int l = 0; l < m; ++l) v += Q[l + (size_t)m * i] * aq[l + (size_t)m * j];
-                            out[i + (size_t)maxdim * j] = v;
-                        }
+                        for (int i = 0; i < r; ++i) out[i + (size_t)maxdim * j] = qaq[i + (size_t)r * j];
                 };
                 reduce(ap, Ap);
                 reduce(am, Am);
                 std::vector<double> br((size_t)maxdim * 3, 0.0);
-                for (int rr = 0; rr < 3; ++rr)
-                    for (int i = 0; i < r; ++i) {
-                        double v = 0.0;
-                        for (int l = 0; l < m; ++l) v += Q[l + (size_t)m * i] * Br[l + (size_t)maxdim * rr];
-                        br[i + (size_t)maxdim * rr] = v;
-                    }
+                host_gemm(true, false, r, 3, m, 1.0, Q.data(), m, Br.data(), maxdim, 0.0, br.data(), maxdim);
                 Br.swap(br);
                 m = r;
+                m0 = std::min(m0, r);
                 if (m + nun > maxdim) nun = maxdim - m;
             }
             if (nun <= 0) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
             STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * nun * 8, hipMemcpyDeviceToDevice, s));
             mnew = orthonormalize(c, V.get(), m, V.get() + n * m, nun, tmp);
-            if (mnew == 0) { done = true; break; }
+            // Every new direction already lies in the space although the
+            // residuals are above the tolerance: the iteration cannot progress.
+            if (mnew == 0)
+                throw Error(STDA_HIP_ERR_NOT_CONVERGED,
+                            "response solver did not converge (no new directions)");
         }
         if (!done) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response solver did not converge");
         // alpha_rc = -2 mu_r^T (X+Y)_c (linear_response.f:147-172)
-        Yd.upload(sol.data(), sol.size(), s);
-        gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
+        if (m > 0) {
+            Yd.upload(coef.data(), (size_t)m * nrhs, s);
+            gemm_nn(c, n, nrhs, m, 1.0, V.get(), n, Yd.get(), m, 0.0, XP.get(), n);
+        } else {
+            STDA_HIP_CHECK(hipMemsetAsync(XP.get(), 0, (size_t)n * nrhs * 8, s));
+        }
         DBuf<double> Ad((size_t)3 * nrhs);
         dgemm_tn(c, 3, nrhs, n, c->mom.get(), n, XP.get(), n, Ad.get());
         std::vector<double> ah((size_t)3 * nrhs);
@@ -3369,8 +3406,9 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
             if (!conv) {
                 status[i] = STDA_HIP_ERR_NOT_CONVERGED;
                 errs[i] = "batched Davidson did not converge";
+            } else {
+                ++b->counts[0];   // the three counts are disjoint: batched OK, single OK, nonzero status
             }
-            ++b->counts[0];
         }
         b->timings[2] = t3.ms();
         items.clear();

@@ -76,7 +76,7 @@ def footprint_bytes(job: Job) -> int:
         maxdim = max(6 * k, k + 64)
     else:
         k = 3 * len(job.omegas)
-        maxdim = min(P, max(40 * k, 256))
+        maxdim = min(P, 2400)    # the response subspace (at most 2400 vectors)
     dav = 8 * P * 3 * min(maxdim, P)
     nsmall = min(P, 4096)
     explicit = 20 * nsmall * nsmall

@@ -479,6 +479,32 @@ def test_many_roots_against_dense():
     np.testing.assert_allclose(sol.omega, w[:k], atol=E_TOL)
 
 
+@pytest.mark.parametrize("system", ["polyene", "cluster30"])
+def test_kshift_xtb_parameters(system):
+    """xTB mode (std2 -xtb, main.f:198-219): a_x 0.5, alpha 2, beta 4 and the K(ia,ia)
+    shift (shftmax 0.5 eV, width 0.1 eV, steepness 4; stda.f:1750-1768) in the screening
+    and PT2 selection: identical CSF list, ed and roots as the oracle."""
+    from stda_amd import synth
+    say(f"{system} xTB parameters with the K shift")
+    if system == "polyene":
+        mol, ints = synth.polyene()
+        thr = 7.0
+    else:
+        mol, ints = synth.cluster(30, seed=3, basis="minimal")
+        thr = 10.0
+    ci = prepare(mol, thr, 0.5, ints)
+    ks = (0.5, 0.1, 4.0)
+    ref = O.run(ci, thr, 0.5, alphak=2.0, betaj=4.0, nroots=8, kshift=ks)
+    plain = O.run(ci, thr, 0.5, alphak=2.0, betaj=4.0, nroots=8)
+    assert ref.nex != plain.nex or not np.array_equal(ref.iconf, plain.iconf)
+    with _hip() as h:
+        h.setup(ci, thr, 0.5, alphak=2.0, betaj=4.0, kshift=ks)
+        _compare_selection(h, ref)
+        sol = h.solve_tda(8)
+    np.testing.assert_allclose(sol.omega, ref.omega, atol=E_TOL)
+    assert_props(sol.props, ref.props)
+
+
 def test_triplet_and_ax0(polyene):
     say("polyene triplet TDA and ax=0 RPA")
     ci = polyene

@@ -310,9 +310,10 @@ def _mirror(mol):
 
 def test_sos_rotation_mirror_image():
     """The SOS optical rotation (sosor.f) of the mirror image is the negative of the
-    molecule's: the sign conventions of the r x nabla and nabla integrals, the velocity
-    correction and the rotation formula hold together.  Energies and oscillator strengths
-    are mirror-invariant; water (achiral) rotates nothing."""
+    molecule's.  This is invariant to a global sign or factor of the r x nabla and nabla
+    integrals and of the rotation formula: it checks only that their relative signs are
+    consistent (the absolute convention of the integrals is pinned by quadrature in
+    test_aoints_quadrature.py).  Energies and oscillator strengths are mirror-invariant."""
     from stda_amd import spectrum as S
     from stda_amd import synth
     mol, _ = synth.cluster(12, seed=5, basis="minimal")
