@@ -28,6 +28,18 @@
 extern "C" {
 #endif
 
+/* Index type of the arrays the reference's driver holds as default Fortran
+ * integers: iaoat, iconf and the -dual atom list.  32-bit in the default build
+ * (libstda_hip.so); 64-bit in the build compiled with -DSTDA_HIP_ILP64
+ * (libstda_hip_ilp64.so), which pairs with std2 built with 8-byte default
+ * integers (meson_options.txt:37-43, meson.build:62-65; lin8 at
+ * stda.f:2183-2190).  Scalars keep their fixed C types in both builds. */
+#ifdef STDA_HIP_ILP64
+typedef int64_t stda_hip_int;
+#else
+typedef int32_t stda_hip_int;
+#endif
+
 enum {
     STDA_HIP_OK = 0,
     STDA_HIP_ERR_ARG = 1,          /* invalid argument / dimension */
@@ -52,7 +64,7 @@ typedef struct {
     int32_t no, nv;         /* window occupied / virtual MOs (moci = no + nv) */
     const double *clow;     /* clow(nao, moci): Loewdin MOs S^1/2 C, stda.f:436-452 */
     const double *epsi;     /* epsi(moci): orbital energies (Hartree), ascending */
-    const int32_t *iaoat;   /* iaoat(nao): 1-based atom of each AO (intslvm.f:66-77) */
+    const stda_hip_int *iaoat; /* iaoat(nao): 1-based atom of each AO (intslvm.f:66-77) */
     const double *xyz;      /* xyz(4, ncent): Bohr; xyz(4,:) = nuclear charge */
     const double *moments;  /* 9 * moci*(moci+1)/2, see above; may be NULL (no properties) */
 } stda_hip_system;
@@ -69,6 +81,10 @@ typedef struct {
 } stda_hip_params;
 
 typedef struct stda_hip_ctx stda_hip_ctx;
+
+/* Bytes of stda_hip_int in this build (4 or 8): a caller checks that it was
+ * linked against the library of its integer size.  No context needed. */
+int stda_hip_index_size(void);
 
 /* Create a context on HIP device `device` (-1: current device). */
 int stda_hip_create(stda_hip_ctx **ctx, int device);
@@ -150,13 +166,13 @@ int stda_hip_prepare_ao(stda_hip_ctx *ctx, int32_t nao, const int64_t *prim_ptr,
  *   pt2_off != 0: no PT2 selection, the primaries only (-dual 1, -rw 1).
  * No counterpart in the reference's RKS driver `stda` itself. */
 int stda_hip_set_selection(stda_hip_ctx *ctx, double thr2_ev, int32_t natoms,
-                           const int32_t *atoms, int32_t pt2_off);
+                           const stda_hip_int *atoms, int32_t pt2_off);
 
 /* CSF list after selection: iconf(nci,2) 1-based (occupied MO, absolute
  * virtual MO index no+1..moci) and ed(nci), the PT2-corrected diagonal
  * (same order as the reference: primaries sorted by energy, then the
  * PT2-selected CSFs in pair order). */
-int stda_hip_get_csf(stda_hip_ctx *ctx, int32_t *iconf, double *ed);
+int stda_hip_get_csf(stda_hip_ctx *ctx, stda_hip_int *iconf, double *ed);
 
 /*
  * TDA eigenproblem (replaces rtdamat + ssyevr, stda.f:1107-1150, and the

@@ -25,6 +25,8 @@ from .constants import HARTREE_EV, HARTREE_EV_PRINT
 from .prep import CoreInputs
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libstda_hip.so")
+# the build with 64-bit index arrays (stda_hip_int = int64_t: iaoat, iconf, -dual atoms)
+LIB_PATH_ILP64 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libstda_hip_ilp64.so")
 
 C_API = [
     "stda_hip_create", "stda_hip_destroy", "stda_hip_error_string", "stda_hip_last_error",
@@ -37,7 +39,7 @@ C_API = [
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
-    "stda_hip_prepare_ao",
+    "stda_hip_prepare_ao", "stda_hip_index_size",
 ]
 
 
@@ -63,19 +65,19 @@ class _Params(ctypes.Structure):
                 ("kshift_steep", ctypes.c_double)]
 
 
-_lib = None
+_libs = {}
 
 
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
-    """Load ``libstda_hip.so``; raises if it has not been built."""
-    global _lib
-    if _lib is not None:
-        return _lib
+    """Load ``libstda_hip.so`` (or the ILP64 build); raises if it has not been built."""
+    if path in _libs:
+        return _libs[path]
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python -m stda_amd.build` "
                            "(there is no CPU fallback)")
     L = ctypes.CDLL(path)
     P, I32, I64, F64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    L.stda_hip_index_size.argtypes = []
     L.stda_hip_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
     L.stda_hip_destroy.argtypes = [P]
     L.stda_hip_error_string.restype = ctypes.c_char_p
@@ -119,7 +121,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_batch_phase_times.argtypes = [P, P]
     L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
                                               ctypes.POINTER(F64), ctypes.POINTER(F64)]
-    _lib = L
+    _libs[path] = L
     return L
 
 
@@ -150,8 +152,12 @@ def _alpha3(a6):
 class StdaHip:
     """One device context of the excited-state core."""
 
-    def __init__(self, device: int = 0):
-        self._L = load_library()
+    def __init__(self, device: int = 0, ilp64: bool = False):
+        """``ilp64`` selects the library built with 64-bit index arrays."""
+        self._L = load_library(LIB_PATH_ILP64 if ilp64 else LIB_PATH)
+        self._idx = np.int64 if ilp64 else np.int32
+        if self._L.stda_hip_index_size() != np.dtype(self._idx).itemsize:
+            raise RuntimeError("library index size does not match the requested model")
         self._h = ctypes.c_void_p()
         self._check(self._L.stda_hip_create(ctypes.byref(self._h), device))
         self.nci = self.nprim = self.nsec = 0
@@ -184,7 +190,7 @@ class StdaHip:
     def set_selection(self, dual=None, pt2_off: bool = False):
         """Selection variants for the next setup (stda_hip_set_selection): dual = (thr2_ev,
         atoms 1-based) as std2 -dual reads them from its 'dual' file; pt2_off: primaries only."""
-        atoms = np.ascontiguousarray([] if dual is None else dual[1], dtype=np.int32)
+        atoms = np.ascontiguousarray([] if dual is None else dual[1], dtype=self._idx)
         thr2 = 0.0 if dual is None else float(dual[0])
         self._check(self._L.stda_hip_set_selection(self._h, thr2, len(atoms),
                                                    _ptr(atoms) if len(atoms) else None,
@@ -196,7 +202,7 @@ class StdaHip:
         """Stage 1: gamma, charges, screening, PT2 selection (stda.f:454-790)."""
         clow = np.asfortranarray(ci.clow, dtype=np.float64)
         eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
-        iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+        iaoat = np.ascontiguousarray(ci.iaoat, dtype=self._idx)
         xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
         mom = np.ascontiguousarray(ci.moments, dtype=np.float64) if ci.moments is not None else None
         self._keep = [clow, eps, iaoat, xyz, mom]
@@ -213,7 +219,7 @@ class StdaHip:
         """Copy the system to the device only (see :meth:`select`)."""
         clow = np.asfortranarray(ci.clow, dtype=np.float64)
         eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
-        iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+        iaoat = np.ascontiguousarray(ci.iaoat, dtype=self._idx)
         xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
         mom = np.ascontiguousarray(ci.moments, dtype=np.float64) if ci.moments is not None else None
         self._keep = [clow, eps, iaoat, xyz, mom]
@@ -249,7 +255,7 @@ class StdaHip:
         return {"ms": ms.value, "launches": n.value, "work": w.value}
 
     def csf(self):
-        iconf = np.zeros(2 * self.nci, dtype=np.int32)
+        iconf = np.zeros(2 * self.nci, dtype=self._idx)
         ed = np.zeros(self.nci)
         self._check(self._L.stda_hip_get_csf(self._h, _ptr(iconf), _ptr(ed)))
         return np.stack([iconf[:self.nci], iconf[self.nci:]], axis=1), ed

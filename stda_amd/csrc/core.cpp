@@ -1805,6 +1805,8 @@ static void rpa_properties(stda_hip_ctx *c, const double *XpY, const double *XmY
 
 extern "C" {
 
+int stda_hip_index_size(void) { return (int)sizeof(stda_hip_int); }
+
 int stda_hip_create(stda_hip_ctx **out, int device) {
     if (!out) return STDA_HIP_ERR_ARG;
     stda_hip_ctx *c = new stda_hip_ctx();
@@ -1903,14 +1905,14 @@ int stda_hip_kernel_stats(const stda_hip_ctx *c, const char *name, double *ms, d
     return STDA_HIP_OK;
 }
 
-int stda_hip_get_csf(stda_hip_ctx *c, int32_t *iconf, double *ed) {
+int stda_hip_get_csf(stda_hip_ctx *c, stda_hip_int *iconf, double *ed) {
     if (!c) return STDA_HIP_ERR_ARG;
     if (!c->ready) return STDA_HIP_ERR_STATE;
     for (int64_t I = 0; I < c->nci; ++I) {
         int64_t p = c->pair_h[I];
         if (iconf) {
-            iconf[I] = (int32_t)(p / c->nv) + 1;
-            iconf[I + c->nci] = (int32_t)(c->no + p % c->nv) + 1;
+            iconf[I] = (stda_hip_int)(p / c->nv) + 1;
+            iconf[I + c->nci] = (stda_hip_int)(c->no + p % c->nv) + 1;
         }
         if (ed) ed[I] = c->ed_h[I];
     }
@@ -2683,10 +2685,14 @@ int stda_hip_prepare_ao(stda_hip_ctx *c, int32_t nao, const int64_t *prim_ptr, c
     })
 }
 
-int stda_hip_set_selection(stda_hip_ctx *c, double thr2_ev, int32_t natoms, const int32_t *atoms,
-                           int32_t pt2_off) {
+int stda_hip_set_selection(stda_hip_ctx *c, double thr2_ev, int32_t natoms,
+                           const stda_hip_int *atoms, int32_t pt2_off) {
     if (!c || natoms < 0 || (natoms > 0 && (!atoms || !(thr2_ev > 0.0)))) return STDA_HIP_ERR_ARG;
-    c->dual_atoms.assign(atoms, atoms + natoms);
+    c->dual_atoms.clear();
+    for (int32_t q = 0; q < natoms; ++q) {
+        if (atoms[q] < 1 || atoms[q] > INT32_MAX) return STDA_HIP_ERR_ARG;
+        c->dual_atoms.push_back((int32_t)atoms[q]);
+    }
     c->dual_thr2_ev = natoms > 0 ? thr2_ev : 0.0;
     c->pt2_off = pt2_off != 0;
     c->ready = false;   // takes effect at the next select

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -17,12 +19,16 @@ def test_header_matches_python_table():
     assert declared_functions() == sorted(C_API)
 
 
-def test_library_exports_every_declared_symbol():
-    from stda_amd.core import LIB_PATH, load_library
-    assert os.path.exists(LIB_PATH), "build libstda_hip.so first (python -m stda_amd.build)"
-    L = load_library()
+@pytest.mark.parametrize("model", ["lp64", "ilp64"])
+def test_library_exports_every_declared_symbol(model):
+    from stda_amd.core import LIB_PATH, LIB_PATH_ILP64, load_library
+    path = LIB_PATH if model == "lp64" else LIB_PATH_ILP64
+    assert os.path.exists(path), "build the libraries first (python -m stda_amd.build)"
+    L = load_library(path)
     for name in declared_functions():
         assert hasattr(L, name), name
+    # the index type of iaoat / iconf / the -dual atom list
+    assert L.stda_hip_index_size() == (4 if model == "lp64" else 8)
 
 
 def test_error_strings_without_device():
