@@ -442,7 +442,7 @@ def stda(ci: CoreInputs, thr: float, thrp: float, ax: float, alphak: float = -10
          betaj: float = -100.0, fthr: float | None = None, nroots: int | None = None,
          rpa: bool = False, triplet: bool = False, velcorr: bool | None = None,
          resp_omegas=None, tol: float = 1e-8, device: int = 0, beta_omegas=None,
-         tpa_states: int | None = None, dual=None, pt2_off: bool = False):
+         tpa_states: int | None = None, dual=None, pt2_off: bool = False, kshift=None):
     """Drop-in for the reference's ``stda`` hot path (stda.f:454-1471).
 
     Returns ``(Solution | list-of-alpha | list-of-beta | list-of-2PA, csf)`` where
@@ -454,13 +454,17 @@ def stda(ci: CoreInputs, thr: float, thrp: float, ax: float, alphak: float = -10
     solve (the reference's srpapack forms them with doubled diagonals in that mode;
     :meth:`StdaHip.tpa` takes any states).  ``dual`` = (thr2_ev, atoms) and ``pt2_off``
     are the selection variants of ``-dual`` / ``-rw 1`` (:meth:`StdaHip.set_selection`).
+    ``kshift`` = (shftmax eV, width eV, steepness) switches on the K(ia,ia) shift of
+    xTB mode (``dokshift``); ``stda_amd.xtb.XTB_PARAMETERS`` holds the whole ``-xtb`` set
+    (a_x, alphak, betaj, kshift) for an input read with ``stda_amd.xtb.read_wfn_xtb``.
     """
     if velcorr is None:
         velcorr = not rpa                 # main.f:102, 662
     with StdaHip(device) as h:
         if dual is not None or pt2_off:
             h.set_selection(dual=dual, pt2_off=pt2_off)
-        h.setup(ci, thr, ax, thrp=thrp, alphak=alphak, betaj=betaj, fthr=fthr, triplet=triplet)
+        h.setup(ci, thr, ax, thrp=thrp, alphak=alphak, betaj=betaj, fthr=fthr, triplet=triplet,
+                kshift=kshift)
         csf = h.csf()
         if beta_omegas is not None:
             return h.beta_shg(beta_omegas), csf
