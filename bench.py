@@ -463,8 +463,8 @@ def run_serve(args):
     cis = synth.mixed_batch(n, lo=10, hi=400, base_seed=2000, basis="minimal", thr=7.0, ax=0.25,
                             workers=gen_workers())
     log(f"# serve workload: {n} molecules (inputs {time.perf_counter() - t0:.1f}s)")
-    rng = np.random.default_rng(2000)
-    kinds = np.where(rng.uniform(size=n) < 0.7, "spectrum", "polar")
+    from stda_amd.serve import request_mix
+    kinds = request_mix(n, seed=2000)[1]
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
     with Server(workers=4, batch_workers=args.workers) as srv:

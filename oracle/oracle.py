@@ -199,11 +199,13 @@ def dual_groups(ci, atoms, thr, thr2_ev, fthr, ax):
 
 
 def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=None, dual=None,
-           pt2_off=False):
+           pt2_off=False, screen_only=False):
     """Screening (stda.f:593-668) then PT2 selection (stda.f:1773-1863).
 
     dual = (thr2_ev, atoms, ax): the dual-threshold screening of std2 -dual
-    (stda-rw_dual.f:658-853); pt2_off: no PT2 selection (-dual 1 / -rw 1)."""
+    (stda-rw_dual.f:658-853); pt2_off: no PT2 selection (-dual 1 / -rw 1);
+    screen_only: return the screening (primaries in sorted order, every secondary in
+    kconf/edpt) without the PT2 step, for ptselect_rows."""
     L = lib()
     no, nv, n = ci.no, ci.nv, ci.ncent
     ak = 0.0 if triplet else 2.0
@@ -231,6 +233,10 @@ def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=Non
     nex, nexpt = nex.value, nexpt.value
     if nex < 1:
         raise RuntimeError("no CSF, increase energy threshold (-e option)")
+    if screen_only:
+        return dict(nex=nex, nexpt=nexpt, iconf_raw=iconf, ed_raw=ed, mx=mx, ak=ak, thr=thr,
+                    kconf=np.stack([kconf[:nexpt], kconf[mx:mx + nexpt]], axis=1),
+                    edpt=edpt[:nexpt].copy())
     if pt2_off:
         nexpt = 0
     incl = np.zeros(max(nexpt, 1), dtype=np.uint8)
@@ -243,6 +249,30 @@ def select(ci, inter, thr_ev, thrp=1e-4, fthr_ev=None, triplet=False, kshift=Non
                 ed=ed[:nci].copy(), ed_raw=ed, mx=mx, ak=ak, thr=thr,
                 kconf=np.stack([kconf[:nexpt], kconf[mx:mx + nexpt]], axis=1),
                 edpt=edpt[:nexpt].copy(), incl=incl[:nexpt].astype(bool))
+
+
+def ptselect_rows(ci, inter, scr, rows, thrp=1e-4):
+    """ptselect (stda.f:1773-1863) for the secondaries ``rows`` (indices into the
+    screening's kconf) only: their inclusion flags.  A secondary's inclusion depends on
+    its own row of couplings to every primary, not on the other secondaries, so the
+    flags of a sample equal those of the full selection; the primaries' PT2 energy
+    lowering (the column sums over all rejected secondaries) is not formed."""
+    L = lib()
+    mx, nex = scr["mx"], scr["nex"]
+    rows = np.asarray(rows, dtype=np.int64)
+    ns = len(rows)
+    kc = np.zeros(2 * mx, dtype=np.int32)
+    kc[:ns] = scr["kconf"][rows, 0]
+    kc[mx:mx + ns] = scr["kconf"][rows, 1]
+    edpt = np.zeros(mx)
+    edpt[:ns] = scr["edpt"][rows]
+    iconf = scr["iconf_raw"].copy()
+    ed = scr["ed_raw"].copy()
+    incl = np.zeros(max(ns, 1), dtype=np.uint8)
+    L.orc_ptselect(nex, ci.ncent, ci.no, ci.nv, ns, mx, _p(iconf), _p(kc), scr["ak"], _p(ed),
+                   _p(edpt), _p(inter["pia"]), _p(inter["qia"]), _p(inter["pij"]),
+                   _p(inter["qab"]), thrp, _p(incl))
+    return incl[:ns].astype(bool)
 
 
 def build_a(ci, inter, sel):

@@ -2051,7 +2051,16 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         STDA_HIP_CHECK(hipSetDevice(c->device));
         Timer t3;
         if (tol <= 0.0) tol = 1.0e-10;
-        if (c->precision == 1) tol = std::max(tol, 1.0e-6);   // FP32 operator noise floor
+        // Mixed precision: the FP32 operator's noise floor is ~1e-7 of the
+        // operator norm times the solution norm, which near resonance exceeds
+        // any useful tolerance.  The iteration runs with the FP32 operator until
+        // its residuals reach max(tol, 1e-6) or stop falling, then the space's
+        // operator images are recomputed in FP64 and it finishes in FP64.
+        bool fp32 = c->precision == 1;
+        if (fp32) tol = std::max(tol, 1.0e-8);
+        const double tol32 = std::max(tol, 1.0e-6);
+        double best32 = 1.0e300;
+        int since_best = 0;
         const int64_t n = c->nci;
         hipStream_t s = c->stream;
         const int nrhs = 3 * nfreq;
@@ -2110,25 +2119,27 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         DBuf<double> oned;
         oned.upload(onev.data(), 3, s);
         for (int it = 0; it < 500 && !done; ++it) {
-            apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
-                      M.get() + n * m, n, c->precision == 1);
-            const int mt = m + mnew;
-            DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
-            dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
-            dgemm_tn(c, mt, mnew, n, V.get(), n, M.get() + n * m, n, Hc.get() + (size_t)mt * mnew);
-            dgemm_tn(c, mnew, 3, n, V.get() + n * m, n, rhs.get(), n, Hc.get() + (size_t)2 * mt * mnew);
-            std::vector<double> hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
-            Hc.download(hc.data(), hc.size(), s);
-            c->sync();
-            for (int j = 0; j < mnew; ++j)
-                for (int i = 0; i < mt; ++i) {
-                    double vp = hc[i + (size_t)mt * j], vm = hc[(size_t)mt * mnew + i + (size_t)mt * j];
-                    Ap[i + (size_t)maxdim * (m + j)] = vp; Ap[(m + j) + (size_t)maxdim * i] = vp;
-                    Am[i + (size_t)maxdim * (m + j)] = vm; Am[(m + j) + (size_t)maxdim * i] = vm;
-                }
-            for (int r = 0; r < 3; ++r)
-                for (int j = 0; j < mnew; ++j) Br[(m + j) + (size_t)maxdim * r] = hc[(size_t)2 * mt * mnew + j + (size_t)mnew * r];
-            m = mt;
+            if (mnew > 0) {   // operator images and reduced matrices of the new block
+                apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
+                          M.get() + n * m, n, fp32);
+                const int mt = m + mnew;
+                DBuf<double> Hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
+                dgemm_tn(c, mt, mnew, n, V.get(), n, P.get() + n * m, n, Hc.get());
+                dgemm_tn(c, mt, mnew, n, V.get(), n, M.get() + n * m, n, Hc.get() + (size_t)mt * mnew);
+                dgemm_tn(c, mnew, 3, n, V.get() + n * m, n, rhs.get(), n, Hc.get() + (size_t)2 * mt * mnew);
+                std::vector<double> hc((size_t)mt * mnew * 2 + (size_t)mnew * 3);
+                Hc.download(hc.data(), hc.size(), s);
+                c->sync();
+                for (int j = 0; j < mnew; ++j)
+                    for (int i = 0; i < mt; ++i) {
+                        double vp = hc[i + (size_t)mt * j], vm = hc[(size_t)mt * mnew + i + (size_t)mt * j];
+                        Ap[i + (size_t)maxdim * (m + j)] = vp; Ap[(m + j) + (size_t)maxdim * i] = vp;
+                        Am[i + (size_t)maxdim * (m + j)] = vm; Am[(m + j) + (size_t)maxdim * i] = vm;
+                    }
+                for (int r = 0; r < 3; ++r)
+                    for (int j = 0; j < mnew; ++j) Br[(m + j) + (size_t)maxdim * r] = hc[(size_t)2 * mt * mnew + j + (size_t)mnew * r];
+                m = mt;
+            }
             // Reduced problem, one factorisation for every frequency: with
             // Am = L L^T and L^T Ap L = Z diag(lam) Z^T (lam: the space's RPA
             // w_n^2), (Ap - w^2 Am^-1) a = V^T r gives a = L Z c with
@@ -2188,8 +2199,37 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             nrm.download(rn.data(), rn.size(), s);
             c->sync();
             std::vector<int> un;
-            for (int j = 0; j < 2 * nrhs; ++j)
-                if (!(rn[j] < tol * std::max(1.0, rhsnorm[j % 3]))) un.push_back(j);
+            const double tol_now = fp32 ? tol32 : tol;
+            double worst = 0.0;
+            for (int j = 0; j < 2 * nrhs; ++j) {
+                const double lim = tol_now * std::max(1.0, rhsnorm[j % 3]);
+                if (!(rn[j] < lim)) un.push_back(j);
+                worst = std::max(worst, std::isfinite(rn[j]) ? rn[j] / lim : 1.0e300);
+            }
+            if (fp32) {
+                if (worst < 0.5 * best32) { best32 = worst; since_best = 0; }
+                else ++since_best;
+            }
+            const bool switch64 = fp32 && ((un.empty() && tol32 > tol) || since_best >= 8 ||
+                                           (m + (int)un.size() > (3 * maxdim) / 4 && m < n));
+            if (switch64) {
+                // FP64 images of the whole space; V^T r is exact already
+                fp32 = false;
+                apply_ops(c, OP_APB, OP_AMB, m, V.get(), n, P.get(), n, M.get(), n, false);
+                DBuf<double> Hf((size_t)m * m * 2);
+                dgemm_tn(c, m, m, n, V.get(), n, P.get(), n, Hf.get());
+                dgemm_tn(c, m, m, n, V.get(), n, M.get(), n, Hf.get() + (size_t)m * m);
+                std::vector<double> hf((size_t)m * m * 2);
+                Hf.download(hf.data(), hf.size(), s);
+                c->sync();
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < m; ++i) {
+                        Ap[i + (size_t)maxdim * j] = hf[i + (size_t)m * j];
+                        Am[i + (size_t)maxdim * j] = hf[(size_t)m * m + i + (size_t)m * j];
+                    }
+                mnew = 0;
+                continue;
+            }
             if (un.empty() || m == n) { done = true; break; }
             // 2x2 diagonal preconditioner of the coupled equations, then keep
             // the unconverged columns

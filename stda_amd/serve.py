@@ -260,6 +260,19 @@ class Server:
                         mean_batch=nbatches[1] / max(nbatches[0], 1))
 
 
+def request_mix(n: int, seed: int = 2000, lo: int = 10, hi: int = 400,
+                polar_fraction: float = 0.3):
+    """Atom counts (log-uniform in [lo, hi], :func:`stda_amd.synth.mixed_sizes`) and
+    job kinds ("spectrum" / "polar") of a serving stream of ``n`` requests; request i
+    is the molecule ``synth.cluster(sizes[i], seed=seed + i, basis="minimal")``.  The
+    kinds come from a stream of their own, independent of the sizes."""
+    from .synth import mixed_sizes
+    sizes = mixed_sizes(n, lo, hi, seed)
+    u = np.random.Generator(np.random.PCG64([seed, 1])).uniform(size=n)
+    kinds = np.where(u < 1.0 - polar_fraction, "spectrum", "polar")
+    return sizes, kinds
+
+
 def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
     """Arrival times (s) of an open-loop Poisson stream of ``rate`` requests/s."""
     rng = np.random.Generator(np.random.PCG64(seed))

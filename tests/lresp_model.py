@@ -6,6 +6,10 @@ convergence behaviour (subspace size, restarts, exits) can be checked on the
 CPU against the oracle's dense solve of the reference's equations
 ((A+B) - w^2 (A-B)^-1)(X+Y) = -2 mu (linear_response.f:90-131).
 
+The model is the FP64 loop; the library's mixed-precision mode runs the same loop with
+the FP32 operator first and switches to this FP64 loop (the space's operator images
+recomputed in FP64) once its residuals stop falling.
+
 ``lresp_model(apply, ed, mu, omegas, maxdim)`` returns ``(alpha, status, info)``
 with ``alpha`` (nfreq, 3, 3) as alpha[f, r, c] = -2 mu_r^T (X+Y)_c, ``status``
 "ok" or "not converged", ``info`` a dict (iterations, restarts, final size).
