@@ -1052,9 +1052,14 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
             const double *fl = pin + o_hc + (size_t)mt * mnew;
             for (int j = 0; j < mnew; ++j) nnew += (alive[m + j] = fl[j] != 0.0 ? 1 : 0);
             flags_pending = false;
-            // no new direction survived: converged to working precision (X and
-            // theta are those of the previous iteration)
-            if (nnew == 0 && it > 0) return;
+            // No new direction survived although residuals were above the
+            // tolerance.  For the FP32 operator this is its working precision
+            // (an FP64 refinement follows, X and theta are those of the previous
+            // iteration); in FP64 the iteration cannot progress: an error.
+            if (nnew == 0 && it > 0) {
+                if (o.fp32 || o.allow_unconverged) return;
+                throw Error(STDA_HIP_ERR_NOT_CONVERGED, "Davidson did not converge (no new directions)");
+            }
         }
         for (int j = 0; j < mnew; ++j)
             for (int i = 0; i < mt; ++i) {
@@ -1142,7 +1147,11 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
     hipStream_t s = c->stream;
     k = (int)std::min<int64_t>(k, n);
     int nguess = (int)std::min<int64_t>(n, std::max(k + 8, k + k / 2));
-    int maxdim = (int)std::min<int64_t>(n, std::max(8 * k, k + 96));
+    // Thick restart: the Ritz pairs of the lowest kt = k + max(8, k/2) roots are
+    // kept, so that roots in a dense cluster (the 400-atom workload has 50 roots
+    // within 0.1 eV) keep their neighbours through a restart.
+    const int kt_max = (int)std::min<int64_t>(n, k + std::max(8, k / 2));
+    int maxdim = (int)std::min<int64_t>(n, std::max(10 * k, 2 * k + 128));
     if (maxdim < nguess) maxdim = nguess;
     DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim);
     DBuf<double> R((size_t)n * 2 * k), PX((size_t)n * k), MX((size_t)n * k), tmp, Yd, nrm(2 * k);
@@ -1239,9 +1248,27 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
         thud.upload(thu.data(), nun, s);
         launch_precondition(s, n, nun, R.get(), c->ed.get(), thud.get(), R.get());
         if (m + nun > maxdim) {
-            // restart: V <- orthonormal basis of span(X+Y, X-Y), P/M transformed alike
-            std::vector<double> G = both;   // (m x 2kk) coefficient block
-            int nc = 2 * kk;
+            // restart: V <- orthonormal basis of span(X+Y, X-Y) of the lowest kt
+            // Ritz pairs, P/M transformed alike
+            const int kt = std::min(m, kt_max);
+            std::vector<double> G((size_t)m * 2 * kt);
+            {
+                std::vector<double> Zt(Hm.begin(), Hm.begin() + (size_t)m * kt);
+                std::vector<double> xt = matmul(sq, Zt, m, m, kt);
+                for (int j = 0; j < kt; ++j) {
+                    const double wj = std::sqrt(std::max(w2[j], 1.0e-300));
+                    for (int i = 0; i < m; ++i) xt[i + (size_t)m * j] /= std::sqrt(wj);
+                }
+                std::vector<double> at = matmul(ap, xt, m, m, kt);
+                for (int j = 0; j < kt; ++j) {
+                    const double wj = std::sqrt(std::max(w2[j], 1.0e-300));
+                    for (int i = 0; i < m; ++i) {
+                        G[i + (size_t)m * j] = xt[i + (size_t)m * j];
+                        G[i + (size_t)m * (kt + j)] = at[i + (size_t)m * j] / wj;
+                    }
+                }
+            }
+            int nc = 2 * kt;
             std::vector<double> S = matmul([&] {
                 std::vector<double> t((size_t)nc * m);
                 for (int j = 0; j < nc; ++j)
@@ -1287,7 +1314,12 @@ void davidson_rpa(stda_hip_ctx *c, int k, double tol, std::vector<double> &omega
         STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * room * 8,
                                       hipMemcpyDeviceToDevice, s));
         mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
-        if (mnew == 0) return;
+        // every new direction lies in the space although residuals are above the
+        // tolerance: the iteration cannot progress
+        if (mnew == 0) {
+            if (o.allow_unconverged) return;
+            throw Error(STDA_HIP_ERR_NOT_CONVERGED, "RPA Davidson did not converge (no new directions)");
+        }
     }
     throw Error(STDA_HIP_ERR_NOT_CONVERGED, "RPA Davidson did not converge");
 }
