@@ -288,6 +288,22 @@ int stda_hip_gspec(stda_hip_ctx *ctx, int32_t nspec, const int64_t *line_offsets
                    const double *y, const double *wei, const double *emin, const double *step,
                    int32_t npts, int32_t flags, double rfaktor, double *e, double *r);
 
+/* stda_hip_gspec without any device allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernel and results
+ * as stda_hip_gspec (host arrays in and out); the call stages its inputs,
+ * parameters and outputs through a fixed device array of 32 MiB per GPU,
+ * reserved when the library is loaded and held until the process ends, and
+ * calls no allocating routine.  It needs, in doubles, each part rounded up to
+ * a multiple of 32: nspec + 1, three times the line count, twice nspec and
+ * twice nspec * npts; calls needing more than 32 MiB return
+ * STDA_HIP_ERR_TOO_LARGE.  Every call on one GPU uses that GPU's array: calls
+ * that overlap on the same GPU (from several threads or contexts) race. */
+int stda_hip_gspec_noalloc(stda_hip_ctx *ctx, int32_t nspec, const int64_t *line_offsets,
+                           const double *x, const double *y, const double *wei,
+                           const double *emin, const double *step, int32_t npts, int32_t flags,
+                           double rfaktor, double *e, double *r);
+
 /* The lines of spec.dat (g_spec.f format 1000, (f16.6, e14.4)) for n grid
  * points: out receives 31 n bytes, each line 30 characters and a newline, no
  * terminating zero.  Host only (no device work, no context).  No counterpart in

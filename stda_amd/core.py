@@ -39,7 +39,7 @@ C_API = [
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
-    "stda_hip_prepare_ao", "stda_hip_index_size",
+    "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc",
 ]
 
 
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_build_bhalf.argtypes = [P, P]
     L.stda_hip_matvec.argtypes = [P, I32, I32, P, P]
     L.stda_hip_gspec.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
+    L.stda_hip_gspec_noalloc.argtypes = [P, I32, P, P, P, P, P, P, I32, I32, F64, P, P]
     L.stda_hip_beta_shg.argtypes = [P, I32, P, F64, P, P]
     L.stda_hip_tpa.argtypes = [P, I32, P, P, P, I64, F64, P, P]
     L.stda_hip_set_selection.argtypes = [P, F64, I32, P, I32]

@@ -1822,6 +1822,39 @@ static void rpa_properties(stda_hip_ctx *c, const double *XpY, const double *XmY
 
 // ============================================================================================
 // C ABI
+// ------------------------------------------------------------------------------------------
+// Fixed device staging area of stda_hip_gspec_noalloc: one copy per GPU, held
+// until the process ends.  In host code `scratch` is only a host-side stand-in
+// for the symbol; its device address is taken once per device, with the device
+// current, by hipGetSymbolAddress at context creation and kept in
+// g_scratch_ptr, which is what copies and kernel arguments use.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's copy of `scratch` (the device must be current).
+void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(STDA_HIP_ERR_ARG, "device index too large");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(STDA_HIP_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                             hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+}
+
+double *scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 #define STDA_TRY(ctx, ...)                                            \
     try {                                                             \
         AllocScope alloc_scope_((ctx)->stream);                       \
@@ -1845,6 +1878,7 @@ int stda_hip_create(stda_hip_ctx **out, int device) {
     try {
         if (device >= 0) STDA_HIP_CHECK(hipSetDevice(device));
         STDA_HIP_CHECK(hipGetDevice(&c->device));
+        scratch_init(c->device);
         STDA_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->owner.stream = c->stream;
         STDA_BLAS_CHECK(rocblas_create_handle(&c->blas));
@@ -2987,6 +3021,61 @@ int stda_hip_gspec(stda_hip_ctx *c, int32_t nspec, const int64_t *line_offsets, 
         RO.download(r, np, st);
         c->sync();
     })
+}
+
+int stda_hip_gspec_noalloc(stda_hip_ctx *c, int32_t nspec, const int64_t *line_offsets,
+                           const double *x, const double *y, const double *wei,
+                           const double *emin, const double *step, int32_t npts, int32_t flags,
+                           double rfaktor, double *e, double *r) {
+    if (!c || nspec < 0 || npts <= 0 || (flags & ~15) != 0) return STDA_HIP_ERR_ARG;
+    if (nspec == 0) return STDA_HIP_OK;
+    if (!line_offsets || !emin || !step || !e || !r) return STDA_HIP_ERR_ARG;
+    if (line_offsets[0] != 0) return STDA_HIP_ERR_ARG;
+    for (int32_t q = 0; q < nspec; ++q)
+        if (line_offsets[q + 1] < line_offsets[q]) return STDA_HIP_ERR_ARG;
+    const int64_t nl = line_offsets[nspec];
+    if (nl > 0 && (!x || !y || !wei)) return STDA_HIP_ERR_ARG;
+    // staging layout in the scratch array (doubles; every part starts on 256 bytes)
+    const size_t np = (size_t)nspec * (size_t)npts;
+    auto up = [](size_t nd) { return (nd + 31) / 32 * 32; };
+    const size_t o_off = 0, o_x = o_off + up((size_t)nspec + 1), o_y = o_x + up((size_t)nl),
+                 o_w = o_y + up((size_t)nl), o_e0 = o_w + up((size_t)nl),
+                 o_st = o_e0 + up((size_t)nspec), o_eo = o_st + up((size_t)nspec),
+                 o_ro = o_eo + up(np), total = o_ro + up(np);
+    if (total * sizeof(double) > kScratchBytes) {
+        c->last_error = "gspec_noalloc: " + std::to_string(total * sizeof(double)) +
+                        " bytes of staging needed, the fixed area holds " +
+                        std::to_string(kScratchBytes);
+        return STDA_HIP_ERR_TOO_LARGE;
+    }
+    double *base = scratch_of(c->device);
+    if (!base) {
+        c->last_error = "gspec_noalloc: no scratch address for this device";
+        return STDA_HIP_ERR_STATE;
+    }
+    try {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t st = c->stream;
+        int64_t *doff = reinterpret_cast<int64_t *>(base + o_off);
+        STDA_HIP_CHECK(hipMemcpyAsync(doff, line_offsets, ((size_t)nspec + 1) * 8, hipMemcpyHostToDevice, st));
+        if (nl > 0) {
+            STDA_HIP_CHECK(hipMemcpyAsync(base + o_x, x, (size_t)nl * 8, hipMemcpyHostToDevice, st));
+            STDA_HIP_CHECK(hipMemcpyAsync(base + o_y, y, (size_t)nl * 8, hipMemcpyHostToDevice, st));
+            STDA_HIP_CHECK(hipMemcpyAsync(base + o_w, wei, (size_t)nl * 8, hipMemcpyHostToDevice, st));
+        }
+        STDA_HIP_CHECK(hipMemcpyAsync(base + o_e0, emin, (size_t)nspec * 8, hipMemcpyHostToDevice, st));
+        STDA_HIP_CHECK(hipMemcpyAsync(base + o_st, step, (size_t)nspec * 8, hipMemcpyHostToDevice, st));
+        // every grid point of every spectrum is written by the kernel: nothing to clear
+        launch_gspec(st, nspec, doff, base + o_x, base + o_y, base + o_w, base + o_e0,
+                     base + o_st, npts, flags, rfaktor, base + o_eo, base + o_ro);
+        STDA_HIP_CHECK(hipMemcpyAsync(e, base + o_eo, np * 8, hipMemcpyDeviceToHost, st));
+        STDA_HIP_CHECK(hipMemcpyAsync(r, base + o_ro, np * 8, hipMemcpyDeviceToHost, st));
+        STDA_HIP_CHECK(hipStreamSynchronize(st));
+    } catch (const Error &ex) {
+        c->last_error = ex.what();
+        return ex.status;
+    }
+    return STDA_HIP_OK;
 }
 
 int stda_hip_set_precision(stda_hip_ctx *c, int32_t precision) {
