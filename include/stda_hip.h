@@ -317,8 +317,24 @@ int stda_hip_format_curve(int64_t n, const double *e, const double *r, char *out
  *     and residual norms in FP64, converged to a residual of 1e-6, followed
  *     by one FP64 Rayleigh-Ritz refinement of the converged roots.  The
  *     response solver uses the FP32 operator with a 1e-6 tolerance.
+ *   2 split-bfloat16: the same, with the operator GEMMs' FP32 operands split
+ *     into two bfloat16 numbers each and multiplied as hi*hi + hi*lo + lo*hi
+ *     on the bfloat16 matrix cores with FP32 accumulation, to a residual of
+ *     3e-5; then the FP32 phase from its vectors and the FP64 refinement.
  * No counterpart in the reference (which is FP32 throughout, stda.f:82-107). */
 int stda_hip_set_precision(stda_hip_ctx *ctx, int32_t precision);
+
+/* The matrix-core GEMM behind every contraction of the operators, on host
+ * arrays: C = alpha op(A) op(B) + beta C, column-major, op = transpose when
+ * ta / tb is nonzero; op(A) is m x k, op(B) k x n.  mode 0: FP64 (the
+ * drop-in's arithmetic); 1: FP32 operands and accumulation; 2: split-bfloat16
+ * with FP32 accumulation (modes 1 and 2 round A, B, C to FP32 first and
+ * return the FP32 result widened).  No counterpart in the reference (its
+ * ssymm/sgemm calls, stda.f:558-589,742); exposes the kernel to tests and to
+ * callers that want it. */
+int stda_hip_gemm(stda_hip_ctx *ctx, int32_t mode, int32_t ta, int32_t tb, int32_t m, int32_t n,
+                  int32_t k, double alpha, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, double beta, double *C, int64_t ldc);
 
 /* Timing of the last setup/solve phases on this context, in milliseconds:
  * [0] gamma+charges+intermediates, [1] screening+sort, [2] PT2,

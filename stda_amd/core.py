@@ -39,7 +39,7 @@ C_API = [
     "stda_hip_batch_phase_times", "stda_hip_batch_kernel_stats", "stda_hip_gspec",
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
-    "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc",
+    "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc", "stda_hip_gemm",
 ]
 
 
@@ -106,6 +106,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_prepare_ao.argtypes = [P, I32, P, P, P, P, P, I32, P, I32, P, P]
     L.stda_hip_timings.argtypes = [P, P]
     L.stda_hip_set_precision.argtypes = [P, I32]
+    L.stda_hip_gemm.argtypes = [P, I32, I32, I32, I32, I32, I32, F64, P, I64, P, I64, F64, P, I64]
     L.stda_hip_load.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params)]
     L.stda_hip_select.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
     L.stda_hip_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
@@ -244,10 +245,26 @@ class StdaHip:
         return self.nci, self.nprim, self.nsec
 
     def set_precision(self, precision: str):
-        """'fp64' (default) or 'mixed' (FP32 contractions, FP64 Davidson numerics
-        and a final FP64 Rayleigh-Ritz refinement)."""
-        code = {"fp64": 0, "mixed": 1}[precision]
+        """'fp64' (default), 'mixed' (FP32 contractions, FP64 Davidson numerics
+        and a final FP64 Rayleigh-Ritz refinement) or 'bf16' (split-bfloat16
+        contractions first, then as 'mixed')."""
+        code = {"fp64": 0, "mixed": 1, "bf16": 2}[precision]
         self._check(self._L.stda_hip_set_precision(self._h, code))
+
+    def gemm(self, A, B, C=None, alpha=1.0, beta=0.0, ta=False, tb=False, mode="fp64"):
+        """C = alpha op(A) op(B) + beta C with the library's matrix-core GEMM
+        (stda_hip_gemm); mode 'fp64', 'fp32' or 'bf16' (split-bfloat16)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        m = A.shape[1] if ta else A.shape[0]
+        k = A.shape[0] if ta else A.shape[1]
+        n = B.shape[0] if tb else B.shape[1]
+        C = np.zeros((m, n), order="F") if C is None else np.asfortranarray(C, dtype=np.float64).copy()
+        code = {"fp64": 0, "fp32": 1, "bf16": 2}[mode]
+        self._check(self._L.stda_hip_gemm(self._h, code, int(ta), int(tb), m, n, k, alpha, _ptr(A),
+                                          max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), beta,
+                                          _ptr(C), max(1, m)))
+        return C
 
     def kernel_stats(self, name: str):
         ms, n, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
@@ -573,6 +590,21 @@ class StdaHipBatch:
         names = ["gram", "jacobi", "ritz", "expand", "final", "unused", "sweeps",
                  "worker_setup", "worker_matrices", "worker_single"]
         return dict(zip(names, t.tolist()))
+
+    def gemm(self, A, B, C=None, alpha=1.0, beta=0.0, ta=False, tb=False, mode="fp64"):
+        """C = alpha op(A) op(B) + beta C with the library's matrix-core GEMM
+        (stda_hip_gemm); mode 'fp64', 'fp32' or 'bf16' (split-bfloat16)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        B = np.asfortranarray(B, dtype=np.float64)
+        m = A.shape[1] if ta else A.shape[0]
+        k = A.shape[0] if ta else A.shape[1]
+        n = B.shape[0] if tb else B.shape[1]
+        C = np.zeros((m, n), order="F") if C is None else np.asfortranarray(C, dtype=np.float64).copy()
+        code = {"fp64": 0, "fp32": 1, "bf16": 2}[mode]
+        self._check(self._L.stda_hip_gemm(self._h, code, int(ta), int(tb), m, n, k, alpha, _ptr(A),
+                                          max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), beta,
+                                          _ptr(C), max(1, m)))
+        return C
 
     def kernel_stats(self, name: str):
         """Per-kernel statistics summed over the workers and molecules of the

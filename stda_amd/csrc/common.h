@@ -28,6 +28,17 @@ struct Error : std::runtime_error {
 constexpr int TK = 4, TC = 8, TJ = 4, TB = 8;
 constexpr int NA_CHUNK = 32;
 
+// --- matrix-core GEMM (gemm.hip) -----------------------------------------------------
+// C_b = alpha op(A_b) op(B_b) + beta C_b (column-major, batch strides sA, sB, sC).
+enum GemmMode { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X3 = 2 };
+void launch_gemm_f64(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                     const double *A, int64_t lda, int64_t sA, const double *B, int64_t ldb,
+                     int64_t sB, double beta, double *C, int64_t ldc, int64_t sC, int batch);
+// mode GEMM_F32 (FP32 MFMA) or GEMM_BF16X3 (split-bfloat16 MFMA, FP32 accumulation)
+void launch_gemm_f32(hipStream_t s, int mode, bool ta, bool tb, int M, int N, int K, float alpha,
+                     const float *A, int64_t lda, int64_t sA, const float *B, int64_t ldb,
+                     int64_t sB, float beta, float *C, int64_t ldc, int64_t sC, int batch);
+
 // --- kernel launchers (kernels.hip) -------------------------------------------------
 void launch_gamma(hipStream_t s, int ncent, const double *xyz4, const double *eta, double ax,
                   double alphak, double betaj, double *gamj, double *gamk);

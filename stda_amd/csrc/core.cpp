@@ -348,7 +348,9 @@ struct stda_hip_ctx {
     // matrix-free work buffers
     DBuf<double> xfull, tbuf, dbuf, gbuf, wbuf, rbuf, ubuf;
     // mixed precision: FP32 copies of the operator data and FP32 work buffers
-    int precision = 0;    // 0 FP64, 1 FP32 operator + FP64 Davidson numerics and refinement
+    int precision = 0;    // 0 FP64, 1 FP32 operator + FP64 Davidson numerics and refinement,
+                          // 2 split-bfloat16 operator, then FP32, then the FP64 refinement
+    int gemm32 = GEMM_F32;   // arithmetic of the FP32 operator's GEMMs in the current phase
     DBuf<float> ct32, gamj32, gamk32;
     DBuf<float> xfull32, tbuf32, dbuf32, gbuf32, wbuf32, rbuf32, ubuf32;
     int nb_max32 = 0;
@@ -701,10 +703,10 @@ void apply_ops_f32(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V,
         EventTimer evm(c);
         launch_scatter_pairs_f32(s, Vb, ldv, c->pair.get(), n, nb, c->xfull32.get(), c->npair);
         sgemm(c->blas, false, false, nao, (int64_t)no * nb, nv, 1.0f, Cv, nao, c->xfull32.get(), nv,
-              0.0f, c->tbuf32.get(), nao);
+              0.0f, c->tbuf32.get(), nao, c->gemm32);
         sgemm_batched(c->blas, false, true, nao, nao, no, 1.0f, c->tbuf32.get(), nao,
                       (int64_t)nao * no, Co, nao, 0, 0.0f, c->dbuf32.get(), nao, (int64_t)nao * nao,
-                      nb);
+                      nb, c->gemm32);
         bool need_u = false;
         int nops = 0;
         for (int pass = 0; pass < 2; ++pass) {
@@ -742,9 +744,9 @@ void apply_ops_f32(stda_hip_ctx *c, int op1, int op2, int nvec, const double *V,
                               c->gbuf32.get(), (int64_t)nao * nao);
             sgemm_batched(c->blas, false, false, nao, no, nao, 1.0f, c->gbuf32.get(), nao,
                           (int64_t)nao * nao, Co, nao, 0, 0.0f, c->wbuf32.get(), nao,
-                          (int64_t)nao * no, nb);
+                          (int64_t)nao * no, nb, c->gemm32);
             sgemm(c->blas, true, false, nv, (int64_t)no * nb, nao, 1.0f, Cv, nao, c->wbuf32.get(),
-                  nao, 0.0f, c->rbuf32.get(), nv);
+                  nao, 0.0f, c->rbuf32.get(), nv, c->gemm32);
             launch_gather_pairs_f32(s, c->rbuf32.get(), c->npair, c->pair.get(), n, nb, corr, Vb,
                                     ldv, out + ld * v0, ld);
         }
@@ -1130,7 +1132,10 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
             } else {
                 mnew = orthonormalize(c, V.get(), m, V.get() + n * m, room, tmp);
                 std::fill(alive.begin() + m, alive.begin() + m + mnew, 1);
-                if (mnew == 0) return;   // no new directions: converged to machine precision
+                if (mnew == 0) {   // no new directions
+                    if (o.fp32 || o.allow_unconverged) return;
+                    throw Error(STDA_HIP_ERR_NOT_CONVERGED, "Davidson did not converge (no new directions)");
+                }
             }
         }
     }
@@ -1775,13 +1780,24 @@ static void tda_properties(stda_hip_ctx *c, const double *X, const std::vector<d
 }
 
 // Mixed precision TDA: FP32 operator to a residual of max(tol, 1e-6), then one
-// FP64 Rayleigh-Ritz step in the span of the converged vectors.
+// FP64 Rayleigh-Ritz step in the span of the converged vectors.  With the
+// split-bfloat16 operator (precision 2) a first phase runs to 3e-5 (about its
+// noise floor) and the FP32 phase starts from its Ritz vectors.
 static void solve_tda_mixed(stda_hip_ctx *c, int nroots, double tol, std::vector<double> &w,
                             DBuf<double> &X) {
     DavOpts lo;
     lo.fp32 = true;
+    if (c->precision == 2) {
+        HostSpan hs(c, "mixed_bf16_phase");
+        c->gemm32 = GEMM_BF16X3;
+        davidson_tda(c, nroots, std::max(tol, 3.0e-5), w, X, lo);
+        c->gemm32 = GEMM_F32;
+        lo.V0 = X.get();
+        lo.nv0 = (int)w.size();
+    }
     {
         HostSpan hs(c, "mixed_fp32_phase");
+        DBuf<double> Xb = std::move(X);
         davidson_tda(c, nroots, std::max(tol, 1.0e-6), w, X, lo);
     }
     DavOpts ref;
@@ -1997,7 +2013,7 @@ int stda_hip_solve_tda(stda_hip_ctx *c, int32_t nroots, int32_t capacity, int32_
         std::vector<double> w;
         DBuf<double> X;
         const int64_t n = c->nci;
-        if (nroots > 0 && c->precision == 1) {
+        if (nroots > 0 && c->precision >= 1) {
             solve_tda_mixed(c, nroots, tol, w, X);
         } else if (nroots > 0) {
             davidson_tda(c, nroots, tol, w, X);
@@ -2050,9 +2066,22 @@ int stda_hip_solve_rpa(stda_hip_ctx *c, int32_t nroots, int32_t capacity, double
         std::vector<double> w;
         DBuf<double> XpY, XmY;
         const int64_t n = c->nci;
-        if (nroots > 0 && c->precision == 1) {
+        if (nroots > 0 && c->precision >= 1) {
             DavOpts lo;
             lo.fp32 = true;
+            DBuf<double> Vb;
+            if (c->precision == 2) {   // split-bfloat16 phase, then FP32 from its vectors
+                HostSpan hs(c, "mixed_bf16_phase");
+                c->gemm32 = GEMM_BF16X3;
+                davidson_rpa(c, nroots, std::max(tol, 3.0e-5), w, XpY, XmY, lo);
+                c->gemm32 = GEMM_F32;
+                const int kb = (int)w.size();
+                Vb.resize((size_t)n * 2 * kb);
+                STDA_HIP_CHECK(hipMemcpyAsync(Vb.get(), XpY.get(), (size_t)n * kb * 8, hipMemcpyDeviceToDevice, c->stream));
+                STDA_HIP_CHECK(hipMemcpyAsync(Vb.get() + n * kb, XmY.get(), (size_t)n * kb * 8, hipMemcpyDeviceToDevice, c->stream));
+                lo.V0 = Vb.get();
+                lo.nv0 = 2 * kb;
+            }
             {
                 HostSpan hs(c, "mixed_fp32_phase");
                 davidson_rpa(c, nroots, std::max(tol, 1.0e-6), w, XpY, XmY, lo);
@@ -2122,9 +2151,10 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         // any useful tolerance.  The iteration runs with the FP32 operator until
         // its residuals reach max(tol, 1e-6) or stop falling, then the space's
         // operator images are recomputed in FP64 and it finishes in FP64.
-        bool fp32 = c->precision == 1;
+        bool fp32 = c->precision >= 1;
         if (fp32) tol = std::max(tol, 1.0e-8);
-        const double tol32 = std::max(tol, 1.0e-6);
+        c->gemm32 = c->precision == 2 ? GEMM_BF16X3 : GEMM_F32;
+        const double tol32 = std::max(tol, c->precision == 2 ? 3.0e-5 : 1.0e-6);
         double best32 = 1.0e300;
         int since_best = 0;
         const int64_t n = c->nci;
@@ -2281,6 +2311,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             if (switch64) {
                 // FP64 images of the whole space; V^T r is exact already
                 fp32 = false;
+                c->gemm32 = GEMM_F32;
                 apply_ops(c, OP_APB, OP_AMB, m, V.get(), n, P.get(), n, M.get(), n, false);
                 DBuf<double> Hf((size_t)m * m * 2);
                 dgemm_tn(c, m, m, n, V.get(), n, P.get(), n, Hf.get());
@@ -3079,9 +3110,47 @@ int stda_hip_gspec_noalloc(stda_hip_ctx *c, int32_t nspec, const int64_t *line_o
 }
 
 int stda_hip_set_precision(stda_hip_ctx *c, int32_t precision) {
-    if (!c || precision < 0 || precision > 1) return STDA_HIP_ERR_ARG;
+    if (!c || precision < 0 || precision > 2) return STDA_HIP_ERR_ARG;
     c->precision = precision;
+    c->gemm32 = GEMM_F32;
     return STDA_HIP_OK;
+}
+
+int stda_hip_gemm(stda_hip_ctx *c, int32_t mode, int32_t ta, int32_t tb, int32_t m, int32_t n,
+                  int32_t k, double alpha, const double *A, int64_t lda, const double *B,
+                  int64_t ldb, double beta, double *C, int64_t ldc) {
+    if (!c || mode < 0 || mode > 2 || m < 0 || n < 0 || k < 0) return STDA_HIP_ERR_ARG;
+    if (m == 0 || n == 0) return STDA_HIP_OK;
+    const int64_t ar = ta ? k : m, ac = ta ? m : k, br = tb ? n : k, bc = tb ? k : n;
+    if (lda < std::max<int64_t>(1, ar) || ldb < std::max<int64_t>(1, br) || ldc < m) return STDA_HIP_ERR_ARG;
+    if ((ac > 0 && !A) || (bc > 0 && !B) || !C) return STDA_HIP_ERR_ARG;
+    STDA_TRY(c, {
+        STDA_HIP_CHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        const size_t na = (size_t)lda * ac, nbb = (size_t)ldb * bc, nc = (size_t)ldc * n;
+        if (mode == 0) {
+            DBuf<double> dA, dB, dC;
+            dA.upload(A, na, s);
+            dB.upload(B, nbb, s);
+            dC.upload(C, nc, s);
+            launch_gemm_f64(s, ta != 0, tb != 0, m, n, k, alpha, dA.get(), lda, 0, dB.get(), ldb, 0,
+                            beta, dC.get(), ldc, 0, 1);
+            dC.download(C, nc, s);
+        } else {
+            std::vector<float> hA(A, A + na), hB(B, B + nbb), hC(C, C + nc);
+            DBuf<float> dA, dB, dC;
+            dA.upload(hA.data(), na, s);
+            dB.upload(hB.data(), nbb, s);
+            dC.upload(hC.data(), nc, s);
+            launch_gemm_f32(s, mode == 2 ? GEMM_BF16X3 : GEMM_F32, ta != 0, tb != 0, m, n, k,
+                            (float)alpha, dA.get(), lda, 0, dB.get(), ldb, 0, (float)beta,
+                            dC.get(), ldc, 0, 1);
+            dC.download(hC.data(), nc, s);
+            c->sync();
+            for (size_t q = 0; q < nc; ++q) C[q] = hC[q];
+        }
+        c->sync();
+    })
 }
 
 int stda_hip_timings(const stda_hip_ctx *c, double *ms6) {

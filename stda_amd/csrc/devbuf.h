@@ -186,16 +186,26 @@ class DBuf {
             throw ::stda::Error(3, std::string(#expr) + ": " + rocblas_status_to_string(st_)); \
     } while (0)
 
-// C = alpha op(A) op(B) + beta C (column-major)
+// C = alpha op(A) op(B) + beta C (column-major) on the matrix cores (gemm.hip),
+// on the stream of the context's BLAS handle (kept for rocSOLVER).
+inline hipStream_t handle_stream(rocblas_handle h) {
+    hipStream_t s = nullptr;
+    STDA_BLAS_CHECK(rocblas_get_stream(h, &s));
+    return s;
+}
+
+inline void gemm_dims(int64_t m, int64_t n, int64_t k, int64_t batch) {
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX || batch > INT32_MAX)
+        throw Error(9, "gemm dimension beyond 2^31");
+}
+
 inline void dgemm(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
                   double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
                   double beta, double *C, int64_t ldc) {
     if (m == 0 || n == 0) return;
-    STDA_BLAS_CHECK(rocblas_dgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-                                  tb ? rocblas_operation_transpose : rocblas_operation_none,
-                                  (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
-                                  (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C,
-                                  (rocblas_int)ldc));
+    gemm_dims(m, n, k, 1);
+    launch_gemm_f64(handle_stream(h), ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, 0, B, ldb,
+                    0, beta, C, ldc, 0, 1);
 }
 
 inline void dgemm_batched(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
@@ -203,34 +213,33 @@ inline void dgemm_batched(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t
                           int64_t ldb, int64_t sb, double beta, double *C, int64_t ldc,
                           int64_t sc, int64_t batch) {
     if (m == 0 || n == 0 || batch == 0) return;
-    STDA_BLAS_CHECK(rocblas_dgemm_strided_batched(
-        h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-        tb ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-        (rocblas_int)k, &alpha, A, (rocblas_int)lda, sa, B, (rocblas_int)ldb, sb, &beta, C,
-        (rocblas_int)ldc, sc, (rocblas_int)batch));
+    gemm_dims(m, n, k, batch);
+    for (int64_t b0 = 0; b0 < batch; b0 += 65535)
+        launch_gemm_f64(handle_stream(h), ta, tb, (int)m, (int)n, (int)k, alpha, A + sa * b0, lda,
+                        sa, B + sb * b0, ldb, sb, beta, C + sc * b0, ldc, sc,
+                        (int)std::min<int64_t>(65535, batch - b0));
 }
 
+// FP32 operands: mode GEMM_F32 or GEMM_BF16X3
 inline void sgemm(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
                   const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
-                  int64_t ldc) {
+                  int64_t ldc, int mode = GEMM_F32) {
     if (m == 0 || n == 0) return;
-    STDA_BLAS_CHECK(rocblas_sgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-                                  tb ? rocblas_operation_transpose : rocblas_operation_none,
-                                  (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
-                                  (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C,
-                                  (rocblas_int)ldc));
+    gemm_dims(m, n, k, 1);
+    launch_gemm_f32(handle_stream(h), mode, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, 0, B,
+                    ldb, 0, beta, C, ldc, 0, 1);
 }
 
 inline void sgemm_batched(rocblas_handle h, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
                           float alpha, const float *A, int64_t lda, int64_t sa, const float *B,
                           int64_t ldb, int64_t sb, float beta, float *C, int64_t ldc, int64_t sc,
-                          int64_t batch) {
+                          int64_t batch, int mode = GEMM_F32) {
     if (m == 0 || n == 0 || batch == 0) return;
-    STDA_BLAS_CHECK(rocblas_sgemm_strided_batched(
-        h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-        tb ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-        (rocblas_int)k, &alpha, A, (rocblas_int)lda, sa, B, (rocblas_int)ldb, sb, &beta, C,
-        (rocblas_int)ldc, sc, (rocblas_int)batch));
+    gemm_dims(m, n, k, batch);
+    for (int64_t b0 = 0; b0 < batch; b0 += 65535)
+        launch_gemm_f32(handle_stream(h), mode, ta, tb, (int)m, (int)n, (int)k, alpha, A + sa * b0,
+                        lda, sa, B + sb * b0, ldb, sb, beta, C + sc * b0, ldc, sc,
+                        (int)std::min<int64_t>(65535, batch - b0));
 }
 
 // Host symmetric eigensolver (LAPACK dsyevd): eigenvalues ascending, vectors

@@ -1,0 +1,263 @@
+// Column-major GEMM on the matrix cores for the contractions of the
+// excited-state operators (the transition-density transforms of the
+// matrix-free A, A+B, A-B and 1/2 B, and the contracted charges p = gamma q):
+//
+//   C_b = alpha op(A_b) op(B_b) + beta C_b,   b = 0 .. batch-1
+//   op(A) is M x K: A[m + k lda], or A[k + m lda] when transposed
+//   op(B) is K x N: B[k + n ldb], or B[n + k ldb] when transposed
+//
+// Three arithmetic modes share one tiling:
+//   F64     FP64 operands and accumulation, v_mfma_f64_16x16x4f64;
+//   F32     FP32 operands and accumulation, v_mfma_f32_16x16x4f32;
+//   BF16X3  FP32 operands split x = hi + lo into two bfloat16 numbers
+//           (round to nearest even; |x - hi - lo| <= 2^-18 |x|), the product
+//           formed as hi hi + hi lo + lo hi by v_mfma_f32_16x16x32_bf16 with
+//           FP32 accumulation (the lo lo term, ~2^-18 relative, is dropped).
+//
+// Work-group: 256 threads (four waves) own a 64 x 64 tile of C; wave w owns
+// rows 32 (w / 2) .. + 32 and columns 32 (w % 2) .. + 32, i.e. 2 x 2 MFMA
+// blocks of 16 x 16.  K advances in chunks (16 for F64/F32, 32 for BF16X3):
+// each thread loads its share of the next chunk into registers (coalesced
+// along the contiguous index of each operand) while the waves multiply the
+// current chunk out of LDS.  Operand layouts (lane l, g = l / 16, c = l % 16):
+//   16x16x4:  A[c][g], B[g][c];       16x16x32 bf16: A[c][8g..8g+7], B[8g..8g+7][c]
+//   FP64 out: register i holds D[4i + g][c];   FP32 out: register i holds D[4g + i][c].
+// Edges are zero-filled on load and masked on store; blockIdx.z is the batch.
+
+#include "common.h"
+
+#include <type_traits>
+
+namespace stda {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int GT = 64;          // tile edge (rows and columns of C)
+constexpr int GTHREADS = 256;
+
+// bfloat16 of x rounded to nearest even (finite inputs), as raw bits
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short b) {
+    return __uint_as_float((unsigned int)b << 16);
+}
+
+template <int MODE> struct GemmTraits;
+template <> struct GemmTraits<GEMM_F64> {
+    using T = double;
+    static constexpr int KC = 16;
+};
+template <> struct GemmTraits<GEMM_F32> {
+    using T = float;
+    static constexpr int KC = 16;
+};
+template <> struct GemmTraits<GEMM_BF16X3> {
+    using T = float;
+    static constexpr int KC = 32;
+};
+
+// LDS: F64/F32 keep [k][m] (65-element rows); BF16X3 keeps [m][k] of hi and lo
+// bf16 (40-element rows, 80 bytes: each lane reads 8 consecutive k as 16 bytes).
+constexpr int PAD_MN = GT + 1;
+constexpr int BK_LD = 32 + 8;
+
+template <int MODE>
+struct GemmSmem;
+template <>
+struct GemmSmem<GEMM_F64> {
+    double a[16][PAD_MN], b[16][PAD_MN];
+};
+template <>
+struct GemmSmem<GEMM_F32> {
+    float a[16][PAD_MN], b[16][PAD_MN];
+};
+template <>
+struct GemmSmem<GEMM_BF16X3> {
+    unsigned short ah[GT][BK_LD], al[GT][BK_LD], bh[GT][BK_LD], bl[GT][BK_LD];
+};
+
+template <int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(GTHREADS)
+gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
+            const typename GemmTraits<MODE>::T *__restrict__ A, int64_t lda, int64_t sA,
+            const typename GemmTraits<MODE>::T *__restrict__ B, int64_t ldb, int64_t sB,
+            typename GemmTraits<MODE>::T beta, typename GemmTraits<MODE>::T *__restrict__ C,
+            int64_t ldc, int64_t sC) {
+    using T = typename GemmTraits<MODE>::T;
+    constexpr int KC = GemmTraits<MODE>::KC;
+    constexpr int PER = GT * KC / GTHREADS;   // elements of each operand per thread and chunk
+    __shared__ __align__(16) GemmSmem<MODE> sm;
+    const int t = threadIdx.x, l = t & 63, g = l >> 4, c = l & 15;
+    const int w = t >> 6, wr = w >> 1, wc = w & 1;
+    const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+    A += sA * blockIdx.z;
+    B += sB * blockIdx.z;
+    C += sC * blockIdx.z;
+
+    // (tile row, k) of this thread's p-th element of the A chunk, and (k, tile column) of B
+    auto a_idx = [&](int p, int &mm, int &kk) {
+        const int e = t + GTHREADS * p;
+        if (TA) { kk = e % KC; mm = e / KC; } else { mm = e % GT; kk = e / GT; }
+    };
+    auto b_idx = [&](int p, int &kk, int &nn) {
+        const int e = t + GTHREADS * p;
+        if (TB) { nn = e % GT; kk = e / GT; } else { kk = e % KC; nn = e / KC; }
+    };
+    T ra[PER], rb[PER];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            int mm, kk;
+            a_idx(p, mm, kk);
+            const int gm = m0 + mm, gk = k0 + kk;
+            ra[p] = (gm < M && gk < K) ? (TA ? A[gk + lda * gm] : A[gm + lda * gk]) : T(0);
+            int kb, nn;
+            b_idx(p, kb, nn);
+            const int gn = n0 + nn, gkb = k0 + kb;
+            rb[p] = (gn < N && gkb < K) ? (TB ? B[gn + ldb * gkb] : B[gkb + ldb * gn]) : T(0);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            int mm, kk, kb, nn;
+            a_idx(p, mm, kk);
+            b_idx(p, kb, nn);
+            if constexpr (MODE == GEMM_BF16X3) {
+                const unsigned short ah = bf16_bits(ra[p]);
+                sm.ah[mm][kk] = ah;
+                sm.al[mm][kk] = bf16_bits(ra[p] - bf16_value(ah));
+                const unsigned short bh = bf16_bits(rb[p]);
+                sm.bh[nn][kb] = bh;
+                sm.bl[nn][kb] = bf16_bits(rb[p] - bf16_value(bh));
+            } else {
+                sm.a[kk][mm] = ra[p];
+                sm.b[kb][nn] = rb[p];
+            }
+        }
+    };
+
+    using Acc = typename std::conditional<MODE == GEMM_F64, v4d, v4f>::type;
+    Acc acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();   // the previous chunk has been consumed
+        store();
+        __syncthreads();
+        if (k0 + KC < K) load(k0 + KC);   // next chunk in flight during the products
+        if constexpr (MODE == GEMM_BF16X3) {
+            v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = wr * 32 + i * 16 + c;
+                ah[i] = *reinterpret_cast<const v8bf *>(&sm.ah[r][8 * g]);
+                al[i] = *reinterpret_cast<const v8bf *>(&sm.al[r][8 * g]);
+                const int q = wc * 32 + i * 16 + c;
+                bh[i] = *reinterpret_cast<const v8bf *>(&sm.bh[q][8 * g]);
+                bl[i] = *reinterpret_cast<const v8bf *>(&sm.bl[q][8 * g]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 4) {
+                T av[2], bv[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    av[i] = sm.a[kk + g][wr * 32 + i * 16 + c];
+                    bv[i] = sm.b[kk + g][wc * 32 + i * 16 + c];
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        if constexpr (MODE == GEMM_F64)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+        }
+    }
+    // epilogue: alpha acc (+ beta C), masked at the edges
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = (MODE == GEMM_F64) ? 4 * e + g : 4 * g + e;
+                const int gm = m0 + wr * 32 + i * 16 + row;
+                const int gn = n0 + wc * 32 + j * 16 + c;
+                if (gm < M && gn < N) {
+                    T v = alpha * acc[i][j][e];
+                    T *p = C + gm + ldc * (int64_t)gn;
+                    if (beta != T(0)) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+template <int MODE, typename T>
+void launch_gemm_mode(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T *A,
+                      int64_t lda, int64_t sA, const T *B, int64_t ldb, int64_t sB, T beta, T *C,
+                      int64_t ldc, int64_t sC, int batch) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    const unsigned gx = (unsigned)((M + GT - 1) / GT), gy = (unsigned)((N + GT - 1) / GT);
+    if (gy > 65535u || batch > 65535)
+        throw Error(9, "gemm: more than 65535 column tiles or batch entries per launch");
+    const dim3 grid(gx, gy, (unsigned)batch);
+    if (!ta && !tb)
+        hipLaunchKernelGGL((gemm_kernel<MODE, false, false>), grid, dim3(GTHREADS), 0, s, M, N, K,
+                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((gemm_kernel<MODE, false, true>), grid, dim3(GTHREADS), 0, s, M, N, K,
+                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((gemm_kernel<MODE, true, false>), grid, dim3(GTHREADS), 0, s, M, N, K,
+                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else
+        hipLaunchKernelGGL((gemm_kernel<MODE, true, true>), grid, dim3(GTHREADS), 0, s, M, N, K,
+                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    STDA_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_gemm_f64(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                     const double *A, int64_t lda, int64_t sA, const double *B, int64_t ldb,
+                     int64_t sB, double beta, double *C, int64_t ldc, int64_t sC, int batch) {
+    launch_gemm_mode<GEMM_F64>(s, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                               sC, batch);
+}
+
+void launch_gemm_f32(hipStream_t s, int mode, bool ta, bool tb, int M, int N, int K, float alpha,
+                     const float *A, int64_t lda, int64_t sA, const float *B, int64_t ldb,
+                     int64_t sB, float beta, float *C, int64_t ldc, int64_t sC, int batch) {
+    if (mode == GEMM_BF16X3)
+        launch_gemm_mode<GEMM_BF16X3>(s, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                                      ldc, sC, batch);
+    else
+        launch_gemm_mode<GEMM_F32>(s, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                                   ldc, sC, batch);
+}
+
+}  // namespace stda
