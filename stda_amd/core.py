@@ -259,7 +259,7 @@ class StdaHip:
         m = A.shape[1] if ta else A.shape[0]
         k = A.shape[0] if ta else A.shape[1]
         n = B.shape[0] if tb else B.shape[1]
-        C = np.zeros((m, n), order="F") if C is None else np.asfortranarray(C, dtype=np.float64).copy()
+        C = np.zeros((m, n), order="F") if C is None else np.array(C, dtype=np.float64, order="F")
         code = {"fp64": 0, "fp32": 1, "bf16": 2}[mode]
         self._check(self._L.stda_hip_gemm(self._h, code, int(ta), int(tb), m, n, k, alpha, _ptr(A),
                                           max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), beta,
@@ -599,7 +599,7 @@ class StdaHipBatch:
         m = A.shape[1] if ta else A.shape[0]
         k = A.shape[0] if ta else A.shape[1]
         n = B.shape[0] if tb else B.shape[1]
-        C = np.zeros((m, n), order="F") if C is None else np.asfortranarray(C, dtype=np.float64).copy()
+        C = np.zeros((m, n), order="F") if C is None else np.array(C, dtype=np.float64, order="F")
         code = {"fp64": 0, "fp32": 1, "bf16": 2}[mode]
         self._check(self._L.stda_hip_gemm(self._h, code, int(ta), int(tb), m, n, k, alpha, _ptr(A),
                                           max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), beta,
