@@ -14,9 +14,10 @@
 //           formed as hi hi + hi lo + lo hi by v_mfma_f32_16x16x32_bf16 with
 //           FP32 accumulation (the lo lo term, ~2^-18 relative, is dropped).
 //
-// Work-group: 256 threads (four waves) own a 64 x 64 tile of C; wave w owns
-// rows 32 (w / 2) .. + 32 and columns 32 (w % 2) .. + 32, i.e. 2 x 2 MFMA
-// blocks of 16 x 16.  K advances in chunks (16 for F64/F32, 32 for BF16X3):
+// Work-group: 256 threads (four waves) own a BT x BT tile of C (BT = 128, or
+// 64 when the 128-tiles would leave compute units idle); wave w owns rows
+// BT/2 (w / 2) .. + BT/2 and columns BT/2 (w % 2) .. + BT/2, i.e. (BT/32)^2
+// MFMA blocks of 16 x 16.  K advances in chunks (16 for F64/F32, 32 for BF16X3):
 // each thread loads its share of the next chunk into registers (coalesced
 // along the contiguous index of each operand) while the waves multiply the
 // current chunk out of LDS.  Operand layouts (lane l, g = l / 16, c = l % 16):
@@ -36,7 +37,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int GT = 64;          // tile edge (rows and columns of C)
 constexpr int GTHREADS = 256;
 
 // bfloat16 of x rounded to nearest even (finite inputs), as raw bits
@@ -54,6 +54,11 @@ template <> struct GemmTraits<GEMM_F64> {
     using T = double;
     static constexpr int KC = 16;
 };
+// K chunk of a mode and tile: FP64 128-tiles take 8 (half the prefetch registers)
+template <int MODE, int BT>
+constexpr int chunk_of() {
+    return (MODE == GEMM_F64 && BT == 128) ? 8 : GemmTraits<MODE>::KC;
+}
 template <> struct GemmTraits<GEMM_F32> {
     using T = float;
     static constexpr int KC = 16;
@@ -63,40 +68,35 @@ template <> struct GemmTraits<GEMM_BF16X3> {
     static constexpr int KC = 32;
 };
 
-// LDS: F64/F32 keep [k][m] (65-element rows); BF16X3 keeps [m][k] of hi and lo
-// bf16 (40-element rows, 80 bytes: each lane reads 8 consecutive k as 16 bytes).
-constexpr int PAD_MN = GT + 1;
+// LDS: F64/F32 keep [k][m] (BT + 1-element rows); BF16X3 keeps [m][k] of hi and
+// lo bf16 (40-element rows, 80 bytes: each lane reads 8 consecutive k as 16 bytes).
 constexpr int BK_LD = 32 + 8;
 
-template <int MODE>
-struct GemmSmem;
-template <>
-struct GemmSmem<GEMM_F64> {
-    double a[16][PAD_MN], b[16][PAD_MN];
+template <int MODE, int BT>
+struct GemmSmem {
+    typename GemmTraits<MODE>::T a[16][BT + 1], b[16][BT + 1];
 };
-template <>
-struct GemmSmem<GEMM_F32> {
-    float a[16][PAD_MN], b[16][PAD_MN];
-};
-template <>
-struct GemmSmem<GEMM_BF16X3> {
-    unsigned short ah[GT][BK_LD], al[GT][BK_LD], bh[GT][BK_LD], bl[GT][BK_LD];
+template <int BT>
+struct GemmSmem<GEMM_BF16X3, BT> {
+    unsigned short ah[BT][BK_LD], al[BT][BK_LD], bh[BT][BK_LD], bl[BT][BK_LD];
 };
 
-template <int MODE, bool TA, bool TB>
-__global__ void __launch_bounds__(GTHREADS)
+template <int MODE, bool TA, bool TB, int BT>
+__global__ void __launch_bounds__(GTHREADS, 2)   // at least two waves per SIMD
 gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
             const typename GemmTraits<MODE>::T *__restrict__ A, int64_t lda, int64_t sA,
             const typename GemmTraits<MODE>::T *__restrict__ B, int64_t ldb, int64_t sB,
             typename GemmTraits<MODE>::T beta, typename GemmTraits<MODE>::T *__restrict__ C,
             int64_t ldc, int64_t sC) {
     using T = typename GemmTraits<MODE>::T;
-    constexpr int KC = GemmTraits<MODE>::KC;
-    constexpr int PER = GT * KC / GTHREADS;   // elements of each operand per thread and chunk
-    __shared__ __align__(16) GemmSmem<MODE> sm;
+    constexpr int KC = chunk_of<MODE, BT>();
+    constexpr int PER = BT * KC / GTHREADS;   // elements of each operand per thread and chunk
+    constexpr int WT = BT / 2;                // wave tile edge
+    constexpr int NB = WT / 16;               // MFMA blocks per wave tile edge
+    __shared__ __align__(16) GemmSmem<MODE, BT> sm;
     const int t = threadIdx.x, l = t & 63, g = l >> 4, c = l & 15;
     const int w = t >> 6, wr = w >> 1, wc = w & 1;
-    const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+    const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
     A += sA * blockIdx.z;
     B += sB * blockIdx.z;
     C += sC * blockIdx.z;
@@ -104,11 +104,11 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
     // (tile row, k) of this thread's p-th element of the A chunk, and (k, tile column) of B
     auto a_idx = [&](int p, int &mm, int &kk) {
         const int e = t + GTHREADS * p;
-        if (TA) { kk = e % KC; mm = e / KC; } else { mm = e % GT; kk = e / GT; }
+        if (TA) { kk = e % KC; mm = e / KC; } else { mm = e % BT; kk = e / BT; }
     };
     auto b_idx = [&](int p, int &kk, int &nn) {
         const int e = t + GTHREADS * p;
-        if (TB) { nn = e % GT; kk = e / GT; } else { kk = e % KC; nn = e / KC; }
+        if (TB) { nn = e % BT; kk = e / BT; } else { kk = e % KC; nn = e / KC; }
     };
     T ra[PER], rb[PER];
     auto load = [&](int k0) {
@@ -145,11 +145,11 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
     };
 
     using Acc = typename std::conditional<MODE == GEMM_F64, v4d, v4f>::type;
-    Acc acc[2][2];
+    Acc acc[NB][NB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+        for (int j = 0; j < NB; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
     load(0);
     for (int k0 = 0; k0 < K; k0 += KC) {
@@ -158,37 +158,38 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
         __syncthreads();
         if (k0 + KC < K) load(k0 + KC);   // next chunk in flight during the products
         if constexpr (MODE == GEMM_BF16X3) {
-            v8bf ah[2], al[2], bh[2], bl[2];
+            v8bf bh[NB], bl[NB];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = wr * 32 + i * 16 + c;
-                ah[i] = *reinterpret_cast<const v8bf *>(&sm.ah[r][8 * g]);
-                al[i] = *reinterpret_cast<const v8bf *>(&sm.al[r][8 * g]);
-                const int q = wc * 32 + i * 16 + c;
-                bh[i] = *reinterpret_cast<const v8bf *>(&sm.bh[q][8 * g]);
-                bl[i] = *reinterpret_cast<const v8bf *>(&sm.bl[q][8 * g]);
+            for (int j = 0; j < NB; ++j) {
+                const int q = wc * WT + j * 16 + c;
+                bh[j] = *reinterpret_cast<const v8bf *>(&sm.bh[q][8 * g]);
+                bl[j] = *reinterpret_cast<const v8bf *>(&sm.bl[q][8 * g]);
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < NB; ++i) {
+                const int r = wr * WT + i * 16 + c;
+                const v8bf ah = *reinterpret_cast<const v8bf *>(&sm.ah[r][8 * g]);
+                const v8bf al = *reinterpret_cast<const v8bf *>(&sm.al[r][8 * g]);
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < NB; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
                 }
+            }
         } else {
 #pragma unroll
             for (int kk = 0; kk < KC; kk += 4) {
-                T av[2], bv[2];
+                T av[NB], bv[NB];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    av[i] = sm.a[kk + g][wr * 32 + i * 16 + c];
-                    bv[i] = sm.b[kk + g][wc * 32 + i * 16 + c];
+                for (int i = 0; i < NB; ++i) {
+                    av[i] = sm.a[kk + g][wr * WT + i * 16 + c];
+                    bv[i] = sm.b[kk + g][wc * WT + i * 16 + c];
                 }
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < NB; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) {
+                    for (int j = 0; j < NB; ++j) {
                         if constexpr (MODE == GEMM_F64)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
                         else
@@ -199,14 +200,14 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
     }
     // epilogue: alpha acc (+ beta C), masked at the edges
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NB; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int row = (MODE == GEMM_F64) ? 4 * e + g : 4 * g + e;
-                const int gm = m0 + wr * 32 + i * 16 + row;
-                const int gn = n0 + wc * 32 + j * 16 + c;
+                const int gm = m0 + wr * WT + i * 16 + row;
+                const int gn = n0 + wc * WT + j * 16 + c;
                 if (gm < M && gn < N) {
                     T v = alpha * acc[i][j][e];
                     T *p = C + gm + ldc * (int64_t)gn;
@@ -221,22 +222,32 @@ void launch_gemm_mode(hipStream_t s, bool ta, bool tb, int M, int N, int K, T al
                       int64_t lda, int64_t sA, const T *B, int64_t ldb, int64_t sB, T beta, T *C,
                       int64_t ldc, int64_t sC, int batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    const unsigned gx = (unsigned)((M + GT - 1) / GT), gy = (unsigned)((N + GT - 1) / GT);
+    // 128-tiles when they still give every compute unit two work-groups
+    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
+    const int bt = big >= 512 ? 128 : 64;
+    const unsigned gx = (unsigned)((M + bt - 1) / bt), gy = (unsigned)((N + bt - 1) / bt);
     if (gy > 65535u || batch > 65535)
         throw Error(9, "gemm: more than 65535 column tiles or batch entries per launch");
     const dim3 grid(gx, gy, (unsigned)batch);
-    if (!ta && !tb)
-        hipLaunchKernelGGL((gemm_kernel<MODE, false, false>), grid, dim3(GTHREADS), 0, s, M, N, K,
-                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else if (!ta && tb)
-        hipLaunchKernelGGL((gemm_kernel<MODE, false, true>), grid, dim3(GTHREADS), 0, s, M, N, K,
-                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else if (ta && !tb)
-        hipLaunchKernelGGL((gemm_kernel<MODE, true, false>), grid, dim3(GTHREADS), 0, s, M, N, K,
-                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else
-        hipLaunchKernelGGL((gemm_kernel<MODE, true, true>), grid, dim3(GTHREADS), 0, s, M, N, K,
-                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+#define STDA_GEMM_LAUNCH(BTV)                                                                   \
+    if (!ta && !tb)                                                                             \
+        hipLaunchKernelGGL((gemm_kernel<MODE, false, false, BTV>), grid, dim3(GTHREADS), 0, s, M, \
+                           N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);              \
+    else if (!ta && tb)                                                                         \
+        hipLaunchKernelGGL((gemm_kernel<MODE, false, true, BTV>), grid, dim3(GTHREADS), 0, s, M,  \
+                           N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);              \
+    else if (ta && !tb)                                                                         \
+        hipLaunchKernelGGL((gemm_kernel<MODE, true, false, BTV>), grid, dim3(GTHREADS), 0, s, M,  \
+                           N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);              \
+    else                                                                                        \
+        hipLaunchKernelGGL((gemm_kernel<MODE, true, true, BTV>), grid, dim3(GTHREADS), 0, s, M,   \
+                           N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    if (bt == 128) {
+        STDA_GEMM_LAUNCH(128)
+    } else {
+        STDA_GEMM_LAUNCH(64)
+    }
+#undef STDA_GEMM_LAUNCH
     STDA_HIP_CHECK(hipGetLastError());
 }
 
