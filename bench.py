@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0
 # of the tall-skinny V^T W products and the V C products) and composite spans
 # (the AO operator products, several GEMMs and element-wise launches; PT2
 # pass 2 with its finishing kernel)
-SINGLE_KERNELS = ("pt2_rows", "batch_matvec", "tsgemm_tn", "tsgemm_nn")
-HBM_KERNELS = ("batch_matvec", "tsgemm_tn", "tsgemm_nn")
+SINGLE_KERNELS = ("pt2_rows", "batch_lockstep", "tsgemm_tn", "tsgemm_nn")
+HBM_KERNELS = ("batch_lockstep", "tsgemm_tn", "tsgemm_nn")
 COMPOSITE_SPANS = ("pt2_cols", "ao_matvec", "ao_matvec_f32")
 EV = 27.211385050
 METRIC = "excited-state solves/s (batched mixed vs plain FP64) at <=1 meV; p50/p99 latency"

@@ -398,8 +398,9 @@ int stda_hip_batch_timings(const stda_hip_batch *b, double *ms5, int64_t *counts
  * [7] setup pipeline, [8] explicit matrices, [9] single-path solves. */
 int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us10);
 /* stda_hip_kernel_stats summed over the workers and the molecules of the last
- * call (names as there, plus "batch_matvec": the batched operator products,
- * work = algorithmic bytes). */
+ * call (names as there, plus "batch_lockstep": the launches of the lock-step
+ * graph, each `poll` iterations of the operator products and Davidson steps of
+ * every active molecule; work = algorithmic bytes of the operator products). */
 int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms, double *launches,
                                 double *work);
 /* Algorithmic HBM bytes of the last batched Davidson launch: per molecule,

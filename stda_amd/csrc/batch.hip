@@ -15,6 +15,8 @@
 // fixed order, so a molecule's result does not depend on the rest of the batch.
 #include "common.h"
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 namespace stda {
@@ -25,9 +27,6 @@ constexpr int BT = 512;            // threads per molecule (8 waves)
 constexpr int NWV = BT / 64;
 constexpr int MD = BATCH_MD;
 constexpr int KM = BATCH_KMAX;
-constexpr int JB = 128;            // vector rows staged per LDS pass of a matvec
-constexpr int CW32 = 32;           // columns per FP32 matvec pass
-constexpr int CW64 = KM;           // columns per FP64 matvec pass
 constexpr int DI = 4, DC = 8;      // tile of the dot-product routine
 constexpr int LDU = MD + 1;        // leading dimension of the LDS matrices (odd: no bank conflicts)
 static_assert((MD / 2) * (MD / 2) % BT == 0 && MD * (MD / 2) % BT == 0, "Jacobi items tile the step");
@@ -583,64 +582,90 @@ __global__ __launch_bounds__(BT) void batch_step_kernel(const BatchMol *__restri
 }
 
 // ---------------------------------------------------------------------------------------
-// Operator products of all active molecules: work-group (molecule, 256-row
-// block).  B_ITER: AV[:, m:m+mnew] = A32 V[:, m:m+mnew] (FP32 products and
-// accumulation); B_REFINE: AX = A64 X; B_VEL: W = B64 V[:, 0:kk] (FP64).
-// A is read down its columns (coalesced over the rows of the block); the
-// vector rows are staged through LDS in the product's precision.
+// Operator products of all active molecules on the matrix cores: work-group
+// (molecule, 64-row block).  B_ITER: AV[:, m:m+mnew] = A32 V[:, m:m+mnew]
+// (FP32 operands and accumulation, v_mfma_f32_16x16x4f32); B_REFINE: AX = A64 X;
+// B_VEL: W = B64 V[:, 0:kk] (FP64, v_mfma_f64_16x16x4f64).  The product is
+// out(64 rows, nc) = A(rows, 0:n) V(0:n, 0:nc): wave w owns rows 16 w .. +16
+// and 32 columns (two 16 x 16 blocks) per pass over the columns; K = n
+// advances in chunks of 16, A (column-major, coalesced over the rows) and the
+// vector rows (widened or rounded to the product's type) staged through LDS.
+// MFMA 16x16x4 layouts (lane l, g = l / 16, c = l % 16): A[c][g], B[g][c];
+// the FP64 result register i holds D[4i + g][c], the FP32 one D[4g + i][c].
 constexpr int MV_BT = 256;
+constexpr int MV_ROWS = BATCH_MV_ROWS, MV_COLS = 32, MV_KC = 16;
+typedef double mv_v4d __attribute__((ext_vector_type(4)));
+typedef float mv_v4f __attribute__((ext_vector_type(4)));
 
-template <typename T, int CW>
-__device__ void matvec_rows(double *stage_mem, int64_t n, int64_t r0, const T *A, const double *Vin,
-                            int nc, double *out) {
-    T *stage = reinterpret_cast<T *>(stage_mem);
-    const int64_t r = r0 + threadIdx.x;
-    for (int c0 = 0; c0 < nc; c0 += CW) {
-        const int cw = min(CW, nc - c0);
-        T acc[CW];
-#pragma unroll
-        for (int cc = 0; cc < CW; ++cc) acc[cc] = (T)0;
-        for (int64_t j0 = 0; j0 < n; j0 += JB) {
-            const int jb = (int)min((int64_t)JB, n - j0);
+template <typename T>
+struct MvSmem {
+    T a[MV_KC][MV_ROWS + 1];
+    T v[MV_KC][MV_COLS + 1];
+};
+
+template <typename T>
+__device__ void matvec_rows_mfma(MvSmem<T> &sm, int64_t n, int64_t r0, const T *A,
+                                 const double *Vin, int nc, double *out) {
+    using Acc = typename std::conditional<sizeof(T) == 8, mv_v4d, mv_v4f>::type;
+    const int t = threadIdx.x, l = t & 63, g = l >> 4, c = l & 15, w = t >> 6;
+    for (int c0 = 0; c0 < nc; c0 += MV_COLS) {
+        Acc acc[2] = {Acc{0, 0, 0, 0}, Acc{0, 0, 0, 0}};
+        for (int64_t k0 = 0; k0 < n; k0 += MV_KC) {
             __syncthreads();
-            for (int e = threadIdx.x; e < JB * CW; e += MV_BT) {
-                const int jj = e % JB, cc = e / JB;
-                stage[jj * CW + cc] = (jj < jb && cc < cw) ? (T)Vin[(j0 + jj) + n * (c0 + cc)] : (T)0;
+            for (int e = t; e < MV_ROWS * MV_KC; e += MV_BT) {
+                const int rr = e % MV_ROWS, kk = e / MV_ROWS;
+                const int64_t gr = r0 + rr, gk = k0 + kk;
+                sm.a[kk][rr] = (gr < n && gk < n) ? A[gr + n * gk] : T(0);
+            }
+            for (int e = t; e < MV_KC * MV_COLS; e += MV_BT) {
+                const int kk = e % MV_KC, cc = e / MV_KC;
+                const int64_t gk = k0 + kk;
+                sm.v[kk][cc] = (gk < n && c0 + cc < nc) ? (T)Vin[gk + n * (c0 + cc)] : T(0);
             }
             __syncthreads();
-            if (r < n) {
-                const T *Ar = A + r + n * j0;
-                for (int jj = 0; jj < jb; ++jj) {
-                    const T a = Ar[n * jj];
 #pragma unroll
-                    for (int cc = 0; cc < CW; ++cc) acc[cc] = fma(a, stage[jj * CW + cc], acc[cc]);
+            for (int kk = 0; kk < MV_KC; kk += 4) {
+                const T a = sm.a[kk + g][16 * w + c];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const T b = sm.v[kk + g][16 * j + c];
+                    if constexpr (sizeof(T) == 8)
+                        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+                    else
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
                 }
             }
         }
-        if (r < n) {
 #pragma unroll
-            for (int cc = 0; cc < CW; ++cc)
-                if (cc < cw) out[r + n * (c0 + cc)] = (double)acc[cc];
-        }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = sizeof(T) == 8 ? 4 * e + g : 4 * g + e;
+                const int64_t gr = r0 + 16 * w + row;
+                const int col = c0 + 16 * j + c;
+                if (gr < n && col < nc) out[gr + n * col] = (double)acc[j][e];
+            }
     }
 }
 
 __global__ __launch_bounds__(MV_BT) void batch_matvec_kernel(const BatchMol *__restrict__ mols,
                                                              const int2 *__restrict__ tiles) {
-    __shared__ double stage[JB * CW64];
-    static_assert(JB * CW32 * sizeof(float) <= sizeof(stage), "FP32 stage fits");
+    __shared__ union {
+        MvSmem<float> f;
+        MvSmem<double> d;
+    } sm;
     const int2 t = tiles[blockIdx.x];
     const BatchMol &M = mols[t.x];
     const int *st = M.state;
     const int phase = st[0];
-    const int64_t n = M.n, r0 = (int64_t)t.y * MV_BT;
+    const int64_t n = M.n, r0 = (int64_t)t.y * MV_ROWS;
     if (phase == B_ITER) {
         const int m = st[1], mnew = st[2];
-        matvec_rows<float, CW32>(stage, n, r0, M.A32, M.V + n * m, mnew, M.AV + n * m);
+        matvec_rows_mfma<float>(sm.f, n, r0, M.A32, M.V + n * m, mnew, M.AV + n * m);
     } else if (phase == B_REFINE) {
-        matvec_rows<double, CW64>(stage, n, r0, M.A64, M.X, st[4], M.AX);
+        matvec_rows_mfma<double>(sm.d, n, r0, M.A64, M.X, st[4], M.AX);
     } else if (phase == B_VEL) {
-        matvec_rows<double, CW64>(stage, n, r0, M.B64, M.V, st[4], M.W);
+        matvec_rows_mfma<double>(sm.d, n, r0, M.B64, M.V, st[4], M.W);
     }
 }
 

@@ -263,7 +263,8 @@ struct BatchMol {
 };
 // one step of every molecule (one work-group each)
 void launch_batch_step(hipStream_t s, int nmol, const BatchMol *mols);
-// operator products of every active molecule; tiles = (molecule, 256-row block)
+// operator products of every active molecule; tiles = (molecule, BATCH_MV_ROWS-row block)
+constexpr int BATCH_MV_ROWS = 64;
 void launch_batch_matvec(hipStream_t s, int ntiles, const BatchMol *mols, const int2 *tiles);
 // Explicit matrix: A(row_csf, col_csf) = ak (ia|jb) - (ij|ab) for every CSF pair
 // (diagonal written later); csf_of_pair maps pair -> CSF index or -1.

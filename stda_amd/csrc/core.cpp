@@ -3191,8 +3191,8 @@ struct stda_hip_batch {
     double timings[5] = {0, 0, 0, 0, 0};
     int64_t counts[3] = {0, 0, 0};       // molecules: batched, single path, failed
     double kernel_bytes = 0.0;           // algorithmic bytes of the batched operator products
-    double matvec_ms = 0.0;              // HIP-event time of the batched operator products
-    double matvec_launches = 0.0;        // and their number of launches
+    double lockstep_ms = 0.0;            // HIP-event time of the lock-step graph launches
+    double lockstep_launches = 0.0;      // and their number (each `group` iterations)
     // summed step phases per molecule (us), 0, Jacobi sweeps, then the
     // workers' summed host time (us) in setup, explicit matrices, single path
     double phase_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -3374,9 +3374,9 @@ int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms,
                                 double *launches, double *work) {
     if (!b || !name) return STDA_HIP_ERR_ARG;
     double t[3] = {0, 0, 0};
-    if (std::string(name) == "batch_matvec") {
-        t[0] = b->matvec_ms;
-        t[1] = b->matvec_launches;
+    if (std::string(name) == "batch_lockstep") {
+        t[0] = b->lockstep_ms;
+        t[1] = b->lockstep_launches;
         t[2] = b->kernel_bytes;
     }
     for (auto *c : b->workers) {
@@ -3423,8 +3423,8 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         for (auto &x : b->timings) x = 0.0;
         for (auto &x : b->phase_us) x = 0.0;
         b->kernel_bytes = 0.0;
-        b->matvec_ms = 0.0;
-        b->matvec_launches = 0.0;
+        b->lockstep_ms = 0.0;
+        b->lockstep_launches = 0.0;
         for (auto *c : b->workers) {
             c->flush_timers();
             c->stats.clear();
@@ -3519,9 +3519,19 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
             DBuf<int2> tilesd;
             int ntiles = 0;
             bool dirty = false;
-            std::vector<std::array<hipEvent_t, 3>> evs;
+            std::vector<std::array<hipEvent_t, 2>> evs;
             int iter = 0;
             bool feeding = true;
+            // The `group` lock-step iterations between two reads of the state
+            // table are one captured graph (two kernels per iteration), rebuilt
+            // when the active set changes.
+            hipGraphExec_t gexec = nullptr;
+            struct GraphGuard {
+                hipGraphExec_t &g;
+                ~GraphGuard() {
+                    if (g) (void)hipGraphExecDestroy(g);
+                }
+            } graph_guard{gexec};
             while (true) {
                 std::vector<std::pair<int, hipEvent_t>> got;
                 {
@@ -3558,24 +3568,38 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                         const int i = active[q];
                         desc.push_back(batch_desc(items[i], tol32, outd.get() + (size_t)i * BATCH_OUT,
                                                   stated.get() + (size_t)i * BATCH_STATE));
-                        const int nblk = (int)((items[i].n + 255) / 256);
+                        const int nblk = (int)((items[i].n + BATCH_MV_ROWS - 1) / BATCH_MV_ROWS);
                         for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
                     }
                     descd.upload(desc.data(), desc.size(), ds);
                     tilesd.upload(tiles.data(), tiles.size(), ds);
                     ntiles = (int)tiles.size();
                     dirty = false;
+                    if (gexec) {
+                        STDA_HIP_CHECK(hipGraphExecDestroy(gexec));
+                        gexec = nullptr;
+                    }
                 }
-                std::array<hipEvent_t, 3> e{};
+                if (!gexec) {
+                    hipGraph_t graph = nullptr;
+                    STDA_HIP_CHECK(hipStreamBeginCapture(ds, hipStreamCaptureModeThreadLocal));
+                    for (int j = 0; j < b->group; ++j) {
+                        launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
+                        launch_batch_step(ds, (int)active.size(), descd.get());
+                    }
+                    STDA_HIP_CHECK(hipStreamEndCapture(ds, &graph));
+                    const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    STDA_HIP_CHECK(ie);
+                }
+                std::array<hipEvent_t, 2> e{};
                 for (auto &x : e) STDA_HIP_CHECK(hipEventCreate(&x));
                 STDA_HIP_CHECK(hipEventRecord(e[0], ds));
-                launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
+                STDA_HIP_CHECK(hipGraphLaunch(gexec, ds));
                 STDA_HIP_CHECK(hipEventRecord(e[1], ds));
-                launch_batch_step(ds, (int)active.size(), descd.get());
-                STDA_HIP_CHECK(hipEventRecord(e[2], ds));
                 evs.push_back(e);
-                ++iter;
-                if (iter % b->group == 0) {
+                iter += b->group;
+                {
                     // poll: drop the finished molecules from the active set
                     STDA_HIP_CHECK(hipMemcpyAsync(stateh.data(), stated.get(),
                                                   stateh.size() * sizeof(int),
@@ -3593,12 +3617,11 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
             }
             STDA_HIP_CHECK(hipStreamSynchronize(ds));
             for (auto &e : evs) {
-                float m0 = 0.f, m1 = 0.f;
+                float m0 = 0.f;
                 STDA_HIP_CHECK(hipEventElapsedTime(&m0, e[0], e[1]));
-                STDA_HIP_CHECK(hipEventElapsedTime(&m1, e[1], e[2]));
-                b->matvec_ms += m0;
-                b->matvec_launches += 1.0;
-                b->timings[4] += m0 + m1;
+                b->lockstep_ms += m0;
+                b->lockstep_launches += 1.0;
+                b->timings[4] += m0;
                 for (auto &x : e) (void)hipEventDestroy(x);
             }
         }
