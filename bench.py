@@ -484,8 +484,8 @@ def run_serve(args):
                     errs[key] = errs.get(key, 0) + 1
             slow = sorted(jobs, key=lambda j: -j.latency)[:3]
             sweep.append({"rate": rate, "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,
-                          "p99_ms": st.p99_ms, "ok": st.ok, "n": st.n, "batches": st.batches,
-                          "mean_batch": st.mean_batch, "errors": errs,
+                          "p99_ms": st.p99_ms, "ok": st.ok, "n": st.n,
+                          "spectra_submitted": st.submitted, "errors": errs,
                           "slowest": [{"kind": j.kind, "natoms": int(j.ci.ncent),
                                        "latency_s": round(j.latency, 2)} for j in slow]})
             log(f"# rate {rate}: {sweep[-1]}")

@@ -403,6 +403,30 @@ int stda_hip_batch_phase_times(const stda_hip_batch *b, double *us10);
  * every active molecule; work = algorithmic bytes of the operator products). */
 int stda_hip_batch_kernel_stats(stda_hip_batch *b, const char *name, double *ms, double *launches,
                                 double *work);
+/* Continuous mode (serving): molecules are submitted one at a time while
+ * earlier ones are being solved.  The object's worker contexts take them in
+ * submission order (setup pipeline; explicit matrices, or the whole
+ * single-molecule mixed solve for larger ones) while a driver thread keeps the
+ * lock-step iterations running: a prepared molecule joins the running set at
+ * the driver's next pass and leaves it with its result at the first poll that
+ * finds it done.  Results equal those of stda_hip_batch_solve_tda for the same
+ * molecule (each molecule's kernels touch only its data and reduce in a fixed
+ * order).  The arrays that sys points to must stay valid until the molecule's
+ * result has been taken.  Up to 2048 molecules are in the lock-step set at
+ * once (more wait for a free row).  While the continuous mode runs,
+ * stda_hip_batch_solve_tda returns STDA_HIP_ERR_STATE.
+ *   submit: queue one molecule; ticket receives its handle.
+ *   result: done = 1 and the outputs (nfound, omega[nroots], props[5 nroots],
+ *           status, as in stda_hip_batch_solve_tda) once the molecule is
+ *           finished, then the ticket is released; done = 0 otherwise.  wait
+ *           != 0 blocks until it is finished.
+ *   stop:   finish every submitted molecule and end the mode's threads. */
+int stda_hip_batch_submit(stda_hip_batch *b, const stda_hip_system *sys,
+                          const stda_hip_params *par, int32_t nroots, double tol,
+                          int32_t velcorr, int64_t *ticket);
+int stda_hip_batch_result(stda_hip_batch *b, int64_t ticket, int32_t wait, int32_t *done,
+                          int32_t *nfound, double *omega, double *props, int32_t *status);
+int stda_hip_batch_stop(stda_hip_batch *b);
 /* Algorithmic HBM bytes of the last batched Davidson launch: per molecule,
  * the FP32 A (4 n^2) once per iteration plus the FP64 A and 0.5 B (8 n^2
  * each) once. */

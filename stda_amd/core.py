@@ -40,6 +40,7 @@ C_API = [
     "stda_hip_beta_shg", "stda_hip_tpa", "stda_hip_set_selection", "stda_hip_props_xyz",
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
     "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc", "stda_hip_gemm",
+    "stda_hip_batch_submit", "stda_hip_batch_result", "stda_hip_batch_stop",
 ]
 
 
@@ -119,6 +120,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                            ctypes.POINTER(_Params), I32, F64, I32, P, P, P, P]
     L.stda_hip_batch_timings.argtypes = [P, P, P]
     L.stda_hip_batch_kernel_bytes.argtypes = [P, ctypes.POINTER(F64)]
+    L.stda_hip_batch_submit.argtypes = [P, ctypes.POINTER(_System), ctypes.POINTER(_Params), I32,
+                                        F64, I32, ctypes.POINTER(I64)]
+    L.stda_hip_batch_result.argtypes = [P, I64, I32, ctypes.POINTER(I32), ctypes.POINTER(I32), P,
+                                        P, ctypes.POINTER(I32)]
+    L.stda_hip_batch_stop.argtypes = [P]
     L.stda_hip_batch_configure.argtypes = [P, I64, I32]
     L.stda_hip_batch_phase_times.argtypes = [P, P]
     L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
@@ -513,6 +519,7 @@ class StdaHipBatch:
     def __init__(self, device: int = 0, workers: int = 8):
         self._L = load_library()
         self._b = ctypes.c_void_p()
+        self._pending = {}
         st = self._L.stda_hip_batch_create(ctypes.byref(self._b), device, workers)
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
@@ -576,6 +583,54 @@ class StdaHipBatch:
     def last_error(self) -> str:
         return self._L.stda_hip_batch_last_error(self._b).decode()
 
+    # ---------------------------------------------------------------- continuous mode
+    def submit(self, ci: CoreInputs, thr: float, ax: float, nroots: int, tol: float = 1e-8,
+               velcorr: bool = True, thrp: float = 1e-4, fthr: float | None = None,
+               triplet: bool = False) -> int:
+        """Queue one molecule for the continuous mode (stda_hip_batch_submit); it joins
+        the running lock-step set as soon as its setup is done.  Returns its ticket."""
+        clow = np.asfortranarray(ci.clow, dtype=np.float64)
+        eps = np.ascontiguousarray(ci.epsi, dtype=np.float64)
+        iaoat = np.ascontiguousarray(ci.iaoat, dtype=np.int32)
+        xyz = np.asfortranarray(ci.xyz, dtype=np.float64)
+        mom = (np.ascontiguousarray(ci.moments, dtype=np.float64)
+               if ci.moments is not None else None)
+        sysd = _System(ci.ncent, ci.nao, ci.no, ci.nv, _ptr(clow), _ptr(eps), _ptr(iaoat),
+                       _ptr(xyz), _ptr(mom))
+        par = _Params(thr, thrp, -1.0 if fthr is None else fthr, ax, -100.0, -100.0,
+                      1 if triplet else 0, 0, 0.5, 0.1, 4.0)
+        t = ctypes.c_int64()
+        st = self._L.stda_hip_batch_submit(self._b, ctypes.byref(sysd), ctypes.byref(par), nroots,
+                                           tol, 1 if velcorr else 0, ctypes.byref(t))
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+        # the library reads these arrays until the molecule is finished
+        self._pending[t.value] = (nroots, (clow, eps, iaoat, xyz, mom))
+        return t.value
+
+    def result(self, ticket: int, wait: bool = True):
+        """``(omega, props, status)`` of a submitted molecule as :meth:`solve_tda` gives
+        them, or None when it is not finished and ``wait`` is false."""
+        nroots = self._pending[ticket][0]
+        done = ctypes.c_int32()
+        nf = ctypes.c_int32()
+        om = np.zeros(nroots)
+        pr = np.zeros((nroots, 5))
+        stt = ctypes.c_int32()
+        st = self._L.stda_hip_batch_result(self._b, ticket, 1 if wait else 0, ctypes.byref(done),
+                                           ctypes.byref(nf), _ptr(om), _ptr(pr),
+                                           ctypes.byref(stt))
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+        if not done.value:
+            return None
+        del self._pending[ticket]
+        return om[:nf.value].copy(), pr[:nf.value, :4].copy(), int(stt.value)
+
+    def stop(self):
+        """Finish every submitted molecule and leave the continuous mode."""
+        self._L.stda_hip_batch_stop(self._b)
+
     def configure(self, nmax_batched: int = 1024, poll: int = 4):
         """CSF limit of the lock-step solver and iterations between polls."""
         st = self._L.stda_hip_batch_configure(self._b, nmax_batched, poll)
@@ -590,21 +645,6 @@ class StdaHipBatch:
         names = ["gram", "jacobi", "ritz", "expand", "final", "unused", "sweeps",
                  "worker_setup", "worker_matrices", "worker_single"]
         return dict(zip(names, t.tolist()))
-
-    def gemm(self, A, B, C=None, alpha=1.0, beta=0.0, ta=False, tb=False, mode="fp64"):
-        """C = alpha op(A) op(B) + beta C with the library's matrix-core GEMM
-        (stda_hip_gemm); mode 'fp64', 'fp32' or 'bf16' (split-bfloat16)."""
-        A = np.asfortranarray(A, dtype=np.float64)
-        B = np.asfortranarray(B, dtype=np.float64)
-        m = A.shape[1] if ta else A.shape[0]
-        k = A.shape[0] if ta else A.shape[1]
-        n = B.shape[0] if tb else B.shape[1]
-        C = np.zeros((m, n), order="F") if C is None else np.array(C, dtype=np.float64, order="F")
-        code = {"fp64": 0, "fp32": 1, "bf16": 2}[mode]
-        self._check(self._L.stda_hip_gemm(self._h, code, int(ta), int(tb), m, n, k, alpha, _ptr(A),
-                                          max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), beta,
-                                          _ptr(C), max(1, m)))
-        return C
 
     def kernel_stats(self, name: str):
         """Per-kernel statistics summed over the workers and molecules of the

@@ -8,6 +8,9 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <cmath>
 #include <cstdio>
@@ -3197,6 +3200,7 @@ struct stda_hip_batch {
     // workers' summed host time (us) in setup, explicit matrices, single path
     double phase_us[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double wall_tick_hz = 1.0e8;
+    struct BatchStream *engine = nullptr;   // continuous mode (stda_hip_batch_submit), or null
 };
 
 namespace {
@@ -3343,6 +3347,7 @@ int stda_hip_batch_create(stda_hip_batch **out, int device, int nworkers) {
 
 int stda_hip_batch_destroy(stda_hip_batch *b) {
     if (!b) return STDA_HIP_OK;
+    (void)stda_hip_batch_stop(b);
     if (b->dstream) {
         block_cache_drain(b->dstream);
         (void)hipStreamSynchronize(b->dstream);
@@ -3417,6 +3422,10 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         return STDA_HIP_ERR_ARG;
     if (tol <= 0.0) tol = 1.0e-8;
     const double tol32 = std::max(tol, 1.0e-6);
+    if (b->engine) {
+        b->last_error = "continuous mode is running (stda_hip_batch_stop first)";
+        return STDA_HIP_ERR_STATE;
+    }
     try {
         Timer t_all;
         for (auto &x : b->counts) x = 0;
@@ -3441,8 +3450,11 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         std::fill(props, props + (size_t)nmol * nroots * 5, 0.0);
         std::vector<int> order(nmol);
         std::iota(order.begin(), order.end(), 0);
+        // largest first (by pair count): the long single-path solves start
+        // early and the short ones fill in behind them, instead of the largest
+        // molecules forming the tail of the call
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-            return (int64_t)sys[a].no * sys[a].nv < (int64_t)sys[c].no * sys[c].nv;
+            return (int64_t)sys[a].no * sys[a].nv > (int64_t)sys[c].no * sys[c].nv;
         });
         // state table and outputs of every molecule, indexed by molecule
         DBuf<int> stated;
@@ -3691,6 +3703,312 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         b->last_error = e.what();
         return STDA_HIP_ERR_DEVICE;
     }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Continuous mode of a batch object.  Submitted molecules are queued; the
+// worker contexts take them one at a time (setup pipeline, explicit matrices,
+// or the whole single-path solve) while a driver thread keeps the lock-step
+// iterations of every prepared molecule running: a molecule joins the active
+// set at the driver's next pass after its setup and leaves it, with its
+// result, at the first poll that finds it done.  Each molecule owns one of
+// kSlots rows of the device state and output tables.
+namespace {
+
+struct BatchTicket {
+    stda_hip_system sys{};
+    stda_hip_params par{};
+    int nroots = 0, velcorr = 1;
+    double tol = 1.0e-8;
+    std::vector<double> omega, props;
+    int32_t nfound = 0, status = STDA_HIP_OK;
+    std::string err;
+    bool done = false;
+    BatchItem item;
+    int slot = -1;
+};
+
+}  // namespace
+
+struct BatchStream {
+    static constexpr int kSlots = 2048;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int64_t> queue;                          // submitted, not yet taken by a worker
+    std::vector<std::pair<int64_t, hipEvent_t>> ready;  // prepared for the lock-step driver
+    std::map<int64_t, std::unique_ptr<BatchTicket>> tickets;
+    int64_t next = 1;
+    int busy = 0;                                       // workers inside a preparation
+    bool stop = false;
+    std::vector<std::thread> threads;
+};
+
+namespace {
+
+void stream_worker(stda_hip_batch *b, BatchStream *E, int wi) {
+    stda_hip_ctx *c = b->workers[wi];
+    AllocScope scope(c->stream);
+    (void)hipSetDevice(c->device);
+    while (true) {
+        int64_t id = 0;
+        BatchTicket *t = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(E->mu);
+            E->cv.wait(lk, [&] { return E->stop || !E->queue.empty(); });
+            if (E->queue.empty()) return;
+            id = E->queue.front();
+            E->queue.pop_front();
+            t = E->tickets[id].get();
+            ++E->busy;
+        }
+        bool batched = false;
+        hipEvent_t ev = nullptr;
+        try {
+            double us3[3] = {0, 0, 0};
+            batched = batch_prepare(c, &t->sys, &t->par, t->nroots, t->tol, t->velcorr,
+                                    b->nmax_batched, t->item, &t->nfound, t->omega.data(),
+                                    t->props.data(), us3);
+            t->status = STDA_HIP_OK;
+            if (batched) {
+                STDA_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                STDA_HIP_CHECK(hipEventRecord(ev, c->stream));
+            }
+        } catch (const std::exception &e) {
+            const Error *er = dynamic_cast<const Error *>(&e);
+            t->status = er ? er->status : STDA_HIP_ERR_DEVICE;
+            t->err = e.what();
+            t->item = BatchItem();
+            batched = false;
+        }
+        std::lock_guard<std::mutex> lk(E->mu);
+        --E->busy;
+        if (batched) E->ready.emplace_back(id, ev);
+        else t->done = true;
+        E->cv.notify_all();
+    }
+}
+
+void stream_driver(stda_hip_batch *b, BatchStream *E) {
+    (void)hipSetDevice(b->device);
+    hipStream_t ds = b->dstream;
+    AllocScope scope(ds);
+    constexpr int S = BatchStream::kSlots;
+    std::string fatal;
+    try {
+        DBuf<int> stated((size_t)S * BATCH_STATE);
+        DBuf<double> outd((size_t)S * BATCH_OUT);
+        std::vector<int> stateh((size_t)S * BATCH_STATE, B_DONE);
+        stated.upload(stateh.data(), stateh.size(), ds);   // free rows read as done
+        std::vector<double> outh(BATCH_OUT);
+        std::vector<int> free_slots;
+        for (int q = S - 1; q >= 0; --q) free_slots.push_back(q);
+        std::vector<BatchTicket *> active;
+        DBuf<BatchMol> descd;
+        DBuf<int2> tilesd;
+        int ntiles = 0;
+        bool dirty = false;
+        hipGraphExec_t gexec = nullptr;
+        struct GraphGuard {
+            hipGraphExec_t &g;
+            ~GraphGuard() {
+                if (g) (void)hipGraphExecDestroy(g);
+            }
+        } guard{gexec};
+        while (true) {
+            std::vector<std::pair<int64_t, hipEvent_t>> got;
+            std::vector<BatchTicket *> gott;
+            {
+                std::unique_lock<std::mutex> lk(E->mu);
+                if (active.empty())
+                    E->cv.wait_for(lk, std::chrono::milliseconds(2),
+                                   [&] { return E->stop || !E->ready.empty(); });
+                if (active.empty() && E->ready.empty() && E->stop && E->queue.empty() &&
+                    E->busy == 0)
+                    break;
+                while (!E->ready.empty() && free_slots.size() > got.size()) {
+                    got.push_back(E->ready.front());
+                    gott.push_back(E->tickets[E->ready.front().first].get());
+                    E->ready.erase(E->ready.begin());
+                }
+            }
+            for (size_t q = 0; q < got.size(); ++q) {
+                STDA_HIP_CHECK(hipStreamWaitEvent(ds, got[q].second, 0));
+                (void)hipEventDestroy(got[q].second);
+                BatchTicket *t = gott[q];
+                t->slot = free_slots.back();
+                free_slots.pop_back();
+                stateh[(size_t)t->slot * BATCH_STATE] = B_INIT;
+                STDA_HIP_CHECK(hipMemcpyAsync(stated.get() + (size_t)t->slot * BATCH_STATE,
+                                              &stateh[(size_t)t->slot * BATCH_STATE], sizeof(int),
+                                              hipMemcpyHostToDevice, ds));
+                active.push_back(t);
+                dirty = true;
+            }
+            if (active.empty()) continue;
+            if (dirty) {
+                std::stable_sort(active.begin(), active.end(),
+                                 [](const BatchTicket *a, const BatchTicket *c) {
+                                     return a->item.n > c->item.n;
+                                 });
+                std::vector<BatchMol> desc;
+                std::vector<int2> tiles;
+                for (int q = 0; q < (int)active.size(); ++q) {
+                    BatchTicket *t = active[q];
+                    desc.push_back(batch_desc(t->item, std::max(t->tol, 1.0e-6),
+                                              outd.get() + (size_t)t->slot * BATCH_OUT,
+                                              stated.get() + (size_t)t->slot * BATCH_STATE));
+                    const int nblk = (int)((t->item.n + BATCH_MV_ROWS - 1) / BATCH_MV_ROWS);
+                    for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
+                }
+                descd.upload(desc.data(), desc.size(), ds);
+                tilesd.upload(tiles.data(), tiles.size(), ds);
+                ntiles = (int)tiles.size();
+                dirty = false;
+                if (gexec) {
+                    STDA_HIP_CHECK(hipGraphExecDestroy(gexec));
+                    gexec = nullptr;
+                }
+            }
+            if (!gexec) {
+                hipGraph_t graph = nullptr;
+                STDA_HIP_CHECK(hipStreamBeginCapture(ds, hipStreamCaptureModeThreadLocal));
+                for (int j = 0; j < b->group; ++j) {
+                    launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
+                    launch_batch_step(ds, (int)active.size(), descd.get());
+                }
+                STDA_HIP_CHECK(hipStreamEndCapture(ds, &graph));
+                const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                STDA_HIP_CHECK(ie);
+            }
+            STDA_HIP_CHECK(hipGraphLaunch(gexec, ds));
+            // poll the state table (one copy)
+            STDA_HIP_CHECK(hipMemcpyAsync(stateh.data(), stated.get(), stateh.size() * sizeof(int),
+                                          hipMemcpyDeviceToHost, ds));
+            STDA_HIP_CHECK(hipStreamSynchronize(ds));
+            std::vector<BatchTicket *> still;
+            for (BatchTicket *t : active) {
+                if (stateh[(size_t)t->slot * BATCH_STATE] != B_DONE) {
+                    still.push_back(t);
+                    continue;
+                }
+                STDA_HIP_CHECK(hipMemcpyAsync(outh.data(), outd.get() + (size_t)t->slot * BATCH_OUT,
+                                              (size_t)BATCH_OUT * 8, hipMemcpyDeviceToHost, ds));
+                STDA_HIP_CHECK(hipStreamSynchronize(ds));
+                const double *o = outh.data();
+                const double *info = o + 20 * BATCH_KMAX;
+                const int kk = (int)info[2];
+                std::vector<double> w(o, o + kk), pr((size_t)5 * kk);
+                tda_props_from_moments(t->item.ak, w, o + BATCH_KMAX,
+                                       t->item.velcorr ? o + 10 * BATCH_KMAX : nullptr, pr.data(),
+                                       nullptr);
+                t->nfound = kk;
+                for (int j = 0; j < kk; ++j) {
+                    t->omega[j] = w[j];
+                    for (int r = 0; r < 5; ++r) t->props[(size_t)5 * j + r] = pr[(size_t)5 * j + r];
+                }
+                if (info[1] == 0.0) {
+                    t->status = STDA_HIP_ERR_NOT_CONVERGED;
+                    t->err = "batched Davidson did not converge";
+                }
+                t->item = BatchItem();   // the device work on it has completed
+                free_slots.push_back(t->slot);
+                std::lock_guard<std::mutex> lk(E->mu);
+                t->done = true;
+                E->cv.notify_all();
+            }
+            if (still.size() != active.size()) {
+                active.swap(still);
+                dirty = true;
+            }
+        }
+    } catch (const std::exception &e) {
+        fatal = e.what();
+    }
+    if (!fatal.empty()) {
+        // the device failed: every open ticket ends with the error
+        std::lock_guard<std::mutex> lk(E->mu);
+        for (auto &kv : E->tickets)
+            if (!kv.second->done) {
+                kv.second->status = STDA_HIP_ERR_DEVICE;
+                kv.second->err = fatal;
+                kv.second->done = true;
+            }
+        E->stop = true;
+        E->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int stda_hip_batch_submit(stda_hip_batch *b, const stda_hip_system *sys,
+                          const stda_hip_params *par, int32_t nroots, double tol,
+                          int32_t velcorr, int64_t *ticket) {
+    if (!b || !sys || !par || !ticket || nroots < 1) return STDA_HIP_ERR_ARG;
+    if (!b->engine) {
+        auto *E = new BatchStream();
+        b->engine = E;
+        for (int w = 0; w < (int)b->workers.size(); ++w)
+            E->threads.emplace_back(stream_worker, b, E, w);
+        E->threads.emplace_back(stream_driver, b, E);
+    }
+    BatchStream *E = b->engine;
+    auto t = std::make_unique<BatchTicket>();
+    t->sys = *sys;
+    t->par = *par;
+    t->nroots = nroots;
+    t->tol = tol > 0.0 ? tol : 1.0e-8;
+    t->velcorr = velcorr;
+    t->omega.assign(nroots, 0.0);
+    t->props.assign((size_t)5 * nroots, 0.0);
+    std::lock_guard<std::mutex> lk(E->mu);
+    if (E->stop) return STDA_HIP_ERR_STATE;
+    const int64_t id = E->next++;
+    E->tickets[id] = std::move(t);
+    E->queue.push_back(id);
+    E->cv.notify_all();
+    *ticket = id;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_result(stda_hip_batch *b, int64_t ticket, int32_t wait, int32_t *done,
+                          int32_t *nfound, double *omega, double *props, int32_t *status) {
+    if (!b || !done) return STDA_HIP_ERR_ARG;
+    BatchStream *E = b->engine;
+    if (!E) return STDA_HIP_ERR_STATE;
+    std::unique_lock<std::mutex> lk(E->mu);
+    auto it = E->tickets.find(ticket);
+    if (it == E->tickets.end()) return STDA_HIP_ERR_ARG;
+    BatchTicket *t = it->second.get();
+    if (wait) E->cv.wait(lk, [&] { return t->done; });
+    *done = t->done ? 1 : 0;
+    if (!t->done) return STDA_HIP_OK;
+    if (nfound) *nfound = t->nfound;
+    if (omega) std::copy(t->omega.begin(), t->omega.end(), omega);
+    if (props) std::copy(t->props.begin(), t->props.end(), props);
+    if (status) *status = t->status;
+    if (t->status != STDA_HIP_OK) b->last_error = t->err;
+    E->tickets.erase(it);
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_stop(stda_hip_batch *b) {
+    if (!b || !b->engine) return STDA_HIP_OK;
+    BatchStream *E = b->engine;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->stop = true;
+        E->cv.notify_all();
+    }
+    for (auto &th : E->threads)
+        if (th.joinable()) th.join();
+    delete E;
+    b->engine = nullptr;
+    return STDA_HIP_OK;
 }
 
 }  // extern "C"

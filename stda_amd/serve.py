@@ -4,20 +4,22 @@ Jobs are sTDA spectrum requests (lowest ``nroots`` roots with transition
 properties) or sTD-DFT polarizability requests (alpha(w) at a list of
 frequencies).  A :class:`Server` holds
 
-* ``batchers`` batch engines (:class:`stda_amd.core.StdaHipBatch`, each with
-  ``batch_workers`` device contexts): spectrum jobs that have arrived are
-  taken from the queue as one micro-batch by whichever engine is free and
-  solved together (setup on the engine's contexts, one launch of the batched
-  Davidson kernel); requests arriving meanwhile form the next micro-batch,
-  which the other engine runs while the first one is still on the device;
-* ``workers`` contexts for polarizability jobs, each job on its own context
-  and host thread.
+* one batch engine (:class:`stda_amd.core.StdaHipBatch`, ``batch_workers``
+  device contexts) in its continuous mode: a spectrum job is submitted the
+  moment it arrives (and fits, below), its setup runs on the next free worker
+  context, and it joins the running lock-step solver at the solver's next pass
+  -- requests are admitted while others are on the GPU, not at batch
+  boundaries -- or, when larger, runs the single-molecule mixed path on that
+  worker; a collector thread returns each result as soon as it is done;
+* ``workers`` contexts for polarizability jobs, each job on its own context and
+  host thread, overlapping with everything else on the device.
 
 Admission is by HBM footprint: a job enters only when its upper-bound device
-footprint fits in what the running jobs leave of ``hbm_budget``.  All device
-state of a job (charges, gamma, CSF space, matrices, Davidson subspaces)
-stays resident from setup to result.  The C calls release the interpreter
-lock, so the engines, the polarizability workers and the device overlap.
+footprint fits in what the running jobs leave of ``hbm_budget``; it waits in
+arrival order otherwise.  All device state of a job (charges, gamma, CSF
+space, matrices, Davidson subspaces) stays resident from setup to result.  The
+C calls release the interpreter lock, so the engine, the polarizability
+workers and the device overlap.
 
 Every kernel of a job touches only that job's data and reduces in a fixed
 order, so a job's result does not depend on what else is running; the GPU
@@ -94,31 +96,27 @@ class RunStats:
     p50_ms: float
     p99_ms: float
     mean_ms: float
-    batches: int = 0
-    mean_batch: float = 0.0
+    submitted: int = 0            # spectrum jobs submitted to the continuous engine
 
 
 class Server:
     """Runs jobs with continuous batching (see the module documentation)."""
 
     def __init__(self, workers: int = 4, hbm_budget: float = 160e9, precision: str = "mixed",
-                 device: int = 0, tol: float = 1e-8, batchers: int = 2, batch_workers: int = 8,
-                 max_batch: int = 512):
+                 device: int = 0, tol: float = 1e-8, batch_workers: int = 8):
         self.ctx = [StdaHip(device) for _ in range(workers)]
         for h in self.ctx:
             h.set_precision(precision)
-        self.engines = [StdaHipBatch(device, batch_workers) for _ in range(batchers)]
+        self.engine = StdaHipBatch(device, batch_workers)
         self.budget = float(hbm_budget)
         self.tol = tol
-        self.max_batch = max_batch
         self._cv = threading.Condition()
         self._used = 0.0
 
     def close(self):
         for h in self.ctx:
             h.close()
-        for e in self.engines:
-            e.close()
+        self.engine.close()
 
     def __enter__(self):
         return self
@@ -140,58 +138,52 @@ class Server:
                 j.arrival = 0.0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        spec_q: deque = deque()
+        spec_wait: deque = deque()       # arrived spectrum jobs that do not fit yet
         polar_q: deque = deque()
         free_ctx = list(self.ctx)
+        outstanding = {}                 # ticket -> (job, footprint)
         done_feeding = threading.Event()
-        nbatches = [0, 0]
         threads = []
+        stats = {"submitted": 0}
 
         def fp_of(j):
             return min(float(footprint_bytes(j)), self.budget)
 
-        def engine_loop(eng: StdaHipBatch):
+        def admit_spectra():             # with self._cv held
+            while spec_wait and self._fits(fp_of(spec_wait[0])):
+                j = spec_wait.popleft()
+                fp = fp_of(j)
+                self._used += fp
+                j.start = now()
+                t = self.engine.submit(j.ci, j.thr, j.ax, j.nroots, tol=self.tol)
+                outstanding[t] = (j, fp)
+                stats["submitted"] += 1
+
+        def collector():
             while True:
                 with self._cv:
-                    while True:
-                        if spec_q and self._fits(fp_of(spec_q[0])):
-                            break
-                        if done_feeding.is_set() and not spec_q:
-                            return
-                        self._cv.wait(0.05)
-                    batch, fps = [], 0.0
-                    while spec_q and len(batch) < self.max_batch:
-                        f = fp_of(spec_q[0])
-                        if batch and not self._fits(f + fps):
-                            break
-                        batch.append(spec_q.popleft())
-                        fps += f
-                    self._used += fps
-                ts = now()
-                for j in batch:
-                    j.start = ts
-                try:
-                    groups = {}
-                    for j in batch:
-                        groups.setdefault((j.thr, j.ax, j.nroots), []).append(j)
-                    for (thr, ax, nroots), grp in groups.items():
-                        res = eng.solve_tda([j.ci for j in grp], thr, ax, nroots, tol=self.tol)
-                        for j, (om, pr, st) in zip(grp, res):
-                            if st == 0:
-                                j.result = (om, pr)
-                            else:
-                                j.error = f"status {st}"
-                except Exception as e:       # reported per job, the service keeps running
-                    for j in batch:
-                        j.error = j.error or str(e)
-                te = now()
-                for j in batch:
-                    j.end = te
+                    tickets = list(outstanding)
+                    if not tickets and not spec_wait and done_feeding.is_set():
+                        return
+                finished = []
+                for t in tickets:
+                    r = self.engine.result(t, wait=False)
+                    if r is not None:
+                        finished.append((t, r))
                 with self._cv:
-                    self._used -= fps
-                    nbatches[0] += 1
-                    nbatches[1] += len(batch)
-                    self._cv.notify_all()
+                    for t, (om, pr, st) in finished:
+                        j, fp = outstanding.pop(t)
+                        j.end = now()
+                        if st == 0:
+                            j.result = (om, pr)
+                        else:
+                            j.error = f"status {st}"
+                        self._used -= fp
+                    if finished:
+                        admit_spectra()
+                        self._cv.notify_all()
+                    else:
+                        self._cv.wait(0.0005)
 
         def polar_job(h: StdaHip, j: Job, fp: float):
             j.start = now()
@@ -204,6 +196,7 @@ class Server:
             with self._cv:
                 self._used -= fp
                 free_ctx.append(h)
+                admit_spectra()
                 self._cv.notify_all()
 
         def polar_loop():
@@ -223,8 +216,8 @@ class Server:
                 t.start()
                 threads.append(t)
 
-        loops = [threading.Thread(target=engine_loop, args=(e,), daemon=True) for e in self.engines]
-        loops.append(threading.Thread(target=polar_loop, daemon=True))
+        loops = [threading.Thread(target=collector, daemon=True),
+                 threading.Thread(target=polar_loop, daemon=True)]
         for t in loops:
             t.start()
         for j in jobs:                      # the arrival process
@@ -233,7 +226,11 @@ class Server:
                 if delay > 0:
                     time.sleep(delay)
             with self._cv:
-                (spec_q if j.kind == "spectrum" else polar_q).append(j)
+                if j.kind == "spectrum":
+                    spec_wait.append(j)
+                    admit_spectra()
+                else:
+                    polar_q.append(j)
                 self._cv.notify_all()
         done_feeding.set()
         with self._cv:
@@ -251,13 +248,13 @@ class Server:
             wait(t)
         for t in list(threads):     # every polarizability job has been started
             wait(t)
+        self.engine.stop()
         wall = now()
         lat = np.array([j.latency for j in jobs]) * 1e3 if jobs else np.zeros(1)
         ok = sum(1 for j in jobs if not j.error)
         return RunStats(n=len(jobs), ok=ok, wall=wall, solves_per_s=ok / wall if wall > 0 else 0.0,
                         p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                        mean_ms=float(lat.mean()), batches=nbatches[0],
-                        mean_batch=nbatches[1] / max(nbatches[0], 1))
+                        mean_ms=float(lat.mean()), submitted=stats["submitted"])
 
 
 def request_mix(n: int, seed: int = 2000, lo: int = 10, hi: int = 400,

@@ -131,3 +131,26 @@ def test_server_alone_equals_in_mix(molecules, plain):
                 for (w1, a1), (w2, a2) in zip(j.result, alone.result):
                     assert w1 == w2
                     np.testing.assert_array_equal(a1, a2)
+
+
+def test_continuous_mode_equals_closed(molecules):
+    """Continuous mode (stda_hip_batch_submit/result): molecules submitted in two waves,
+    the second while the first are in the running lock-step set, give bit for bit the
+    results of the closed batch call; the closed call works again after stop."""
+    import time
+    from stda_amd.core import StdaHipBatch
+    with StdaHipBatch(0, workers=3) as b:
+        closed = b.solve_tda(molecules, 7.0, 0.25, nroots=20)
+        first = [b.submit(ci, 7.0, 0.25, 20) for ci in molecules[:6]]
+        time.sleep(0.002)
+        second = [b.submit(ci, 7.0, 0.25, 20) for ci in molecules[6:]]
+        got = [b.result(t) for t in first + second]
+        b.stop()
+        again = b.solve_tda(molecules[:2], 7.0, 0.25, nroots=20)
+    for i, ((om, pr, st), (om0, pr0, st0)) in enumerate(zip(got, closed)):
+        assert st == st0, (i, st, st0)
+        np.testing.assert_array_equal(om, om0)
+        np.testing.assert_array_equal(pr, pr0)
+    for (om, pr, st), (om0, pr0, st0) in zip(again, closed[:2]):
+        assert st == st0
+        np.testing.assert_array_equal(om, om0)
