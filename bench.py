@@ -71,6 +71,41 @@ def single_workload(name: str):
     return ci, p
 
 
+GEMM_PROBE_SHAPES = [  # (M, N, K, transA, transB): the operator GEMMs of the 400-atom cluster
+    (2340, 12800, 1540, 0, 0),   # T = Cv X^T for 16 vectors (nao x no*16, K = nv)
+    (2340, 2340, 800, 0, 1),     # D = T Co^T per vector
+    (2340, 800, 2340, 0, 0),     # W = G Co per vector
+    (1540, 12800, 2340, 1, 0),   # R = Cv^T W for 16 vectors
+    (640, 6400, 250, 0, 0),      # the same transforms of a 200-atom minimal-basis molecule
+    (640, 640, 390, 0, 1),
+]
+
+
+def gemm_probe():
+    """The operator-shape GEMMs through the library's matrix-core kernel (FP64, FP32,
+    split-bfloat16; three calls each, host arrays) and, for comparison, through torch's
+    vendor GEMMs on device tensors (FP64, FP32).  Run under rocprofv3 to read each
+    kernel's duration; prints the shapes in order."""
+    import torch
+    from stda_amd.core import StdaHip
+    rng = np.random.default_rng(0)
+    with StdaHip(0) as h:
+        for (m, n, k, ta, tb) in GEMM_PROBE_SHAPES:
+            A = rng.standard_normal((k, m) if ta else (m, k))
+            B = rng.standard_normal((n, k) if tb else (k, n))
+            for mode in ("fp64", "fp32", "bf16"):
+                for _ in range(3):
+                    h.gemm(A, B, ta=bool(ta), tb=bool(tb), mode=mode)
+            tA, tB = torch.tensor(A, device="cuda"), torch.tensor(B, device="cuda")
+            oA, oB = (tA.t() if ta else tA), (tB.t() if tb else tB)
+            for dt in (torch.float64, torch.float32):
+                a, b = oA.to(dt), oB.to(dt)
+                for _ in range(3):
+                    a @ b
+                torch.cuda.synchronize()
+            print(f"gemm probe {m}x{n}x{k} ta={ta} tb={tb}", flush=True)
+
+
 def copy_probe(nbytes: int):
     """One device-to-device copy of nbytes (a kernel of known HBM traffic:
     nbytes read + nbytes written), used to scale the TCC byte counters."""
@@ -524,9 +559,14 @@ def main():
                     help="single-molecule workloads only")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters)")
+    ap.add_argument("--gemm-probe", action="store_true",
+                    help="only run the operator-shape GEMMs (library kernel and vendor GEMM)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.gemm_probe:
+        gemm_probe()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU "

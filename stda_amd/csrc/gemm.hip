@@ -27,6 +27,10 @@
 
 #include "common.h"
 
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 namespace stda {
@@ -218,13 +222,88 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
 }
 
 template <int MODE, typename T>
+void launch_tile(hipStream_t s, int bt, bool ta, bool tb, int M, int N, int K, T alpha, const T *A,
+                 int64_t lda, int64_t sA, const T *B, int64_t ldb, int64_t sB, T beta, T *C,
+                 int64_t ldc, int64_t sC, int batch);
+
+// Tile-size autotuner.  The first call of a shape class (mode, transposes, and
+// the binary orders of M, N, K and the batch count) that is large enough to
+// matter times both tile sizes on its own operands, writing into a scratch
+// output, and keeps the faster one for the rest of the process; smaller
+// classes, and a class while another thread tunes it, use the size rule
+// (128-tiles when they still give every compute unit two work-groups).
+using TileKey = std::tuple<int, int, int, int, int, int, int>;
+std::mutex g_tile_mu;
+std::map<TileKey, int> g_tile_choice;   // 0: being tuned
+
+int rule_tile(int M, int N, int batch) {
+    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
+    return big >= 512 ? 128 : 64;
+}
+
+int ilog2(int64_t v) {
+    int r = 0;
+    while (v > 1) { v >>= 1; ++r; }
+    return r;
+}
+
+template <int MODE, typename T>
+int choose_tile(hipStream_t s, bool ta, bool tb, int M, int N, int K, const T *A, int64_t lda,
+                int64_t sA, const T *B, int64_t ldb, int64_t sB, int64_t ldc, int64_t sC, int batch) {
+    const double flops = 2.0 * M * (double)N * K * batch;
+    if (flops < 2.0e9) return rule_tile(M, N, batch);
+    const TileKey key{MODE, ta, tb, ilog2(M), ilog2(N), ilog2(K), ilog2(batch)};
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_tile_choice.find(key);
+        if (it != g_tile_choice.end()) return it->second ? it->second : rule_tile(M, N, batch);
+        g_tile_choice[key] = 0;
+    }
+    int best = rule_tile(M, N, batch);
+    void *scratch = nullptr;
+    const size_t bytes = sizeof(T) * (size_t)(ldc * (int64_t)(N - 1) + M + sC * (int64_t)(batch - 1));
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    bool ok = hipMallocAsync(&scratch, bytes, s) == hipSuccess;
+    for (auto &x : e) ok = ok && hipEventCreate(&x) == hipSuccess;
+    if (ok) {
+        float ms[2] = {0.f, 0.f};
+        const int cand[2] = {64, 128};
+        for (int c = 0; c < 2 && ok; ++c) {
+            // one untimed run (code load, caches), then the timed one
+            launch_tile<MODE>(s, cand[c], ta, tb, M, N, K, T(1), A, lda, sA, B, ldb, sB, T(0),
+                              static_cast<T *>(scratch), ldc, sC, batch);
+            ok = hipEventRecord(e[0], s) == hipSuccess;
+            launch_tile<MODE>(s, cand[c], ta, tb, M, N, K, T(1), A, lda, sA, B, ldb, sB, T(0),
+                              static_cast<T *>(scratch), ldc, sC, batch);
+            ok = ok && hipEventRecord(e[1], s) == hipSuccess &&
+                 hipEventSynchronize(e[1]) == hipSuccess &&
+                 hipEventElapsedTime(&ms[c], e[0], e[1]) == hipSuccess;
+        }
+        if (ok) best = ms[1] < ms[0] ? 128 : 64;
+    }
+    for (auto &x : e)
+        if (x) (void)hipEventDestroy(x);
+    if (scratch) (void)hipFreeAsync(scratch, s);
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_choice[key] = best;
+    return best;
+}
+
+template <int MODE, typename T>
 void launch_gemm_mode(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T *A,
                       int64_t lda, int64_t sA, const T *B, int64_t ldb, int64_t sB, T beta, T *C,
                       int64_t ldc, int64_t sC, int batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    // 128-tiles when they still give every compute unit two work-groups
-    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
-    const int bt = big >= 512 ? 128 : 64;
+    const int bt = choose_tile<MODE>(s, ta, tb, M, N, K, A, lda, sA, B, ldb, sB, ldc, sC, batch);
+    launch_tile<MODE>(s, bt, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                      batch);
+}
+
+template <int MODE, typename T>
+void launch_tile(hipStream_t s, int bt, bool ta, bool tb, int M, int N, int K, T alpha, const T *A,
+                 int64_t lda, int64_t sA, const T *B, int64_t ldb, int64_t sB, T beta, T *C,
+                 int64_t ldc, int64_t sC, int batch) {
     const unsigned gx = (unsigned)((M + bt - 1) / bt), gy = (unsigned)((N + bt - 1) / bt);
     if (gy > 65535u || batch > 65535)
         throw Error(9, "gemm: more than 65535 column tiles or batch entries per launch");
