@@ -555,7 +555,7 @@ def main():
     ap.add_argument("--rates", default="50,200,800,2000")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "mixed", "bf16"], default="fp64",
                     help="single-molecule workloads only")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters)")
