@@ -325,6 +325,7 @@ def run_mixed(args):
     log(f"# plain FP64 sequential: {plain_wall:.3f}s, {nok}/{n} molecules with CSFs")
     with StdaHipBatch(0, workers=args.workers) as b:
         b.configure(args.nmax_batched, args.poll)
+        b.configure_wide(not args.no_wide, two_phase=args.two_phase)
         for w in range(args.warmup):
             b.solve_tda(cis, thr, ax, nroots)
             log(f"# warmup {w + 1}/{args.warmup} done")
@@ -335,7 +336,8 @@ def run_mixed(args):
             res = b.solve_tda(cis, thr, ax, nroots)
             walls.append(time.perf_counter() - ts)
             phases.append(b.timings())
-            log(f"# step {s + 1}/{args.steps}: {walls[-1]:.3f}s {phases[-1]}")
+            log(f"# step {s + 1}/{args.steps}: {walls[-1]:.3f}s {phases[-1]} wide {b.wide_count()}")
+        nwide = b.wide_count()
         total = time.perf_counter() - tstart
         kst = {k: b.kernel_stats(k) for k in SINGLE_KERNELS + COMPOSITE_SPANS}
         phase_us = b.phase_times()
@@ -397,7 +399,7 @@ def run_mixed(args):
                                f"basis, Ethr 7 eV, ax 0.25, 20 roots each, batched mixed precision",
                    "molecules": n, "molecules_with_csf": nok, "workers": args.workers,
                    "nmax_batched": args.nmax_batched, "poll": args.poll, "host": host_info(),
-                   "paths": cnt, "phases_ms": t,
+                   "paths": dict(cnt, wide_of_batched=nwide), "phases_ms": t,
                    "phase_ms_summed": {k: round(v / 1e3, 2) for k, v in phase_us.items()
                                        if k not in ("sweeps", "unused")}},
         "roofline": roofline,
@@ -550,6 +552,11 @@ def main():
     ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--nmax-batched", type=int, default=1024,
                     help="CSF limit of the lock-step solver (0: every molecule on the worker path)")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="molecules above --nmax-batched on the single-molecule path instead of "
+                         "the matrix-free lock step")
+    ap.add_argument("--two-phase", action="store_true",
+                    help="lock-step iterations start once every molecule is prepared")
     ap.add_argument("--poll", type=int, default=4,
                     help="lock-step iterations between reads of the state table")
     ap.add_argument("--rates", default="50,200,800,2000")

@@ -29,8 +29,8 @@ FORTRAN = os.path.join(ROOT, "fortran")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 FLANG = os.environ.get("FLANG", os.path.join(ROCM, "lib", "llvm", "bin", "flang"))
 
-KERNEL_SOURCES = ["kernels.hip", "gemm.hip", "batch.hip", "spectrum.hip", "hyper.hip",
-                  "transform.hip"]
+KERNEL_SOURCES = ["kernels.hip", "gemm.hip", "batch.hip", "wide.hip", "spectrum.hip",
+                  "hyper.hip", "transform.hip"]
 HOST_SOURCE = "core.cpp"
 HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
             # packed FP32 instructions stay off (hipcc's host pass warns about the feature)

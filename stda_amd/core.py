@@ -41,6 +41,7 @@ C_API = [
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
     "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc", "stda_hip_gemm",
     "stda_hip_batch_submit", "stda_hip_batch_result", "stda_hip_batch_stop",
+    "stda_hip_batch_configure_wide", "stda_hip_batch_wide_count",
 ]
 
 
@@ -126,6 +127,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                         P, ctypes.POINTER(I32)]
     L.stda_hip_batch_stop.argtypes = [P]
     L.stda_hip_batch_configure.argtypes = [P, I64, I32]
+    L.stda_hip_batch_configure_wide.argtypes = [P, I32]
+    L.stda_hip_batch_wide_count.argtypes = [P, ctypes.POINTER(I64)]
     L.stda_hip_batch_phase_times.argtypes = [P, P]
     L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
                                               ctypes.POINTER(F64), ctypes.POINTER(F64)]
@@ -636,6 +639,20 @@ class StdaHipBatch:
         st = self._L.stda_hip_batch_configure(self._b, nmax_batched, poll)
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+
+    def configure_wide(self, enable: bool = True, two_phase: bool = False):
+        """Molecules above the CSF limit in the matrix-free lock step (True)
+        or on the single-molecule path of their worker (False); two_phase:
+        the lock step of a call starts once every molecule is prepared."""
+        st = self._L.stda_hip_batch_configure_wide(self._b, (2 if two_phase else 1) if enable else 0)
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+
+    def wide_count(self) -> int:
+        """Molecules of the last call solved in the matrix-free lock step."""
+        v = ctypes.c_int64(0)
+        self._L.stda_hip_batch_wide_count(self._b, ctypes.byref(v))
+        return int(v.value)
 
     def phase_times(self):
         """Batched-kernel phase times summed over the molecules (us) and the

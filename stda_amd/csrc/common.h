@@ -237,7 +237,7 @@ void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, con
 constexpr int BATCH_MD = 64;                        // subspace limit
 constexpr int BATCH_KMAX = 24;                      // roots per molecule
 constexpr int BATCH_OUT = 20 * BATCH_KMAX + 12;     // doubles of output per molecule
-constexpr int BATCH_STATE = 8;                      // ints of state per molecule
+constexpr int BATCH_STATE = 64;                     // ints of state per molecule
 // Output of one molecule: omega[KMAX] | T[9 KMAX] = mom^T X | T2[9 KMAX] = mom^T Xnew
 // | FP64 residual norms[KMAX] | info[12]: iterations, converged, roots found, last
 // FP32 residual, wall-clock ticks of the step phases (Gram, subspace eigensolve,
@@ -263,6 +263,130 @@ struct BatchMol {
 };
 // one step of every molecule (one work-group each)
 void launch_batch_step(hipStream_t s, int nmol, const BatchMol *mols);
+
+// --- grouped GEMM (gemm.hip) -----------------------------------------------------------
+// One problem of a grouped launch: C_b = alpha op(A_b) op(B_b) + beta C_b with
+// operand b at A + b sA (and B, C likewise).  The count follows `kind`:
+// GC_FIXED uses `count`; GC_OP32 / GC_OP64 read the vector count of the FP32 /
+// FP64 operator product of a lock-step molecule from its state (wide_op_count).
+// cols_per_count > 0: one entry with N = cols_per_count * count columns;
+// otherwise count batch entries of N columns.
+enum GemmCount { GC_FIXED = 0, GC_OP32 = 1, GC_OP64 = 2 };
+struct GemmProb {
+    int M, N, K;
+    int cols_per_count;
+    int kind, count;
+    int gm, gn, gb;            // 64 x 64 tile grid of the launch: rows, columns, batch entries
+    int epi;                   // 1: C(r, c) = aJ gam(A(r), A(c)) (alpha AB)(r, c) (+ aK u(A(c)), r = c)
+    const int *state;
+    const void *A, *B;
+    void *C;
+    int64_t lda, ldb, ldc, sA, sB, sC;
+    double alpha, beta;
+    // epilogue: gam (na x na), atom of each row/column, u (na per batch entry)
+    const void *ep_gam;
+    const int *ep_atom;
+    const void *ep_u;
+    int ep_na, ep_pad;
+    double ep_aJ, ep_aK;
+};
+// Work-group b of a grouped launch belongs to item p with start[p] <= b <
+// start[p + 1] (start: nitems + 1 ascending offsets, start[0] = 0).
+__device__ inline int grouped_item(const int *start, int nitems, int b) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// start[p] .. start[p + 1]: the gm * gn * gb tiles of problem p (tile row fastest)
+void launch_gemm_grouped(hipStream_t s, int mode, bool ta, bool tb, int nprob,
+                         const GemmProb *probs, const int *start, int ntiles);
+
+// --- wide lock-step molecules (wide.hip): matrix-free operator, grouped kernels ---------
+constexpr int WIDE_RC = 1024;                        // rows of a chunk tile
+constexpr int WIDE_MD = 96;                          // subspace limit (LDS Jacobi)
+constexpr int WIDE_PART = WIDE_MD * 40;              // doubles of partial sums per chunk
+constexpr int WIDE_AUX = 16384;                      // doubles of coefficients per molecule
+constexpr int WIDE_GB = 32;                          // AO columns per gbuild work-group
+constexpr int WIDE_MAXATOMS = 1024;                  // atoms of a wide molecule (LDS bound)
+// State beyond the shared fields (phase, m, mnew, iterations, roots kept):
+enum WideState {
+    WS_NEXT = 5, WS_RESTART = 6, WS_ROOM = 7, WS_ACC1 = 8, WS_ACC2 = 9, WS_FINISH = 10,
+    WS_NUN = 11, WS_MP = 12, WS_UN = 16, WS_KEEP = 40
+};
+// Coefficients in WideMol::aux
+enum WideAux {
+    WA_Y = 0,                                      // WIDE_MD x KMAX Ritz coefficients
+    WA_TH = WA_Y + WIDE_MD * BATCH_KMAX,           // KMAX Ritz values (ascending)
+    WA_C = WA_TH + BATCH_KMAX,                     // WIDE_MD x KMAX projection coefficients
+    WA_L = WA_C + WIDE_MD * BATCH_KMAX,            // KMAX x KMAX Cholesky factor
+    WA_NR0 = WA_L + BATCH_KMAX * BATCH_KMAX,       // KMAX squared norms of the new directions
+    WA_YP = 6144,                                  // WIDE_MD x WIDE_MD eigenvectors of the last subspace
+    WA_END = WA_YP + WIDE_MD * WIDE_MD
+};
+static_assert(WA_NR0 + BATCH_KMAX <= WA_YP, "aux layout");
+static_assert(WA_END <= WIDE_AUX, "aux area");
+struct WideMol {
+    int64_t n, npair;          // CSFs; occupied x virtual pairs of the window
+    int nao, no, nv, na;
+    int k, nguess, maxdim, maxiter;
+    int nvb;                   // vectors of the FP32 operator buffers (>= nguess, k)
+    int velcorr, nchunk, pad;
+    double tol;                // residual tolerance of the FP32-operator iterations
+    double ak, ax;
+    const float *C32;          // nao x (no + nv) Loewdin MOs, occupied then virtual
+    const double *C64;
+    const float *gj32, *gk32;  // na x na gamma^J, gamma^K
+    const double *gj64, *gk64;
+    const int32_t *atom_of;    // nao
+    const int32_t *atom_ptr;   // na + 1
+    const int32_t *pair;       // n: CSF -> pair index (occupied * nv + virtual)
+    const int32_t *guess;      // nguess start rows
+    const double *corr;        // n: diagonal term of A added to the coupling part
+    const double *diag;        // n: diagonal of A (preconditioner)
+    const double *mom;         // n x 9 CSF moments or null
+    // FP32 operator work (nvb vectors): transition densities in the pair grid,
+    // nao x no half transforms, nao x nao densities (G in place), pair-grid results
+    float *xf32, *t32, *d32, *r32, *u32;
+    // FP64 operator work (k vectors): the same, G separate (exchange term)
+    double *xf64, *t64, *d64, *g64, *r64, *u64;
+    double *V, *AV;            // n x maxdim (maxdim <= WIDE_MD)
+    double *X, *AX, *W;        // n x BATCH_KMAX
+    double *H;                 // WIDE_MD x WIDE_MD
+    double *aux;               // WIDE_AUX
+    double *part;              // nchunk x WIDE_PART
+    double *out;               // BATCH_OUT
+    int *state;                // BATCH_STATE
+};
+// vector count of a wide molecule's FP32 (B_ITER: the new block) or FP64
+// (B_REFINE, B_VEL: the kept roots) operator product this iteration
+__host__ __device__ inline int wide_op_count(const int *st, bool fp64) {
+    const int ph = st[0];
+    if (!fp64) return ph == B_ITER ? st[2] : 0;
+    return (ph == B_REFINE || ph == B_VEL) ? st[4] : 0;
+}
+__host__ __device__ inline int gemm_prob_count(const GemmProb &P) {
+    if (P.kind == GC_OP32) return wide_op_count(P.state, false);
+    if (P.kind == GC_OP64) return wide_op_count(P.state, true);
+    return P.count;
+}
+// Work-group offsets of the grouped wide kernels, nmol + 1 entries each
+// (grouped_item): row chunks (nchunk per molecule), vectors (nvb), and
+// G-build tiles (nvb * ceil(nao / WIDE_GB): vector, then AO column block).
+struct WideStarts {
+    const int *chunk, *vec, *gb;
+    int nchunks, nvecs, ngb;
+};
+// operator product launches of one lock-step iteration (scatter, atom sums, G build, gather)
+void launch_wide_scatter(hipStream_t s, int nmol, const WideMol *mols, WideStarts st);
+void launch_wide_atomdiag(hipStream_t s, int nmol, const WideMol *mols, WideStarts st, bool fp64);
+void launch_wide_gbuild(hipStream_t s, int nmol, const WideMol *mols, WideStarts st);
+void launch_wide_gather(hipStream_t s, int nmol, const WideMol *mols, WideStarts st);
+// the Davidson step of every wide molecule (a fixed sequence of grouped launches)
+void launch_wide_step(hipStream_t s, int nmol, const WideMol *mols, WideStarts st);
 // operator products of every active molecule; tiles = (molecule, BATCH_MV_ROWS-row block)
 constexpr int BATCH_MV_ROWS = 64;
 void launch_batch_matvec(hipStream_t s, int ntiles, const BatchMol *mols, const int2 *tiles);

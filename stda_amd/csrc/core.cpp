@@ -3187,12 +3187,15 @@ struct stda_hip_batch {
     std::vector<stda_hip_ctx *> workers;
     hipStream_t dstream = nullptr;       // the driver's stream
     std::string last_error;
-    int64_t nmax_batched = 1024;         // CSF limit of the batched solver
+    int64_t nmax_batched = 1024;         // CSF limit of the explicit-operator lock step
+    bool wide = true;                    // larger molecules in the matrix-free lock step
+    bool two_phase = false;              // lock step only after the setup of every molecule
     int group = 4;                       // iterations between polls of the state table
     // ms: worker phase (wall), driver loop (wall), property tables, total,
     // summed HIP-event time of the batched launches (matvec + step)
     double timings[5] = {0, 0, 0, 0, 0};
     int64_t counts[3] = {0, 0, 0};       // molecules: batched, single path, failed
+    int64_t nwide = 0;                   // of the batched: in the matrix-free lock step
     double kernel_bytes = 0.0;           // algorithmic bytes of the batched operator products
     double lockstep_ms = 0.0;            // HIP-event time of the lock-step graph launches
     double lockstep_launches = 0.0;      // and their number (each `group` iterations)
@@ -3213,7 +3216,203 @@ struct BatchItem {
     DBuf<float> A32;
     DBuf<double> A64, B64, diag, mom, work;
     DBuf<int32_t> guess;
+    // wide molecules (matrix-free lock step, wide.hip): copies of the
+    // operator data of the context and the operator work buffers
+    bool wide = false;
+    int64_t npair = 0;
+    int nao = 0, no = 0, nv = 0, na = 0, nvb = 0, nchunk = 0;
+    double ax = 0.0;
+    DBuf<float> C32, gj32, gk32, xf32, t32, d32, r32, u32;
+    DBuf<double> C64, gj64, gk64, corr, xf64, t64, d64, g64, r64, u64, aux, part;
+    DBuf<int32_t> atom_of, atom_ptr, pair;
 };
+
+template <typename T>
+void copy_device(DBuf<T> &dst, const DBuf<T> &src, size_t count, hipStream_t s) {
+    dst.resize(count);
+    if (count)
+        STDA_HIP_CHECK(hipMemcpyAsync(dst.get(), src.get(), count * sizeof(T),
+                                      hipMemcpyDeviceToDevice, s));
+}
+
+// Hand-over of a prepared molecule to the wide lock-step solver: its
+// operator data is copied out of the context (which goes on to the next
+// molecule) and the work buffers are sized (FP32 products for nvb vectors,
+// FP64 for k).  The pair-grid buffers are cleared once: only the CSF pairs
+// are ever written into them.
+void wide_prepare(stda_hip_ctx *c, int k, bool velcorr, BatchItem &it) {
+    const int64_t n = c->nci;
+    hipStream_t s = c->stream;
+    it.wide = true;
+    it.n = n;
+    it.k = k;
+    it.ak = c->ak;
+    it.ax = c->ax;
+    it.moments = c->have_moments;
+    it.velcorr = velcorr && c->have_moments;
+    it.maxdim = (int)std::min<int64_t>(n, WIDE_MD);
+    it.nguess = std::min(it.maxdim, std::max(k + 8, k + k / 2));
+    it.nvb = std::max(it.nguess, k);
+    it.nao = c->nao;
+    it.no = c->no;
+    it.nv = c->nv;
+    it.na = c->ncent;
+    it.npair = c->npair;
+    it.nchunk = (int)((n + WIDE_RC - 1) / WIDE_RC);
+    ensure_fp32(c);
+    const size_t nao = it.nao, moci = (size_t)it.no + it.nv, na = it.na, np = it.npair;
+    copy_device(it.C32, c->ct32, nao * moci, s);
+    copy_device(it.C64, c->ct, nao * moci, s);
+    copy_device(it.gj32, c->gamj32, na * na, s);
+    copy_device(it.gk32, c->gamk32, na * na, s);
+    copy_device(it.gj64, c->gamj, na * na, s);
+    copy_device(it.gk64, c->gamk, na * na, s);
+    copy_device(it.atom_of, c->atom_of, nao, s);
+    copy_device(it.atom_ptr, c->atom_ptr, na + 1, s);
+    copy_device(it.pair, c->pair, (size_t)n, s);
+    copy_device(it.corr, c->corr, (size_t)n, s);
+    copy_device(it.diag, c->ed, (size_t)n, s);
+    if (it.moments) copy_device(it.mom, c->mom, (size_t)9 * n, s);
+    std::vector<int32_t> g = lowest_diag_rows(c->ed_h, it.nguess);
+    it.guess.upload(g.data(), g.size(), s);
+    const size_t vb = it.nvb, kb = k, no = it.no;
+    it.xf32.resize(np * vb);
+    it.xf32.zero(s);
+    it.t32.resize(nao * no * vb);
+    it.d32.resize(nao * nao * vb);
+    it.r32.resize(np * vb);
+    it.u32.resize(na * vb);
+    it.xf64.resize(np * kb);
+    it.xf64.zero(s);
+    it.t64.resize(nao * no * kb);
+    it.d64.resize(nao * nao * kb);
+    it.g64.resize(nao * nao * kb);
+    it.r64.resize(np * kb);
+    it.u64.resize(na * kb);
+    it.work.resize((size_t)n * (2 * it.maxdim + 3 * BATCH_KMAX) + (size_t)WIDE_MD * WIDE_MD);
+    it.aux.resize(WIDE_AUX);
+    it.part.resize((size_t)it.nchunk * WIDE_PART);
+}
+
+WideMol wide_desc(BatchItem &it, double tol32, double *out, int *state) {
+    WideMol d{};
+    const int64_t n = it.n;
+    d.n = n;
+    d.npair = it.npair;
+    d.nao = it.nao;
+    d.no = it.no;
+    d.nv = it.nv;
+    d.na = it.na;
+    d.k = it.k;
+    d.nguess = it.nguess;
+    d.maxdim = it.maxdim;
+    d.maxiter = 300;
+    d.nvb = it.nvb;
+    d.velcorr = it.velcorr ? 1 : 0;
+    d.nchunk = it.nchunk;
+    d.tol = tol32;
+    d.ak = it.ak;
+    d.ax = it.ax;
+    d.C32 = it.C32.get();
+    d.C64 = it.C64.get();
+    d.gj32 = it.gj32.get();
+    d.gk32 = it.gk32.get();
+    d.gj64 = it.gj64.get();
+    d.gk64 = it.gk64.get();
+    d.atom_of = it.atom_of.get();
+    d.atom_ptr = it.atom_ptr.get();
+    d.pair = it.pair.get();
+    d.guess = it.guess.get();
+    d.corr = it.corr.get();
+    d.diag = it.diag.get();
+    d.mom = it.moments ? it.mom.get() : nullptr;
+    d.xf32 = it.xf32.get();
+    d.t32 = it.t32.get();
+    d.d32 = it.d32.get();
+    d.r32 = it.r32.get();
+    d.u32 = it.u32.get();
+    d.xf64 = it.xf64.get();
+    d.t64 = it.t64.get();
+    d.d64 = it.d64.get();
+    d.g64 = it.g64.get();
+    d.r64 = it.r64.get();
+    d.u64 = it.u64.get();
+    double *w = it.work.get();
+    d.V = w;
+    d.AV = w + n * it.maxdim;
+    d.X = w + 2 * n * it.maxdim;
+    d.AX = d.X + n * BATCH_KMAX;
+    d.W = d.AX + n * BATCH_KMAX;
+    d.H = d.W + n * BATCH_KMAX;
+    d.aux = it.aux.get();
+    d.part = it.part.get();
+    d.out = out;
+    d.state = state;
+    return d;
+}
+
+// The four transforms of a wide molecule's operator product as grouped GEMM
+// problems, FP32 (iterations, `cap` = nvb vectors) and FP64 (refinement,
+// velocity correction, k vectors): index 2 * stage + precision, stages
+// T = Cv X^T (NN), D = T Co^T (NT), W = G Co (NN), R = Cv^T W (TN).
+void wide_gemm_probs(const WideMol &d, GemmProb (&P)[8]) {
+    const int nao = d.nao, no = d.no, nv = d.nv;
+    auto tiles = [](int64_t x) { return (int)((x + 63) / 64); };
+    for (int prec = 0; prec < 2; ++prec) {
+        const bool f64 = prec == 1;
+        const int cap = f64 ? d.k : d.nvb;
+        const size_t es = f64 ? 8 : 4;
+        const char *C = static_cast<const char *>(f64 ? (const void *)d.C64 : (const void *)d.C32);
+        const void *Co = C, *Cv = C + es * (size_t)nao * no;
+        void *xf = f64 ? (void *)d.xf64 : (void *)d.xf32;
+        void *t = f64 ? (void *)d.t64 : (void *)d.t32;
+        void *D = f64 ? (void *)d.d64 : (void *)d.d32;
+        void *G = f64 ? (void *)d.g64 : (void *)d.d32;   // FP32: G in place of D
+        void *R = f64 ? (void *)d.r64 : (void *)d.r32;
+        GemmProb base{};
+        base.kind = f64 ? GC_OP64 : GC_OP32;
+        base.state = d.state;
+        base.alpha = 1.0;
+        base.beta = 0.0;
+        // T = Cv X^T: nao x (no * count), K = nv
+        GemmProb g1 = base;
+        g1.M = nao; g1.N = no; g1.K = nv; g1.cols_per_count = no;
+        g1.A = Cv; g1.lda = nao; g1.B = xf; g1.ldb = nv; g1.C = t; g1.ldc = nao;
+        g1.gm = tiles(nao); g1.gn = tiles((int64_t)no * cap); g1.gb = 1;
+        // D_v = T_v Co^T: nao x nao, K = no, per vector.  FP32 (operator A):
+        // stored as G = -gamJ o D + ak diag(u) (epilogue); FP64: D itself
+        GemmProb g2 = base;
+        g2.M = nao; g2.N = nao; g2.K = no;
+        g2.A = t; g2.lda = nao; g2.sA = (int64_t)nao * no;
+        g2.B = Co; g2.ldb = nao; g2.C = D; g2.ldc = nao; g2.sC = (int64_t)nao * nao;
+        g2.gm = tiles(nao); g2.gn = tiles(nao); g2.gb = cap;
+        if (!f64) {
+            g2.epi = 1;
+            g2.ep_gam = d.gj32;
+            g2.ep_atom = d.atom_of;
+            g2.ep_u = d.u32;
+            g2.ep_na = d.na;
+            g2.ep_aJ = -1.0;
+            g2.ep_aK = d.ak;
+        }
+        // W_v = G_v Co: nao x no, K = nao, per vector (into the T buffer)
+        GemmProb g3 = base;
+        g3.M = nao; g3.N = no; g3.K = nao;
+        g3.A = G; g3.lda = nao; g3.sA = (int64_t)nao * nao;
+        g3.B = Co; g3.ldb = nao; g3.C = t; g3.ldc = nao; g3.sC = (int64_t)nao * no;
+        g3.gm = tiles(nao); g3.gn = tiles(no); g3.gb = cap;
+        // R = Cv^T W: nv x (no * count), K = nao
+        GemmProb g4 = base;
+        g4.M = nv; g4.N = no; g4.K = nao; g4.cols_per_count = no;
+        g4.A = Cv; g4.lda = nao; g4.B = t; g4.ldb = nao; g4.C = R; g4.ldc = nv;
+        g4.gm = tiles(nv); g4.gn = tiles((int64_t)no * cap); g4.gb = 1;
+        P[0 + prec] = g1;
+        P[2 + prec] = g2;
+        P[4 + prec] = g3;
+        P[6 + prec] = g4;
+    }
+}
+
 
 // 0.5 B = 0.5 ak (ia|jb)_K - 0.5 ax (ib|ja)_K for all CSF pairs (rtdacorr,
 // apbtrafo.f:580-637), n x n column-major on the device
@@ -3232,7 +3431,7 @@ void build_bhalf_device(stda_hip_ctx *c, DBuf<double> &B) {
 
 // one molecule of the setup phase; returns true when it goes to the batched solver
 bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
-                   int nroots, double tol, int velcorr, int64_t nmax, BatchItem &it,
+                   int nroots, double tol, int velcorr, int64_t nmax, bool wide, BatchItem &it,
                    int32_t *nfound, double *omega, double *props, double *us3) {
     Timer t0;
     do_load(c, sys, par);
@@ -3242,6 +3441,11 @@ bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_p
     const int64_t n = c->nci;
     const int k = (int)std::min<int64_t>(nroots, n);
     hipStream_t s = c->stream;
+    if (n > nmax && wide && k <= BATCH_KMAX && c->ncent <= WIDE_MAXATOMS) {
+        wide_prepare(c, k, velcorr != 0, it);
+        us3[1] += 1e3 * t1.ms();
+        return true;
+    }
     if (n > nmax || k > BATCH_KMAX) {
         struct Add {
             double *d;
@@ -3314,6 +3518,149 @@ BatchMol batch_desc(BatchItem &it, double tol32, double *out, int *state) {
     return d;
 }
 
+// The device description of an active lock-step set: explicit-operator
+// molecules (BatchMol: one work-group each per step, 64-row tiles for the
+// products) and wide ones (WideMol: grouped kernels and GEMMs), and the
+// captured graph of `group` iterations over both.
+struct LockstepSet {
+    struct Entry {
+        BatchItem *item;
+        double tol32;
+        double *out;
+        int *state;
+    };
+    DBuf<BatchMol> descd;
+    DBuf<int2> tilesd;
+    int nsmall = 0, ntiles = 0;
+    DBuf<WideMol> wdescd;
+    DBuf<int> wstart;       // chunk | vector | G-build offsets, nwide + 1 each
+    DBuf<GemmProb> gprob;   // 8 groups of nwide problems
+    DBuf<int> gstart;       // 8 groups of nwide + 1 offsets
+    int gtotal[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int nwide = 0;
+    WideStarts ws{};
+    int gemm32 = GEMM_F32;
+    hipGraphExec_t gexec = nullptr;
+
+    LockstepSet() = default;
+    LockstepSet(const LockstepSet &) = delete;
+    LockstepSet &operator=(const LockstepSet &) = delete;
+    ~LockstepSet() { drop_graph(); }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+    bool empty() const { return nsmall == 0 && nwide == 0; }
+
+    void build(std::vector<Entry> entries, hipStream_t ds) {
+        drop_graph();
+        // largest first: their work-groups start first
+        std::stable_sort(entries.begin(), entries.end(),
+                         [](const Entry &a, const Entry &c) { return a.item->n > c.item->n; });
+        std::vector<BatchMol> desc;
+        std::vector<int2> tiles;
+        std::vector<WideMol> wdesc;
+        for (const Entry &e : entries) {
+            if (e.item->wide) {
+                wdesc.push_back(wide_desc(*e.item, e.tol32, e.out, e.state));
+                continue;
+            }
+            const int q = (int)desc.size();
+            desc.push_back(batch_desc(*e.item, e.tol32, e.out, e.state));
+            const int nblk = (int)((e.item->n + BATCH_MV_ROWS - 1) / BATCH_MV_ROWS);
+            for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
+        }
+        nsmall = (int)desc.size();
+        ntiles = (int)tiles.size();
+        if (nsmall) {
+            descd.upload(desc.data(), desc.size(), ds);
+            tilesd.upload(tiles.data(), tiles.size(), ds);
+        }
+        nwide = (int)wdesc.size();
+        if (nwide) {
+            wdescd.upload(wdesc.data(), wdesc.size(), ds);
+            std::vector<int> st((size_t)3 * (nwide + 1), 0);
+            int *ck = st.data(), *vc = ck + nwide + 1, *gb = vc + nwide + 1;
+            for (int q = 0; q < nwide; ++q) {
+                const WideMol &d = wdesc[q];
+                ck[q + 1] = ck[q] + d.nchunk;
+                vc[q + 1] = vc[q] + d.nvb;
+                gb[q + 1] = gb[q] + d.nvb * ((d.nao + WIDE_GB - 1) / WIDE_GB);
+            }
+            wstart.upload(st.data(), st.size(), ds);
+            ws.chunk = wstart.get();
+            ws.vec = ws.chunk + nwide + 1;
+            ws.gb = ws.vec + nwide + 1;
+            ws.nchunks = ck[nwide];
+            ws.nvecs = vc[nwide];
+            ws.ngb = gb[nwide];
+            std::vector<GemmProb> probs((size_t)8 * nwide);
+            std::vector<int> gs((size_t)8 * (nwide + 1), 0);
+            // the device addresses of the states are those in the descriptors
+            for (int q = 0; q < nwide; ++q) {
+                GemmProb P[8];
+                wide_gemm_probs(wdesc[q], P);
+                for (int g = 0; g < 8; ++g) probs[(size_t)g * nwide + q] = P[g];
+            }
+            for (int g = 0; g < 8; ++g) {
+                int *o = gs.data() + (size_t)g * (nwide + 1);
+                for (int q = 0; q < nwide; ++q) {
+                    const GemmProb &P = probs[(size_t)g * nwide + q];
+                    const int64_t cnt = o[q] + (int64_t)P.gm * P.gn * P.gb;
+                    if (cnt > INT32_MAX) throw Error(STDA_HIP_ERR_ARG, "wide lock-step: too many GEMM tiles");
+                    o[q + 1] = (int)cnt;
+                }
+                gtotal[g] = o[nwide];
+            }
+            gprob.upload(probs.data(), probs.size(), ds);
+            gstart.upload(gs.data(), gs.size(), ds);
+        }
+    }
+
+    // one lock-step iteration of both kinds
+    void iteration(hipStream_t ds) {
+        if (nsmall) {
+            launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
+            launch_batch_step(ds, nsmall, descd.get());
+        }
+        if (nwide) {
+            const WideMol *w = wdescd.get();
+            auto gemm = [&](int g, int mode, bool ta, bool tb) {
+                launch_gemm_grouped(ds, mode, ta, tb, nwide, gprob.get() + (size_t)g * nwide,
+                                    gstart.get() + (size_t)g * (nwide + 1), gtotal[g]);
+            };
+            launch_wide_scatter(ds, nwide, w, ws);
+            gemm(0, gemm32, false, false);
+            gemm(1, GEMM_F64, false, false);
+            launch_wide_atomdiag(ds, nwide, w, ws, false);
+            gemm(2, gemm32, false, true);
+            gemm(3, GEMM_F64, false, true);
+            launch_wide_atomdiag(ds, nwide, w, ws, true);
+            launch_wide_gbuild(ds, nwide, w, ws);
+            gemm(4, gemm32, false, false);
+            gemm(5, GEMM_F64, false, false);
+            gemm(6, gemm32, true, false);
+            gemm(7, GEMM_F64, true, false);
+            launch_wide_gather(ds, nwide, w, ws);
+            launch_wide_step(ds, nwide, w, ws);
+        }
+    }
+
+    // `group` iterations as one graph launch (captured on first use after a build)
+    void launch(hipStream_t ds, int group) {
+        if (!gexec) {
+            hipGraph_t graph = nullptr;
+            STDA_HIP_CHECK(hipStreamBeginCapture(ds, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < group; ++j) iteration(ds);
+            STDA_HIP_CHECK(hipStreamEndCapture(ds, &graph));
+            const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            STDA_HIP_CHECK(ie);
+        }
+        STDA_HIP_CHECK(hipGraphLaunch(gexec, ds));
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -3366,6 +3713,19 @@ int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t po
     if (!b || nmax_batched < 0 || poll < 1) return STDA_HIP_ERR_ARG;
     b->nmax_batched = nmax_batched;
     b->group = poll;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable) {
+    if (!b || enable < 0 || enable > 2) return STDA_HIP_ERR_ARG;
+    b->wide = enable != 0;
+    b->two_phase = enable == 2;
+    return STDA_HIP_OK;
+}
+
+int stda_hip_batch_wide_count(const stda_hip_batch *b, int64_t *nwide) {
+    if (!b || !nwide) return STDA_HIP_ERR_ARG;
+    *nwide = b->nwide;
     return STDA_HIP_OK;
 }
 
@@ -3429,6 +3789,7 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
     try {
         Timer t_all;
         for (auto &x : b->counts) x = 0;
+        b->nwide = 0;
         for (auto &x : b->timings) x = 0.0;
         for (auto &x : b->phase_us) x = 0.0;
         b->kernel_bytes = 0.0;
@@ -3483,7 +3844,7 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                     const int i = order[q];
                     try {
                         batched[i] = batch_prepare(c, &sys[i], &par[i], nroots, tol, velcorr,
-                                                   b->nmax_batched, items[i], &nfound[i],
+                                                   b->nmax_batched, b->wide, items[i], &nfound[i],
                                                    omega + (size_t)i * nroots,
                                                    props + (size_t)i * nroots * 5, wus[wi].data());
                         status[i] = STDA_HIP_OK;
@@ -3527,28 +3888,18 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
         {
             AllocScope scope(ds);
             std::vector<int> active;
-            DBuf<BatchMol> descd;
-            DBuf<int2> tilesd;
-            int ntiles = 0;
             bool dirty = false;
             std::vector<std::array<hipEvent_t, 2>> evs;
             int iter = 0;
             bool feeding = true;
             // The `group` lock-step iterations between two reads of the state
-            // table are one captured graph (two kernels per iteration), rebuilt
-            // when the active set changes.
-            hipGraphExec_t gexec = nullptr;
-            struct GraphGuard {
-                hipGraphExec_t &g;
-                ~GraphGuard() {
-                    if (g) (void)hipGraphExecDestroy(g);
-                }
-            } graph_guard{gexec};
+            // table are one captured graph, rebuilt when the active set changes.
+            LockstepSet set;
             while (true) {
                 std::vector<std::pair<int, hipEvent_t>> got;
                 {
                     std::unique_lock<std::mutex> lk(mu);
-                    if (active.empty() && ready.empty() && workers_left > 0)
+                    if ((active.empty() || b->two_phase) && workers_left > 0)
                         cv.wait_for(lk, std::chrono::milliseconds(2));
                     got.swap(ready);
                     feeding = workers_left > 0;
@@ -3570,44 +3921,19 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                     }
                     continue;
                 }
+                if (b->two_phase && feeding) continue;   // lock step once every molecule is prepared
                 if (dirty) {
-                    // largest first: their work-groups start first
-                    std::stable_sort(active.begin(), active.end(),
-                                     [&](int a, int c) { return items[a].n > items[c].n; });
-                    std::vector<BatchMol> desc;
-                    std::vector<int2> tiles;
-                    for (int q = 0; q < (int)active.size(); ++q) {
-                        const int i = active[q];
-                        desc.push_back(batch_desc(items[i], tol32, outd.get() + (size_t)i * BATCH_OUT,
-                                                  stated.get() + (size_t)i * BATCH_STATE));
-                        const int nblk = (int)((items[i].n + BATCH_MV_ROWS - 1) / BATCH_MV_ROWS);
-                        for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
-                    }
-                    descd.upload(desc.data(), desc.size(), ds);
-                    tilesd.upload(tiles.data(), tiles.size(), ds);
-                    ntiles = (int)tiles.size();
+                    std::vector<LockstepSet::Entry> ent;
+                    for (int q : active)
+                        ent.push_back({&items[q], tol32, outd.get() + (size_t)q * BATCH_OUT,
+                                       stated.get() + (size_t)q * BATCH_STATE});
+                    set.build(std::move(ent), ds);
                     dirty = false;
-                    if (gexec) {
-                        STDA_HIP_CHECK(hipGraphExecDestroy(gexec));
-                        gexec = nullptr;
-                    }
-                }
-                if (!gexec) {
-                    hipGraph_t graph = nullptr;
-                    STDA_HIP_CHECK(hipStreamBeginCapture(ds, hipStreamCaptureModeThreadLocal));
-                    for (int j = 0; j < b->group; ++j) {
-                        launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
-                        launch_batch_step(ds, (int)active.size(), descd.get());
-                    }
-                    STDA_HIP_CHECK(hipStreamEndCapture(ds, &graph));
-                    const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(graph);
-                    STDA_HIP_CHECK(ie);
                 }
                 std::array<hipEvent_t, 2> e{};
                 for (auto &x : e) STDA_HIP_CHECK(hipEventCreate(&x));
                 STDA_HIP_CHECK(hipEventRecord(e[0], ds));
-                STDA_HIP_CHECK(hipGraphLaunch(gexec, ds));
+                set.launch(ds, b->group);
                 STDA_HIP_CHECK(hipEventRecord(e[1], ds));
                 evs.push_back(e);
                 iter += b->group;
@@ -3661,7 +3987,10 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
             const bool conv = info[1] != 0.0;
             // the FP32 A once per iteration, the FP64 A and 0.5 B once
             const double n2 = (double)items[i].n * (double)items[i].n;
-            b->kernel_bytes += info[0] * 4.0 * n2 + 8.0 * n2 + (items[i].velcorr ? 8.0 * n2 : 0.0);
+            if (!items[i].wide)
+                b->kernel_bytes += info[0] * 4.0 * n2 + 8.0 * n2 + (items[i].velcorr ? 8.0 * n2 : 0.0);
+            else
+                ++b->nwide;
             for (int ph = 0; ph < 5; ++ph) b->phase_us[ph] += info[4 + ph] * us_per_tick;
             b->phase_us[6] += info[10];
             std::vector<double> w(o, o + kk);
@@ -3768,7 +4097,7 @@ void stream_worker(stda_hip_batch *b, BatchStream *E, int wi) {
         try {
             double us3[3] = {0, 0, 0};
             batched = batch_prepare(c, &t->sys, &t->par, t->nroots, t->tol, t->velcorr,
-                                    b->nmax_batched, t->item, &t->nfound, t->omega.data(),
+                                    b->nmax_batched, b->wide, t->item, &t->nfound, t->omega.data(),
                                     t->props.data(), us3);
             t->status = STDA_HIP_OK;
             if (batched) {
@@ -3805,17 +4134,8 @@ void stream_driver(stda_hip_batch *b, BatchStream *E) {
         std::vector<int> free_slots;
         for (int q = S - 1; q >= 0; --q) free_slots.push_back(q);
         std::vector<BatchTicket *> active;
-        DBuf<BatchMol> descd;
-        DBuf<int2> tilesd;
-        int ntiles = 0;
         bool dirty = false;
-        hipGraphExec_t gexec = nullptr;
-        struct GraphGuard {
-            hipGraphExec_t &g;
-            ~GraphGuard() {
-                if (g) (void)hipGraphExecDestroy(g);
-            }
-        } guard{gexec};
+        LockstepSet set;
         while (true) {
             std::vector<std::pair<int64_t, hipEvent_t>> got;
             std::vector<BatchTicket *> gott;
@@ -3848,42 +4168,15 @@ void stream_driver(stda_hip_batch *b, BatchStream *E) {
             }
             if (active.empty()) continue;
             if (dirty) {
-                std::stable_sort(active.begin(), active.end(),
-                                 [](const BatchTicket *a, const BatchTicket *c) {
-                                     return a->item.n > c->item.n;
-                                 });
-                std::vector<BatchMol> desc;
-                std::vector<int2> tiles;
-                for (int q = 0; q < (int)active.size(); ++q) {
-                    BatchTicket *t = active[q];
-                    desc.push_back(batch_desc(t->item, std::max(t->tol, 1.0e-6),
-                                              outd.get() + (size_t)t->slot * BATCH_OUT,
-                                              stated.get() + (size_t)t->slot * BATCH_STATE));
-                    const int nblk = (int)((t->item.n + BATCH_MV_ROWS - 1) / BATCH_MV_ROWS);
-                    for (int r = 0; r < nblk; ++r) tiles.push_back(make_int2(q, r));
-                }
-                descd.upload(desc.data(), desc.size(), ds);
-                tilesd.upload(tiles.data(), tiles.size(), ds);
-                ntiles = (int)tiles.size();
+                std::vector<LockstepSet::Entry> ent;
+                for (BatchTicket *t : active)
+                    ent.push_back({&t->item, std::max(t->tol, 1.0e-6),
+                                   outd.get() + (size_t)t->slot * BATCH_OUT,
+                                   stated.get() + (size_t)t->slot * BATCH_STATE});
+                set.build(std::move(ent), ds);
                 dirty = false;
-                if (gexec) {
-                    STDA_HIP_CHECK(hipGraphExecDestroy(gexec));
-                    gexec = nullptr;
-                }
             }
-            if (!gexec) {
-                hipGraph_t graph = nullptr;
-                STDA_HIP_CHECK(hipStreamBeginCapture(ds, hipStreamCaptureModeThreadLocal));
-                for (int j = 0; j < b->group; ++j) {
-                    launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
-                    launch_batch_step(ds, (int)active.size(), descd.get());
-                }
-                STDA_HIP_CHECK(hipStreamEndCapture(ds, &graph));
-                const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                STDA_HIP_CHECK(ie);
-            }
-            STDA_HIP_CHECK(hipGraphLaunch(gexec, ds));
+            set.launch(ds, b->group);
             // poll the state table (one copy)
             STDA_HIP_CHECK(hipMemcpyAsync(stateh.data(), stated.get(), stateh.size() * sizeof(int),
                                           hipMemcpyDeviceToHost, ds));

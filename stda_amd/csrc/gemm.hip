@@ -85,25 +85,37 @@ struct GemmSmem<GEMM_BF16X3, BT> {
     unsigned short ah[BT][BK_LD], al[BT][BK_LD], bh[BT][BK_LD], bl[BT][BK_LD];
 };
 
-template <int MODE, bool TA, bool TB, int BT>
-__global__ void __launch_bounds__(GTHREADS, 2)   // at least two waves per SIMD
-gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
-            const typename GemmTraits<MODE>::T *__restrict__ A, int64_t lda, int64_t sA,
-            const typename GemmTraits<MODE>::T *__restrict__ B, int64_t ldb, int64_t sB,
-            typename GemmTraits<MODE>::T beta, typename GemmTraits<MODE>::T *__restrict__ C,
-            int64_t ldc, int64_t sC) {
+// Epilogue of the transition-density transform D = T Co^T of the matrix-free
+// operator: the stored element is G(nu, mu) = aJ gamJ(A(nu), A(mu)) D(nu, mu)
+// (+ aK u(A(mu)) on the diagonal), i.e. the Coulomb-type and atom-diagonal
+// parts of G formed as D leaves the matrix cores (row nu = tile row, mu = column).
+template <typename T>
+struct GammaEpi {
+    const T *gam;
+    const int *atom_of;
+    const T *u;
+    int na;
+    T aJ, aK;
+};
+
+// One BT x BT tile of C at (m0, n0), computed by the whole work-group.
+template <int MODE, bool TA, bool TB, int BT, bool EPI = false>
+__device__ __forceinline__ void gemm_tile(GemmSmem<MODE, BT> &sm, int M, int N, int K,
+                                          typename GemmTraits<MODE>::T alpha,
+                                          const typename GemmTraits<MODE>::T *__restrict__ A,
+                                          int64_t lda,
+                                          const typename GemmTraits<MODE>::T *__restrict__ B,
+                                          int64_t ldb, typename GemmTraits<MODE>::T beta,
+                                          typename GemmTraits<MODE>::T *__restrict__ C,
+                                          int64_t ldc, int m0, int n0,
+                                          GammaEpi<typename GemmTraits<MODE>::T> ge = {}) {
     using T = typename GemmTraits<MODE>::T;
     constexpr int KC = chunk_of<MODE, BT>();
     constexpr int PER = BT * KC / GTHREADS;   // elements of each operand per thread and chunk
     constexpr int WT = BT / 2;                // wave tile edge
     constexpr int NB = WT / 16;               // MFMA blocks per wave tile edge
-    __shared__ __align__(16) GemmSmem<MODE, BT> sm;
     const int t = threadIdx.x, l = t & 63, g = l >> 4, c = l & 15;
     const int w = t >> 6, wr = w >> 1, wc = w & 1;
-    const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
-    A += sA * blockIdx.z;
-    B += sB * blockIdx.z;
-    C += sC * blockIdx.z;
 
     // (tile row, k) of this thread's p-th element of the A chunk, and (k, tile column) of B
     auto a_idx = [&](int p, int &mm, int &kk) {
@@ -215,10 +227,69 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
                 if (gm < M && gn < N) {
                     T v = alpha * acc[i][j][e];
                     T *p = C + gm + ldc * (int64_t)gn;
-                    if (beta != T(0)) v += beta * *p;
+                    if constexpr (EPI) {
+                        const int am = ge.atom_of[gm], an = ge.atom_of[gn];
+                        v = ge.aJ * ge.gam[am + (int64_t)ge.na * an] * v;
+                        if (gm == gn) v += ge.aK * ge.u[an];
+                    } else if (beta != T(0)) {
+                        v += beta * *p;
+                    }
                     *p = v;
                 }
             }
+}
+
+template <int MODE, bool TA, bool TB, int BT>
+__global__ void __launch_bounds__(GTHREADS, 2)   // at least two waves per SIMD
+gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
+            const typename GemmTraits<MODE>::T *__restrict__ A, int64_t lda, int64_t sA,
+            const typename GemmTraits<MODE>::T *__restrict__ B, int64_t ldb, int64_t sB,
+            typename GemmTraits<MODE>::T beta, typename GemmTraits<MODE>::T *__restrict__ C,
+            int64_t ldc, int64_t sC) {
+    __shared__ __align__(16) GemmSmem<MODE, BT> sm;
+    gemm_tile<MODE, TA, TB, BT>(sm, M, N, K, alpha, A + sA * blockIdx.z, lda, B + sB * blockIdx.z,
+                                ldb, beta, C + sC * blockIdx.z, ldc, (int)blockIdx.x * BT,
+                                (int)blockIdx.y * BT);
+}
+
+// Grouped form: the work-groups start[p] .. start[p + 1] compute the 64 x 64
+// tiles of problem p (tile row fastest, then tile column, then batch entry).
+// A problem's column or batch count can follow the state of a lock-step
+// molecule (GemmProb::kind), so that one captured launch serves every
+// iteration; tiles beyond the current count return at once.
+template <int MODE, bool TA, bool TB>
+__global__ void __launch_bounds__(GTHREADS, 2)
+gemm_grouped_kernel(const GemmProb *__restrict__ probs, const int *__restrict__ start, int nprob) {
+    using T = typename GemmTraits<MODE>::T;
+    constexpr int BT = 64;
+    __shared__ __align__(16) GemmSmem<MODE, BT> sm;
+    const int p = grouped_item(start, nprob, (int)blockIdx.x);
+    const GemmProb &P = probs[p];
+    const int local = (int)blockIdx.x - start[p];
+    const int tm = local % P.gm, rest = local / P.gm;
+    const int tn = rest % P.gn, b = rest / P.gn;
+    const int cnt = gemm_prob_count(P);
+    int N = P.N;
+    if (P.cols_per_count > 0) {
+        N = P.cols_per_count * cnt;
+        if (b != 0) return;
+    } else if (b >= cnt) {
+        return;
+    }
+    if (tn * BT >= N) return;
+    const T *A = static_cast<const T *>(P.A) + P.sA * (int64_t)b;
+    const T *B = static_cast<const T *>(P.B) + P.sB * (int64_t)b;
+    T *C = static_cast<T *>(P.C) + P.sC * (int64_t)b;
+    if (P.epi) {
+        GammaEpi<T> ge{static_cast<const T *>(P.ep_gam), P.ep_atom,
+                       static_cast<const T *>(P.ep_u) + (int64_t)P.ep_na * b, P.ep_na,
+                       (T)P.ep_aJ, (T)P.ep_aK};
+        gemm_tile<MODE, TA, TB, BT, true>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb,
+                                          (T)P.beta, C, P.ldc, tm * BT, tn * BT, ge);
+    } else {
+        gemm_tile<MODE, TA, TB, BT>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb, (T)P.beta, C,
+                                    P.ldc, tm * BT, tn * BT);
+    }
 }
 
 template <int MODE, typename T>
@@ -337,6 +408,33 @@ void launch_gemm_f64(hipStream_t s, bool ta, bool tb, int M, int N, int K, doubl
                      int64_t sB, double beta, double *C, int64_t ldc, int64_t sC, int batch) {
     launch_gemm_mode<GEMM_F64>(s, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                                sC, batch);
+}
+
+void launch_gemm_grouped(hipStream_t s, int mode, bool ta, bool tb, int nprob,
+                         const GemmProb *probs, const int *start, int ntiles) {
+    if (ntiles <= 0 || nprob <= 0) return;
+#define STDA_GROUPED(MD)                                                                        \
+    if (!ta && !tb)                                                                             \
+        hipLaunchKernelGGL((gemm_grouped_kernel<MD, false, false>), dim3(ntiles), dim3(GTHREADS), \
+                           0, s, probs, start, nprob);                                                 \
+    else if (!ta && tb)                                                                         \
+        hipLaunchKernelGGL((gemm_grouped_kernel<MD, false, true>), dim3(ntiles), dim3(GTHREADS),  \
+                           0, s, probs, start, nprob);                                                 \
+    else if (ta && !tb)                                                                         \
+        hipLaunchKernelGGL((gemm_grouped_kernel<MD, true, false>), dim3(ntiles), dim3(GTHREADS),  \
+                           0, s, probs, start, nprob);                                                 \
+    else                                                                                        \
+        hipLaunchKernelGGL((gemm_grouped_kernel<MD, true, true>), dim3(ntiles), dim3(GTHREADS),   \
+                           0, s, probs, start, nprob);
+    if (mode == GEMM_F64) {
+        STDA_GROUPED(GEMM_F64)
+    } else if (mode == GEMM_BF16X3) {
+        STDA_GROUPED(GEMM_BF16X3)
+    } else {
+        STDA_GROUPED(GEMM_F32)
+    }
+#undef STDA_GROUPED
+    STDA_HIP_CHECK(hipGetLastError());
 }
 
 void launch_gemm_f32(hipStream_t s, int mode, bool ta, bool tb, int M, int N, int K, float alpha,

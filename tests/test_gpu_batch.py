@@ -60,8 +60,10 @@ def test_batch_matches_fp64(molecules, plain):
     say(f"batch of {len(molecules)} molecules, 4 workers")
     with StdaHipBatch(0, workers=4) as b:
         b.configure(nmax_batched=4096)     # the two ~80-atom molecules take the single path
+        b.configure_wide(False)
         res = b.solve_tda(molecules, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
         t, cnt = b.timings()
+        assert b.wide_count() == 0
     say(f"  paths {cnt}, timings {t}")
     assert cnt["batched"] >= 8 and cnt["single"] == 2
     for i, ((om, pr, st), ref) in enumerate(zip(res, plain)):
@@ -75,6 +77,45 @@ def test_batch_matches_fp64(molecules, plain):
             f"max|dfL|={np.abs(pr[:, 0] - rpr[:, 0]).max():.2e}")
         np.testing.assert_allclose(om, rom, atol=E_MIX, rtol=0)
         np.testing.assert_allclose(pr[:, :2], rpr[:, :2], atol=F_MIX, rtol=0)
+
+
+def test_wide_matches_fp64(molecules, plain):
+    """Matrix-free lock step: every molecule above 128 CSFs in the grouped
+    kernels (FP32 operator products, FP64 Davidson numerics and refinement)."""
+    from stda_amd.core import StdaHipBatch
+    say(f"wide lock step: {len(molecules)} molecules, nmax_batched 128")
+    with StdaHipBatch(0, workers=3) as b:
+        b.configure(nmax_batched=128)
+        res = b.solve_tda(molecules, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
+        t, cnt = b.timings()
+        nw = b.wide_count()
+    say(f"  paths {cnt}, wide {nw}, timings {t}")
+    assert nw >= 6 and cnt["single"] == 0
+    for i, ((om, pr, st), ref) in enumerate(zip(res, plain)):
+        if isinstance(ref, int):
+            assert st == ref, (i, st, ref)
+            continue
+        assert st == 0, (i, st)
+        rom, rpr, nci = ref
+        assert len(om) == len(rom) == min(20, nci)
+        say(f"  mol {i}: nci={nci} max|dE|={np.abs(om - rom).max() * EV:.2e} eV "
+            f"max|dfL|={np.abs(pr[:, 0] - rpr[:, 0]).max():.2e} "
+            f"max|dfV|={np.abs(pr[:, 1] - rpr[:, 1]).max():.2e}")
+        np.testing.assert_allclose(om, rom, atol=E_MIX, rtol=0)
+        np.testing.assert_allclose(pr[:, :2], rpr[:, :2], atol=F_MIX, rtol=0)
+
+
+def test_wide_alone_equals_in_batch(molecules):
+    from stda_amd.core import StdaHipBatch
+    with StdaHipBatch(0, workers=3) as b:
+        b.configure(nmax_batched=128)
+        mix = b.solve_tda(molecules, 7.0, 0.25, nroots=20)
+        for i in (1, len(molecules) - 1):
+            alone = b.solve_tda([molecules[i]], 7.0, 0.25, nroots=20)[0]
+            assert b.wide_count() == (1 if alone[2] == 0 else 0)
+            np.testing.assert_array_equal(alone[0], mix[i][0])
+            np.testing.assert_array_equal(alone[1], mix[i][1])
+            assert alone[2] == mix[i][2]
 
 
 def test_batch_matches_oracle(molecules):
