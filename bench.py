@@ -325,7 +325,7 @@ def run_mixed(args):
     log(f"# plain FP64 sequential: {plain_wall:.3f}s, {nok}/{n} molecules with CSFs")
     with StdaHipBatch(0, workers=args.workers) as b:
         b.configure(args.nmax_batched, args.poll)
-        b.configure_wide(not args.no_wide, two_phase=args.two_phase)
+        b.configure_wide(not args.no_wide, two_phase=args.two_phase, nmax_wide=args.nmax_wide)
         for w in range(args.warmup):
             b.solve_tda(cis, thr, ax, nroots)
             log(f"# warmup {w + 1}/{args.warmup} done")
@@ -398,7 +398,8 @@ def run_mixed(args):
         "config": {"workload": f"mixed{n}: {n} molecules, natoms log-uniform [10,200], minimal "
                                f"basis, Ethr 7 eV, ax 0.25, 20 roots each, batched mixed precision",
                    "molecules": n, "molecules_with_csf": nok, "workers": args.workers,
-                   "nmax_batched": args.nmax_batched, "poll": args.poll, "host": host_info(),
+                   "nmax_batched": args.nmax_batched, "nmax_wide": args.nmax_wide,
+                   "wide": not args.no_wide, "poll": args.poll, "host": host_info(),
                    "paths": dict(cnt, wide_of_batched=nwide), "phases_ms": t,
                    "phase_ms_summed": {k: round(v / 1e3, 2) for k, v in phase_us.items()
                                        if k not in ("sweeps", "unused")}},
@@ -549,12 +550,15 @@ def main():
     ap.add_argument("--workload", default="mixed512",
                     help="mixed512 (default), cluster400, cluster100, polyene, serve")
     ap.add_argument("--nmol", type=int, default=512)
-    ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--nmax-batched", type=int, default=1024,
                     help="CSF limit of the lock-step solver (0: every molecule on the worker path)")
     ap.add_argument("--no-wide", action="store_true",
                     help="molecules above --nmax-batched on the single-molecule path instead of "
                          "the matrix-free lock step")
+    ap.add_argument("--nmax-wide", type=int, default=16384,
+                    help="CSF limit of the matrix-free lock step (0: none; larger molecules take "
+                         "the single-molecule path)")
     ap.add_argument("--two-phase", action="store_true",
                     help="lock-step iterations start once every molecule is prepared")
     ap.add_argument("--poll", type=int, default=4,

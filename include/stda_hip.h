@@ -384,14 +384,15 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
 /* Molecules with at most nmax_batched CSFs go to the lock-step solver, whose
  * driver reads the state table back every `poll` iterations. */
 int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t poll);
-/* Larger molecules (more than nmax_batched CSFs, at most 24 roots and 1024
- * atoms) join the lock step too, with the matrix-free operator (enable = 1,
- * the default), or take the single-molecule mixed solve on their worker
- * (enable = 0).  enable = 2: as 1, with the lock-step iterations of a call
- * starting only once every molecule is prepared (setup and solve then do not
- * share the device).  No counterpart in the reference (its driver solves one
- * molecule per call: stda.f:53, main.f:928). */
-int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable);
+/* Larger molecules (more than nmax_batched and at most nmax_wide CSFs, at
+ * most 24 roots and 1024 atoms) join the lock step too, with the matrix-free
+ * operator (enable = 1, the default), or take the single-molecule mixed solve
+ * on their worker (enable = 0; so do the molecules above nmax_wide;
+ * nmax_wide <= 0: no limit).  enable = 2: as 1, with the lock-step iterations
+ * of a call starting only once every molecule is prepared (setup and solve
+ * then do not share the device).  No counterpart in the reference (its
+ * driver solves one molecule per call: stda.f:53, main.f:928). */
+int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable, int64_t nmax_wide);
 /* molecules of the last call solved in the matrix-free lock step (a subset of
  * counts3[0] of stda_hip_batch_timings) */
 int stda_hip_batch_wide_count(const stda_hip_batch *b, int64_t *nwide);

@@ -127,7 +127,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                         P, ctypes.POINTER(I32)]
     L.stda_hip_batch_stop.argtypes = [P]
     L.stda_hip_batch_configure.argtypes = [P, I64, I32]
-    L.stda_hip_batch_configure_wide.argtypes = [P, I32]
+    L.stda_hip_batch_configure_wide.argtypes = [P, I32, I64]
     L.stda_hip_batch_wide_count.argtypes = [P, ctypes.POINTER(I64)]
     L.stda_hip_batch_phase_times.argtypes = [P, P]
     L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
@@ -640,11 +640,13 @@ class StdaHipBatch:
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
 
-    def configure_wide(self, enable: bool = True, two_phase: bool = False):
-        """Molecules above the CSF limit in the matrix-free lock step (True)
-        or on the single-molecule path of their worker (False); two_phase:
-        the lock step of a call starts once every molecule is prepared."""
-        st = self._L.stda_hip_batch_configure_wide(self._b, (2 if two_phase else 1) if enable else 0)
+    def configure_wide(self, enable: bool = True, two_phase: bool = False, nmax_wide: int = 0):
+        """Molecules above the explicit-operator CSF limit (and at most
+        nmax_wide CSFs; 0: no limit) in the matrix-free lock step (True) or on
+        the single-molecule path of their worker (False); two_phase: the lock
+        step of a call starts once every molecule is prepared."""
+        st = self._L.stda_hip_batch_configure_wide(self._b, (2 if two_phase else 1) if enable else 0,
+                                                   int(nmax_wide))
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
 

@@ -270,14 +270,17 @@ void launch_batch_step(hipStream_t s, int nmol, const BatchMol *mols);
 // GC_FIXED uses `count`; GC_OP32 / GC_OP64 read the vector count of the FP32 /
 // FP64 operator product of a lock-step molecule from its state (wide_op_count).
 // cols_per_count > 0: one entry with N = cols_per_count * count columns;
-// otherwise count batch entries of N columns.
+// otherwise count batch entries of N columns, bpw of them in turn per
+// work-group (gb groups of them in the grid).
 enum GemmCount { GC_FIXED = 0, GC_OP32 = 1, GC_OP64 = 2 };
 struct GemmProb {
     int M, N, K;
     int cols_per_count;
     int kind, count;
-    int gm, gn, gb;            // 64 x 64 tile grid of the launch: rows, columns, batch entries
+    int gm, gn, gb;            // 64 x 64 tile grid of the launch: rows, columns, batch groups
+    int bpw;                   // batch entries per work-group (in turn), gb = ceil(max count / bpw)
     int epi;                   // 1: C(r, c) = aJ gam(A(r), A(c)) (alpha AB)(r, c) (+ aK u(A(c)), r = c)
+    int pad2;
     const int *state;
     const void *A, *B;
     void *C;

@@ -3190,6 +3190,7 @@ struct stda_hip_batch {
     int64_t nmax_batched = 1024;         // CSF limit of the explicit-operator lock step
     bool wide = true;                    // larger molecules in the matrix-free lock step
     bool two_phase = false;              // lock step only after the setup of every molecule
+    int64_t nmax_wide = INT64_MAX;       // CSF limit of the matrix-free lock step
     int group = 4;                       // iterations between polls of the state table
     // ms: worker phase (wall), driver loop (wall), property tables, total,
     // summed HIP-event time of the batched launches (matvec + step)
@@ -3369,23 +3370,34 @@ void wide_gemm_probs(const WideMol &d, GemmProb (&P)[8]) {
         void *D = f64 ? (void *)d.d64 : (void *)d.d32;
         void *G = f64 ? (void *)d.g64 : (void *)d.d32;   // FP32: G in place of D
         void *R = f64 ? (void *)d.r64 : (void *)d.r32;
+        // FP64 (a few products per molecule): 4 vectors in turn per work-group,
+        // so that the launch stays small while no molecule needs it
+        const int bpw = f64 ? 4 : 1, gbat = (cap + bpw - 1) / bpw;
         GemmProb base{};
+        base.bpw = bpw;
         base.kind = f64 ? GC_OP64 : GC_OP32;
         base.state = d.state;
         base.alpha = 1.0;
         base.beta = 0.0;
-        // T = Cv X^T: nao x (no * count), K = nv
+        // T = Cv X^T: nao x (no * count), K = nv (FP64: per vector)
         GemmProb g1 = base;
         g1.M = nao; g1.N = no; g1.K = nv; g1.cols_per_count = no;
         g1.A = Cv; g1.lda = nao; g1.B = xf; g1.ldb = nv; g1.C = t; g1.ldc = nao;
         g1.gm = tiles(nao); g1.gn = tiles((int64_t)no * cap); g1.gb = 1;
+        if (f64) {
+            g1.cols_per_count = 0;
+            g1.sB = (int64_t)nv * no;
+            g1.sC = (int64_t)nao * no;
+            g1.gn = tiles(no);
+            g1.gb = gbat;
+        }
         // D_v = T_v Co^T: nao x nao, K = no, per vector.  FP32 (operator A):
         // stored as G = -gamJ o D + ak diag(u) (epilogue); FP64: D itself
         GemmProb g2 = base;
         g2.M = nao; g2.N = nao; g2.K = no;
         g2.A = t; g2.lda = nao; g2.sA = (int64_t)nao * no;
         g2.B = Co; g2.ldb = nao; g2.C = D; g2.ldc = nao; g2.sC = (int64_t)nao * nao;
-        g2.gm = tiles(nao); g2.gn = tiles(nao); g2.gb = cap;
+        g2.gm = tiles(nao); g2.gn = tiles(nao); g2.gb = gbat;
         if (!f64) {
             g2.epi = 1;
             g2.ep_gam = d.gj32;
@@ -3400,12 +3412,19 @@ void wide_gemm_probs(const WideMol &d, GemmProb (&P)[8]) {
         g3.M = nao; g3.N = no; g3.K = nao;
         g3.A = G; g3.lda = nao; g3.sA = (int64_t)nao * nao;
         g3.B = Co; g3.ldb = nao; g3.C = t; g3.ldc = nao; g3.sC = (int64_t)nao * no;
-        g3.gm = tiles(nao); g3.gn = tiles(no); g3.gb = cap;
+        g3.gm = tiles(nao); g3.gn = tiles(no); g3.gb = gbat;
         // R = Cv^T W: nv x (no * count), K = nao
         GemmProb g4 = base;
         g4.M = nv; g4.N = no; g4.K = nao; g4.cols_per_count = no;
         g4.A = Cv; g4.lda = nao; g4.B = t; g4.ldb = nao; g4.C = R; g4.ldc = nv;
         g4.gm = tiles(nv); g4.gn = tiles((int64_t)no * cap); g4.gb = 1;
+        if (f64) {
+            g4.cols_per_count = 0;
+            g4.sB = (int64_t)nao * no;
+            g4.sC = (int64_t)nv * no;
+            g4.gn = tiles(no);
+            g4.gb = gbat;
+        }
         P[0 + prec] = g1;
         P[2 + prec] = g2;
         P[4 + prec] = g3;
@@ -3431,7 +3450,7 @@ void build_bhalf_device(stda_hip_ctx *c, DBuf<double> &B) {
 
 // one molecule of the setup phase; returns true when it goes to the batched solver
 bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_params *par,
-                   int nroots, double tol, int velcorr, int64_t nmax, bool wide, BatchItem &it,
+                   int nroots, double tol, int velcorr, int64_t nmax, int64_t nmax_wide, BatchItem &it,
                    int32_t *nfound, double *omega, double *props, double *us3) {
     Timer t0;
     do_load(c, sys, par);
@@ -3441,7 +3460,7 @@ bool batch_prepare(stda_hip_ctx *c, const stda_hip_system *sys, const stda_hip_p
     const int64_t n = c->nci;
     const int k = (int)std::min<int64_t>(nroots, n);
     hipStream_t s = c->stream;
-    if (n > nmax && wide && k <= BATCH_KMAX && c->ncent <= WIDE_MAXATOMS) {
+    if (n > nmax && n <= nmax_wide && k <= BATCH_KMAX && c->ncent <= WIDE_MAXATOMS) {
         wide_prepare(c, k, velcorr != 0, it);
         us3[1] += 1e3 * t1.ms();
         return true;
@@ -3542,10 +3561,26 @@ struct LockstepSet {
     int gemm32 = GEMM_F32;
     hipGraphExec_t gexec = nullptr;
 
-    LockstepSet() = default;
+    // the explicit-operator molecules run on a side branch of the graph, beside the wide ones
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+
+    LockstepSet() {
+        STDA_HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        STDA_HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        STDA_HIP_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    }
     LockstepSet(const LockstepSet &) = delete;
     LockstepSet &operator=(const LockstepSet &) = delete;
-    ~LockstepSet() { drop_graph(); }
+    ~LockstepSet() {
+        drop_graph();
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+    }
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
@@ -3617,11 +3652,18 @@ struct LockstepSet {
         }
     }
 
-    // one lock-step iteration of both kinds
+    // one lock-step iteration of both kinds (the explicit-operator molecules
+    // on the side stream when there are wide ones: two branches of the graph)
     void iteration(hipStream_t ds) {
+        const bool both = nsmall && nwide;
+        if (both) {
+            STDA_HIP_CHECK(hipEventRecord(fork, ds));
+            STDA_HIP_CHECK(hipStreamWaitEvent(side, fork, 0));
+        }
         if (nsmall) {
-            launch_batch_matvec(ds, ntiles, descd.get(), tilesd.get());
-            launch_batch_step(ds, nsmall, descd.get());
+            hipStream_t ss = both ? side : ds;
+            launch_batch_matvec(ss, ntiles, descd.get(), tilesd.get());
+            launch_batch_step(ss, nsmall, descd.get());
         }
         if (nwide) {
             const WideMol *w = wdescd.get();
@@ -3643,6 +3685,10 @@ struct LockstepSet {
             gemm(7, GEMM_F64, true, false);
             launch_wide_gather(ds, nwide, w, ws);
             launch_wide_step(ds, nwide, w, ws);
+        }
+        if (both) {
+            STDA_HIP_CHECK(hipEventRecord(join, side));
+            STDA_HIP_CHECK(hipStreamWaitEvent(ds, join, 0));
         }
     }
 
@@ -3716,10 +3762,11 @@ int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t po
     return STDA_HIP_OK;
 }
 
-int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable) {
+int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable, int64_t nmax_wide) {
     if (!b || enable < 0 || enable > 2) return STDA_HIP_ERR_ARG;
     b->wide = enable != 0;
     b->two_phase = enable == 2;
+    b->nmax_wide = nmax_wide > 0 ? nmax_wide : INT64_MAX;
     return STDA_HIP_OK;
 }
 
@@ -3844,7 +3891,7 @@ int stda_hip_batch_solve_tda(stda_hip_batch *b, int32_t nmol, const stda_hip_sys
                     const int i = order[q];
                     try {
                         batched[i] = batch_prepare(c, &sys[i], &par[i], nroots, tol, velcorr,
-                                                   b->nmax_batched, b->wide, items[i], &nfound[i],
+                                                   b->nmax_batched, b->wide ? b->nmax_wide : 0, items[i], &nfound[i],
                                                    omega + (size_t)i * nroots,
                                                    props + (size_t)i * nroots * 5, wus[wi].data());
                         status[i] = STDA_HIP_OK;
@@ -4097,7 +4144,7 @@ void stream_worker(stda_hip_batch *b, BatchStream *E, int wi) {
         try {
             double us3[3] = {0, 0, 0};
             batched = batch_prepare(c, &t->sys, &t->par, t->nroots, t->tol, t->velcorr,
-                                    b->nmax_batched, b->wide, t->item, &t->nfound, t->omega.data(),
+                                    b->nmax_batched, b->wide ? b->nmax_wide : 0, t->item, &t->nfound, t->omega.data(),
                                     t->props.data(), us3);
             t->status = STDA_HIP_OK;
             if (batched) {

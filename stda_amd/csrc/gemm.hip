@@ -253,43 +253,53 @@ gemm_kernel(int M, int N, int K, typename GemmTraits<MODE>::T alpha,
 }
 
 // Grouped form: the work-groups start[p] .. start[p + 1] compute the 64 x 64
-// tiles of problem p (tile row fastest, then tile column, then batch entry).
+// tiles of problem p (tile row fastest, then tile column, then batch group).
 // A problem's column or batch count can follow the state of a lock-step
 // molecule (GemmProb::kind), so that one captured launch serves every
 // iteration; tiles beyond the current count return at once.
+template <int MODE, bool TA, bool TB, int BT>
+__device__ __forceinline__ void grouped_tile(GemmSmem<MODE, BT> &sm, const GemmProb &P, int local) {
+    using T = typename GemmTraits<MODE>::T;
+    const int tm = local % P.gm, rest = local / P.gm;
+    const int tn = rest % P.gn, bg = rest / P.gn;
+    const int cnt = gemm_prob_count(P);
+    int N = P.N, b0 = bg, b1 = bg + 1;
+    if (P.cols_per_count > 0) {
+        N = P.cols_per_count * cnt;
+        if (bg != 0) return;
+    } else {   // batch entries bg * bpw .. + bpw of this work-group, in turn
+        b0 = bg * P.bpw;
+        b1 = min(cnt, b0 + P.bpw);
+        if (b0 >= b1) return;
+    }
+    if (tn * BT >= N) return;
+    for (int b = b0; b < b1; ++b) {
+        const T *A = static_cast<const T *>(P.A) + P.sA * (int64_t)b;
+        const T *B = static_cast<const T *>(P.B) + P.sB * (int64_t)b;
+        T *C = static_cast<T *>(P.C) + P.sC * (int64_t)b;
+        if (b > b0) __syncthreads();   // LDS of the previous entry consumed
+        if (P.epi) {
+            GammaEpi<T> ge{static_cast<const T *>(P.ep_gam), P.ep_atom,
+                           static_cast<const T *>(P.ep_u) + (int64_t)P.ep_na * b, P.ep_na,
+                           (T)P.ep_aJ, (T)P.ep_aK};
+            gemm_tile<MODE, TA, TB, BT, true>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb,
+                                              (T)P.beta, C, P.ldc, tm * BT, tn * BT, ge);
+        } else {
+            gemm_tile<MODE, TA, TB, BT>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb,
+                                        (T)P.beta, C, P.ldc, tm * BT, tn * BT);
+        }
+    }
+}
+
+// 64 x 64 tiles throughout: on the 300-600 AO transforms of the lock step the
+// edge waste of 128-tiles, and their LDS, cost more than they gain (measured).
 template <int MODE, bool TA, bool TB>
 __global__ void __launch_bounds__(GTHREADS, 2)
 gemm_grouped_kernel(const GemmProb *__restrict__ probs, const int *__restrict__ start, int nprob) {
-    using T = typename GemmTraits<MODE>::T;
-    constexpr int BT = 64;
-    __shared__ __align__(16) GemmSmem<MODE, BT> sm;
+    __shared__ __align__(16) GemmSmem<MODE, 64> sm;
     const int p = grouped_item(start, nprob, (int)blockIdx.x);
     const GemmProb &P = probs[p];
-    const int local = (int)blockIdx.x - start[p];
-    const int tm = local % P.gm, rest = local / P.gm;
-    const int tn = rest % P.gn, b = rest / P.gn;
-    const int cnt = gemm_prob_count(P);
-    int N = P.N;
-    if (P.cols_per_count > 0) {
-        N = P.cols_per_count * cnt;
-        if (b != 0) return;
-    } else if (b >= cnt) {
-        return;
-    }
-    if (tn * BT >= N) return;
-    const T *A = static_cast<const T *>(P.A) + P.sA * (int64_t)b;
-    const T *B = static_cast<const T *>(P.B) + P.sB * (int64_t)b;
-    T *C = static_cast<T *>(P.C) + P.sC * (int64_t)b;
-    if (P.epi) {
-        GammaEpi<T> ge{static_cast<const T *>(P.ep_gam), P.ep_atom,
-                       static_cast<const T *>(P.ep_u) + (int64_t)P.ep_na * b, P.ep_na,
-                       (T)P.ep_aJ, (T)P.ep_aK};
-        gemm_tile<MODE, TA, TB, BT, true>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb,
-                                          (T)P.beta, C, P.ldc, tm * BT, tn * BT, ge);
-    } else {
-        gemm_tile<MODE, TA, TB, BT>(sm, P.M, N, P.K, (T)P.alpha, A, P.lda, B, P.ldb, (T)P.beta, C,
-                                    P.ldc, tm * BT, tn * BT);
-    }
+    grouped_tile<MODE, TA, TB, 64>(sm, P, (int)blockIdx.x - start[p]);
 }
 
 template <int MODE, typename T>

@@ -95,11 +95,11 @@ __device__ void block_sum(BatchSmem &sm, double (&v)[NV], int nv, double *out) {
 // LDX), m <= MDX: cyclic Jacobi (two-sided rotations, Golub & Van Loan,
 // Matrix Computations, Alg. 8.5.2/8.5.3) with the round-robin schedule, the
 // m/2 disjoint rotations of a step applied in parallel.  A rotation is
-// skipped when |a_pq| < 1e-17 (|a_pp| + |a_qq|); the sweeps stop when a whole
-// sweep rotates nothing.  Out: sm.theta (Jacobi order), eigenvectors in the
+// skipped when |a_pq| < rtol (|a_pp| + |a_qq|) (rtol = 1e-17 unless given); the
+// sweeps stop when a whole sweep rotates nothing.  Out: sm.theta (Jacobi order), eigenvectors in the
 // columns of sm.y, sm.perm = indices in ascending eigenvalue order.
 template <int MDX>
-__device__ int jacobi_t(LockSmem<MDX> &sm, int m) {
+__device__ int jacobi_t(LockSmem<MDX> &sm, int m, double rtol = 1.0e-17) {
     constexpr int LDX = LockSmem<MDX>::LDX;
     const int tid = threadIdx.x;
     for (int e = tid; e < LDX * MDX; e += BT) sm.y[e] = (e % LDX == e / LDX) ? 1.0 : 0.0;
@@ -124,7 +124,7 @@ __device__ int jacobi_t(LockSmem<MDX> &sm, int m) {
                 if (b < m) {
                     const double app = sm.u[a + LDX * a], aqq = sm.u[b + LDX * b];
                     const double apq = sm.u[a + LDX * b];
-                    if (fabs(apq) > 1e-300 && fabs(apq) >= 1e-17 * (fabs(app) + fabs(aqq))) {
+                    if (fabs(apq) > 1e-300 && fabs(apq) >= rtol * (fabs(app) + fabs(aqq))) {
                         const double tau = (aqq - app) / (2.0 * apq);
                         t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);

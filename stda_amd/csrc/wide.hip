@@ -536,7 +536,7 @@ __global__ __launch_bounds__(BT) void wide_sub_kernel(const WideMol *__restrict_
         }
     }
     __syncthreads();
-    const int sw = jacobi_t<WMD>(sm, m);
+    const int sw = jacobi_t<WMD>(sm, m, 1.0e-14);
     const int kk = min(M.k, m);
     if (tid < kk) aux[WA_TH + tid] = sm.theta[sm.perm[tid]];
     if (ph == B_ITER) {
@@ -659,35 +659,40 @@ __global__ __launch_bounds__(WT) void wide_ctl_kernel(const WideMol *__restrict_
         if (tid == 0) st[WS_NEXT] = M.mom ? B_REFINE : B_DONE;   // moments follow
         return;
     }
-    if (tid != 0) return;
-    const int m = st[1], it = st[3] + 1;
-    int nun = 0;
-    double rmax = 0.0;
-    for (int c = 0; c < kk; ++c) {
-        rmax = fmax(rmax, rn[c]);
-        if (!(rn[c] < M.tol)) st[WS_UN + nun++] = c;
+    __shared__ int rs_flag;
+    if (tid == 0) {
+        const int m = st[1], it = st[3] + 1;
+        int nun = 0;
+        double rmax = 0.0;
+        for (int c = 0; c < kk; ++c) {
+            rmax = fmax(rmax, rn[c]);
+            if (!(rn[c] < M.tol)) st[WS_UN + nun++] = c;
+        }
+        const bool finish = (nun == 0 && kk == M.k) || m == M.n || it >= M.maxiter;
+        st[3] = it;
+        info[0] = (double)it;
+        info[3] = rmax;
+        st[WS_NUN] = nun;
+        rs_flag = 0;
+        if (finish) {
+            info[1] = (nun == 0 || m == M.n) ? 1.0 : 0.0;
+            info[2] = (double)kk;
+            st[WS_FINISH] = 1;
+            st[WS_NEXT] = B_REFINE;
+        } else {
+            const bool restart = m + nun > M.maxdim;
+            const int mb = restart ? kk : m;
+            st[WS_RESTART] = restart ? 1 : 0;
+            if (restart) st[WS_MP] = 0;   // basis = Ritz vectors, H diagonal: no rotation needed
+            st[1] = mb;
+            st[WS_ROOM] = min(nun, M.maxdim - mb);
+            st[WS_NEXT] = B_ITER;
+            rs_flag = restart ? 1 : 0;
+        }
     }
-    const bool finish = (nun == 0 && kk == M.k) || m == M.n || it >= M.maxiter;
-    st[3] = it;
-    info[0] = (double)it;
-    info[3] = rmax;
-    st[WS_NUN] = nun;
-    if (finish) {
-        info[1] = (nun == 0 || m == M.n) ? 1.0 : 0.0;
-        info[2] = (double)kk;
-        st[WS_FINISH] = 1;
-        st[WS_NEXT] = B_REFINE;
-        return;
-    }
-    const bool restart = m + nun > M.maxdim;
-    const int mb = restart ? kk : m;
-    st[WS_RESTART] = restart ? 1 : 0;
-    if (restart) st[WS_MP] = 0;   // basis = Ritz vectors, H diagonal: no rotation needed
-    st[1] = mb;
-    st[WS_ROOM] = min(nun, M.maxdim - mb);
-    st[WS_NEXT] = B_ITER;
-    if (restart)
-        for (int e = 0; e < WMD * WMD; ++e) {
+    __syncthreads();
+    if (rs_flag)
+        for (int e = tid; e < WMD * WMD; e += WT) {
             const int i = e % WMD, j = e / WMD;
             M.H[e] = (i == j && i < kk) ? M.aux[WA_TH + i] : 0.0;
         }
