@@ -40,6 +40,10 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
+# HBM bytes per launch of PT2 pass 1 over the mixed512 molecules, from the
+# FETCH_SIZE / WRITE_SIZE counter runs of profiles/pt2_pipe_mixed512_pmc.md
+# (copy-calibrated); None until measured
+PT2_MIXED_TRAFFIC = None
 HBM_PEAK_GBS = 8000.0
 # HIP-event timed statistics of the library: single kernels (one launch per
 # record: the PT2 row pass, the lock-step operator kernel, the slice partials
@@ -226,7 +230,7 @@ def run_single(args):
 
 
 # --------------------------------------------------------------------------- batched
-PLAIN_KERNELS = ("tsgemm_tn", "tsgemm_nn")
+PLAIN_KERNELS = ("tsgemm_tn", "tsgemm_nn", "pt2_rows")
 
 
 def plain_fp64(cis, thr, ax, nroots, kst=None):
@@ -258,16 +262,22 @@ def plain_fp64(cis, thr, ax, nroots, kst=None):
 
 
 def plain_rates(kst):
-    """Per-launch HIP-event time and achieved HBM rate of the tall-skinny
-    products in the sequential plain pass (algorithmic bytes 8 n (m + p))."""
+    """Per-launch HIP-event time and achieved rate of the kernels timed in the
+    sequential plain pass: the tall-skinny products (HBM, algorithmic bytes
+    8 n (m + p)) and PT2 pass 1 (FP64 MFMA, 4 Na Nsec Nprim flops)."""
     out = {}
     for k, st in kst.items():
         if st["ms"] <= 0:
             continue
-        gbs = st["work"] / (st["ms"] * 1e-3) / 1e9
-        out[k] = {"ms": round(st["ms"], 2), "launches": int(st["launches"]),
-                  "avg_launch_ms": round(st["ms"] / max(st["launches"], 1.0), 4),
-                  "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        d = {"ms": round(st["ms"], 2), "launches": int(st["launches"]),
+             "avg_launch_ms": round(st["ms"] / max(st["launches"], 1.0), 4)}
+        if k in HBM_KERNELS:
+            gbs = st["work"] / (st["ms"] * 1e-3) / 1e9
+            d.update(achieved_GBps=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+        else:
+            tf = st["work"] / (st["ms"] * 1e-3) / 1e12
+            d.update(achieved_TFLOPs=round(tf, 3), frac=round(tf / FP64_PEAK_TFLOPS, 4))
+        out[k] = d
     return out
 
 
@@ -351,27 +361,23 @@ def run_mixed(args):
     except Exception as exc:
         prep = {"error": repr(exc)}
     t, cnt = phases[-1]
-    # dominant single kernel of the timed steps (HIP-event time on the
-    # launching stream, summed over the workers; with concurrent workers a
-    # launch's time includes sharing the device, so these rates are lower
-    # bounds).  The operator products and pass 2 of PT2 are several launches
-    # each: listed, never taken as the dominant kernel.
-    dom = max(SINGLE_KERNELS, key=lambda k: kst[k]["ms"])
-    st = kst[dom]
-    per = st["ms"] / max(st["launches"], 1.0)
-    if dom in HBM_KERNELS:
-        achieved = st["work"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0.0
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
-    else:
-        achieved = st["work"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
-        roofline = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
-                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4)}
-    roofline.update({"traffic": None, "avg_launch_ms": round(per, 4),
-                     "work_per_launch": st["work"] / max(st["launches"], 1.0),
-                     "kernels": {k: {"ms": round(v["ms"], 2), "launches": int(v["launches"])}
-                                 for k, v in kst.items()}})
+    # Roofline: PT2 pass 1 (pt2_pipe_kernel<0>), the kernel with the largest
+    # summed device time in the batched step (rocprof, profiles/).  Its rate is
+    # taken from the sequential plain FP64 pass over the same molecules, where
+    # one stream runs at a time and an event pair brackets the kernel alone;
+    # in the batched step 16 worker streams and the lock-step graph share the
+    # device, so the workers' event times there (listed under "batched") also
+    # hold the waits for it.  Algorithmic work: 4 Na Nsec Nprim flops.
+    pst = plain_kst.get("pt2_rows", {"ms": 0.0, "launches": 0.0, "work": 0.0})
+    achieved = pst["work"] / (pst["ms"] * 1e-3) / 1e12 if pst["ms"] > 0 else 0.0
+    roofline = {"kernel": "pt2_rows (pt2_pipe_kernel<0>, PT2 pass 1)", "bound": "mfma",
+                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": PT2_MIXED_TRAFFIC,
+                "avg_launch_ms": round(pst["ms"] / max(pst["launches"], 1.0), 4),
+                "work_per_launch": pst["work"] / max(pst["launches"], 1.0),
+                "measured_in": "sequential plain FP64 pass (one stream)",
+                "batched": {k: {"ms": round(v["ms"], 2), "launches": int(v["launches"])}
+                            for k, v in kst.items()}}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_batch(cis, thr, ax, nroots, args.cpu_budget)
