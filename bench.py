@@ -40,10 +40,10 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
-# HBM bytes per launch of PT2 pass 1 over the mixed512 molecules, from the
-# FETCH_SIZE / WRITE_SIZE counter runs of profiles/pt2_pipe_mixed512_pmc.md
-# (copy-calibrated); None until measured
-PT2_MIXED_TRAFFIC = None
+# HBM bytes per launch of PT2 pass 1 over the mixed512 molecules in the plain
+# pass: FETCH_SIZE (x2) + WRITE_SIZE (x1) counters, scaled by a 1 GiB copy
+# (profiles/pt2_pipe_mixed512_pmc.md)
+PT2_MIXED_TRAFFIC = 7.12e8
 HBM_PEAK_GBS = 8000.0
 # HIP-event timed statistics of the library: single kernels (one launch per
 # record: the PT2 row pass, the lock-step operator kernel, the slice partials
@@ -325,14 +325,31 @@ def run_mixed(args):
     from stda_amd.core import StdaHipBatch
     n, thr, ax, nroots = args.nmol, 7.0, 0.25, 20
     t0 = time.perf_counter()
-    cis = synth.mixed_batch(n, lo=10, hi=200, base_seed=1000, basis="minimal", thr=thr, ax=ax,
-                            workers=gen_workers())
+    cis = None
+    cache = args.inputs_cache
+    if cache and os.path.exists(cache):
+        # written by this script (below); the inputs are plain arrays
+        import pickle
+        with open(cache, "rb") as f:
+            cis = pickle.load(f)
+    if cis is None or len(cis) != n:
+        cis = synth.mixed_batch(n, lo=10, hi=200, base_seed=1000, basis="minimal", thr=thr, ax=ax,
+                                workers=gen_workers())
+        if cache:
+            import pickle
+            with open(cache, "wb") as f:
+                pickle.dump(cis, f)
     log(f"# workload mixed{n}: {n} molecules, natoms {min(c.ncent for c in cis)}-"
         f"{max(c.ncent for c in cis)} (inputs {time.perf_counter() - t0:.1f}s)")
+    if args.inputs_only:
+        return None
     plain_kst = {}
     ref, plain_wall, plain_lat = plain_fp64(cis, thr, ax, nroots, plain_kst)
     nok = sum(1 for r in ref if not isinstance(r, int))
     log(f"# plain FP64 sequential: {plain_wall:.3f}s, {nok}/{n} molecules with CSFs")
+    if args.plain_only:   # for counter profiles of the plain pass (one thread, one stream)
+        return {"plain_fp64": {"value": nok / plain_wall, "unit": "solves/s",
+                               "kernels": plain_rates(plain_kst)}}
     with StdaHipBatch(0, workers=args.workers) as b:
         b.configure(args.nmax_batched, args.poll)
         b.configure_wide(not args.no_wide, two_phase=args.two_phase, nmax_wide=args.nmax_wide)
@@ -548,6 +565,44 @@ def run_serve(args):
     }
 
 
+def rocprof_summary(db_path: str, top: int = 25):
+    """Per-kernel statistics of a `rocprofv3 --kernel-trace` database
+    (rocpd SQLite) of a bench run, as text: the whole process, the plain FP64
+    pass (the stream of the first PT2 pass-1 launch, up to its last launch)
+    and the rest (the batched steps)."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    cur = con.cursor()
+    rows = cur.execute("select name, stream_id, start, end from kernels order by start").fetchall()
+    if not rows:
+        return "no kernels"
+    first = next((r for r in rows if "pt2_pipe_kernel" in r[0]), None)
+    plain_sid = first[1] if first else None
+    plain_end = max((r[3] for r in rows if r[1] == plain_sid), default=0) if first else 0
+
+    def table(sel, title):
+        agg = {}
+        for n, _, s0, e0 in sel:
+            k = n.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+            a = agg.setdefault(k, [0, 0.0, float("inf"), 0.0])
+            d = (e0 - s0) / 1e3
+            a[0] += 1
+            a[1] += d
+            a[2] = min(a[2], d)
+            a[3] = max(a[3], d)
+        out = [f"## {title}", "", "| kernel | launches | total ms | avg us | min us | max us |",
+               "|---|---|---|---|---|---|"]
+        for k, a in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+            out.append(f"| `{k}` | {a[0]} | {a[1] / 1e3:.2f} | {a[1] / a[0]:.1f} | {a[2]:.1f} | {a[3]:.1f} |")
+        return "\n".join(out)
+
+    plain = [r for r in rows if r[1] == plain_sid and r[3] <= plain_end]
+    rest = [r for r in rows if not (r[1] == plain_sid and r[3] <= plain_end)]
+    return "\n\n".join([table(rows, "whole process"),
+                          table(plain, "plain FP64 pass (one stream)"),
+                          table(rest, "batched steps (worker streams and the lock-step graph)")])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -576,9 +631,21 @@ def main():
                     help="single-molecule workloads only")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates HBM counters)")
+    ap.add_argument("--inputs-cache", default="",
+                    help="mixed workload: read the generated molecules from this file, or write "
+                         "them there (a run under a counter profile cannot fork the generator)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="mixed workload: only the sequential plain FP64 pass (for counter profiles)")
+    ap.add_argument("--inputs-only", action="store_true",
+                    help="with --inputs-cache: generate and write the molecules, then exit")
+    ap.add_argument("--rocprof-summary", default="",
+                    help="print per-kernel statistics of a rocprofv3 kernel-trace database and exit")
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only run the operator-shape GEMMs (library kernel and vendor GEMM)")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        print(rocprof_summary(args.rocprof_summary))
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -593,6 +660,8 @@ def main():
         if args.workload != "mixed512":
             args.nmol = int(args.workload[5:])
         line = run_mixed(args)
+        if line is None:   # --inputs-only
+            return
     elif args.workload == "serve":
         line = run_serve(args)
     else:
