@@ -106,36 +106,28 @@ struct TileSet {
     const int32_t *vir;   // ntiles x T_vir virtual indices (-1 = none)
     int ntiles;
 };
-// PT2 pass 1: row sums of pt over all primaries for every secondary row tile.
-void launch_pt2_rows(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
-                     const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak,
-                     double *rowsum);
-// PT2 pass 2: column sums of pt over the rows selected by rowmask/rowbit.
-void launch_pt2_cols(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
-                     const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak,
-                     double *colsum);
-// The same two passes on the matrix cores (FP64 MFMA) with 8 x 8 x 8 x 8 tiles:
-// rows/cols tile lists hold PT2_T occupied and PT2_T virtual indices each.
+// PT2 on the matrix cores (FP64 MFMA) with 8 x 8 x 8 x 8 tiles: rows/cols
+// tile lists hold PT2_T occupied and PT2_T virtual indices each.
 constexpr int PT2_T = 8;
-void launch_pt2_rows_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                          const double *pK, const double *q, const double *pJ, const double *qab,
-                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                          const unsigned char *colmask, unsigned char colbit, double ak,
-                          double *rowsum);
-void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                          const double *pK, const double *q, const double *pJ, const double *qab,
-                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                          const unsigned char *colmask, unsigned char colbit, double ak,
-                          double *colsum);
-// Software-pipelined PT2 passes (mode 0: row sums, 1: column sums).
-void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int ncent, int nv,
+constexpr int PT2_SLOTS = PT2_T * PT2_T;
+// Pass 1: for every secondary of the row tiles its sum of pt over all
+// primaries.  drow/dcol (PT2_SLOTS per tile) hold the de of each tile slot,
+// NaN where the slot is not a secondary (rows) or primary (cols).  The column
+// tiles are split over nsplit work-groups per row tile; rowpart holds
+// nsplit x rows.ntiles x PT2_SLOTS partial sums, summed in split order into
+// rowsum[pair] for the secondaries.  With ptm non-null every pt term is also
+// stored at ptm[(PT2_SLOTS rt + row slot) * ldp + PT2_SLOTS ct + col slot]
+// (0 for slots that are not a secondary-primary pair), ldp = PT2_SLOTS *
+// cols.ntiles.
+void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak, double *out);
+                     const double *drow, const double *dcol, double ak, int nsplit,
+                     double *rowpart, double *rowsum, double *ptm, int64_t ldp);
+// Pass 2 from the stored terms: colsum[c] = sum over the rows rrow[0..nrej)
+// (ascending) of ptm[r * ldp + c] for every column c < ldp, the rows split
+// over nsplit work-groups per column block and the splits summed in order.
+void launch_pt2_colsum(hipStream_t s, int nrej, const int32_t *rrow, int64_t ldp,
+                       const double *ptm, int nsplit, double *part, double *colsum);
 // PT2 pass 2 over a sparse row set: row tile = one occupied k (rk) with 16
 // virtual c (rc, -1 padded); column tile = 16 occupied j (cj) x 16 virtual b (cb).
 void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int32_t *rc, int nct,

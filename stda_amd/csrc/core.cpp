@@ -14,6 +14,7 @@
 #include <mutex>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <optional>
@@ -51,6 +52,11 @@ static const double kEta[94] = {
 
 constexpr double kHartreeEv = 27.211385050;   // stda.f:142
 constexpr double kRot = 235.7220;             // stda.f:1300
+// PT2 pass 1: work-groups resident at a time (256 compute units x 2 of the
+// kernel's 77 KiB LDS footprint), the unit of its split choice.
+constexpr int kPt2Resident = 512;
+// PT2: largest matrix of pass-1 terms kept in device memory for pass 2.
+constexpr size_t kPt2StoreBytes = size_t(2) << 30;
 
 // Symmetric eigensolver for the small subspace problems.  The image's LAPACK
 // (MKL's dsyevd) is loaded at run time by absolute path with RTLD_LOCAL, so
@@ -1551,35 +1557,82 @@ static void do_select(stda_hip_ctx *c) {
     std::vector<double> pt2(c->npair, 0.0);
     std::vector<unsigned char> incl(c->npair, 0);
     if (c->nsec > 0) {
-        // tiles of the (k,c) grid holding secondaries and of the (j,b) grid holding primaries
-        auto make_tiles = [&](int tocc, int tvir, unsigned char bit, std::vector<int32_t> &occ,
-                              std::vector<int32_t> &vir) {
+        // tiles of the (k,c) grid holding secondaries and of the (j,b) grid
+        // holding primaries; tile_of maps a block (k / PT2_T, c / PT2_T) to its tile
+        const int nvb = (nv + PT2_T - 1) / PT2_T;
+        auto make_tiles = [&](unsigned char bit, std::vector<int32_t> &occ,
+                              std::vector<int32_t> &vir, std::vector<int32_t> &tile_of) {
+            tile_of.assign((size_t)((no + PT2_T - 1) / PT2_T) * nvb, -1);
             int nt = 0;
-            for (int k0 = 0; k0 < no; k0 += tocc)
-                for (int c0 = 0; c0 < nv; c0 += tvir) {
+            for (int k0 = 0; k0 < no; k0 += PT2_T)
+                for (int c0 = 0; c0 < nv; c0 += PT2_T) {
                     bool any = false;
-                    for (int k = k0; k < std::min(no, k0 + tocc) && !any; ++k)
-                        for (int cc = c0; cc < std::min(nv, c0 + tvir); ++cc)
+                    for (int k = k0; k < std::min(no, k0 + PT2_T) && !any; ++k)
+                        for (int cc = c0; cc < std::min(nv, c0 + PT2_T); ++cc)
                             if (fl[(int64_t)k * nv + cc] & bit) { any = true; break; }
                     if (!any) continue;
-                    for (int k = 0; k < tocc; ++k) occ.push_back(k0 + k < no ? k0 + k : -1);
-                    for (int cc = 0; cc < tvir; ++cc) vir.push_back(c0 + cc < nv ? c0 + cc : -1);
+                    tile_of[(size_t)(k0 / PT2_T) * nvb + c0 / PT2_T] = nt;
+                    for (int k = 0; k < PT2_T; ++k) occ.push_back(k0 + k < no ? k0 + k : -1);
+                    for (int cc = 0; cc < PT2_T; ++cc) vir.push_back(c0 + cc < nv ? c0 + cc : -1);
                     ++nt;
                 }
             return nt;
         };
-        std::vector<int32_t> ro, rv, co, cv;
-        int nrt = make_tiles(PT2_T, PT2_T, 2, ro, rv);
-        int nct = make_tiles(PT2_T, PT2_T, 1, co, cv);
+        // de of every tile slot; NaN where the slot's pair lacks `bit`
+        auto slot_de = [&](int nt, const std::vector<int32_t> &occ, const std::vector<int32_t> &vir,
+                           unsigned char bit) {
+            std::vector<double> d((size_t)nt * PT2_SLOTS, std::nan(""));
+            for (int t = 0; t < nt; ++t)
+                for (int x = 0; x < PT2_T; ++x)
+                    for (int y = 0; y < PT2_T; ++y) {
+                        const int k = occ[(size_t)t * PT2_T + x], cc = vir[(size_t)t * PT2_T + y];
+                        if (k < 0 || cc < 0) continue;
+                        const int64_t p = (int64_t)k * nv + cc;
+                        if (fl[p] & bit) d[(size_t)t * PT2_SLOTS + x * PT2_T + y] = deh[p];
+                    }
+            return d;
+        };
+        // slot of pair p in the tile grid of tile_of
+        auto slot_of = [&](const std::vector<int32_t> &tile_of, int64_t p) {
+            const int k = (int)(p / nv), cc = (int)(p % nv);
+            const int t = tile_of[(size_t)(k / PT2_T) * nvb + cc / PT2_T];
+            return (int64_t)t * PT2_SLOTS + (k % PT2_T) * PT2_T + cc % PT2_T;
+        };
+        std::vector<int32_t> ro, rv, co, cv, rtile, ctile;
+        const int nrt = make_tiles(2, ro, rv, rtile);
+        const int nct = make_tiles(1, co, cv, ctile);
+        const std::vector<double> drh = slot_de(nrt, ro, rv, 2), dch = slot_de(nct, co, cv, 1);
         DBuf<int32_t> rod, rvd, cod, cvd;
+        DBuf<double> drd, dcd;
         rod.upload(ro.data(), ro.size(), s); rvd.upload(rv.data(), rv.size(), s);
         cod.upload(co.data(), co.size(), s); cvd.upload(cv.data(), cv.size(), s);
+        drd.upload(drh.data(), drh.size(), s); dcd.upload(dch.data(), dch.size(), s);
         TileSet rows{rod.get(), rvd.get(), nrt}, cols{cod.get(), cvd.get(), nct};
-        DBuf<double> rowsum(c->npair);
+        // Column tiles split so that row tiles x splits fill the device in the
+        // fewest whole rounds (kPt2Resident work-groups resident at a time);
+        // a function of the tile counts only, so a molecule's sums do not
+        // depend on what else runs.
+        int nsplit = 1;
+        {
+            double best = std::ceil((double)nrt / kPt2Resident);
+            for (int ns = 2; ns <= std::min(16, nct / 2); ++ns) {
+                const double t = std::ceil((double)nrt * ns / kPt2Resident) / ns;
+                if (t < best - 1e-9) { best = t; nsplit = ns; }
+            }
+        }
+        // every pt term kept for pass 2 when the (secondary slot x primary
+        // slot) matrix fits kPt2StoreBytes
+        // (STDA_HIP_PT2_STORE_MAX, bytes, overrides the limit)
+        const int64_t ldp = (int64_t)PT2_SLOTS * nct;
+        double store_max = (double)kPt2StoreBytes;
+        if (const char *e = std::getenv("STDA_HIP_PT2_STORE_MAX")) store_max = std::strtod(e, nullptr);
+        const bool store = (double)PT2_SLOTS * nrt * (double)ldp * 8.0 <= store_max;
+        DBuf<double> ptm(store ? (size_t)PT2_SLOTS * nrt * (size_t)ldp : 0);
+        DBuf<double> rowsum(c->npair), rowpart((size_t)nsplit * nrt * PT2_SLOTS);
         EventTimer ev1(c);
-        launch_pt2_pipe(s, 0, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                        c->qab.get(), c->de.get(), c->flags.get(), 2, c->flags.get(), 1, c->ak,
-                        rowsum.get());
+        launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                        c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
+                        rowsum.get(), ptm.get(), ldp);
         ev1.finish("pt2_rows", 4.0 * na * (double)c->nsec * (double)c->nprim);
         std::vector<double> rs(c->npair);
         rowsum.download(rs.data(), rs.size(), s);
@@ -1589,7 +1642,28 @@ static void do_select(stda_hip_ctx *c) {
             if (rs[p] > c->thrp) incl[p] = 1;
             else { fl[p] |= 4; ++nexcl; }
         }
-        if (nexcl > 0) {
+        if (nexcl > 0 && store) {
+            // pass 2 from the stored terms: column sums over the rejected rows
+            std::vector<int32_t> rrow;
+            rrow.reserve((size_t)nexcl);
+            for (int32_t p : sec)
+                if (fl[p] & 4) rrow.push_back((int32_t)slot_of(rtile, p));
+            std::sort(rrow.begin(), rrow.end());
+            DBuf<int32_t> rrowd;
+            rrowd.upload(rrow.data(), rrow.size(), s);
+            const int64_t ncb = (ldp + 255) / 256;
+            const int nsp2 = (int)std::max<int64_t>(
+                1, std::min<int64_t>(nexcl, (1024 + ncb - 1) / ncb));
+            DBuf<double> part((size_t)nsp2 * ldp), colsum((size_t)ldp);
+            EventTimer ev2(c);
+            launch_pt2_colsum(s, (int)nexcl, rrowd.get(), ldp, ptm.get(), nsp2, part.get(),
+                              colsum.get());
+            ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
+            std::vector<double> cs((size_t)ldp);
+            colsum.download(cs.data(), cs.size(), s);
+            c->sync();
+            for (int32_t p : prim) pt2[p] = cs[(size_t)slot_of(ctile, p)];
+        } else if (nexcl > 0) {
             c->flags.upload(fl.data(), fl.size(), s);
             // rejected secondaries grouped by occupied k, 16 virtual c per row tile
             std::vector<int32_t> rk, rc;
@@ -1624,12 +1698,12 @@ static void do_select(stda_hip_ctx *c) {
             DBuf<double> colsum(c->npair);
             // split the row stream so that about 1024 work-groups are in flight
             const int nrt2 = (int)rk.size();
-            const int nsplit = std::max(1, std::min(nrt2, (1024 + nct2 - 1) / std::max(nct2, 1)));
-            DBuf<double> part((size_t)nsplit * nct2 * 256);
+            const int nsp2 = std::max(1, std::min(nrt2, (1024 + nct2 - 1) / std::max(nct2, 1)));
+            DBuf<double> part((size_t)nsp2 * nct2 * 256);
             EventTimer ev2(c);
             launch_pt2_cols_sparse(s, nrt2, rkd.get(), rcd.get(), nct2, cjd.get(), cbd.get(), na,
                                    nv, c->pia.get(), c->qia.get(), c->pij.get(), c->qab.get(),
-                                   c->de.get(), c->flags.get(), 4, c->flags.get(), 1, c->ak, nsplit,
+                                   c->de.get(), c->flags.get(), 4, c->flags.get(), 1, c->ak, nsp2,
                                    part.get(), colsum.get());
             ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs(c->npair);

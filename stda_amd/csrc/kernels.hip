@@ -4,11 +4,13 @@
 // pair p = i*nv + a; charge arrays are (ncent, npairs) with the atom index
 // fastest; packed occupied/virtual pairs use lin0(x,y) = max*(max+1)/2 + min,
 // the 0-based form of the reference's lin() (stda.f:2174-2181).
+#include "../../include/stda_hip.h"
 #include "common.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 namespace stda {
 
@@ -624,129 +626,6 @@ __device__ __forceinline__ void load_tile_ids(TileSmem &sm, TileSet rows, int rt
     __syncthreads();
 }
 
-// PT2 pass 1 (row sums) -- one block per row tile, looping over column tiles.
-__global__ void __launch_bounds__(NT) pt2_rows_kernel(TileSet rows, TileSet cols, int ncent,
-                                                      int no, int nv, const double *pK,
-                                                      const double *q, const double *pJ,
-                                                      const double *qab, const double *de,
-                                                      const unsigned char *rowmask,
-                                                      unsigned char rowbit,
-                                                      const unsigned char *colmask,
-                                                      unsigned char colbit, double ak,
-                                                      double *rowsum) {
-    __shared__ TileSmem sm;
-    int rt = blockIdx.x;
-    int t = threadIdx.x;
-    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
-    double part[TK] = {0.0, 0.0, 0.0, 0.0};
-    for (int ct = 0; ct < cols.ntiles; ++ct) {
-        load_tile_ids(sm, rows, rt, cols, ct);
-        double accK[TK], accJ[TK];
-        tile_contract(sm, ncent, nv, pK, q, pJ, qab, accK, accJ);
-        int jj = sm.occc[j], bb = sm.virc[b];
-        int cc = sm.virr[c];
-        if (jj < 0 || bb < 0 || cc < 0) continue;
-        int64_t pc = (int64_t)jj * nv + bb;
-        if (!(colmask[pc] & colbit)) continue;
-        double dcol = de[pc];
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            int kk = sm.occr[k];
-            if (kk < 0) continue;
-            int64_t pr = (int64_t)kk * nv + cc;
-            if (!(rowmask[pr] & rowbit)) continue;
-            double amat = ak * accK[k] - accJ[k];
-            part[k] += amat * amat / (de[pr] - dcol + 1.0e-10);
-        }
-    }
-    // reduce over the TJ*TB = 32 threads sharing c (consecutive lanes)
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-        double v = part[k];
-        for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 32);
-        part[k] = v;
-    }
-    if ((t % (TB * TJ)) == 0) {
-        int cc = rows.vir[rt * TC + c];
-        for (int k = 0; k < TK; ++k) {
-            int kk = rows.occ[rt * TK + k];
-            if (kk < 0 || cc < 0) continue;
-            int64_t pr = (int64_t)kk * nv + cc;
-            if (rowmask[pr] & rowbit) rowsum[pr] = part[k];
-        }
-    }
-}
-
-// PT2 pass 2 (column sums over the selected rows) -- one block per column tile.
-__global__ void __launch_bounds__(NT) pt2_cols_kernel(TileSet rows, TileSet cols, int ncent,
-                                                      int no, int nv, const double *pK,
-                                                      const double *q, const double *pJ,
-                                                      const double *qab, const double *de,
-                                                      const unsigned char *rowmask,
-                                                      unsigned char rowbit,
-                                                      const unsigned char *colmask,
-                                                      unsigned char colbit, double ak,
-                                                      double *colsum) {
-    __shared__ TileSmem sm;
-    __shared__ double red[NT];
-    int ctile = blockIdx.x;
-    int t = threadIdx.x;
-    int b = t % TB, j = (t / TB) % TJ, c = t / (TB * TJ);
-    double part = 0.0;
-    for (int rt = 0; rt < rows.ntiles; ++rt) {
-        load_tile_ids(sm, rows, rt, cols, ctile);
-        double accK[TK], accJ[TK];
-        tile_contract(sm, ncent, nv, pK, q, pJ, qab, accK, accJ);
-        int jj = sm.occc[j], bb = sm.virc[b];
-        int cc = sm.virr[c];
-        if (jj < 0 || bb < 0 || cc < 0) continue;
-        int64_t pc = (int64_t)jj * nv + bb;
-        if (!(colmask[pc] & colbit)) continue;
-        double dcol = de[pc];
-#pragma unroll
-        for (int k = 0; k < TK; ++k) {
-            int kk = sm.occr[k];
-            if (kk < 0) continue;
-            int64_t pr = (int64_t)kk * nv + cc;
-            if (!(rowmask[pr] & rowbit)) continue;
-            double amat = ak * accK[k] - accJ[k];
-            part += amat * amat / (de[pr] - dcol + 1.0e-10);
-        }
-    }
-    // reduce over the TC threads sharing (j, b): stride TB*TJ
-    red[t] = part;
-    __syncthreads();
-    if (t < TB * TJ) {
-        double s = 0.0;
-        for (int cI = 0; cI < TC; ++cI) s += red[t + cI * TB * TJ];
-        int jj = cols.occ[ctile * TJ + j], bb = cols.vir[ctile * TB + b];
-        if (jj >= 0 && bb >= 0) {
-            int64_t pc = (int64_t)jj * nv + bb;
-            if (colmask[pc] & colbit) colsum[pc] = s;
-        }
-    }
-}
-
-void launch_pt2_rows(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
-                     const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak,
-                     double *rowsum) {
-    if (rows.ntiles == 0) return;
-    hipLaunchKernelGGL(pt2_rows_kernel, dim3(rows.ntiles), dim3(NT), 0, s, rows, cols, ncent, no,
-                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, rowsum);
-}
-
-void launch_pt2_cols(hipStream_t s, TileSet rows, TileSet cols, int ncent, int no, int nv,
-                     const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak,
-                     double *colsum) {
-    if (cols.ntiles == 0) return;
-    hipLaunchKernelGGL(pt2_cols_kernel, dim3(cols.ntiles), dim3(NT), 0, s, rows, cols, ncent, no,
-                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
-}
-
 // Explicit A: one block per (row tile, column tile); stores the coupling of
 // every pair of CSFs (the diagonal is overwritten afterwards with ed).
 __global__ void __launch_bounds__(NT) build_a_kernel(TileSet tiles, int ncent, int no, int nv,
@@ -837,262 +716,6 @@ constexpr int M_T = 8;          // per-dimension tile size
 constexpr int M_CH = 16;        // atoms per staged chunk (4 MFMA k-steps)
 constexpr int M_LD = 64 + 1;    // padded LDS row (doubles)
 
-struct MfmaSmem {
-    double pk[M_CH][M_LD];      // (k,c) = 8k + c
-    double qc[M_CH][M_LD];      // (j,b) = 8j + b
-    double pj[M_CH][M_LD];      // (k,j) = 8k + j
-    double qv[M_CH][M_LD];      // (c,b) = 8c + b
-    double xch[4][256];         // per-wave J block in tuple order
-    double de_r[64], de_c[64];
-    unsigned char ok_r[64], ok_c[64];
-    int occr[M_T], virr[M_T], occc[M_T], virc[M_T];
-    double red[4][64];
-};
-
-__device__ __forceinline__ void mfma_load_ids(MfmaSmem &sm, TileSet rows, int rt, TileSet cols,
-                                              int ct, int nv, const double *de,
-                                              const unsigned char *rowmask, unsigned char rowbit,
-                                              const unsigned char *colmask, unsigned char colbit) {
-    __syncthreads();
-    int t = threadIdx.x;
-    if (t < M_T) {
-        sm.occr[t] = rows.occ[rt * M_T + t];
-        sm.virr[t] = rows.vir[rt * M_T + t];
-        sm.occc[t] = cols.occ[ct * M_T + t];
-        sm.virc[t] = cols.vir[ct * M_T + t];
-    }
-    __syncthreads();
-    if (t < 64) {
-        int k = sm.occr[t / M_T], c = sm.virr[t % M_T];
-        bool ok = k >= 0 && c >= 0;
-        int64_t p = ok ? (int64_t)k * nv + c : 0;
-        ok = ok && (rowmask[p] & rowbit);
-        sm.ok_r[t] = ok;
-        sm.de_r[t] = ok ? de[p] : 0.0;
-    } else if (t < 128) {
-        int u = t - 64;
-        int j = sm.occc[u / M_T], b = sm.virc[u % M_T];
-        bool ok = j >= 0 && b >= 0;
-        int64_t p = ok ? (int64_t)j * nv + b : 0;
-        ok = ok && (colmask[p] & colbit);
-        sm.ok_c[u] = ok;
-        sm.de_c[u] = ok ? de[p] : 0.0;
-    }
-}
-
-// stage atoms [a0, a0+na) of the four operand sets into LDS (zero-padded)
-__device__ __forceinline__ void mfma_stage(MfmaSmem &sm, int a0, int na, int ncent, int nv,
-                                           const double *__restrict__ pK,
-                                           const double *__restrict__ q,
-                                           const double *__restrict__ pJ,
-                                           const double *__restrict__ qab) {
-    int t = threadIdx.x;
-    int a = t % M_CH;            // atom within the chunk (fastest: coalesced)
-    int vb = t / M_CH;           // 0..15
-    bool in = a < na;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        int v = vb + 16 * r;     // 0..63
-        int x = v / M_T, y = v % M_T;
-        {   // pK (k,c)
-            int k = sm.occr[x], c = sm.virr[y];
-            sm.pk[a][v] = (in && k >= 0 && c >= 0)
-                              ? pK[a0 + a + (int64_t)ncent * ((int64_t)k * nv + c)] : 0.0;
-        }
-        {   // q (j,b)
-            int j = sm.occc[x], b = sm.virc[y];
-            sm.qc[a][v] = (in && j >= 0 && b >= 0)
-                              ? q[a0 + a + (int64_t)ncent * ((int64_t)j * nv + b)] : 0.0;
-        }
-        {   // pJ (k,j)
-            int k = sm.occr[x], j = sm.occc[y];
-            sm.pj[a][v] = (in && k >= 0 && j >= 0) ? pJ[a0 + a + (int64_t)ncent * lin0(k, j)] : 0.0;
-        }
-        {   // qab (c,b)
-            int c = sm.virr[x], b = sm.virc[y];
-            sm.qv[a][v] = (in && c >= 0 && b >= 0) ? qab[a0 + a + (int64_t)ncent * lin0(c, b)] : 0.0;
-        }
-    }
-}
-
-// accJ[blk], accK[blk]: blk = 2*ch + bh
-__device__ __forceinline__ void mfma_contract(MfmaSmem &sm, int ncent, int nv, const double *pK,
-                                              const double *q, const double *pJ,
-                                              const double *qab, v4d accJ[4], v4d accK[4]) {
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int kh = w >> 1, jh = w & 1;
-    int r16 = lane & 15, g = lane >> 4;
-    int hi = r16 >> 2, lo = r16 & 3;
-    for (int b = 0; b < 4; ++b) {
-        accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
-        accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
-    // operand column indices (constant over atoms)
-    int iPJ = M_T * (4 * kh + hi) + (4 * jh + lo);            // (k,j)
-    int iPK0 = M_T * (4 * kh + hi) + lo, iPK1 = iPK0 + 4;     // (k,c) for ch = 0,1
-    int iQ0 = M_T * (4 * jh + hi) + lo, iQ1 = iQ0 + 4;        // (j,b) for bh = 0,1
-    int iQV[4];
-    for (int ch = 0; ch < 2; ++ch)
-        for (int bh = 0; bh < 2; ++bh) iQV[2 * ch + bh] = M_T * (4 * ch + hi) + 4 * bh + lo;
-    for (int a0 = 0; a0 < ncent; a0 += M_CH) {
-        int na = min(M_CH, ncent - a0);
-        __syncthreads();
-        mfma_stage(sm, a0, na, ncent, nv, pK, q, pJ, qab);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < M_CH / 4; ++s) {
-            int arow = 4 * s + g;
-            double aJ = sm.pj[arow][iPJ];
-            double aK0 = sm.pk[arow][iPK0], aK1 = sm.pk[arow][iPK1];
-            double bQ0 = sm.qc[arow][iQ0], bQ1 = sm.qc[arow][iQ1];
-#pragma unroll
-            for (int blk = 0; blk < 4; ++blk) {
-                double bV = sm.qv[arow][iQV[blk]];
-                accJ[blk] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, bV, accJ[blk], 0, 0, 0);
-            }
-            accK[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ0, accK[0], 0, 0, 0);
-            accK[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ1, accK[1], 0, 0, 0);
-            accK[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ0, accK[2], 0, 0, 0);
-            accK[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ1, accK[3], 0, 0, 0);
-        }
-    }
-}
-
-// pt values of this lane in the K layout: pt[blk][i] for row (k,c) =
-// (4kh + i, 4ch + g) and column (j,b) = (4jh + hi, 4bh + lo).
-__device__ __forceinline__ void mfma_epilogue(MfmaSmem &sm, const v4d accJ[4], const v4d accK[4],
-                                              double ak, double pt[4][4]) {
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int kh = w >> 1, jh = w & 1;
-    int r16 = lane & 15, g = lane >> 4;
-    int hi = r16 >> 2, lo = r16 & 3;
-    for (int blk = 0; blk < 4; ++blk) {
-        int ch = blk >> 1, bh = blk & 1;
-        // J block in tuple order (k', c', j', b'): lane holds k' = i, j' = g, c' = hi, b' = lo
-        for (int i = 0; i < 4; ++i) sm.xch[w][((i * 4 + hi) * 4 + g) * 4 + lo] = accJ[blk][i];
-        __builtin_amdgcn_wave_barrier();
-        for (int i = 0; i < 4; ++i) {
-            double J = sm.xch[w][((i * 4 + g) * 4 + hi) * 4 + lo];   // k'=i, c'=g, j'=hi, b'=lo
-            int rr = M_T * (4 * kh + i) + 4 * ch + g;                 // (k,c)
-            int cc = M_T * (4 * jh + hi) + 4 * bh + lo;               // (j,b)
-            double amat = ak * accK[blk][i] - J;
-            pt[blk][i] = (sm.ok_r[rr] && sm.ok_c[cc])
-                             ? amat * amat / (sm.de_r[rr] - sm.de_c[cc] + 1.0e-10) : 0.0;
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-__global__ void __launch_bounds__(256) pt2_rows_mfma_kernel(
-    TileSet rows, TileSet cols, int ncent, int nv, const double *pK, const double *q,
-    const double *pJ, const double *qab, const double *de, const unsigned char *rowmask,
-    unsigned char rowbit, const unsigned char *colmask, unsigned char colbit, double ak,
-    double *rowsum) {
-    __shared__ MfmaSmem sm;
-    int rt = blockIdx.x;
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int kh = w >> 1;
-    int g = lane >> 4;
-    double racc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};     // rows (4kh+i, 4ch+g)
-    for (int ct = 0; ct < cols.ntiles; ++ct) {
-        mfma_load_ids(sm, rows, rt, cols, ct, nv, de, rowmask, rowbit, colmask, colbit);
-        v4d accJ[4], accK[4];
-        mfma_contract(sm, ncent, nv, pK, q, pJ, qab, accJ, accK);
-        double pt[4][4];
-        mfma_epilogue(sm, accJ, accK, ak, pt);
-        for (int blk = 0; blk < 4; ++blk)
-            for (int i = 0; i < 4; ++i) racc[blk >> 1][i] += pt[blk][i];
-    }
-    // reduce over the 16 lanes of a row group (columns), then over the two j-halves
-    for (int ch = 0; ch < 2; ++ch)
-        for (int i = 0; i < 4; ++i) {
-            double v = racc[ch][i];
-            for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
-            racc[ch][i] = v;
-        }
-    __syncthreads();
-    if ((lane & 15) == 0)
-        for (int ch = 0; ch < 2; ++ch)
-            for (int i = 0; i < 4; ++i) sm.red[w][g * 8 + ch * 4 + i] = racc[ch][i];
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        int t = threadIdx.x;           // row (k,c) = 8k + c
-        int k = t / M_T, c = t % M_T;
-        int kh2 = k / 4, i = k % 4, ch = c / 4, gg = c % 4;
-        double s = sm.red[2 * kh2][gg * 8 + ch * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + ch * 4 + i];
-        int kk = rows.occ[rt * M_T + k], cc = rows.vir[rt * M_T + c];
-        if (kk >= 0 && cc >= 0) {
-            int64_t p = (int64_t)kk * nv + cc;
-            if (rowmask[p] & rowbit) rowsum[p] = s;
-        }
-    }
-    (void)kh;
-}
-
-__global__ void __launch_bounds__(256) pt2_cols_mfma_kernel(
-    TileSet rows, TileSet cols, int ncent, int nv, const double *pK, const double *q,
-    const double *pJ, const double *qab, const double *de, const unsigned char *rowmask,
-    unsigned char rowbit, const unsigned char *colmask, unsigned char colbit, double ak,
-    double *colsum) {
-    __shared__ MfmaSmem sm;
-    int ct = blockIdx.x;
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int r16 = lane & 15;
-    double cacc[2] = {0.0, 0.0};     // columns (4jh+hi, 4bh+lo), bh = 0,1
-    for (int rt = 0; rt < rows.ntiles; ++rt) {
-        mfma_load_ids(sm, rows, rt, cols, ct, nv, de, rowmask, rowbit, colmask, colbit);
-        v4d accJ[4], accK[4];
-        mfma_contract(sm, ncent, nv, pK, q, pJ, qab, accJ, accK);
-        double pt[4][4];
-        mfma_epilogue(sm, accJ, accK, ak, pt);
-        for (int blk = 0; blk < 4; ++blk)
-            for (int i = 0; i < 4; ++i) cacc[blk & 1] += pt[blk][i];
-    }
-    // reduce over the 4 row groups (lanes 16 apart), then over the two k-halves
-    for (int bh = 0; bh < 2; ++bh) {
-        double v = cacc[bh];
-        v += __shfl_xor(v, 16, 64);
-        v += __shfl_xor(v, 32, 64);
-        cacc[bh] = v;
-    }
-    __syncthreads();
-    if (lane < 16)
-        for (int bh = 0; bh < 2; ++bh) sm.red[w][r16 * 2 + bh] = cacc[bh];
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        int t = threadIdx.x;           // column (j,b) = 8j + b
-        int j = t / M_T, b = t % M_T;
-        int jh = j / 4, hi = j % 4, bh = b / 4, lo = b % 4;
-        int r = hi * 4 + lo;
-        double s = sm.red[jh][r * 2 + bh] + sm.red[2 + jh][r * 2 + bh];
-        int jj = cols.occ[ct * M_T + j], bb = cols.vir[ct * M_T + b];
-        if (jj >= 0 && bb >= 0) {
-            int64_t p = (int64_t)jj * nv + bb;
-            if (colmask[p] & colbit) colsum[p] = s;
-        }
-    }
-}
-
-void launch_pt2_rows_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                          const double *pK, const double *q, const double *pJ, const double *qab,
-                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                          const unsigned char *colmask, unsigned char colbit, double ak,
-                          double *rowsum) {
-    if (rows.ntiles == 0) return;
-    hipLaunchKernelGGL(pt2_rows_mfma_kernel, dim3(rows.ntiles), dim3(256), 0, s, rows, cols, ncent,
-                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, rowsum);
-}
-
-void launch_pt2_cols_mfma(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                          const double *pK, const double *q, const double *pJ, const double *qab,
-                          const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                          const unsigned char *colmask, unsigned char colbit, double ak,
-                          double *colsum) {
-    if (cols.ntiles == 0) return;
-    hipLaunchKernelGGL(pt2_cols_mfma_kernel, dim3(cols.ntiles), dim3(256), 0, s, rows, cols, ncent,
-                       nv, pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, colsum);
-}
-
 }  // namespace stda
 
 namespace stda {
@@ -1140,12 +763,12 @@ void launch_sum_slices(hipStream_t s, int64_t len, int nchunk, const double *W, 
 namespace stda {
 
 // ----------------------------------------------------------------------------
-// Software-pipelined form of the PT2 passes.  A work-group owns one tile of
-// its side (a secondary row tile in pass 1, a primary column tile in pass 2)
-// and streams the tiles of the other side.  The (tile, atom chunk) steps are
-// flattened; while the matrix cores work on step s from one LDS buffer, the
-// operands of step s+1 are fetched into registers and then written to the
-// other buffer, so a single barrier per step orders everything.
+// Software-pipelined PT2 pass 1 (stda.f:1800-1838).  A work-group owns one
+// secondary row tile and streams a range of primary column tiles.  The
+// (tile, atom chunk) steps are flattened; while the matrix cores work on step
+// s from one LDS buffer, the operands of step s+1 are fetched into registers
+// and then written to the other buffer, so a single barrier per step orders
+// everything.
 struct PipeBuf {
     double pk[M_CH][M_LD];
     double qc[M_CH][M_LD];
@@ -1194,20 +817,27 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return x * q + min(x, r) + i;
 }
 
-template <int MODE>   // 0: row sums (pass 1), 1: column sums (pass 2)
+// Work-group item = split sp x row tile own; it streams column tiles
+// [ct0, ct1) of that split.  de of the tile slots comes from drow/dcol (NaN
+// for slots outside the selection), read once per row tile and prefetched
+// with each column tile.
 __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     TileSet rows, TileSet cols, int ncent, int nv, const double *__restrict__ pK,
     const double *__restrict__ q, const double *__restrict__ pJ, const double *__restrict__ qab,
-    const double *__restrict__ de, const unsigned char *__restrict__ rowmask, unsigned char rowbit,
-    const unsigned char *__restrict__ colmask, unsigned char colbit, double ak, double *out) {
+    const double *__restrict__ drow, const double *__restrict__ dcol, double ak, int nsplit,
+    double *__restrict__ rowpart, double *__restrict__ ptm, int64_t ldp) {
     __shared__ PipeSmem sm;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so the
-    // tiles are regrouped to give each XCD a contiguous range of owned tiles;
-    // neighbouring tiles share operand columns, which then hit that XCD's L2.
-    const int own = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int nother = MODE == 0 ? cols.ntiles : rows.ntiles;
+    // items are regrouped to give each XCD a contiguous range (split-major):
+    // the work-groups of one XCD then stream the same column tiles, whose
+    // operands hit that XCD's L2.
+    const int nown = rows.ntiles;
+    const int item = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int sp = item / nown, own = item % nown;
+    const int ct0 = (int)((int64_t)cols.ntiles * sp / nsplit);
+    const int ct1 = (int)((int64_t)cols.ntiles * (sp + 1) / nsplit);
     const int nch = (ncent + M_CH - 1) / M_CH;
-    const int64_t nsteps = (int64_t)nother * nch;
+    const int64_t nsteps = (int64_t)(ct1 - ct0) * nch;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int kh = w >> 1, jh = w & 1;
     const int r16 = lane & 15, g = lane >> 4, hi = r16 >> 2, lo = r16 & 3;
@@ -1217,6 +847,14 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     const int iQ0 = M_T * (4 * jh + hi) + lo;
     const int iQV0 = M_T * hi + lo;   // + 32 ch + 4 bh
     const double *src[4] = {pK, q, pJ, qab};
+
+    // this lane's rows (4kh + i, 4chv + g) and columns (4jh + hi, 4bh + lo)
+    double dr[2][4], dc[2] = {0.0, 0.0}, dcn[2] = {0.0, 0.0};
+#pragma unroll
+    for (int chv = 0; chv < 2; ++chv)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            dr[chv][i] = drow[(int64_t)own * PT2_SLOTS + (4 * kh + i) * M_T + 4 * chv + g];
 
     int32_t idx[4][4];
     double pre[4][4];
@@ -1240,13 +878,19 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
         }
     };
     auto set_tile = [&](int other) {
-        if (MODE == 0) stage_cols(rows, own, cols, other, nv, idx);
-        else stage_cols(rows, other, cols, own, nv, idx);
+        stage_cols(rows, own, cols, other, nv, idx);
+#pragma unroll
+        for (int bh = 0; bh < 2; ++bh)
+            dcn[bh] = dcol[(int64_t)other * PT2_SLOTS + (4 * jh + hi) * M_T + 4 * bh + lo];
     };
 
-    set_tile(0);
-    fetch(0);
-    store(0);
+    if (nsteps > 0) {
+        set_tile(ct0);
+        fetch(0);
+        store(0);
+        dc[0] = dcn[0];
+        dc[1] = dcn[1];
+    }
     __syncthreads();
 
     v4d accJ[4], accK[4];
@@ -1255,12 +899,12 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
         accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
         accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
     }
-    // MODE 0: partial row sums of rows (4kh+i, 4ch+g); MODE 1: column sums (4jh+hi, 4bh+lo)
+    // partial row sums of rows (4kh+i, 4chv+g), index 4 chv + i
     double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     for (int64_t step = 0; step < nsteps; ++step) {
         const int ch = (int)(step % nch);
-        const int other = (int)(step / nch);
+        const int other = ct0 + (int)(step / nch);
         const int cur = (int)(step & 1);
         const bool more = step + 1 < nsteps;
         if (more) {
@@ -1285,14 +929,9 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
             accK[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ1, accK[3], 0, 0, 0);
         }
         if (ch == nch - 1) {
-            const int rt = MODE == 0 ? own : other, ct = MODE == 0 ? other : own;
-            const int jj = cols.occ[ct * M_T + 4 * jh + hi];
 #pragma unroll
             for (int blk = 0; blk < 4; ++blk) {
                 const int chv = blk >> 1, bh = blk & 1;
-                const double dc = pair_de(jj, cols.vir[ct * M_T + 4 * bh + lo], nv, de, colmask,
-                                          colbit);
-                const int cc = rows.vir[rt * M_T + 4 * chv + g];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     sm.xch[w][((i * 4 + hi) * 4 + g) * 4 + lo] = accJ[blk][i];
@@ -1301,82 +940,120 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
                 for (int i = 0; i < 4; ++i) {
                     double J = sm.xch[w][((i * 4 + g) * 4 + hi) * 4 + lo];
                     double amat = ak * accK[blk][i] - J;
-                    double dr = pair_de(rows.occ[rt * M_T + 4 * kh + i], cc, nv, de, rowmask, rowbit);
-                    double pt = amat * amat / (dr - dc + 1.0e-10);
-                    if (!__builtin_isnan(pt)) {
-                        if (MODE == 0) part[chv * 4 + i] += pt;
-                        else part[bh] += pt;
-                    }
+                    double pt = amat * amat / (dr[chv][i] - dc[bh] + 1.0e-10);
+                    const bool ok = !__builtin_isnan(pt);
+                    if (ok) part[chv * 4 + i] += pt;
+                    if (ptm)
+                        ptm[((int64_t)own * PT2_SLOTS + (4 * kh + i) * M_T + 4 * chv + g) * ldp +
+                            (int64_t)other * PT2_SLOTS + (4 * jh + hi) * M_T + 4 * bh + lo] =
+                            ok ? pt : 0.0;
                 }
                 __builtin_amdgcn_wave_barrier();
                 accJ[blk] = v4d{0.0, 0.0, 0.0, 0.0};
                 accK[blk] = v4d{0.0, 0.0, 0.0, 0.0};
             }
+            dc[0] = dcn[0];
+            dc[1] = dcn[1];
         }
         if (more) store(cur ^ 1);
         __syncthreads();
     }
 
-    if (MODE == 0) {
 #pragma unroll
-        for (int q8 = 0; q8 < 8; ++q8) {
-            double v = part[q8];
-            for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-            part[q8] = v;
-        }
-        if (r16 == 0)
-            for (int q8 = 0; q8 < 8; ++q8) sm.red[w][g * 8 + q8] = part[q8];   // q8 = 4 c2 + i
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            int t = threadIdx.x;
-            int k = t / M_T, c = t % M_T;
-            int kh2 = k / 4, i = k % 4, c2 = c / 4, gg = c % 4;
-            double sum = sm.red[2 * kh2][gg * 8 + c2 * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + c2 * 4 + i];
-            int kk = rows.occ[own * M_T + k], cv = rows.vir[own * M_T + c];
-            if (kk >= 0 && cv >= 0) {
-                int64_t p = (int64_t)kk * nv + cv;
-                if (rowmask[p] & rowbit) out[p] = sum;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int bh = 0; bh < 2; ++bh) {
-            double v = part[bh];
-            v += __shfl_xor(v, 16, 64);
-            v += __shfl_xor(v, 32, 64);
-            part[bh] = v;
-        }
-        if (lane < 16)
-            for (int bh = 0; bh < 2; ++bh) sm.red[w][r16 * 2 + bh] = part[bh];
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            int t = threadIdx.x;
-            int j = t / M_T, b = t % M_T;
-            int jh2 = j / 4, hh = j % 4, bh = b / 4, ll = b % 4;
-            int r = hh * 4 + ll;
-            double sum = sm.red[jh2][r * 2 + bh] + sm.red[2 + jh2][r * 2 + bh];
-            int jv = cols.occ[own * M_T + j], bv = cols.vir[own * M_T + b];
-            if (jv >= 0 && bv >= 0) {
-                int64_t p = (int64_t)jv * nv + bv;
-                if (colmask[p] & colbit) out[p] = sum;
-            }
-        }
+    for (int q8 = 0; q8 < 8; ++q8) {
+        double v = part[q8];
+        for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+        part[q8] = v;
+    }
+    if (r16 == 0)
+        for (int q8 = 0; q8 < 8; ++q8) sm.red[w][g * 8 + q8] = part[q8];   // q8 = 4 c2 + i
+    __syncthreads();
+    if (threadIdx.x < PT2_SLOTS) {
+        int t = threadIdx.x;
+        int k = t / M_T, c = t % M_T;
+        int kh2 = k / 4, i = k % 4, c2 = c / 4, gg = c % 4;
+        double sum = sm.red[2 * kh2][gg * 8 + c2 * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + c2 * 4 + i];
+        rowpart[((int64_t)sp * nown + own) * PT2_SLOTS + t] = sum;
     }
 }
 
-void launch_pt2_pipe(hipStream_t s, int mode, TileSet rows, TileSet cols, int ncent, int nv,
+// rowsum[pair] = sum over the splits, in order, of the partial row sums of
+// each secondary slot.
+__global__ void pt2_rows_finish_kernel(TileSet rows, int nv, int nsplit,
+                                       const double *__restrict__ drow,
+                                       const double *__restrict__ rowpart,
+                                       double *__restrict__ rowsum) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nslot = (int64_t)rows.ntiles * PT2_SLOTS;
+    if (e >= nslot || __builtin_isnan(drow[e])) return;
+    const int64_t rt = e / PT2_SLOTS;
+    const int slot = (int)(e % PT2_SLOTS);
+    const int k = rows.occ[rt * M_T + slot / M_T], c = rows.vir[rt * M_T + slot % M_T];
+    double s = rowpart[e];
+    for (int sp = 1; sp < nsplit; ++sp) s += rowpart[sp * nslot + e];
+    rowsum[(int64_t)k * nv + c] = s;
+}
+
+void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *de, const unsigned char *rowmask, unsigned char rowbit,
-                     const unsigned char *colmask, unsigned char colbit, double ak, double *out) {
-    int nown = mode == 0 ? rows.ntiles : cols.ntiles;
-    int noth = mode == 0 ? cols.ntiles : rows.ntiles;
-    if (nown == 0 || noth == 0) return;
-    if (mode == 0)
-        hipLaunchKernelGGL(pt2_pipe_kernel<0>, dim3(nown), dim3(256), 0, s, rows, cols, ncent, nv,
-                           pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, out);
-    else
-        hipLaunchKernelGGL(pt2_pipe_kernel<1>, dim3(nown), dim3(256), 0, s, rows, cols, ncent, nv,
-                           pK, q, pJ, qab, de, rowmask, rowbit, colmask, colbit, ak, out);
+                     const double *drow, const double *dcol, double ak, int nsplit,
+                     double *rowpart, double *rowsum, double *ptm, int64_t ldp) {
+    if (rows.ntiles == 0 || cols.ntiles == 0) return;
+    if (nsplit < 1 || nsplit > cols.ntiles)
+        throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1: split count outside [1, column tiles]");
+    const int64_t nitem = (int64_t)rows.ntiles * nsplit;
+    if (nitem > INT32_MAX) throw Error(STDA_HIP_ERR_TOO_LARGE, "PT2 pass 1: too many row tiles");
+    hipLaunchKernelGGL(pt2_pipe_kernel, dim3((unsigned)nitem), dim3(256), 0, s, rows, cols, ncent,
+                       nv, pK, q, pJ, qab, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
+    const int64_t nslot = (int64_t)rows.ntiles * PT2_SLOTS;
+    hipLaunchKernelGGL(pt2_rows_finish_kernel, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0,
+                       s, rows, nv, nsplit, drow, rowpart, rowsum);
+}
+
+// Pass 2 from the stored terms: block (cb, sp) sums columns [256 cb, +256)
+// over the rows of split sp; every row is a contiguous line of ldp terms.
+__global__ void __launch_bounds__(256) pt2_colsum_kernel(int nrej, const int32_t *__restrict__ rrow,
+                                                         int64_t ldp, const double *__restrict__ ptm,
+                                                         int nsplit, double *__restrict__ part) {
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int sp = blockIdx.y;
+    const int r0 = (int)((int64_t)nrej * sp / nsplit), r1 = (int)((int64_t)nrej * (sp + 1) / nsplit);
+    if (col >= ldp) return;
+    double s = 0.0;
+    int r = r0;
+    for (; r + 4 <= r1; r += 4) {
+        const double a0 = ptm[(int64_t)rrow[r] * ldp + col];
+        const double a1 = ptm[(int64_t)rrow[r + 1] * ldp + col];
+        const double a2 = ptm[(int64_t)rrow[r + 2] * ldp + col];
+        const double a3 = ptm[(int64_t)rrow[r + 3] * ldp + col];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+    }
+    for (; r < r1; ++r) s += ptm[(int64_t)rrow[r] * ldp + col];
+    part[(int64_t)sp * ldp + col] = s;
+}
+
+__global__ void pt2_colsum_finish_kernel(int64_t ldp, int nsplit, const double *__restrict__ part,
+                                         double *__restrict__ colsum) {
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ldp) return;
+    double s = part[col];
+    for (int sp = 1; sp < nsplit; ++sp) s += part[(int64_t)sp * ldp + col];
+    colsum[col] = s;
+}
+
+void launch_pt2_colsum(hipStream_t s, int nrej, const int32_t *rrow, int64_t ldp,
+                       const double *ptm, int nsplit, double *part, double *colsum) {
+    if (ldp <= 0) return;
+    if (nsplit < 1 || nsplit > std::max(nrej, 1))
+        throw Error(STDA_HIP_ERR_ARG, "PT2 pass 2: split count outside [1, rejected rows]");
+    const unsigned nb = (unsigned)((ldp + 255) / 256);
+    hipLaunchKernelGGL(pt2_colsum_kernel, dim3(nb, nsplit), dim3(256), 0, s, nrej, rrow, ldp, ptm,
+                       nsplit, part);
+    hipLaunchKernelGGL(pt2_colsum_finish_kernel, dim3(nb), dim3(256), 0, s, ldp, nsplit, part,
+                       colsum);
 }
 
 }  // namespace stda

@@ -458,6 +458,32 @@ def test_cluster_selection_and_roots(cluster30):
     assert_props(sol.props, ref.props)
 
 
+@pytest.fixture(scope="module")
+def cluster60():
+    from stda_amd import synth
+    mol, ints = synth.cluster(60, seed=1004, basis="minimal")
+    return prepare(mol, 7.0, 0.25, ints)
+
+
+@pytest.mark.parametrize("store_max", [None, "0"])
+def test_pt2_pass2_forms(cluster60, monkeypatch, store_max):
+    """PT2 pass 2 (the primaries' lowering by the rejected secondaries, 640
+    of 5686 here) from the stored pass-1 terms (default) and by recomputing
+    the couplings of the rejected rows (the form used when the matrix of
+    pass-1 terms exceeds its memory limit, forced with a limit of 0 bytes):
+    both give the oracle's selection and PT2-lowered diagonal."""
+    if store_max is not None:
+        monkeypatch.setenv("STDA_HIP_PT2_STORE_MAX", store_max)
+    ci = cluster60
+    r = O.select(ci, O.intermediates(ci, 0.25), 7.0)
+    assert r["nexpt"] - r["npt"] > 100
+    ref = type("Sel", (), {"nex": r["nex"], "nci": r["nci"], "iconf": r["iconf"], "ed": r["ed"]})
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        assert h.kernel_stats("pt2_cols")["launches"] >= 1
+        _compare_selection(h, ref)
+
+
 def test_many_roots_against_dense():
     """50 roots: a Davidson subspace of up to 300 vectors, beyond the shapes
     of the hand-written tall-skinny kernels (split-K Gram products, V C with
