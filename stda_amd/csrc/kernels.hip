@@ -1286,21 +1286,38 @@ __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__r
                                                        int ref_stride, double thr,
                                                        double *__restrict__ LinvT,
                                                        double *__restrict__ alive) {
-    __shared__ double L[64 * 65];
-    __shared__ double Z[64 * 65];   // first the staged rows of G, then the inverse factor
+    // L and Z (the inverse factor) share one buffer, which first holds the
+    // staged rows of G
+    constexpr int SBUF = 2 * 64 * 65;
+    __shared__ double buf[SBUF];
+    double *L = buf, *Z = buf + 64 * 65;
     __shared__ int keep[64];
+    __shared__ double dg[64], lim[64];
     const int tid = threadIdx.x;
-    // S = G[m:m+nt, :] - G[0:m, :]^T G[0:m, :]: the correction over 32-row
-    // chunks of G staged in LDS, each thread owning up to 16 elements of S
+    if (tid < nt) lim[tid] = thr * (ref ? ref[(int64_t)tid * ref_stride] : 1.0);
+    // S = G[m:m+nt, :] - G[0:m, :]^T G[0:m, :]: the correction over chunks of
+    // SBUF / (nt + 1) rows of G (all of them for m <= 395 at nt = 20), loaded
+    // eight per thread before any is stored, each thread owning up to 16
+    // elements of S
+    const int ld = nt + 1, rows_per = SBUF / ld;
     double acc[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-    for (int q0 = 0; q0 < m; q0 += 32) {
-        const int mq = min(32, m - q0);
+    for (int q0 = 0; q0 < m; q0 += rows_per) {
+        const int mq = min(rows_per, m - q0), tot = mq * nt;
         __syncthreads();
-        for (int e = tid; e < 32 * nt; e += 256) {
-            const int q = e % 32, c = e / 32;
-            Z[q * 65 + c] = q < mq ? G[(q0 + q) + (int64_t)ldg * c] : 0.0;
+        for (int e0 = 0; e0 < tot; e0 += 256 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + tid + 256 * u;
+                v[u] = e < tot ? G[(q0 + e % mq) + (int64_t)ldg * (e / mq)] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + tid + 256 * u;
+                if (e < tot) buf[(e % mq) * ld + e / mq] = v[u];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -1309,11 +1326,12 @@ __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__r
             if (e < nt * nt) {
                 const int r = e % nt, c = e / nt;
                 double s = 0.0;
-                for (int q = 0; q < mq; ++q) s += Z[q * 65 + r] * Z[q * 65 + c];
+                for (int q = 0; q < mq; ++q) s += buf[q * ld + r] * buf[q * ld + c];
                 acc[u] -= s;
             }
         }
     }
+    __syncthreads();
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const int e = tid + 256 * u;
@@ -1323,42 +1341,54 @@ __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__r
         }
     }
     __syncthreads();
-    // factor: thread j finishes column j's pivot, threads i > j its column
-    const int i = tid;
+    // Right-looking factor, one barrier per step: step j takes the pivot of
+    // column j and subtracts l_ij l_kj (l = column j over its root, formed
+    // where it is used) from the trailing lower triangle; column j itself is
+    // scaled in step j + 1, when nothing reads it.  Every element sees the
+    // same operations in the same order as in the left-looking form.
     for (int j = 0; j < nt; ++j) {
-        if (i == j) {
-            double d = L[j * 65 + j];
-            for (int p = 0; p < j; ++p) d -= L[j * 65 + p] * L[j * 65 + p];
-            const double lim = thr * (ref ? ref[(int64_t)j * ref_stride] : 1.0);
-            const bool kp = d > lim && d > 0.0;
+        const double d = L[j * 65 + j];
+        const bool kp = d > lim[j] && d > 0.0;
+        const double ljj = kp ? sqrt(d) : 0.0;
+        if (j > 0 && tid >= j && tid < nt) {
+            const int pj = j - 1;
+            L[tid * 65 + pj] = keep[pj] ? L[tid * 65 + pj] / dg[pj] : 0.0;
+        }
+        if (tid == 0) {
             keep[j] = kp ? 1 : 0;
-            L[j * 65 + j] = kp ? sqrt(d) : 0.0;
+            dg[j] = ljj;
         }
-        __syncthreads();
-        if (i > j && i < nt) {
-            double x = 0.0;
-            if (keep[j]) {
-                x = L[i * 65 + j];
-                for (int p = 0; p < j; ++p) x -= L[i * 65 + p] * L[j * 65 + p];
-                x /= L[j * 65 + j];
+        if (kp) {
+            const int w = nt - j - 1;
+            for (int e = tid; e < w * w; e += 256) {
+                const int i = j + 1 + e % w, k = j + 1 + e / w;
+                if (k <= i) L[i * 65 + k] -= (L[i * 65 + j] / ljj) * (L[k * 65 + j] / ljj);
             }
-            L[i * 65 + j] = x;
         }
         __syncthreads();
     }
-    // inverse on the kept indices: thread q solves column q
-    if (i < nt) {
-        const int q = i;
-        for (int r = 0; r < nt; ++r) {
-            double z = 0.0;
-            if (r >= q && keep[r] && keep[q]) {
-                z = r == q ? 1.0 : 0.0;
-                for (int p = q; p < r; ++p) z -= L[r * 65 + p] * Z[p * 65 + q];
-                z /= L[r * 65 + r];
-            }
-            Z[r * 65 + q] = z;
-        }
+    if (tid < nt) L[tid * 65 + tid] = dg[tid];
+    // inverse on the kept indices, row by row (forward substitution over all
+    // columns at once): step r subtracts L_ir (row r over its pivot) from the
+    // rows below; row r is divided in step r + 1
+    for (int e = tid; e < nt * nt; e += 256) {
+        const int r = e / nt, q = e % nt;
+        Z[r * 65 + q] = (r == q && keep[r]) ? 1.0 : 0.0;
     }
+    __syncthreads();
+    for (int r = 0; r < nt; ++r) {
+        if (r > 0 && tid < r && keep[r - 1] && keep[tid]) Z[(r - 1) * 65 + tid] /= dg[r - 1];
+        if (keep[r]) {
+            const double lrr = dg[r];
+            const int w = nt - r - 1, h = r + 1;
+            for (int e = tid; e < w * h; e += 256) {
+                const int i = r + 1 + e / h, q = e % h;
+                if (keep[i] && keep[q]) Z[i * 65 + q] -= L[i * 65 + r] * (Z[r * 65 + q] / lrr);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < nt && keep[nt - 1] && keep[tid]) Z[(nt - 1) * 65 + tid] /= dg[nt - 1];
     __syncthreads();
     for (int e = tid; e < nt * nt; e += 256) LinvT[e] = Z[(e / nt) * 65 + e % nt];
     if (tid < nt) alive[tid] = keep[tid] ? 1.0 : 0.0;
@@ -1367,6 +1397,7 @@ __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__r
 void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
                      int ref_stride, double thr, double *LinvT, double *alive) {
     if (nt <= 0) return;
+    if (nt > 64) throw Error(STDA_HIP_ERR_ARG, "chol_inv: more than 64 columns");
     hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(256), 0, s, nt, G, m, ldg, ref, ref_stride,
                        thr, LinvT, alive);
 }
