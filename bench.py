@@ -42,8 +42,9 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 # HBM bytes per launch of PT2 pass 1 over the mixed512 molecules in the plain
 # pass: FETCH_SIZE (x2) + WRITE_SIZE (x1) counters, scaled by a 1 GiB copy
-# (profiles/pt2_pipe_mixed512_pmc.md)
-PT2_MIXED_TRAFFIC = 7.12e8
+# (profiles/pt2_pipe_mixed512_pmc.md); the writes are the pass-1 terms kept
+# for pass 2
+PT2_MIXED_TRAFFIC = 9.38e8
 HBM_PEAK_GBS = 8000.0
 # HIP-event timed statistics of the library: single kernels (one launch per
 # record: the PT2 row pass, the lock-step operator kernel, the slice partials
@@ -378,7 +379,7 @@ def run_mixed(args):
     except Exception as exc:
         prep = {"error": repr(exc)}
     t, cnt = phases[-1]
-    # Roofline: PT2 pass 1 (pt2_pipe_kernel<0>), the kernel with the largest
+    # Roofline: PT2 pass 1 (pt2_pipe_kernel), the kernel with the largest
     # summed device time in the batched step (rocprof, profiles/).  Its rate is
     # taken from the sequential plain FP64 pass over the same molecules, where
     # one stream runs at a time and an event pair brackets the kernel alone;
@@ -387,7 +388,7 @@ def run_mixed(args):
     # hold the waits for it.  Algorithmic work: 4 Na Nsec Nprim flops.
     pst = plain_kst.get("pt2_rows", {"ms": 0.0, "launches": 0.0, "work": 0.0})
     achieved = pst["work"] / (pst["ms"] * 1e-3) / 1e12 if pst["ms"] > 0 else 0.0
-    roofline = {"kernel": "pt2_rows (pt2_pipe_kernel<0>, PT2 pass 1)", "bound": "mfma",
+    roofline = {"kernel": "pt2_rows (pt2_pipe_kernel, PT2 pass 1)", "bound": "mfma",
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": PT2_MIXED_TRAFFIC,
                 "avg_launch_ms": round(pst["ms"] / max(pst["launches"], 1.0), 4),

@@ -113,16 +113,18 @@ constexpr int PT2_SLOTS = PT2_T * PT2_T;
 // Pass 1: for every secondary of the row tiles its sum of pt over all
 // primaries.  drow/dcol (PT2_SLOTS per tile) hold the de of each tile slot,
 // NaN where the slot is not a secondary (rows) or primary (cols).  The column
-// tiles are split over nsplit work-groups per row tile; rowpart holds
-// nsplit x rows.ntiles x PT2_SLOTS partial sums, summed in split order into
-// rowsum[pair] for the secondaries.  With ptm non-null every pt term is also
-// stored at ptm[(PT2_SLOTS rt + row slot) * ldp + PT2_SLOTS ct + col slot]
-// (0 for slots that are not a secondary-primary pair), ldp = PT2_SLOTS *
-// cols.ntiles.
+// tiles are split over nsplit work-groups per row tile; rowpart receives
+// nsplit x rows.ntiles x PT2_SLOTS partial sums.  With ptm non-null every pt
+// term is also stored at ptm[(PT2_SLOTS rt + row slot) * ldp + PT2_SLOTS ct +
+// col slot] (0 for slots that are not a secondary-primary pair), ldp =
+// PT2_SLOTS * cols.ntiles.
 void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *drow, const double *dcol, double ak, int nsplit,
-                     double *rowpart, double *rowsum, double *ptm, int64_t ldp);
+                     double *rowpart, double *ptm, int64_t ldp);
+// rowsum[pair] of every secondary slot: its partial sums added in split order.
+void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,
+                            const double *rowpart, double *rowsum);
 // Pass 2 from the stored terms: colsum[c] = sum over the rows rrow[0..nrej)
 // (ascending) of ptm[r * ldp + c] for every column c < ldp, the rows split
 // over nsplit work-groups per column block and the splits summed in order.

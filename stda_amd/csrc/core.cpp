@@ -1632,8 +1632,9 @@ static void do_select(stda_hip_ctx *c) {
         EventTimer ev1(c);
         launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
                         c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
-                        rowsum.get(), ptm.get(), ldp);
+                        ptm.get(), ldp);
         ev1.finish("pt2_rows", 4.0 * na * (double)c->nsec * (double)c->nprim);
+        launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
         std::vector<double> rs(c->npair);
         rowsum.download(rs.data(), rs.size(), s);
         c->sync();

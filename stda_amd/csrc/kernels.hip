@@ -997,7 +997,7 @@ __global__ void pt2_rows_finish_kernel(TileSet rows, int nv, int nsplit,
 void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *drow, const double *dcol, double ak, int nsplit,
-                     double *rowpart, double *rowsum, double *ptm, int64_t ldp) {
+                     double *rowpart, double *ptm, int64_t ldp) {
     if (rows.ntiles == 0 || cols.ntiles == 0) return;
     if (nsplit < 1 || nsplit > cols.ntiles)
         throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1: split count outside [1, column tiles]");
@@ -1005,7 +1005,12 @@ void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int n
     if (nitem > INT32_MAX) throw Error(STDA_HIP_ERR_TOO_LARGE, "PT2 pass 1: too many row tiles");
     hipLaunchKernelGGL(pt2_pipe_kernel, dim3((unsigned)nitem), dim3(256), 0, s, rows, cols, ncent,
                        nv, pK, q, pJ, qab, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
+}
+
+void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,
+                            const double *rowpart, double *rowsum) {
     const int64_t nslot = (int64_t)rows.ntiles * PT2_SLOTS;
+    if (nslot == 0) return;
     hipLaunchKernelGGL(pt2_rows_finish_kernel, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0,
                        s, rows, nv, nsplit, drow, rowpart, rowsum);
 }
@@ -1035,12 +1040,22 @@ __global__ void __launch_bounds__(256) pt2_colsum_kernel(int nrej, const int32_t
     part[(int64_t)sp * ldp + col] = s;
 }
 
+// colsum[c] = the split partials of column c added in split order (loads
+// issued eight at a time)
 __global__ void pt2_colsum_finish_kernel(int64_t ldp, int nsplit, const double *__restrict__ part,
                                          double *__restrict__ colsum) {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ldp) return;
     double s = part[col];
-    for (int sp = 1; sp < nsplit; ++sp) s += part[(int64_t)sp * ldp + col];
+    int sp = 1;
+    for (; sp + 8 <= nsplit; sp += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)(sp + u) * ldp + col];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; sp < nsplit; ++sp) s += part[(int64_t)sp * ldp + col];
     colsum[col] = s;
 }
 
