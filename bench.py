@@ -47,11 +47,13 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 PT2_MIXED_TRAFFIC = 9.38e8
 HBM_PEAK_GBS = 8000.0
 # HIP-event timed statistics of the library: single kernels (one launch per
-# record: the PT2 row pass, the lock-step operator kernel, the slice partials
+# record: the PT2 row pass (FP64; FP32 with the FP64 recheck of row tiles
+# near the threshold in mixed precision), the lock-step operator kernel, the slice partials
 # of the tall-skinny V^T W products and the V C products) and composite spans
 # (the AO operator products, several GEMMs and element-wise launches; PT2
 # pass 2 with its finishing kernel)
-SINGLE_KERNELS = ("pt2_rows", "batch_lockstep", "tsgemm_tn", "tsgemm_nn")
+SINGLE_KERNELS = ("pt2_rows", "pt2_rows_f32", "pt2_rows_recheck", "batch_lockstep", "tsgemm_tn",
+                  "tsgemm_nn")
 HBM_KERNELS = ("batch_lockstep", "tsgemm_tn", "tsgemm_nn")
 COMPOSITE_SPANS = ("pt2_cols", "ao_matvec", "ao_matvec_f32")
 EV = 27.211385050

@@ -122,6 +122,11 @@ void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int n
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *drow, const double *dcol, double ak, int nsplit,
                      double *rowpart, double *ptm, int64_t ldp);
+// The same with FP32 operands and MFMA (pt and its sums in FP64).
+void launch_pt2_pipe_f32(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                         const float *pK, const float *q, const float *pJ, const float *qab,
+                         const double *drow, const double *dcol, double ak, int nsplit,
+                         double *rowpart, double *ptm, int64_t ldp);
 // rowsum[pair] of every secondary slot: its partial sums added in split order.
 void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,
                             const double *rowpart, double *rowsum);

@@ -57,6 +57,11 @@ constexpr double kRot = 235.7220;             // stda.f:1300
 constexpr int kPt2Resident = 512;
 // PT2: largest matrix of pass-1 terms kept in device memory for pass 2.
 constexpr size_t kPt2StoreBytes = size_t(2) << 30;
+// PT2 pass 1 in FP32: a secondary whose row sum lies within this fraction of
+// thrp has its row tile evaluated again in FP64.  The FP32 sums carry
+// relative errors of order 1e-6 (sums of up to a few hundred products per
+// coupling); the margin is a thousand times that.
+constexpr double kPt2Recheck = 1.0e-3;
 
 // Symmetric eigensolver for the small subspace problems.  The image's LAPACK
 // (MKL's dsyevd) is loaded at run time by absolute path with RTLD_LOCAL, so
@@ -1629,15 +1634,72 @@ static void do_select(stda_hip_ctx *c) {
         const bool store = (double)PT2_SLOTS * nrt * (double)ldp * 8.0 <= store_max;
         DBuf<double> ptm(store ? (size_t)PT2_SLOTS * nrt * (size_t)ldp : 0);
         DBuf<double> rowsum(c->npair), rowpart((size_t)nsplit * nrt * PT2_SLOTS);
-        EventTimer ev1(c);
-        launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                        c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
-                        ptm.get(), ldp);
-        ev1.finish("pt2_rows", 4.0 * na * (double)c->nsec * (double)c->nprim);
-        launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
         std::vector<double> rs(c->npair);
-        rowsum.download(rs.data(), rs.size(), s);
-        c->sync();
+        const double work1 = 4.0 * na * (double)c->nsec * (double)c->nprim;
+        if (c->precision == 0) {
+            EventTimer ev1(c);
+            launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                            c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
+                            ptm.get(), ldp);
+            ev1.finish("pt2_rows", work1);
+            launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
+            rowsum.download(rs.data(), rs.size(), s);
+            c->sync();
+        } else {
+            // Mixed precision: pass 1 on FP32 copies of the operands (the
+            // reference's own real*4 couplings, stda.f:1781-1788), then the row
+            // tiles holding a secondary whose sum lies within kPt2Recheck of thrp
+            // again in FP64, with the same split: their sums are then those of
+            // the FP64 pass bit for bit, so the selection is the FP64 one.
+            const int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
+            DBuf<float> pk32((size_t)na * c->npair), q32((size_t)na * c->npair);
+            DBuf<float> pj32((size_t)na * nij), qab32((size_t)na * nab);
+            launch_to_f32(s, c->pia.get(), (int64_t)na * c->npair, pk32.get());
+            launch_to_f32(s, c->qia.get(), (int64_t)na * c->npair, q32.get());
+            launch_to_f32(s, c->pij.get(), (int64_t)na * nij, pj32.get());
+            launch_to_f32(s, c->qab.get(), (int64_t)na * nab, qab32.get());
+            EventTimer ev1(c);
+            launch_pt2_pipe_f32(s, rows, cols, na, nv, pk32.get(), q32.get(), pj32.get(),
+                                qab32.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
+                                ptm.get(), ldp);
+            ev1.finish("pt2_rows_f32", work1);
+            launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
+            rowsum.download(rs.data(), rs.size(), s);
+            c->sync();
+            std::vector<char> redo(nrt, 0);
+            int nredo = 0;
+            for (int32_t p : sec)
+                if (std::fabs(rs[p] - c->thrp) <= kPt2Recheck * c->thrp) {
+                    const int t = (int)(slot_of(rtile, p) / PT2_SLOTS);
+                    nredo += redo[t] ? 0 : 1;
+                    redo[t] = 1;
+                }
+            if (nredo > 0) {
+                std::vector<int32_t> ro2, rv2;
+                std::vector<double> dr2;
+                for (int t = 0; t < nrt; ++t) {
+                    if (!redo[t]) continue;
+                    ro2.insert(ro2.end(), ro.begin() + (size_t)t * PT2_T, ro.begin() + (size_t)(t + 1) * PT2_T);
+                    rv2.insert(rv2.end(), rv.begin() + (size_t)t * PT2_T, rv.begin() + (size_t)(t + 1) * PT2_T);
+                    dr2.insert(dr2.end(), drh.begin() + (size_t)t * PT2_SLOTS,
+                               drh.begin() + (size_t)(t + 1) * PT2_SLOTS);
+                }
+                DBuf<int32_t> ro2d, rv2d;
+                DBuf<double> dr2d, part2((size_t)nsplit * nredo * PT2_SLOTS);
+                ro2d.upload(ro2.data(), ro2.size(), s);
+                rv2d.upload(rv2.data(), rv2.size(), s);
+                dr2d.upload(dr2.data(), dr2.size(), s);
+                TileSet rows2{ro2d.get(), rv2d.get(), nredo};
+                EventTimer ev3(c);
+                launch_pt2_pipe(s, rows2, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
+                                c->qab.get(), dr2d.get(), dcd.get(), c->ak, nsplit, part2.get(),
+                                nullptr, 0);
+                ev3.finish("pt2_rows_recheck", 4.0 * na * (double)nredo * PT2_SLOTS * (double)c->nprim);
+                launch_pt2_rows_finish(s, rows2, nv, nsplit, dr2d.get(), part2.get(), rowsum.get());
+                rowsum.download(rs.data(), rs.size(), s);
+                c->sync();
+            }
+        }
         int64_t nexcl = 0;
         for (int32_t p : sec) {
             if (rs[p] > c->thrp) incl[p] = 1;

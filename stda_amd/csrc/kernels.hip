@@ -769,16 +769,35 @@ namespace stda {
 // s from one LDS buffer, the operands of step s+1 are fetched into registers
 // and then written to the other buffer, so a single barrier per step orders
 // everything.
+template <typename T>
 struct PipeBuf {
-    double pk[M_CH][M_LD];
-    double qc[M_CH][M_LD];
-    double pj[M_CH][M_LD];
-    double qv[M_CH][M_LD];
+    T pk[M_CH][M_LD];
+    T qc[M_CH][M_LD];
+    T pj[M_CH][M_LD];
+    T qv[M_CH][M_LD];
 };
+template <typename T>
 struct PipeSmem {
-    PipeBuf buf[2];
-    double xch[4][256];
+    PipeBuf<T> buf[2];
+    T xch[4][256];
     double red[4][64];
+};
+typedef float v4f __attribute__((ext_vector_type(4)));
+// MFMA 16x16x4 step and accumulator type per operand type.  Output layouts
+// (lane l, g = l / 16, c = l % 16): FP64 register i holds D[4i + g][c], FP32
+// register i holds D[4g + i][c].
+template <typename T> struct PipeMfma;
+template <> struct PipeMfma<double> {
+    typedef v4d acc;
+    static __device__ __forceinline__ acc step(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct PipeMfma<float> {
+    typedef v4f acc;
+    static __device__ __forceinline__ acc step(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
 };
 
 // Column index (pair or packed pair) of this thread's 16 staged elements, or -1.
@@ -820,13 +839,24 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // Work-group item = split sp x row tile own; it streams column tiles
 // [ct0, ct1) of that split.  de of the tile slots comes from drow/dcol (NaN
 // for slots outside the selection), read once per row tile and prefetched
-// with each column tile.
+// with each column tile.  T = double: FP64 operands and MFMA; T = float:
+// FP32 operands and MFMA (the reference's own precision for these couplings,
+// stda.f:1781-1788), pt and its sums in FP64 either way.
+//
+// A lane's K values are the rows (k, c) = (4kh + kk, 4chv + cc) and columns
+// (j, b) = (4jh + hi, 4bh + lo) of block blk = 2chv + bh, with (kk, cc) =
+// (i, g) for the FP64 layout and (g, i) for the FP32 one (register i); its J
+// values are (kk, jj = the other of i and g, cc = hi, bb = lo), moved to the
+// K lanes through LDS.
+template <typename T>
 __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
-    TileSet rows, TileSet cols, int ncent, int nv, const double *__restrict__ pK,
-    const double *__restrict__ q, const double *__restrict__ pJ, const double *__restrict__ qab,
+    TileSet rows, TileSet cols, int ncent, int nv, const T *__restrict__ pK,
+    const T *__restrict__ q, const T *__restrict__ pJ, const T *__restrict__ qab,
     const double *__restrict__ drow, const double *__restrict__ dcol, double ak, int nsplit,
     double *__restrict__ rowpart, double *__restrict__ ptm, int64_t ldp) {
-    __shared__ PipeSmem sm;
+    typedef typename PipeMfma<T>::acc Acc;
+    constexpr bool F64 = sizeof(T) == 8;
+    __shared__ PipeSmem<T> sm;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so the
     // items are regrouped to give each XCD a contiguous range (split-major):
     // the work-groups of one XCD then stream the same column tiles, whose
@@ -846,18 +876,21 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     const int iPK0 = M_T * (4 * kh + hi) + lo;
     const int iQ0 = M_T * (4 * jh + hi) + lo;
     const int iQV0 = M_T * hi + lo;   // + 32 ch + 4 bh
-    const double *src[4] = {pK, q, pJ, qab};
+    const T *src[4] = {pK, q, pJ, qab};
+    // (kk, cc) of register i: the row slot (4kh + kk) * 8 + 4chv + cc
+    auto kk_of = [&](int i) { return F64 ? i : g; };
+    auto cc_of = [&](int i) { return F64 ? g : i; };
 
-    // this lane's rows (4kh + i, 4chv + g) and columns (4jh + hi, 4bh + lo)
+    // this lane's rows and columns (4jh + hi, 4bh + lo)
     double dr[2][4], dc[2] = {0.0, 0.0}, dcn[2] = {0.0, 0.0};
 #pragma unroll
     for (int chv = 0; chv < 2; ++chv)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            dr[chv][i] = drow[(int64_t)own * PT2_SLOTS + (4 * kh + i) * M_T + 4 * chv + g];
+            dr[chv][i] = drow[(int64_t)own * PT2_SLOTS + (4 * kh + kk_of(i)) * M_T + 4 * chv + cc_of(i)];
 
     int32_t idx[4][4];
-    double pre[4][4];
+    T pre[4][4];
     auto fetch = [&](int ch) {
         int a = ch * M_CH + sa;
         bool in = a < ncent;
@@ -865,7 +898,7 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
         for (int o = 0; o < 4; ++o)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                pre[o][r] = (in && idx[o][r] >= 0) ? src[o][(int64_t)ncent * idx[o][r] + a] : 0.0;
+                pre[o][r] = (in && idx[o][r] >= 0) ? src[o][(int64_t)ncent * idx[o][r] + a] : T(0);
     };
     auto store = [&](int b) {
 #pragma unroll
@@ -893,13 +926,13 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     }
     __syncthreads();
 
-    v4d accJ[4], accK[4];
+    Acc accJ[4], accK[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        accJ[b] = v4d{0.0, 0.0, 0.0, 0.0};
-        accK[b] = v4d{0.0, 0.0, 0.0, 0.0};
+        accJ[b] = Acc{0, 0, 0, 0};
+        accK[b] = Acc{0, 0, 0, 0};
     }
-    // partial row sums of rows (4kh+i, 4chv+g), index 4 chv + i
+    // partial row sums of register i's rows of half chv, index 4 chv + i
     double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     for (int64_t step = 0; step < nsteps; ++step) {
@@ -912,45 +945,48 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
             if (ch2 == 0) set_tile(other + 1);
             fetch(ch2);
         }
-        const PipeBuf &B = sm.buf[cur];
+        const PipeBuf<T> &B = sm.buf[cur];
 #pragma unroll
         for (int s4 = 0; s4 < M_CH / 4; ++s4) {
             int arow = 4 * s4 + g;
-            double aJ = B.pj[arow][iPJ];
-            double aK0 = B.pk[arow][iPK0], aK1 = B.pk[arow][iPK0 + 4];
-            double bQ0 = B.qc[arow][iQ0], bQ1 = B.qc[arow][iQ0 + 4];
-            accJ[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0], accJ[0], 0, 0, 0);
-            accJ[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 4], accJ[1], 0, 0, 0);
-            accJ[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 32], accJ[2], 0, 0, 0);
-            accJ[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aJ, B.qv[arow][iQV0 + 36], accJ[3], 0, 0, 0);
-            accK[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ0, accK[0], 0, 0, 0);
-            accK[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK0, bQ1, accK[1], 0, 0, 0);
-            accK[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ0, accK[2], 0, 0, 0);
-            accK[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aK1, bQ1, accK[3], 0, 0, 0);
+            T aJ = B.pj[arow][iPJ];
+            T aK0 = B.pk[arow][iPK0], aK1 = B.pk[arow][iPK0 + 4];
+            T bQ0 = B.qc[arow][iQ0], bQ1 = B.qc[arow][iQ0 + 4];
+            accJ[0] = PipeMfma<T>::step(aJ, B.qv[arow][iQV0], accJ[0]);
+            accJ[1] = PipeMfma<T>::step(aJ, B.qv[arow][iQV0 + 4], accJ[1]);
+            accJ[2] = PipeMfma<T>::step(aJ, B.qv[arow][iQV0 + 32], accJ[2]);
+            accJ[3] = PipeMfma<T>::step(aJ, B.qv[arow][iQV0 + 36], accJ[3]);
+            accK[0] = PipeMfma<T>::step(aK0, bQ0, accK[0]);
+            accK[1] = PipeMfma<T>::step(aK0, bQ1, accK[1]);
+            accK[2] = PipeMfma<T>::step(aK1, bQ0, accK[2]);
+            accK[3] = PipeMfma<T>::step(aK1, bQ1, accK[3]);
         }
         if (ch == nch - 1) {
 #pragma unroll
             for (int blk = 0; blk < 4; ++blk) {
                 const int chv = blk >> 1, bh = blk & 1;
+                // J of register i: (kk, jj, cc, bb) = (kk_of(i), cc_of(i), hi, lo)
+                // stored at ((kk * 4 + cc) * 4 + jj) * 4 + bb
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    sm.xch[w][((i * 4 + hi) * 4 + g) * 4 + lo] = accJ[blk][i];
+                    sm.xch[w][((kk_of(i) * 4 + hi) * 4 + cc_of(i)) * 4 + lo] = accJ[blk][i];
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    double J = sm.xch[w][((i * 4 + g) * 4 + hi) * 4 + lo];
-                    double amat = ak * accK[blk][i] - J;
-                    double pt = amat * amat / (dr[chv][i] - dc[bh] + 1.0e-10);
+                    const T J = sm.xch[w][((kk_of(i) * 4 + cc_of(i)) * 4 + hi) * 4 + lo];
+                    const T amat = T(ak) * accK[blk][i] - J;
+                    const double am = (double)amat;
+                    double pt = am * am / (dr[chv][i] - dc[bh] + 1.0e-10);
                     const bool ok = !__builtin_isnan(pt);
                     if (ok) part[chv * 4 + i] += pt;
                     if (ptm)
-                        ptm[((int64_t)own * PT2_SLOTS + (4 * kh + i) * M_T + 4 * chv + g) * ldp +
+                        ptm[((int64_t)own * PT2_SLOTS + (4 * kh + kk_of(i)) * M_T + 4 * chv + cc_of(i)) * ldp +
                             (int64_t)other * PT2_SLOTS + (4 * jh + hi) * M_T + 4 * bh + lo] =
                             ok ? pt : 0.0;
                 }
                 __builtin_amdgcn_wave_barrier();
-                accJ[blk] = v4d{0.0, 0.0, 0.0, 0.0};
-                accK[blk] = v4d{0.0, 0.0, 0.0, 0.0};
+                accJ[blk] = Acc{0, 0, 0, 0};
+                accK[blk] = Acc{0, 0, 0, 0};
             }
             dc[0] = dcn[0];
             dc[1] = dcn[1];
@@ -966,13 +1002,16 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
         part[q8] = v;
     }
     if (r16 == 0)
-        for (int q8 = 0; q8 < 8; ++q8) sm.red[w][g * 8 + q8] = part[q8];   // q8 = 4 c2 + i
+        for (int q8 = 0; q8 < 8; ++q8) sm.red[w][g * 8 + q8] = part[q8];   // q8 = 4 chv + i
     __syncthreads();
     if (threadIdx.x < PT2_SLOTS) {
-        int t = threadIdx.x;
-        int k = t / M_T, c = t % M_T;
-        int kh2 = k / 4, i = k % 4, c2 = c / 4, gg = c % 4;
-        double sum = sm.red[2 * kh2][gg * 8 + c2 * 4 + i] + sm.red[2 * kh2 + 1][gg * 8 + c2 * 4 + i];
+        // slot t = 8 k + c, k = 4 kh2 + kk, c = 4 c2 + cc: the lane group g and
+        // register i that hold it (g = cc, i = kk in FP64; g = kk, i = cc in FP32)
+        const int t = threadIdx.x;
+        const int k = t / M_T, c = t % M_T;
+        const int kh2 = k / 4, kk = k % 4, c2 = c / 4, cc = c % 4;
+        const int gg = F64 ? cc : kk, ii = F64 ? kk : cc;
+        const double sum = sm.red[2 * kh2][gg * 8 + c2 * 4 + ii] + sm.red[2 * kh2 + 1][gg * 8 + c2 * 4 + ii];
         rowpart[((int64_t)sp * nown + own) * PT2_SLOTS + t] = sum;
     }
 }
@@ -994,17 +1033,34 @@ __global__ void pt2_rows_finish_kernel(TileSet rows, int nv, int nsplit,
     rowsum[(int64_t)k * nv + c] = s;
 }
 
-void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                     const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *drow, const double *dcol, double ak, int nsplit,
-                     double *rowpart, double *ptm, int64_t ldp) {
+template <typename T>
+static void pt2_pipe_launch(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                            const T *pK, const T *q, const T *pJ, const T *qab, const double *drow,
+                            const double *dcol, double ak, int nsplit, double *rowpart, double *ptm,
+                            int64_t ldp) {
     if (rows.ntiles == 0 || cols.ntiles == 0) return;
     if (nsplit < 1 || nsplit > cols.ntiles)
         throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1: split count outside [1, column tiles]");
     const int64_t nitem = (int64_t)rows.ntiles * nsplit;
     if (nitem > INT32_MAX) throw Error(STDA_HIP_ERR_TOO_LARGE, "PT2 pass 1: too many row tiles");
-    hipLaunchKernelGGL(pt2_pipe_kernel, dim3((unsigned)nitem), dim3(256), 0, s, rows, cols, ncent,
-                       nv, pK, q, pJ, qab, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
+    hipLaunchKernelGGL(pt2_pipe_kernel<T>, dim3((unsigned)nitem), dim3(256), 0, s, rows, cols,
+                       ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
+}
+
+void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                     const double *pK, const double *q, const double *pJ, const double *qab,
+                     const double *drow, const double *dcol, double ak, int nsplit,
+                     double *rowpart, double *ptm, int64_t ldp) {
+    pt2_pipe_launch<double>(s, rows, cols, ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit,
+                            rowpart, ptm, ldp);
+}
+
+void launch_pt2_pipe_f32(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
+                         const float *pK, const float *q, const float *pJ, const float *qab,
+                         const double *drow, const double *dcol, double ak, int nsplit,
+                         double *rowpart, double *ptm, int64_t ldp) {
+    pt2_pipe_launch<float>(s, rows, cols, ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit,
+                           rowpart, ptm, ldp);
 }
 
 void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,

@@ -484,6 +484,23 @@ def test_pt2_pass2_forms(cluster60, monkeypatch, store_max):
         _compare_selection(h, ref)
 
 
+def test_mixed_precision_selection_is_fp64(cluster60):
+    """Mixed precision runs PT2 pass 1 on FP32 operands and evaluates again in
+    FP64 every row tile with a secondary near the threshold: the CSF list is
+    the FP64 one, and the PT2-lowered diagonal (pass 2 from the FP32-derived
+    terms) agrees to 1e-8 Hartree."""
+    ci = cluster60
+    with _hip() as h:
+        h.setup(ci, 7.0, 0.25)
+        ic64, ed64 = h.csf()
+        h.set_precision("mixed")
+        h.setup(ci, 7.0, 0.25)
+        assert h.kernel_stats("pt2_rows_f32")["launches"] == 1
+        ic32, ed32 = h.csf()
+    np.testing.assert_array_equal(ic32, ic64)
+    np.testing.assert_allclose(ed32, ed64, atol=1e-8, rtol=0)
+
+
 def test_many_roots_against_dense():
     """50 roots: a Davidson subspace of up to 300 vectors, beyond the shapes
     of the hand-written tall-skinny kernels (split-K Gram products, V C with
