@@ -58,10 +58,12 @@ constexpr int kPt2Resident = 512;
 // PT2: largest matrix of pass-1 terms kept in device memory for pass 2.
 constexpr size_t kPt2StoreBytes = size_t(2) << 30;
 // PT2 pass 1 in FP32: a secondary whose row sum lies within this fraction of
-// thrp has its row tile evaluated again in FP64.  The FP32 sums carry
-// relative errors of order 1e-6 (sums of up to a few hundred products per
-// coupling); the margin is a thousand times that.
-constexpr double kPt2Recheck = 1.0e-3;
+// thrp has its row tile evaluated again in FP64.  The couplings are FP32 dot
+// products over the atoms, accumulated in order in 4-atom steps: relative
+// error at most about (Na / 4) 2^-24, 3e-6 for 200 atoms; the row sums of
+// 60-200-atom clusters measure at most 6e-8 against FP64.  The margin is 30
+// times the bound.
+constexpr double kPt2Recheck = 1.0e-4;
 
 // Symmetric eigensolver for the small subspace problems.  The image's LAPACK
 // (MKL's dsyevd) is loaded at run time by absolute path with RTLD_LOCAL, so

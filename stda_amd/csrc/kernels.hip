@@ -1645,16 +1645,18 @@ namespace stda {
 // C = A^T B once per 32 columns of A).
 //
 // C (m x p) = A^T B, A (n x m), B (n x p): work-group (s, ib) reduces the
-// rows of slice s for the 32 columns ib*32.. of A into part[s*m*p + i + m*j].
-// Rows are staged through LDS in 64-row chunks with coalesced loads, the next
-// chunk's loads in flight (registers) while the current one is multiplied;
-// the four waves take the chunk's rows in four interleaved sets and lane l
-// owns C(2*(l%16) + {0,1}, l/16 + 4u), u < NJ; the wave partials are added in
-// wave order at the end.  tsgemm_reduce_kernel then sums the slices in slice
-// order.
+// rows of slice s for the 32 columns ib*32.. of A into part[s*m*p + i + m*j],
+// on the matrix cores (v_mfma_f64_16x16x4f64, the rows as the reduction
+// index).  Rows are staged through LDS in 64-row chunks with coalesced loads,
+// the next chunk's loads in flight (registers) while the current one is
+// multiplied; the chunk's 16 steps of 4 rows are dealt to the four waves
+// (step s to wave s % 4).  Lane l = 16g + c supplies A^T[16mb + c][row 4s + g]
+// and B[row 4s + g][16pb + c]; register i of accumulator (mb, pb) holds
+// C(16mb + 4i + g, 16pb + c).  The wave partials are added in wave order at
+// the end.  tsgemm_reduce_kernel then sums the slices in slice order.
 constexpr int TS_ROWS = 32, TS_CH = 64;
 
-template <int NJ>
+template <int PB>
 __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
                                                                 const double *__restrict__ A,
                                                                 int m1,
@@ -1664,15 +1666,15 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
                                                                 int64_t ldb, int64_t rows,
                                                                 double *__restrict__ part) {
     // the m columns of A: A's first m1 columns, then those of A2 (same lda)
-    constexpr int PJ = 4 * NJ, QA = TS_CH * 32 / 256, QB = TS_CH * PJ / 256;
+    constexpr int PJ = 16 * PB, QA = TS_CH * 32 / 256, QB = TS_CH * PJ / 256;
     // LDS: the staged chunk (As[64][33], Bs[64][PJ+1]), then the wave partials
-    constexpr int LDS_STAGE = TS_CH * 33 + TS_CH * (PJ + 1), LDS_RED = 3 * 64 * 2 * NJ;
+    constexpr int LDS_STAGE = TS_CH * 33 + TS_CH * (PJ + 1), LDS_RED = 3 * 64 * 8 * PB;
     __shared__ double lds[LDS_STAGE > LDS_RED ? LDS_STAGE : LDS_RED];
     double *As = lds, *Bs = lds + TS_CH * 33;
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i0 = 32 * blockIdx.y, mi = min(32, m - i0);
-    const int ii = 2 * (l & 15), jg = l >> 4;
+    const int c = l & 15, g = l >> 4;
     const int64_t rb = (int64_t)blockIdx.x * rows, re = min(n, rb + rows);
     double pa[QA], pb[QB];
     auto fetch = [&](int64_t r0) {
@@ -1689,12 +1691,13 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
             pb[q] = (rr < nr && j < p) ? B[(r0 + rr) + ldb * j] : 0.0;
         }
     };
-    double acc[2][NJ];
+    v4d acc[2][PB];
 #pragma unroll
-    for (int u = 0; u < NJ; ++u) acc[0][u] = acc[1][u] = 0.0;
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int u = 0; u < PB; ++u) acc[mb][u] = v4d{0.0, 0.0, 0.0, 0.0};
     if (rb < re) fetch(rb);
     for (int64_t r0 = rb; r0 < re; r0 += TS_CH) {
-        const int nr = (int)min((int64_t)TS_CH, re - r0);
         __syncthreads();   // the previous chunk is consumed
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
@@ -1708,37 +1711,45 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
         }
         __syncthreads();
         if (r0 + TS_CH < re) fetch(r0 + TS_CH);
-        for (int rr = w; rr < nr; rr += 4) {
-            const double a0 = As[rr * 33 + ii], a1 = As[rr * 33 + ii + 1];
+        // rows past the slice end are zero in the staged chunk
 #pragma unroll
-            for (int u = 0; u < NJ; ++u) {
-                const double b = Bs[rr * (PJ + 1) + jg + 4 * u];
-                acc[0][u] = fma(a0, b, acc[0][u]);
-                acc[1][u] = fma(a1, b, acc[1][u]);
+        for (int st = 0; st < TS_CH / 16; ++st) {
+            const int rr = 4 * (w + 4 * st) + g;
+            const double a0 = As[rr * 33 + c], a1 = As[rr * 33 + 16 + c];
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const double b = Bs[rr * (PJ + 1) + 16 * u + c];
+                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, acc[0][u], 0, 0, 0);
+                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, acc[1][u], 0, 0, 0);
             }
         }
     }
     __syncthreads();   // every wave is done with the staged chunk
-    double *red = lds;  // red[(w-1)][v][l], v < 2*NJ
+    double *red = lds;  // red[(w-1)][v][l], v = 4 (PB mb + u) + i < 8 PB
     if (w > 0) {
 #pragma unroll
-        for (int u = 0; u < NJ; ++u) {
-            red[((w - 1) * 2 * NJ + u) * 64 + l] = acc[0][u];
-            red[((w - 1) * 2 * NJ + NJ + u) * 64 + l] = acc[1][u];
-        }
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int u = 0; u < PB; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    red[((w - 1) * 8 * PB + 4 * (PB * mb + u) + i) * 64 + l] = acc[mb][u][i];
     }
     __syncthreads();
     if (w == 0) {
         double *dst = part + (int64_t)blockIdx.x * m * p;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int u = 0; u < NJ; ++u) {
-                const int i = ii + h, j = jg + 4 * u, v = h * NJ + u;
-                const double s = ((acc[h][u] + red[(0 * 2 * NJ + v) * 64 + l]) +
-                                  red[(1 * 2 * NJ + v) * 64 + l]) + red[(2 * 2 * NJ + v) * 64 + l];
-                if (i < mi && j < p) dst[(i0 + i) + (int64_t)m * j] = s;
-            }
+            for (int u = 0; u < PB; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int v = 4 * (PB * mb + u) + i;
+                    const int ii = 16 * mb + 4 * i + g, j = 16 * u + c;
+                    const double s = ((acc[mb][u][i] + red[(0 * 8 * PB + v) * 64 + l]) +
+                                      red[(1 * 8 * PB + v) * 64 + l]) + red[(2 * 8 * PB + v) * 64 + l];
+                    if (ii < mi && j < p) dst[(i0 + ii) + (int64_t)m * j] = s;
+                }
     }
 }
 
@@ -1800,14 +1811,12 @@ void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A
     const int64_t rows = tsgemm_tn_rows(n, m, p);
     const dim3 grid(ns, (m + 31) / 32);
     if (!A2) A2 = A1;
-    if (p <= 8)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
-    else if (p <= 16)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+    if (p <= 16)
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<1>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
     else if (p <= 32)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<8>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
     else
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<16>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
 }
 
 void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
@@ -1830,14 +1839,15 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
     launch_tsgemm_tn_reduce(s, n, m, p, part, C);
 }
 
-// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= PJ <= 32
-// (PJ a multiple of 4).  A
-// work-group covers 64 rows; its four waves take the inner index in four
-// interleaved sets (i = g, g+4, ...), each lane loading all of its A values of
-// a 128-index chunk before the multiply-adds (32 loads in flight), and the
-// partial rows are added in wave order through LDS.  C is staged through LDS
-// in 128-row chunks.  beta == 0 never reads out.
-template <int PJ>
+// out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32, on the
+// matrix cores (v_mfma_f64_16x16x4f64).  A work-group covers 64 rows, wave w
+// the 16-row block 16w..16w+15, and NB column blocks of 16.  C is staged
+// through LDS in 128-row chunks; each lane loads its 32 A values of a chunk
+// (row 16w + c of the block, inner indices g + 4u: lane l = 16g + c supplies
+// operand A[c][g] of MFMA step u) before the multiplies, so 32 loads per lane
+// are in flight.  Register i of accumulator j holds row 16w + 4i + g, column
+// 16j + c.  beta == 0 never reads out.
+template <int NB>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A0,
                                                         const double *__restrict__ A1, int64_t lda,
@@ -1847,24 +1857,26 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
     // blockIdx.y selects the product: (A0, out0) or (A1, out1), same C
     const double *__restrict__ A = blockIdx.y ? A1 : A0;
     double *out = blockIdx.y ? out1 : out0;
-    constexpr int LDS_C = 128 * PJ, LDS_RED = 3 * PJ * 64;
-    __shared__ __align__(16) double lds[LDS_C > LDS_RED ? LDS_C : LDS_RED];
-    const int tid = threadIdx.x, l = tid & 63;
-    const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t r = (int64_t)blockIdx.x * 64 + l;
-    const bool live = r < n;
-    double acc[PJ];
+    constexpr int PJ = 16 * NB;
+    __shared__ double cs[128 * PJ];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * 64 + 16 * w;
+    const int64_t row = r0 + c;
+    const bool live = row < n;
+    v4d acc[NB];
 #pragma unroll
-    for (int j = 0; j < PJ; ++j) acc[j] = 0.0;
+    for (int j = 0; j < NB; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
     for (int i0 = 0; i0 < m; i0 += 128) {
         const int mi = min(128, m - i0);
         __syncthreads();
         for (int e = tid; e < 128 * PJ; e += 256) {
             const int i = e / PJ, j = e % PJ;
-            lds[e] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
+            cs[e] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
         }
         double a[32];
-        const double *ap = A + (live ? r : 0) + lda * (int64_t)(i0 + g);
+        const double *ap = A + (live ? row : 0) + lda * (int64_t)(i0 + g);
 #pragma unroll
         for (int u = 0; u < 32; ++u) {
             a[u] = (live && g + 4 * u < mi) ? *ap : 0.0;
@@ -1873,31 +1885,26 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 32; ++u) {
-            const int i = g + 4 * u;
-            if (i < mi) {
-                const double *ci = lds + i * PJ;
+            if (4 * u < mi) {
 #pragma unroll
-                for (int j = 0; j < PJ; ++j) acc[j] = fma(a[u], ci[j], acc[j]);
+                for (int j = 0; j < NB; ++j)
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], cs[(4 * u + g) * PJ + 16 * j + c],
+                                                                  acc[j], 0, 0, 0);
             }
         }
     }
-    __syncthreads();   // every wave is done with the staged C
-    if (g > 0) {
 #pragma unroll
-        for (int j = 0; j < PJ; ++j) lds[((g - 1) * PJ + j) * 64 + l] = acc[j];
-    }
-    __syncthreads();
-    if (g == 0 && live) {
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int j = 0; j < PJ; ++j)
-            if (j < p) {
-                const double sum = ((acc[j] + lds[(0 * PJ + j) * 64 + l]) + lds[(1 * PJ + j) * 64 + l]) +
-                                   lds[(2 * PJ + j) * 64 + l];
-                double v = alpha * sum;
-                if (beta != 0.0) v += beta * out[r + ldo * j];
-                out[r + ldo * j] = v;
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = r0 + 4 * i + g;
+            const int col = 16 * j + c;
+            if (r < n && col < p) {
+                double v = alpha * acc[j][i];
+                if (beta != 0.0) v += beta * out[r + ldo * col];
+                out[r + ldo * col] = v;
             }
-    }
+        }
 }
 
 void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A0,
@@ -1910,16 +1917,12 @@ void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, con
         const double *Cj = C + ldc * j0;
         double *oj0 = out0 + ldo * j0;
         double *oj1 = out1 ? out1 + ldo * j0 : oj0;
-#define STDA_NN_CASE(W)                                                                        \
-    case W / 4:                                                                                \
-        hipLaunchKernelGGL(tsgemm_nn_kernel<W>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,      \
-                           A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);                    \
-        break;
-        switch ((pj + 3) / 4) {
-            STDA_NN_CASE(4) STDA_NN_CASE(8) STDA_NN_CASE(12) STDA_NN_CASE(16)
-            STDA_NN_CASE(20) STDA_NN_CASE(24) STDA_NN_CASE(28) STDA_NN_CASE(32)
-        }
-#undef STDA_NN_CASE
+        if (pj <= 16)
+            hipLaunchKernelGGL(tsgemm_nn_kernel<1>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,
+                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);
+        else
+            hipLaunchKernelGGL(tsgemm_nn_kernel<2>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,
+                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);
     }
 }
 
