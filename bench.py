@@ -356,6 +356,7 @@ def run_mixed(args):
     with StdaHipBatch(0, workers=args.workers) as b:
         b.configure(args.nmax_batched, args.poll)
         b.configure_wide(not args.no_wide, two_phase=args.two_phase, nmax_wide=args.nmax_wide)
+        b.set_precision(args.batch_precision)
         for w in range(args.warmup):
             b.solve_tda(cis, thr, ax, nroots)
             log(f"# warmup {w + 1}/{args.warmup} done")
@@ -413,7 +414,10 @@ def run_mixed(args):
         "p99_ms": float(np.percentile(np.array(walls) * 1e3, 99)),
         "latency_note": "closed batch: every request of a step completes with the step",
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 operator products + f64 Davidson numerics and refinement",
+        "dtype": ("f32 operator products + f64 Davidson numerics and refinement"
+                  if args.batch_precision == "mixed" else
+                  "split-bf16 then f32 operator products on the single path, f32 in the lock "
+                  "steps + f64 Davidson numerics and refinement"),
         "plain_fp64": {"value": plain, "unit": "solves/s", "wall_s": plain_wall,
                        "kernels": plain_rates(plain_kst),
                        "p50_ms": float(np.percentile(plain_lat * 1e3, 50)),
@@ -425,6 +429,7 @@ def run_mixed(args):
                                f"basis, Ethr 7 eV, ax 0.25, 20 roots each, batched mixed precision",
                    "molecules": n, "molecules_with_csf": nok, "workers": args.workers,
                    "nmax_batched": args.nmax_batched, "nmax_wide": args.nmax_wide,
+                   "batch_precision": args.batch_precision,
                    "wide": not args.no_wide, "poll": args.poll, "host": host_info(),
                    "paths": dict(cnt, wide_of_batched=nwide), "phases_ms": t,
                    "phase_ms_summed": {k: round(v / 1e3, 2) for k, v in phase_us.items()
@@ -623,6 +628,9 @@ def main():
     ap.add_argument("--nmax-wide", type=int, default=16384,
                     help="CSF limit of the matrix-free lock step (0: none; larger molecules take "
                          "the single-molecule path)")
+    ap.add_argument("--batch-precision", choices=["mixed", "bf16"], default="mixed",
+                    help="mixed512: operator products of the single-path solves (FP32, or a "
+                         "split-bfloat16 phase before the FP32 one)")
     ap.add_argument("--two-phase", action="store_true",
                     help="lock-step iterations start once every molecule is prepared")
     ap.add_argument("--poll", type=int, default=4,

@@ -393,6 +393,13 @@ int stda_hip_batch_configure(stda_hip_batch *b, int64_t nmax_batched, int32_t po
  * then do not share the device).  No counterpart in the reference (its
  * driver solves one molecule per call: stda.f:53, main.f:928). */
 int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable, int64_t nmax_wide);
+/* Arithmetic of the operator products of the molecules solved on the
+ * single-molecule path of the workers: 1 FP32 (the default), 2 split-bfloat16
+ * (a split-bfloat16 phase to 3e-5, then FP32 to 1e-6, then the FP64 step, as
+ * stda_hip_set_precision 2).  The lock steps keep FP32 products; PT2 pass 1
+ * runs in FP32 with the FP64 recheck in both.  No counterpart in the
+ * reference. */
+int stda_hip_batch_set_precision(stda_hip_batch *b, int32_t precision);
 /* molecules of the last call solved in the matrix-free lock step (a subset of
  * counts3[0] of stda_hip_batch_timings) */
 int stda_hip_batch_wide_count(const stda_hip_batch *b, int64_t *nwide);

@@ -41,7 +41,7 @@ C_API = [
     "stda_hip_format_curve", "stda_hip_transform", "stda_hip_ao_ints",
     "stda_hip_prepare_ao", "stda_hip_index_size", "stda_hip_gspec_noalloc", "stda_hip_gemm",
     "stda_hip_batch_submit", "stda_hip_batch_result", "stda_hip_batch_stop",
-    "stda_hip_batch_configure_wide", "stda_hip_batch_wide_count",
+    "stda_hip_batch_configure_wide", "stda_hip_batch_wide_count", "stda_hip_batch_set_precision",
 ]
 
 
@@ -128,6 +128,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.stda_hip_batch_stop.argtypes = [P]
     L.stda_hip_batch_configure.argtypes = [P, I64, I32]
     L.stda_hip_batch_configure_wide.argtypes = [P, I32, I64]
+    L.stda_hip_batch_set_precision.argtypes = [P, I32]
     L.stda_hip_batch_wide_count.argtypes = [P, ctypes.POINTER(I64)]
     L.stda_hip_batch_phase_times.argtypes = [P, P]
     L.stda_hip_batch_kernel_stats.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(F64),
@@ -647,6 +648,14 @@ class StdaHipBatch:
         step of a call starts once every molecule is prepared."""
         st = self._L.stda_hip_batch_configure_wide(self._b, (2 if two_phase else 1) if enable else 0,
                                                    int(nmax_wide))
+        if st != 0:
+            raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
+
+    def set_precision(self, precision: str):
+        """Operator products of the single-path solves: "mixed" (FP32, the
+        default) or "bf16" (split-bfloat16 phase, then FP32, then FP64)."""
+        code = {"mixed": 1, "bf16": 2}[precision]
+        st = self._L.stda_hip_batch_set_precision(self._b, code)
         if st != 0:
             raise StdaHipError(st, self._L.stda_hip_error_string(st).decode())
 

@@ -3909,6 +3909,16 @@ int stda_hip_batch_configure_wide(stda_hip_batch *b, int32_t enable, int64_t nma
     return STDA_HIP_OK;
 }
 
+int stda_hip_batch_set_precision(stda_hip_batch *b, int32_t precision) {
+    if (!b || precision < 1 || precision > 2) return STDA_HIP_ERR_ARG;
+    if (b->engine) {
+        b->last_error = "continuous mode is running (stda_hip_batch_stop first)";
+        return STDA_HIP_ERR_STATE;
+    }
+    for (auto *c : b->workers) c->precision = precision;
+    return STDA_HIP_OK;
+}
+
 int stda_hip_batch_wide_count(const stda_hip_batch *b, int64_t *nwide) {
     if (!b || !nwide) return STDA_HIP_ERR_ARG;
     *nwide = b->nwide;

@@ -79,6 +79,29 @@ def test_batch_matches_fp64(molecules, plain):
         np.testing.assert_allclose(pr[:, :2], rpr[:, :2], atol=F_MIX, rtol=0)
 
 
+def test_batch_bf16_single_path(molecules, plain):
+    """Split-bfloat16 operator products on the single-molecule path of the
+    workers (stda_hip_batch_set_precision 2): same bar against plain FP64."""
+    from stda_amd.core import StdaHipBatch
+    with StdaHipBatch(0, workers=3) as b:
+        b.configure(nmax_batched=0)
+        b.configure_wide(False)
+        b.set_precision("bf16")
+        res = b.solve_tda(molecules, 7.0, 0.25, nroots=20, tol=1e-8, velcorr=True)
+        t, cnt = b.timings()
+    say(f"  bf16 single path: paths {cnt}")
+    assert cnt["single"] >= 10
+    for i, ((om, pr, st), ref) in enumerate(zip(res, plain)):
+        if isinstance(ref, int):
+            assert st == ref, (i, st, ref)
+            continue
+        assert st == 0, (i, st)
+        rom, rpr, nci = ref
+        assert len(om) == len(rom) == min(20, nci)
+        np.testing.assert_allclose(om, rom, atol=E_MIX, rtol=0)
+        np.testing.assert_allclose(pr[:, :2], rpr[:, :2], atol=F_MIX, rtol=0)
+
+
 def test_wide_matches_fp64(molecules, plain):
     """Matrix-free lock step: every molecule above 128 CSFs in the grouped
     kernels (FP32 operator products, FP64 Davidson numerics and refinement)."""
