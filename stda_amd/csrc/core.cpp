@@ -374,6 +374,10 @@ struct stda_hip_ctx {
     DBuf<double> tnbuf;   // split-K partial products of dgemm_tn
     DBuf<double> tn2buf;  // the two blocks of dgemm_tn2 on its split-K path
     DBuf<double> normbuf; // slice partials of column norms
+    // PT2 selection: the kept pass-1 terms and the FP32 operand copies, kept
+    // (and grown) from one molecule to the next instead of allocated per call
+    DBuf<double> pt2_terms;
+    DBuf<float> pt2_pk32, pt2_q32, pt2_pj32, pt2_qab32;
     int nb_max = 0;
     int nb_chunk = 0;     // vectors per operator block (from the free memory at first use)
     bool xfull_clean = false;
@@ -1634,7 +1638,8 @@ static void do_select(stda_hip_ctx *c) {
         double store_max = (double)kPt2StoreBytes;
         if (const char *e = std::getenv("STDA_HIP_PT2_STORE_MAX")) store_max = std::strtod(e, nullptr);
         const bool store = (double)PT2_SLOTS * nrt * (double)ldp * 8.0 <= store_max;
-        DBuf<double> ptm(store ? (size_t)PT2_SLOTS * nrt * (size_t)ldp : 0);
+        if (store) c->pt2_terms.ensure((size_t)PT2_SLOTS * nrt * (size_t)ldp);
+        double *ptm = store ? c->pt2_terms.get() : nullptr;
         DBuf<double> rowsum(c->npair), rowpart((size_t)nsplit * nrt * PT2_SLOTS);
         std::vector<double> rs(c->npair);
         const double work1 = 4.0 * na * (double)c->nsec * (double)c->nprim;
@@ -1642,7 +1647,7 @@ static void do_select(stda_hip_ctx *c) {
             EventTimer ev1(c);
             launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
                             c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
-                            ptm.get(), ldp);
+                            ptm, ldp);
             ev1.finish("pt2_rows", work1);
             launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
             rowsum.download(rs.data(), rs.size(), s);
@@ -1654,16 +1659,18 @@ static void do_select(stda_hip_ctx *c) {
             // again in FP64, with the same split: their sums are then those of
             // the FP64 pass bit for bit, so the selection is the FP64 one.
             const int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
-            DBuf<float> pk32((size_t)na * c->npair), q32((size_t)na * c->npair);
-            DBuf<float> pj32((size_t)na * nij), qab32((size_t)na * nab);
-            launch_to_f32(s, c->pia.get(), (int64_t)na * c->npair, pk32.get());
-            launch_to_f32(s, c->qia.get(), (int64_t)na * c->npair, q32.get());
-            launch_to_f32(s, c->pij.get(), (int64_t)na * nij, pj32.get());
-            launch_to_f32(s, c->qab.get(), (int64_t)na * nab, qab32.get());
+            c->pt2_pk32.ensure((size_t)na * c->npair);
+            c->pt2_q32.ensure((size_t)na * c->npair);
+            c->pt2_pj32.ensure((size_t)na * nij);
+            c->pt2_qab32.ensure((size_t)na * nab);
+            launch_to_f32(s, c->pia.get(), (int64_t)na * c->npair, c->pt2_pk32.get());
+            launch_to_f32(s, c->qia.get(), (int64_t)na * c->npair, c->pt2_q32.get());
+            launch_to_f32(s, c->pij.get(), (int64_t)na * nij, c->pt2_pj32.get());
+            launch_to_f32(s, c->qab.get(), (int64_t)na * nab, c->pt2_qab32.get());
             EventTimer ev1(c);
-            launch_pt2_pipe_f32(s, rows, cols, na, nv, pk32.get(), q32.get(), pj32.get(),
-                                qab32.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
-                                ptm.get(), ldp);
+            launch_pt2_pipe_f32(s, rows, cols, na, nv, c->pt2_pk32.get(), c->pt2_q32.get(),
+                                c->pt2_pj32.get(), c->pt2_qab32.get(), drd.get(), dcd.get(), c->ak,
+                                nsplit, rowpart.get(), ptm, ldp);
             ev1.finish("pt2_rows_f32", work1);
             launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
             rowsum.download(rs.data(), rs.size(), s);
@@ -1721,7 +1728,7 @@ static void do_select(stda_hip_ctx *c) {
                 1, std::min<int64_t>(nexcl, (1024 + ncb - 1) / ncb));
             DBuf<double> part((size_t)nsp2 * ldp), colsum((size_t)ldp);
             EventTimer ev2(c);
-            launch_pt2_colsum(s, (int)nexcl, rrowd.get(), ldp, ptm.get(), nsp2, part.get(),
+            launch_pt2_colsum(s, (int)nexcl, rrowd.get(), ldp, ptm, nsp2, part.get(),
                               colsum.get());
             ev2.finish("pt2_cols", 4.0 * na * (double)nexcl * (double)c->nprim);
             std::vector<double> cs((size_t)ldp);
