@@ -122,11 +122,16 @@ void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int n
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *drow, const double *dcol, double ak, int nsplit,
                      double *rowpart, double *ptm, int64_t ldp);
-// The same with FP32 operands and MFMA (pt and its sums in FP64).
+// The same with FP32 operands and MFMA (pt and its sums in FP64); the
+// operand columns are lda >= ncent apart, lda a multiple of 4, zero-padded
+// (launch_to_f32_padded).
 void launch_pt2_pipe_f32(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                          const float *pK, const float *q, const float *pJ, const float *qab,
-                         const double *drow, const double *dcol, double ak, int nsplit,
-                         double *rowpart, double *ptm, int64_t ldp);
+                         int64_t lda, const double *drow, const double *dcol, double ak,
+                         int nsplit, double *rowpart, double *ptm, int64_t ldp);
+// y (ld x cols, FP32) = x (rows x cols, ld rows, FP64), rows rows..ld zero
+void launch_to_f32_padded(hipStream_t s, const double *x, int rows, int64_t cols, int ld,
+                          float *y);
 // rowsum[pair] of every secondary slot: its partial sums added in split order.
 void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,
                             const double *rowpart, double *rowsum);

@@ -1659,18 +1659,20 @@ static void do_select(stda_hip_ctx *c) {
             // again in FP64, with the same split: their sums are then those of
             // the FP64 pass bit for bit, so the selection is the FP64 one.
             const int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
-            c->pt2_pk32.ensure((size_t)na * c->npair);
-            c->pt2_q32.ensure((size_t)na * c->npair);
-            c->pt2_pj32.ensure((size_t)na * nij);
-            c->pt2_qab32.ensure((size_t)na * nab);
-            launch_to_f32(s, c->pia.get(), (int64_t)na * c->npair, c->pt2_pk32.get());
-            launch_to_f32(s, c->qia.get(), (int64_t)na * c->npair, c->pt2_q32.get());
-            launch_to_f32(s, c->pij.get(), (int64_t)na * nij, c->pt2_pj32.get());
-            launch_to_f32(s, c->qab.get(), (int64_t)na * nab, c->pt2_qab32.get());
+            // columns padded to a multiple of 4 atoms (16-byte loads)
+            const int na4 = (na + 3) / 4 * 4;
+            c->pt2_pk32.ensure((size_t)na4 * c->npair);
+            c->pt2_q32.ensure((size_t)na4 * c->npair);
+            c->pt2_pj32.ensure((size_t)na4 * nij);
+            c->pt2_qab32.ensure((size_t)na4 * nab);
+            launch_to_f32_padded(s, c->pia.get(), na, c->npair, na4, c->pt2_pk32.get());
+            launch_to_f32_padded(s, c->qia.get(), na, c->npair, na4, c->pt2_q32.get());
+            launch_to_f32_padded(s, c->pij.get(), na, nij, na4, c->pt2_pj32.get());
+            launch_to_f32_padded(s, c->qab.get(), na, nab, na4, c->pt2_qab32.get());
             EventTimer ev1(c);
             launch_pt2_pipe_f32(s, rows, cols, na, nv, c->pt2_pk32.get(), c->pt2_q32.get(),
-                                c->pt2_pj32.get(), c->pt2_qab32.get(), drd.get(), dcd.get(), c->ak,
-                                nsplit, rowpart.get(), ptm, ldp);
+                                c->pt2_pj32.get(), c->pt2_qab32.get(), na4, drd.get(), dcd.get(),
+                                c->ak, nsplit, rowpart.get(), ptm, ldp);
             ev1.finish("pt2_rows_f32", work1);
             launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
             rowsum.download(rs.data(), rs.size(), s);

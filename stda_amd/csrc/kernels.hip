@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 namespace stda {
 
@@ -769,16 +770,16 @@ namespace stda {
 // s from one LDS buffer, the operands of step s+1 are fetched into registers
 // and then written to the other buffer, so a single barrier per step orders
 // everything.
-template <typename T>
+template <typename T, int CH>
 struct PipeBuf {
-    T pk[M_CH][M_LD];
-    T qc[M_CH][M_LD];
-    T pj[M_CH][M_LD];
-    T qv[M_CH][M_LD];
+    T pk[CH][M_LD];
+    T qc[CH][M_LD];
+    T pj[CH][M_LD];
+    T qv[CH][M_LD];
 };
-template <typename T>
+template <typename T, int CH>
 struct PipeSmem {
-    PipeBuf<T> buf[2];
+    PipeBuf<T, CH> buf[2];
     T xch[4][256];
     double red[4][64];
 };
@@ -800,15 +801,15 @@ template <> struct PipeMfma<float> {
     }
 };
 
-// Column index (pair or packed pair) of this thread's 16 staged elements, or -1.
-// o = 0 pK(k,c), 1 q(j,b), 2 pJ(k,j), 3 qab(c,b); the row tile gives k, c and
-// the column tile j, b.
+// Column index (pair or packed pair) of this thread's VPT staged vectors of
+// each operand (vectors vb + VS r), or -1.  o = 0 pK(k,c), 1 q(j,b),
+// 2 pJ(k,j), 3 qab(c,b); the row tile gives k, c and the column tile j, b.
+template <int VS, int VPT>
 __device__ __forceinline__ void stage_cols(TileSet rows, int rt, TileSet cols, int ct, int nv,
-                                           int32_t idx[4][4]) {
-    int vb = threadIdx.x / M_CH;
+                                           int vb, int32_t idx[4][VPT]) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        int v = vb + 16 * r, x = v / M_T, y = v % M_T;
+    for (int r = 0; r < VPT; ++r) {
+        int v = vb + VS * r, x = v / M_T, y = v % M_T;
         int kx = rows.occ[rt * M_T + x], cy = rows.vir[rt * M_T + y];
         int jx = cols.occ[ct * M_T + x], by = cols.vir[ct * M_T + y];
         int jy = cols.occ[ct * M_T + y], cx = rows.vir[rt * M_T + x];
@@ -851,12 +852,20 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 template <typename T>
 __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     TileSet rows, TileSet cols, int ncent, int nv, const T *__restrict__ pK,
-    const T *__restrict__ q, const T *__restrict__ pJ, const T *__restrict__ qab,
+    const T *__restrict__ q, const T *__restrict__ pJ, const T *__restrict__ qab, int64_t lda,
     const double *__restrict__ drow, const double *__restrict__ dcol, double ak, int nsplit,
     double *__restrict__ rowpart, double *__restrict__ ptm, int64_t ldp) {
     typedef typename PipeMfma<T>::acc Acc;
     constexpr bool F64 = sizeof(T) == 8;
-    __shared__ PipeSmem<T> sm;
+    // atoms per step: FP32 steps take 32 (the same 2048 matrix-core cycles per
+    // wave as the FP64 steps of 16, and the same LDS bytes); each FP32 load
+    // brings 4 consecutive atoms of a vector (lda a multiple of 4, the padding
+    // zero), each FP64 load one.  TPV threads share a vector, VS vectors per
+    // pass, VPT passes cover the 64 vectors of an operand.
+    constexpr int W = F64 ? 1 : 4;
+    constexpr int CH = F64 ? M_CH : 2 * M_CH, TPV = CH / W, VS = 256 / TPV, VPT = 64 / VS;
+    typedef typename std::conditional<F64, double, v4f>::type LoadT;
+    __shared__ PipeSmem<T, CH> sm;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so the
     // items are regrouped to give each XCD a contiguous range (split-major):
     // the work-groups of one XCD then stream the same column tiles, whose
@@ -866,12 +875,12 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     const int sp = item / nown, own = item % nown;
     const int ct0 = (int)((int64_t)cols.ntiles * sp / nsplit);
     const int ct1 = (int)((int64_t)cols.ntiles * (sp + 1) / nsplit);
-    const int nch = (ncent + M_CH - 1) / M_CH;
+    const int nch = (ncent + CH - 1) / CH;
     const int64_t nsteps = (int64_t)(ct1 - ct0) * nch;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int kh = w >> 1, jh = w & 1;
     const int r16 = lane & 15, g = lane >> 4, hi = r16 >> 2, lo = r16 & 3;
-    const int sa = threadIdx.x % M_CH, svb = threadIdx.x / M_CH;
+    const int sa = threadIdx.x % TPV, svb = threadIdx.x / TPV;
     const int iPJ = M_T * (4 * kh + hi) + (4 * jh + lo);
     const int iPK0 = M_T * (4 * kh + hi) + lo;
     const int iQ0 = M_T * (4 * jh + hi) + lo;
@@ -889,29 +898,40 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
         for (int i = 0; i < 4; ++i)
             dr[chv][i] = drow[(int64_t)own * PT2_SLOTS + (4 * kh + kk_of(i)) * M_T + 4 * chv + cc_of(i)];
 
-    int32_t idx[4][4];
-    T pre[4][4];
+    int32_t idx[4][VPT];
+    LoadT pre[4][VPT];
     auto fetch = [&](int ch) {
-        int a = ch * M_CH + sa;
-        bool in = a < ncent;
+        const int a = ch * CH + W * sa;
+        const bool in = a < ncent;
 #pragma unroll
         for (int o = 0; o < 4; ++o)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                pre[o][r] = (in && idx[o][r] >= 0) ? src[o][(int64_t)ncent * idx[o][r] + a] : T(0);
+            for (int r = 0; r < VPT; ++r)
+                pre[o][r] = (in && idx[o][r] >= 0)
+                                ? *reinterpret_cast<const LoadT *>(src[o] + lda * idx[o][r] + a)
+                                : LoadT(0);
     };
     auto store = [&](int b) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int v = svb + 16 * r;
-            sm.buf[b].pk[sa][v] = pre[0][r];
-            sm.buf[b].qc[sa][v] = pre[1][r];
-            sm.buf[b].pj[sa][v] = pre[2][r];
-            sm.buf[b].qv[sa][v] = pre[3][r];
+        for (int r = 0; r < VPT; ++r) {
+            const int v = svb + VS * r;
+#pragma unroll
+            for (int u = 0; u < W; ++u) {
+                T x[4];
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    if constexpr (F64) x[o] = pre[o][r];
+                    else x[o] = pre[o][r][u];
+                }
+                sm.buf[b].pk[W * sa + u][v] = x[0];
+                sm.buf[b].qc[W * sa + u][v] = x[1];
+                sm.buf[b].pj[W * sa + u][v] = x[2];
+                sm.buf[b].qv[W * sa + u][v] = x[3];
+            }
         }
     };
     auto set_tile = [&](int other) {
-        stage_cols(rows, own, cols, other, nv, idx);
+        stage_cols<VS, VPT>(rows, own, cols, other, nv, svb, idx);
 #pragma unroll
         for (int bh = 0; bh < 2; ++bh)
             dcn[bh] = dcol[(int64_t)other * PT2_SLOTS + (4 * jh + hi) * M_T + 4 * bh + lo];
@@ -945,9 +965,9 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
             if (ch2 == 0) set_tile(other + 1);
             fetch(ch2);
         }
-        const PipeBuf<T> &B = sm.buf[cur];
+        const PipeBuf<T, CH> &B = sm.buf[cur];
 #pragma unroll
-        for (int s4 = 0; s4 < M_CH / 4; ++s4) {
+        for (int s4 = 0; s4 < CH / 4; ++s4) {
             int arow = 4 * s4 + g;
             T aJ = B.pj[arow][iPJ];
             T aK0 = B.pk[arow][iPK0], aK1 = B.pk[arow][iPK0 + 4];
@@ -1035,32 +1055,52 @@ __global__ void pt2_rows_finish_kernel(TileSet rows, int nv, int nsplit,
 
 template <typename T>
 static void pt2_pipe_launch(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
-                            const T *pK, const T *q, const T *pJ, const T *qab, const double *drow,
-                            const double *dcol, double ak, int nsplit, double *rowpart, double *ptm,
-                            int64_t ldp) {
+                            const T *pK, const T *q, const T *pJ, const T *qab, int64_t lda,
+                            const double *drow, const double *dcol, double ak, int nsplit,
+                            double *rowpart, double *ptm, int64_t ldp) {
     if (rows.ntiles == 0 || cols.ntiles == 0) return;
     if (nsplit < 1 || nsplit > cols.ntiles)
         throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1: split count outside [1, column tiles]");
     const int64_t nitem = (int64_t)rows.ntiles * nsplit;
     if (nitem > INT32_MAX) throw Error(STDA_HIP_ERR_TOO_LARGE, "PT2 pass 1: too many row tiles");
     hipLaunchKernelGGL(pt2_pipe_kernel<T>, dim3((unsigned)nitem), dim3(256), 0, s, rows, cols,
-                       ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
+                       ncent, nv, pK, q, pJ, qab, lda, drow, dcol, ak, nsplit, rowpart, ptm, ldp);
 }
 
 void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
                      const double *drow, const double *dcol, double ak, int nsplit,
                      double *rowpart, double *ptm, int64_t ldp) {
-    pt2_pipe_launch<double>(s, rows, cols, ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit,
-                            rowpart, ptm, ldp);
+    pt2_pipe_launch<double>(s, rows, cols, ncent, nv, pK, q, pJ, qab, ncent, drow, dcol, ak,
+                            nsplit, rowpart, ptm, ldp);
 }
 
 void launch_pt2_pipe_f32(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                          const float *pK, const float *q, const float *pJ, const float *qab,
-                         const double *drow, const double *dcol, double ak, int nsplit,
-                         double *rowpart, double *ptm, int64_t ldp) {
-    pt2_pipe_launch<float>(s, rows, cols, ncent, nv, pK, q, pJ, qab, drow, dcol, ak, nsplit,
+                         int64_t lda, const double *drow, const double *dcol, double ak,
+                         int nsplit, double *rowpart, double *ptm, int64_t ldp) {
+    if (lda % 4 != 0 || lda < ncent)
+        throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1 (FP32): operand rows must be padded to 4");
+    pt2_pipe_launch<float>(s, rows, cols, ncent, nv, pK, q, pJ, qab, lda, drow, dcol, ak, nsplit,
                            rowpart, ptm, ldp);
+}
+
+// y (ld x cols) = x (rows x cols, ld rows) in FP32, the rows rows..ld zero
+__global__ void to_f32_padded_kernel(const double *__restrict__ x, int rows, int64_t cols, int ld,
+                                     float *__restrict__ y) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)ld * cols) return;
+    const int64_t c = e / ld;
+    const int r = (int)(e % ld);
+    y[e] = r < rows ? (float)x[(int64_t)rows * c + r] : 0.0f;
+}
+
+void launch_to_f32_padded(hipStream_t s, const double *x, int rows, int64_t cols, int ld,
+                          float *y) {
+    const int64_t n = (int64_t)ld * cols;
+    if (n == 0) return;
+    hipLaunchKernelGGL(to_f32_padded_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       x, rows, cols, ld, y);
 }
 
 void launch_pt2_rows_finish(hipStream_t s, TileSet rows, int nv, int nsplit, const double *drow,
