@@ -113,15 +113,18 @@ constexpr int PT2_SLOTS = PT2_T * PT2_T;
 // Pass 1: for every secondary of the row tiles its sum of pt over all
 // primaries.  drow/dcol (PT2_SLOTS per tile) hold the de of each tile slot,
 // NaN where the slot is not a secondary (rows) or primary (cols).  The column
-// tiles are split over nsplit work-groups per row tile; rowpart receives
+// tiles are split over nsplit work-groups per row tile; the operand columns
+// are lda >= ncent apart, lda even, zero-padded (launch_pad_rows); rowpart receives
 // nsplit x rows.ntiles x PT2_SLOTS partial sums.  With ptm non-null every pt
 // term is also stored at ptm[(PT2_SLOTS rt + row slot) * ldp + PT2_SLOTS ct +
 // col slot] (0 for slots that are not a secondary-primary pair), ldp =
 // PT2_SLOTS * cols.ntiles.
 void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *drow, const double *dcol, double ak, int nsplit,
+                     int64_t lda, const double *drow, const double *dcol, double ak, int nsplit,
                      double *rowpart, double *ptm, int64_t ldp);
+// y (ld x cols) = x (rows x cols, ld rows) with the rows rows..ld zero
+void launch_pad_rows(hipStream_t s, const double *x, int rows, int64_t cols, int ld, double *y);
 // The same with FP32 operands and MFMA (pt and its sums in FP64); the
 // operand columns are lda >= ncent apart, lda a multiple of 4, zero-padded
 // (launch_to_f32_padded).

@@ -378,6 +378,7 @@ struct stda_hip_ctx {
     // (and grown) from one molecule to the next instead of allocated per call
     DBuf<double> pt2_terms;
     DBuf<float> pt2_pk32, pt2_q32, pt2_pj32, pt2_qab32;
+    DBuf<double> pt2_pk64, pt2_q64, pt2_pj64, pt2_qab64;   // odd atom counts: padded to even
     int nb_max = 0;
     int nb_chunk = 0;     // vectors per operator block (from the free memory at first use)
     bool xfull_clean = false;
@@ -1643,11 +1644,27 @@ static void do_select(stda_hip_ctx *c) {
         DBuf<double> rowsum(c->npair), rowpart((size_t)nsplit * nrt * PT2_SLOTS);
         std::vector<double> rs(c->npair);
         const double work1 = 4.0 * na * (double)c->nsec * (double)c->nprim;
+        const int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
+        // FP64 operands with an even column stride (16-byte loads): the arrays
+        // themselves for an even atom count, zero-padded copies otherwise
+        const double *op64[4] = {c->pia.get(), c->qia.get(), c->pij.get(), c->qab.get()};
+        int na2 = na;
+        auto fp64_operands = [&]() {
+            if (na % 2 == 0 || na2 != na) return;
+            na2 = na + 1;
+            DBuf<double> *buf[4] = {&c->pt2_pk64, &c->pt2_q64, &c->pt2_pj64, &c->pt2_qab64};
+            const int64_t ncol[4] = {c->npair, c->npair, nij, nab};
+            for (int o = 0; o < 4; ++o) {
+                buf[o]->ensure((size_t)na2 * ncol[o]);
+                launch_pad_rows(s, op64[o], na, ncol[o], na2, buf[o]->get());
+                op64[o] = buf[o]->get();
+            }
+        };
         if (c->precision == 0) {
+            fp64_operands();
             EventTimer ev1(c);
-            launch_pt2_pipe(s, rows, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                            c->qab.get(), drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(),
-                            ptm, ldp);
+            launch_pt2_pipe(s, rows, cols, na, nv, op64[0], op64[1], op64[2], op64[3], na2,
+                            drd.get(), dcd.get(), c->ak, nsplit, rowpart.get(), ptm, ldp);
             ev1.finish("pt2_rows", work1);
             launch_pt2_rows_finish(s, rows, nv, nsplit, drd.get(), rowpart.get(), rowsum.get());
             rowsum.download(rs.data(), rs.size(), s);
@@ -1658,7 +1675,6 @@ static void do_select(stda_hip_ctx *c) {
             // tiles holding a secondary whose sum lies within kPt2Recheck of thrp
             // again in FP64, with the same split: their sums are then those of
             // the FP64 pass bit for bit, so the selection is the FP64 one.
-            const int64_t nij = (int64_t)no * (no + 1) / 2, nab = (int64_t)nv * (nv + 1) / 2;
             // columns padded to a multiple of 4 atoms (16-byte loads)
             const int na4 = (na + 3) / 4 * 4;
             c->pt2_pk32.ensure((size_t)na4 * c->npair);
@@ -1701,10 +1717,10 @@ static void do_select(stda_hip_ctx *c) {
                 rv2d.upload(rv2.data(), rv2.size(), s);
                 dr2d.upload(dr2.data(), dr2.size(), s);
                 TileSet rows2{ro2d.get(), rv2d.get(), nredo};
+                fp64_operands();
                 EventTimer ev3(c);
-                launch_pt2_pipe(s, rows2, cols, na, nv, c->pia.get(), c->qia.get(), c->pij.get(),
-                                c->qab.get(), dr2d.get(), dcd.get(), c->ak, nsplit, part2.get(),
-                                nullptr, 0);
+                launch_pt2_pipe(s, rows2, cols, na, nv, op64[0], op64[1], op64[2], op64[3], na2,
+                                dr2d.get(), dcd.get(), c->ak, nsplit, part2.get(), nullptr, 0);
                 ev3.finish("pt2_rows_recheck", 4.0 * na * (double)nredo * PT2_SLOTS * (double)c->nprim);
                 launch_pt2_rows_finish(s, rows2, nv, nsplit, dr2d.get(), part2.get(), rowsum.get());
                 rowsum.download(rs.data(), rs.size(), s);

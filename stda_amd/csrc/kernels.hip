@@ -784,6 +784,7 @@ struct PipeSmem {
     double red[4][64];
 };
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 // MFMA 16x16x4 step and accumulator type per operand type.  Output layouts
 // (lane l, g = l / 16, c = l % 16): FP64 register i holds D[4i + g][c], FP32
 // register i holds D[4g + i][c].
@@ -858,13 +859,13 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
     typedef typename PipeMfma<T>::acc Acc;
     constexpr bool F64 = sizeof(T) == 8;
     // atoms per step: FP32 steps take 32 (the same 2048 matrix-core cycles per
-    // wave as the FP64 steps of 16, and the same LDS bytes); each FP32 load
-    // brings 4 consecutive atoms of a vector (lda a multiple of 4, the padding
-    // zero), each FP64 load one.  TPV threads share a vector, VS vectors per
-    // pass, VPT passes cover the 64 vectors of an operand.
-    constexpr int W = F64 ? 1 : 4;
+    // wave as the FP64 steps of 16, and the same LDS bytes); each load brings
+    // 16 bytes, i.e. W consecutive atoms of a vector (lda a multiple of W, the
+    // padding zero).  TPV threads share a vector, VS vectors per pass, VPT
+    // passes cover the 64 vectors of an operand.
+    constexpr int W = F64 ? 2 : 4;
     constexpr int CH = F64 ? M_CH : 2 * M_CH, TPV = CH / W, VS = 256 / TPV, VPT = 64 / VS;
-    typedef typename std::conditional<F64, double, v4f>::type LoadT;
+    typedef typename std::conditional<F64, v2d, v4f>::type LoadT;
     __shared__ PipeSmem<T, CH> sm;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so the
     // items are regrouped to give each XCD a contiguous range (split-major):
@@ -919,10 +920,7 @@ __global__ void __launch_bounds__(256, 2) pt2_pipe_kernel(
             for (int u = 0; u < W; ++u) {
                 T x[4];
 #pragma unroll
-                for (int o = 0; o < 4; ++o) {
-                    if constexpr (F64) x[o] = pre[o][r];
-                    else x[o] = pre[o][r][u];
-                }
+                for (int o = 0; o < 4; ++o) x[o] = pre[o][r][u];
                 sm.buf[b].pk[W * sa + u][v] = x[0];
                 sm.buf[b].qc[W * sa + u][v] = x[1];
                 sm.buf[b].pj[W * sa + u][v] = x[2];
@@ -1069,10 +1067,29 @@ static void pt2_pipe_launch(hipStream_t s, TileSet rows, TileSet cols, int ncent
 
 void launch_pt2_pipe(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
                      const double *pK, const double *q, const double *pJ, const double *qab,
-                     const double *drow, const double *dcol, double ak, int nsplit,
+                     int64_t lda, const double *drow, const double *dcol, double ak, int nsplit,
                      double *rowpart, double *ptm, int64_t ldp) {
-    pt2_pipe_launch<double>(s, rows, cols, ncent, nv, pK, q, pJ, qab, ncent, drow, dcol, ak,
+    if (lda % 2 != 0 || lda < ncent)
+        throw Error(STDA_HIP_ERR_ARG, "PT2 pass 1: operand rows must be padded to 2");
+    pt2_pipe_launch<double>(s, rows, cols, ncent, nv, pK, q, pJ, qab, lda, drow, dcol, ak,
                             nsplit, rowpart, ptm, ldp);
+}
+
+// y (ld x cols) = x (rows x cols, ld rows), the rows rows..ld zero
+__global__ void pad_rows_kernel(const double *__restrict__ x, int rows, int64_t cols, int ld,
+                                double *__restrict__ y) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)ld * cols) return;
+    const int64_t c = e / ld;
+    const int r = (int)(e % ld);
+    y[e] = r < rows ? x[(int64_t)rows * c + r] : 0.0;
+}
+
+void launch_pad_rows(hipStream_t s, const double *x, int rows, int64_t cols, int ld, double *y) {
+    const int64_t n = (int64_t)ld * cols;
+    if (n == 0) return;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x,
+                       rows, cols, ld, y);
 }
 
 void launch_pt2_pipe_f32(hipStream_t s, TileSet rows, TileSet cols, int ncent, int nv,
