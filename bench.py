@@ -40,11 +40,12 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix
 # HBM bytes per launch of PT2 pass 1 over the mixed512 molecules in the plain
 # pass: FETCH_SIZE (x2) + WRITE_SIZE (x1) counters, scaled by a 1 GiB copy
 # (profiles/pt2_pipe_mixed512_pmc.md); the writes are the pass-1 terms kept
 # for pass 2
-PT2_MIXED_TRAFFIC = 9.38e8
+PT2_MIXED_TRAFFIC = 9.34e8
 HBM_PEAK_GBS = 8000.0
 # HIP-event timed statistics of the library: single kernels (one launch per
 # record: the PT2 row pass (FP64; FP32 with the FP64 recheck of row tiles
@@ -173,8 +174,9 @@ def run_single(args):
         nci, nprim, nsec = h.select()
         sol = h.solve_tda(p["nroots"], velcorr=True, vectors=False)
         lat.append(time.perf_counter() - ts)
-        stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_cols", "ao_matvec",
-                                                       "ao_matvec_f32")})
+        stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_rows_f32",
+                                                       "pt2_rows_recheck", "pt2_cols",
+                                                       "ao_matvec", "ao_matvec_f32")})
         log(f"# step {s + 1}/{args.steps}: {lat[-1]:.3f}s nci={nci} timings={sol.timings_ms}")
     total = time.perf_counter() - tstart
     root_err = None
@@ -189,9 +191,11 @@ def run_single(args):
     dom = max(last, key=lambda k: last[k]["ms"])
     st = last[dom]
     achieved = st["work"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
+    # the FP32 kernels against the FP32 matrix peak, the others against FP64
+    peak = FP32_PEAK_TFLOPS if dom in ("pt2_rows_f32", "ao_matvec_f32") else FP64_PEAK_TFLOPS
     roofline = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
                 "avg_launch_ms": round(st["ms"] / max(st["launches"], 1), 3),
                 "work_per_launch": st["work"] / max(st["launches"], 1)}
     prof = os.path.join(ROOT, "profiles", "pt2_pipe_cluster400_pmc.md")
