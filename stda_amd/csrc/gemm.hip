@@ -23,6 +23,8 @@
 // current chunk out of LDS.  Operand layouts (lane l, g = l / 16, c = l % 16):
 //   16x16x4:  A[c][g], B[g][c];       16x16x32 bf16: A[c][8g..8g+7], B[8g..8g+7][c]
 //   FP64 out: register i holds D[4i + g][c];   FP32 out: register i holds D[4g + i][c].
+// The tiles are formed as C^T blocks (B^T A^T: the B values as the first
+// operand), so that lane c holds row c of C and the stores are coalesced.
 // Edges are zero-filled on load and masked on store; blockIdx.z is the batch.
 
 #include "common.h"
@@ -188,9 +190,9 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<MODE, BT> &sm, int M, int N, 
                 const v8bf al = *reinterpret_cast<const v8bf *>(&sm.al[r][8 * g]);
 #pragma unroll
                 for (int j = 0; j < NB; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
                 }
             }
         } else {
@@ -207,23 +209,27 @@ __device__ __forceinline__ void gemm_tile(GemmSmem<MODE, BT> &sm, int M, int N, 
 #pragma unroll
                     for (int j = 0; j < NB; ++j) {
                         if constexpr (MODE == GEMM_F64)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
                         else
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[i][j], 0, 0, 0);
                     }
             }
         }
     }
-    // epilogue: alpha acc (+ beta C), masked at the edges
+    // epilogue: alpha acc (+ beta C), masked at the edges.  The products above
+    // take B as the MFMA's first operand, so acc[i][j] holds the transposed
+    // 16 x 16 block: lane c is the row of C and register e a column, and the
+    // 16 lanes of a group store 16 consecutive rows of one column (coalesced
+    // in column-major C)
 #pragma unroll
     for (int i = 0; i < NB; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int row = (MODE == GEMM_F64) ? 4 * e + g : 4 * g + e;
-                const int gm = m0 + wr * WT + i * 16 + row;
-                const int gn = n0 + wc * WT + j * 16 + c;
+                const int col = (MODE == GEMM_F64) ? 4 * e + g : 4 * g + e;
+                const int gm = m0 + wr * WT + i * 16 + c;
+                const int gn = n0 + wc * WT + j * 16 + col;
                 if (gm < M && gn < N) {
                     T v = alpha * acc[i][j][e];
                     T *p = C + gm + ldc * (int64_t)gn;
