@@ -1901,9 +1901,10 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
 // the 16-row block 16w..16w+15, and NB column blocks of 16.  C is staged
 // through LDS in 128-row chunks; each lane loads its 32 A values of a chunk
 // (row 16w + c of the block, inner indices g + 4u: lane l = 16g + c supplies
-// operand A[c][g] of MFMA step u) before the multiplies, so 32 loads per lane
-// are in flight.  Register i of accumulator j holds row 16w + 4i + g, column
-// 16j + c.  beta == 0 never reads out.
+// the A value of row c, inner index g, of MFMA step u) before the multiplies,
+// so 32 loads per lane are in flight.  The products take the C values as the
+// first operand (the transposed block), so register i of accumulator j holds
+// row 16w + c, column 16j + 4i + g.  beta == 0 never reads out.
 template <int NB>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A0,
@@ -1945,17 +1946,19 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
             if (4 * u < mi) {
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
-                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], cs[(4 * u + g) * PJ + 16 * j + c],
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cs[(4 * u + g) * PJ + 16 * j + c], a[u],
                                                                   acc[j], 0, 0, 0);
             }
         }
     }
+    // the C values are the first operand, so acc[j] holds the transposed block:
+    // lane c is the row (coalesced stores), register i with g the column
 #pragma unroll
     for (int j = 0; j < NB; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t r = r0 + 4 * i + g;
-            const int col = 16 * j + c;
+            const int64_t r = r0 + c;
+            const int col = 16 * j + 4 * i + g;
             if (r < n && col < p) {
                 double v = alpha * acc[j][i];
                 if (beta != 0.0) v += beta * out[r + ldo * col];
