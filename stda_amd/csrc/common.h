@@ -155,8 +155,11 @@ void launch_pt2_cols_sparse(hipStream_t s, int nrt, const int32_t *rk, const int
 // dropped: LinvT = L^-T on the kept columns, alive[j] = 1.0 if column j is kept
 // (pivot > thr * ref[j * ref_stride], or > thr when ref is null).  S =
 // G[m:m+nt, :] - G[0:m, :]^T G[0:m, :] with G ((m + nt) x nt, ld ldg).
+// skip (device, optional; needs ref = squared norms before the projection):
+// 1.0 when T LinvT needs no further projection against V, else 0.0.
 void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
-                     int ref_stride, double thr, double *LinvT, double *alive);
+                     int ref_stride, double thr, double *LinvT, double *alive,
+                     double *skip = nullptr);
 // column 2-norms of an (n x k) block through nsl row slices (part: nsl * k)
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
                             int nsl, double *norms);
@@ -223,22 +226,24 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
 // the two launches of launch_tsgemm_tn: slice partials, then their ordered sum
 void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
                               int64_t lda, const double *B, int64_t ldb, double *part);
-// slice partials of [A1 A2]^T B (m1 + m2 columns, same lda)
+// slice partials of [A1 A2]^T B (m1 + m2 columns, same lda); with skip1
+// (device) non-null and *skip1 != 0 the A1 rows come out zero, A1 unread
 void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A1, int m2,
                                const double *A2, int p, int64_t lda, const double *B, int64_t ldb,
-                               double *part);
+                               double *part, const double *skip1 = nullptr);
 void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
                              double *C);
 // out (n x p) = alpha A C + beta out with A (n x m), C (m x p); any m, p
-// (32 columns per launch)
+// (32 columns per launch); with skip (device) non-null and *skip != 0 the
+// launches return at once and out is left as it is
 void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
                       int64_t lda, const double *C, int64_t ldc, double beta, double *out,
-                      int64_t ldo);
+                      int64_t ldo, const double *skip = nullptr);
 // two such products with the same C in one launch: out0 = alpha A0 C + beta
 // out0, out1 = alpha A1 C + beta out1 (A1 null: the first only)
 void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A0,
                        const double *A1, int64_t lda, const double *C, int64_t ldc, double beta,
-                       double *out0, double *out1, int64_t ldo);
+                       double *out0, double *out1, int64_t ldo, const double *skip = nullptr);
 
 // --- batched Davidson (batch.hip): molecules in lock step ------------------------------
 constexpr int BATCH_MD = 64;                        // subspace limit

@@ -962,16 +962,21 @@ int orthonormalize(stda_hip_ctx *c, const double *V, int m, double *T, int nt, D
 // also carries the Gram matrix of the projected block (T^T T - C^T C), so the
 // Cholesky factors need no product of their own.  Dependent directions
 // become zero columns marked dead (alive[j] = 0.0, device): a direction is
-// dropped when less than 1e-7 of its norm is new.  T2: n x nt scratch; work:
-// 3 (m + nt) nt + nt^2 doubles.
+// dropped when less than 1e-7 of its norm is new.  The third projection is
+// left out on the device when the first factor shows that it cannot matter
+// (a flag from chol_inv_kernel: the inverse factor amplifies what the first
+// two projections left along V by at most 1e3; the V rows of its product then
+// come out zero and its update returns at once).  T2: n x nt scratch; work:
+// 3 (m + nt) nt + nt^2 + 1 doubles.
 void orthonormalize_dev(stda_hip_ctx *c, const double *V, int m, double *T, int nt, double *T2,
                         double *work, double *alive) {
     const int64_t n = c->nci;
     if (nt == 0) return;
     const int ld = m + nt;
     double *G1 = work, *G2 = G1 + (size_t)ld * nt, *G3 = G2 + (size_t)ld * nt;
-    double *Li = G3 + (size_t)ld * nt;
+    double *Li = G3 + (size_t)ld * nt, *skip = Li + (size_t)nt * nt;
     hipStream_t s = c->stream;
+    const bool cond = m > 0 && ld <= TS_MMAX && nt <= 32;   // hand-written kernels take the flag
     dgemm_tn2(c, m, V, nt, T, nt, n, n, T, n, G1);   // [V T]^T T: projections and squared norms
     const double *Gs = G1;
     if (m > 0) {
@@ -980,10 +985,17 @@ void orthonormalize_dev(stda_hip_ctx *c, const double *V, int m, double *T, int 
         gemm_nn(c, n, nt, m, -1.0, V, n, G2, ld, 1.0, T, n);
         Gs = G2;
     }
-    launch_chol_inv(s, nt, Gs, m, ld, G1 + m, ld + 1, 1.0e-14, Li, alive);
+    launch_chol_inv(s, nt, Gs, m, ld, G1 + m, ld + 1, 1.0e-14, Li, alive, cond ? skip : nullptr);
     gemm_nn(c, n, nt, nt, 1.0, T, n, Li, nt, 0.0, T2, n);
-    dgemm_tn2(c, m, V, nt, T2, nt, n, n, T2, n, G3);
-    if (m > 0) gemm_nn(c, n, nt, m, -1.0, V, n, G3, ld, 1.0, T2, n);
+    if (cond) {   // not timed: how much of it runs is known on the device only
+        c->tnbuf.ensure((size_t)tsgemm_tn_slices(n, ld, nt) * ld * nt);
+        launch_tsgemm_tn2_partial(s, n, m, V, nt, T2, nt, n, T2, n, c->tnbuf.get(), skip);
+        launch_tsgemm_tn_reduce(s, n, ld, nt, c->tnbuf.get(), G3);
+        launch_tsgemm_nn(s, n, m, nt, -1.0, V, n, G3, ld, 1.0, T2, n, skip);
+    } else {
+        dgemm_tn2(c, m, V, nt, T2, nt, n, n, T2, n, G3);
+        if (m > 0) gemm_nn(c, n, nt, m, -1.0, V, n, G3, ld, 1.0, T2, n);
+    }
     launch_chol_inv(s, nt, G3, m, ld, nullptr, 0, 1.0e-14, Li, alive);
     gemm_nn(c, n, nt, nt, 1.0, T2, n, Li, nt, 0.0, T, n);
 }
@@ -1025,7 +1037,7 @@ void davidson_tda(stda_hip_ctx *c, int k, double tol, std::vector<double> &theta
     DBuf<double> ow;
     if (devortho) {
         T2.resize((size_t)n * k);
-        ow.resize(3 * (size_t)(maxdim + k) * k + (size_t)k * k);
+        ow.resize(3 * (size_t)(maxdim + k) * k + (size_t)k * k + 1);
     }
     X.resize((size_t)n * k);
     // pinned staging: Gram block and alive flags | Y and theta | residual norms

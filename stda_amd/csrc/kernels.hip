@@ -1409,11 +1409,19 @@ __global__ void col_norm_final_kernel(int nsl, int k, const double *part, double
 // their projection T - V G[0:m, :] against orthonormal V: S = G[m:m+nt, :] -
 // G[0:m, :]^T G[0:m, :] is the Gram matrix of the projected block (m = 0:
 // G = T^T T).
+//
+// With skip non-null (and ref the squared norms of the directions before
+// their projection), skip[0] = 1.0 when max_k sum_j |Linv(k, j)| |t_j| <= 1e3,
+// else 0.0.  After two projections the part of t_j along V is a few ulps of
+// |t_j|, and column k of T LinvT carries at most that sum of them: at the
+// bound it stays below about 1e-13 and another projection of T LinvT against
+// V would change nothing that matters, so the caller may leave it out.
 __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__restrict__ G, int m,
                                                        int ldg, const double *__restrict__ ref,
                                                        int ref_stride, double thr,
                                                        double *__restrict__ LinvT,
-                                                       double *__restrict__ alive) {
+                                                       double *__restrict__ alive,
+                                                       double *__restrict__ skip) {
     // L and Z (the inverse factor) share one buffer, which first holds the
     // staged rows of G
     constexpr int SBUF = 2 * 64 * 65;
@@ -1520,14 +1528,31 @@ __global__ __launch_bounds__(256) void chol_inv_kernel(int nt, const double *__r
     __syncthreads();
     for (int e = tid; e < nt * nt; e += 256) LinvT[e] = Z[(e / nt) * 65 + e % nt];
     if (tid < nt) alive[tid] = keep[tid] ? 1.0 : 0.0;
+    if (skip && ref) {
+        // row k of Z holds the coefficients of column k of T LinvT; lim[]
+        // is no longer read, so it takes the row sums
+        if (tid < nt) {
+            double a = 0.0;
+            for (int j = 0; j <= tid; ++j)
+                a += fabs(Z[tid * 65 + j]) * sqrt(fmax(ref[(int64_t)j * ref_stride], 0.0));
+            lim[tid] = a;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0;
+            for (int k = 0; k < nt; ++k) a = fmax(a, lim[k]);
+            skip[0] = a <= 1.0e3 ? 1.0 : 0.0;   // NaN rows fail the test
+        }
+    }
 }
 
 void launch_chol_inv(hipStream_t s, int nt, const double *G, int m, int ldg, const double *ref,
-                     int ref_stride, double thr, double *LinvT, double *alive) {
+                     int ref_stride, double thr, double *LinvT, double *alive, double *skip) {
     if (nt <= 0) return;
     if (nt > 64) throw Error(STDA_HIP_ERR_ARG, "chol_inv: more than 64 columns");
+    if (skip && !ref) throw Error(STDA_HIP_ERR_ARG, "chol_inv: the skip test needs the norms");
     hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(256), 0, s, nt, G, m, ldg, ref, ref_stride,
-                       thr, LinvT, alive);
+                       thr, LinvT, alive, skip);
 }
 
 void launch_col_norms_split(hipStream_t s, int64_t n, int k, const double *R, double *part,
@@ -1721,8 +1746,10 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
                                                                 int64_t lda,
                                                                 const double *__restrict__ B,
                                                                 int64_t ldb, int64_t rows,
-                                                                double *__restrict__ part) {
-    // the m columns of A: A's first m1 columns, then those of A2 (same lda)
+                                                                double *__restrict__ part,
+                                                                const double *__restrict__ skip1) {
+    // the m columns of A: A's first m1 columns, then those of A2 (same lda);
+    // columns below lo read as zero (skip1 set: A's columns are not read)
     constexpr int PJ = 16 * PB, QA = TS_CH * 32 / 256, QB = TS_CH * PJ / 256;
     // LDS: the staged chunk (As[64][33], Bs[64][PJ+1]), then the wave partials
     constexpr int LDS_STAGE = TS_CH * 33 + TS_CH * (PJ + 1), LDS_RED = 3 * 64 * 8 * PB;
@@ -1733,6 +1760,12 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
     const int i0 = 32 * blockIdx.y, mi = min(32, m - i0);
     const int c = l & 15, g = l >> 4;
     const int64_t rb = (int64_t)blockIdx.x * rows, re = min(n, rb + rows);
+    const int lo = (skip1 && *skip1 != 0.0) ? m1 : 0;
+    if (i0 + mi <= lo) {   // every column of this block reads as zero
+        double *dst = part + (int64_t)blockIdx.x * m * p;
+        for (int e = tid; e < mi * p; e += 256) dst[(i0 + e % mi) + (int64_t)m * (e / mi)] = 0.0;
+        return;
+    }
     double pa[QA], pb[QB];
     auto fetch = [&](int64_t r0) {
         const int nr = (int)min((int64_t)TS_CH, re - r0);
@@ -1740,7 +1773,7 @@ __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m
         for (int q = 0; q < QA; ++q) {
             const int e = tid + 256 * q, rr = e % TS_CH, i = e / TS_CH, col = i0 + i;
             const double *ac = col < m1 ? A + lda * col : A2 + lda * (col - m1);
-            pa[q] = (rr < nr && i < mi) ? ac[r0 + rr] : 0.0;
+            pa[q] = (rr < nr && i < mi && col >= lo) ? ac[r0 + rr] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
@@ -1861,7 +1894,7 @@ int tsgemm_tn_slices(int64_t n, int m, int p) {
 
 void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A1, int m2,
                                const double *A2, int p, int64_t lda, const double *B, int64_t ldb,
-                               double *part) {
+                               double *part, const double *skip1) {
     const int m = m1 + m2;
     const int ns = tsgemm_tn_slices(n, m, p);
     if (ns == 0) return;
@@ -1869,16 +1902,16 @@ void launch_tsgemm_tn2_partial(hipStream_t s, int64_t n, int m1, const double *A
     const dim3 grid(ns, (m + 31) / 32);
     if (!A2) A2 = A1;
     if (p <= 16)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<1>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<1>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part, skip1);
     else if (p <= 32)
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<2>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part, skip1);
     else
-        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part);
+        hipLaunchKernelGGL(tsgemm_tn_partial_kernel<4>, grid, dim3(256), 0, s, n, m, p, A1, m1, A2, lda, B, ldb, rows, part, skip1);
 }
 
 void launch_tsgemm_tn_partial(hipStream_t s, int64_t n, int m, int p, const double *A,
                               int64_t lda, const double *B, int64_t ldb, double *part) {
-    launch_tsgemm_tn2_partial(s, n, m, A, 0, nullptr, p, lda, B, ldb, part);
+    launch_tsgemm_tn2_partial(s, n, m, A, 0, nullptr, p, lda, B, ldb, part, nullptr);
 }
 
 void launch_tsgemm_tn_reduce(hipStream_t s, int64_t n, int m, int p, const double *part,
@@ -1911,7 +1944,8 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
                                                         const double *__restrict__ A1, int64_t lda,
                                                         const double *__restrict__ C, int64_t ldc,
                                                         double beta, double *out0, double *out1,
-                                                        int64_t ldo) {
+                                                        int64_t ldo, const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;   // the caller's flag: this product is not needed
     // blockIdx.y selects the product: (A0, out0) or (A1, out1), same C
     const double *__restrict__ A = blockIdx.y ? A1 : A0;
     double *out = blockIdx.y ? out1 : out0;
@@ -1969,7 +2003,7 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
 
 void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A0,
                        const double *A1, int64_t lda, const double *C, int64_t ldc, double beta,
-                       double *out0, double *out1, int64_t ldo) {
+                       double *out0, double *out1, int64_t ldo, const double *skip) {
     if (n == 0 || p == 0) return;
     const dim3 grid((unsigned)((n + 63) / 64), A1 ? 2 : 1);
     for (int j0 = 0; j0 < p; j0 += 32) {
@@ -1979,17 +2013,17 @@ void launch_tsgemm_nn2(hipStream_t s, int64_t n, int m, int p, double alpha, con
         double *oj1 = out1 ? out1 + ldo * j0 : oj0;
         if (pj <= 16)
             hipLaunchKernelGGL(tsgemm_nn_kernel<1>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,
-                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);
+                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo, skip);
         else
             hipLaunchKernelGGL(tsgemm_nn_kernel<2>, grid, dim3(256), 0, s, n, m, pj, alpha, A0,
-                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo);
+                               A1 ? A1 : A0, lda, Cj, ldc, beta, oj0, oj1, ldo, skip);
     }
 }
 
 void launch_tsgemm_nn(hipStream_t s, int64_t n, int m, int p, double alpha, const double *A,
                       int64_t lda, const double *C, int64_t ldc, double beta, double *out,
-                      int64_t ldo) {
-    launch_tsgemm_nn2(s, n, m, p, alpha, A, nullptr, lda, C, ldc, beta, out, nullptr, ldo);
+                      int64_t ldo, const double *skip) {
+    launch_tsgemm_nn2(s, n, m, p, alpha, A, nullptr, lda, C, ldc, beta, out, nullptr, ldo, skip);
 }
 
 }  // namespace stda
