@@ -553,7 +553,7 @@ def run_serve(args):
             errs = {}
             for j in jobs:
                 if j.error:
-                    key = f"{j.kind}: {j.error[:60]}"
+                    key = f"{j.kind}: {j.error[:110]}"
                     errs[key] = errs.get(key, 0) + 1
             slow = sorted(jobs, key=lambda j: -j.latency)[:3]
             sweep.append({"rate": rate, "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,

@@ -672,9 +672,7 @@ int chunk_vectors(stda_hip_ctx *c) {
     if (c->nb_chunk > 0) return c->nb_chunk;
     size_t per = sizeof(double) * ((size_t)2 * c->npair + (size_t)2 * c->nao * c->no +
                                    (size_t)2 * c->nao * c->nao + c->ncent);
-    size_t freeb = 0, total = 0;
-    (void)hipMemGetInfo(&freeb, &total);
-    size_t budget = std::min<size_t>(freeb / 2, (size_t)24 << 30);
+    size_t budget = std::min<size_t>(device_available_bytes() / 2, (size_t)24 << 30);
     int nb = (int)std::max<size_t>(1, budget / per);
     c->nb_chunk = std::min(nb, 64);
     return c->nb_chunk;
@@ -2353,18 +2351,23 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         // the space that resolves the poles near w must be kept whole, so the
         // space is sized not to restart in practice (a 60 000-CSF molecule
         // with 200 poles below w converges at ~600 vectors): V, P = (A+B)V and
-        // M = (A-B)V take 24 n bytes per vector, given up to an eighth of the
-        // free HBM (at most 32 GiB) and 2400 vectors.
-        size_t freeb = 0, totalb = 0;
-        (void)hipMemGetInfo(&freeb, &totalb);
-        const size_t budget = std::min<size_t>(freeb / 8, (size_t)32 << 30);
+        // M = (A-B)V take 24 n bytes per vector, allowed up to an eighth of
+        // the available HBM beyond the runtime's reserve (at most 32 GiB) and
+        // 2400 vectors.  They are allocated as the space grows (below).
+        const size_t budget = std::min<size_t>(device_available_bytes() / 8, (size_t)32 << 30);
         const int need = 3 + 4 * nrhs;   // initial block, kept solutions, one expansion
         int64_t md = std::min<int64_t>((int64_t)(budget / (24.0 * (double)n)), 2400);
         int maxdim = (int)std::min<int64_t>(n, std::max<int64_t>(md, need));
         // STDA_HIP_LRESP_MAXDIM caps the subspace (tests use it to force restarts)
         if (const char *e = getenv("STDA_HIP_LRESP_MAXDIM"))
             if (atoi(e) > 0) maxdim = (int)std::min<int64_t>(maxdim, std::max(atoi(e), need));
-        DBuf<double> V((size_t)n * maxdim), P((size_t)n * maxdim), M((size_t)n * maxdim), tmp;
+        // V, P and M hold `cap` columns and grow (doubling, up to maxdim) when
+        // the space needs more, so that a solve holds the memory it uses and
+        // concurrent solves share the device; when the device cannot hold
+        // more (STDA_HIP_ERR_NOMEM) the space stops growing and restarts at
+        // `limit` = its current size instead
+        int cap = (int)std::min<int64_t>(maxdim, std::max(2 * need, 128)), limit = maxdim;
+        DBuf<double> V((size_t)n * cap), P((size_t)n * cap), M((size_t)n * cap), tmp;
         DBuf<double> R((size_t)n * 2 * nrhs), XP((size_t)n * nrhs), nrm(2 * nrhs), Yd;
         // initial space: the right-hand sides preconditioned by the diagonal
         STDA_HIP_CHECK(hipMemcpyAsync(V.get(), rhs.get(), (size_t)n * 3 * 8, hipMemcpyDeviceToDevice, s));
@@ -2393,6 +2396,27 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
         std::vector<double> onev(3, 1.0);
         DBuf<double> oned;
         oned.upload(onev.data(), 3, s);
+        auto grow = [&](int want, int keep) {   // room for `want` columns, the first `keep` kept
+            int nc = cap;
+            while (nc < want && nc < maxdim) nc = (int)std::min<int64_t>(maxdim, 2 * (int64_t)nc);
+            if (nc <= cap) return;
+            try {
+                DBuf<double> nV((size_t)n * nc), nP((size_t)n * nc), nM((size_t)n * nc);
+                const size_t bytes = (size_t)n * keep * 8;
+                if (bytes) {
+                    STDA_HIP_CHECK(hipMemcpyAsync(nV.get(), V.get(), bytes, hipMemcpyDeviceToDevice, s));
+                    STDA_HIP_CHECK(hipMemcpyAsync(nP.get(), P.get(), bytes, hipMemcpyDeviceToDevice, s));
+                    STDA_HIP_CHECK(hipMemcpyAsync(nM.get(), M.get(), bytes, hipMemcpyDeviceToDevice, s));
+                }
+                V = std::move(nV);
+                P = std::move(nP);
+                M = std::move(nM);
+                cap = nc;
+            } catch (const Error &e) {
+                if (e.status != STDA_HIP_ERR_NOMEM) throw;
+                limit = cap;
+            }
+        };
         for (int it = 0; it < 500 && !done; ++it) {
             if (mnew > 0) {   // operator images and reduced matrices of the new block
                 apply_ops(c, OP_APB, OP_AMB, mnew, V.get() + n * m, n, P.get() + n * m, n,
@@ -2486,7 +2510,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                 else ++since_best;
             }
             const bool switch64 = fp32 && ((un.empty() && tol32 > tol) || since_best >= 8 ||
-                                           (m + (int)un.size() > (3 * maxdim) / 4 && m < n));
+                                           (m + (int)un.size() > (3 * limit) / 4 && m < n));
             if (switch64) {
                 // FP64 images of the whole space; V^T r is exact already
                 fp32 = false;
@@ -2514,7 +2538,8 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
             for (int q = 0; q < nun; ++q)
                 if (un[q] != q)
                     STDA_HIP_CHECK(hipMemcpyAsync(R.get() + n * q, R.get() + n * un[q], n * 8, hipMemcpyDeviceToDevice, s));
-            if (m + nun > maxdim) {
+            if (m + nun > cap) grow(m + nun, m);
+            if (m + nun > limit) {
                 // Restart within half the space: the initial block, the current
                 // X+Y and X-Y of every right-hand side, then the space's RPA
                 // pairs (X+Y = L z, X-Y ~ L^-T z) from the lowest up to 1.25
@@ -2549,7 +2574,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                     e[i] = 0.0;
                 }
                 for (int col = 0; col < 2 * nrhs; ++col) add_col(coef.data() + (size_t)m * col);
-                const int room = std::max(0, (maxdim / 2 - m0 - 2 * nrhs) / 2);
+                const int room = std::max(0, (limit / 2 - m0 - 2 * nrhs) / 2);
                 for (int q = 0, added = 0; q < m && added < room && wmax > 0.0; ++q) {
                     if (!(lam[q] > 0.0) || lam[q] > 1.5625 * wmax * wmax) break;
                     add_col(W.data() + (size_t)m * q);
@@ -2578,7 +2603,7 @@ int stda_hip_lresp(stda_hip_ctx *c, int32_t nfreq, const double *omega, double t
                 Br.swap(br);
                 m = r;
                 m0 = std::min(m0, r);
-                if (m + nun > maxdim) nun = maxdim - m;
+                if (m + nun > limit) nun = limit - m;
             }
             if (nun <= 0) throw Error(STDA_HIP_ERR_NOT_CONVERGED, "response subspace exhausted");
             STDA_HIP_CHECK(hipMemcpyAsync(V.get() + n * m, R.get(), (size_t)n * nun * 8, hipMemcpyDeviceToDevice, s));

@@ -5,10 +5,12 @@
 
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 #include <utility>
 #include <vector>
 
+#include "../../include/stda_hip.h"
 #include "common.h"
 
 namespace stda {
@@ -92,6 +94,64 @@ inline BlockCache &block_cache() {
 
 inline void block_cache_drain(hipStream_t s) { block_cache().drain(s); }
 
+// Device memory left free for the runtime's own allocations: the scratch of
+// a kernel with a private segment is allocated when it is dispatched,
+// outside the stream-ordered pool, and a dispatch that cannot get it aborts
+// the process.  The pool keeps released blocks (its release threshold is
+// unlimited), so a new block that would leave less than this free first
+// returns the cached blocks and the pool's unused memory to the device, and
+// fails with STDA_HIP_ERR_NOMEM if that is not enough.
+constexpr size_t kDeviceReserveBytes = size_t(4) << 30;
+
+// Device memory a new buffer can draw on beyond the reserve: what the device
+// has free plus what the pool holds unused (hipMemGetInfo counts all of the
+// pool's memory as used, so after busy phases it alone understates what is
+// there).  For sizing work buffers to the memory at hand.
+inline size_t device_available_bytes() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    uint64_t resv = 0, used = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &resv) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess &&
+        resv > used)
+        fr += (size_t)(resv - used);
+    (void)hipGetLastError();
+    return fr > kDeviceReserveBytes ? fr - kDeviceReserveBytes : 0;
+}
+
+inline void *device_alloc(size_t bytes, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) cs = hipStreamCaptureStatusNone;
+    size_t fr = 0, tot = 0;
+    if (cs == hipStreamCaptureStatusNone && hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        fr < bytes + kDeviceReserveBytes) {
+        block_cache().drain_all();
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        // the released blocks return to the pool in stream order: wait for
+        // them, then hand every unused byte of the pool back
+        if (hipDeviceSynchronize() == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+            (void)hipMemPoolTrimTo(pool, 0);
+        (void)hipGetLastError();
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDeviceReserveBytes)
+            throw Error(STDA_HIP_ERR_NOMEM,
+                        "device allocation of " + std::to_string(bytes >> 20) +
+                            " MiB would leave less than the runtime's reserve free (" +
+                            std::to_string(fr >> 20) + " MiB free)");
+    }
+    void *q = nullptr;
+    if (hipMallocAsync(&q, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        block_cache().drain_all();
+        STDA_HIP_CHECK(hipMallocAsync(&q, bytes, s));
+    }
+    return q;
+}
+
 // Stream that orders the device allocations of the calling thread.  Every
 // entry point sets it to its context's stream (AllocScope), so that buffers
 // come from the device's stream-ordered pool (hipMallocAsync/hipFreeAsync)
@@ -143,11 +203,7 @@ class DBuf {
         void *q = block_cache().take(n * sizeof(T), s_, &got);
         if (!q) {
             got = n * sizeof(T);
-            if (hipMallocAsync(&q, got, s_) != hipSuccess) {
-                (void)hipGetLastError();
-                block_cache().drain_all();
-                STDA_HIP_CHECK(hipMallocAsync(&q, got, s_));
-            }
+            q = device_alloc(got, s_);
         }
         p_ = static_cast<T *>(q);
         n_ = n;
