@@ -3,6 +3,8 @@
 
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -100,8 +102,20 @@ inline void block_cache_drain(hipStream_t s) { block_cache().drain(s); }
 // the process.  The pool keeps released blocks (its release threshold is
 // unlimited), so a new block that would leave less than this free first
 // returns the cached blocks and the pool's unused memory to the device, and
-// fails with STDA_HIP_ERR_NOMEM if that is not enough.
-constexpr size_t kDeviceReserveBytes = size_t(4) << 30;
+// fails with STDA_HIP_ERR_NOMEM if that is not enough.  4 GiB unless the
+// environment's STDA_HIP_DEVICE_RESERVE_MB (read once per process) says
+// otherwise, e.g. for a GPU shared with other processes.
+inline size_t device_reserve_bytes() {
+    static const size_t r = [] {
+        const char *e = std::getenv("STDA_HIP_DEVICE_RESERVE_MB");
+        if (e && *e) {
+            const double mb = std::strtod(e, nullptr);
+            if (mb >= 0.0) return (size_t)std::min(mb * 1048576.0, 1.0e18);
+        }
+        return size_t(4) << 30;
+    }();
+    return r;
+}
 
 // Device memory a new buffer can draw on beyond the reserve: what the device
 // has free plus what the pool holds unused (hipMemGetInfo counts all of the
@@ -119,15 +133,17 @@ inline size_t device_available_bytes() {
         resv > used)
         fr += (size_t)(resv - used);
     (void)hipGetLastError();
-    return fr > kDeviceReserveBytes ? fr - kDeviceReserveBytes : 0;
+    const size_t rs = device_reserve_bytes();
+    return fr > rs ? fr - rs : 0;
 }
 
 inline void *device_alloc(size_t bytes, hipStream_t s) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess) cs = hipStreamCaptureStatusNone;
     size_t fr = 0, tot = 0;
+    const size_t rs = device_reserve_bytes();
     if (cs == hipStreamCaptureStatusNone && hipMemGetInfo(&fr, &tot) == hipSuccess &&
-        fr < bytes + kDeviceReserveBytes) {
+        fr < bytes + rs) {
         block_cache().drain_all();
         int dev = 0;
         hipMemPool_t pool = nullptr;
@@ -137,7 +153,7 @@ inline void *device_alloc(size_t bytes, hipStream_t s) {
             hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
             (void)hipMemPoolTrimTo(pool, 0);
         (void)hipGetLastError();
-        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kDeviceReserveBytes)
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + rs)
             throw Error(STDA_HIP_ERR_NOMEM,
                         "device allocation of " + std::to_string(bytes >> 20) +
                             " MiB would leave less than the runtime's reserve free (" +

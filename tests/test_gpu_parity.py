@@ -204,6 +204,40 @@ def test_lresp_restarts(cluster30, monkeypatch):
         np.testing.assert_allclose(a, ar, rtol=0, atol=1e-6 * np.abs(ar).max())
 
 
+def test_device_reserve_nomem():
+    """A device allocation that would leave less than the runtime's reserve
+    free fails with STDA_HIP_ERR_NOMEM instead of taking the device's last
+    memory.  The reserve is set beyond the device's size for a fresh
+    process (it is read once per process), so every new device buffer
+    refuses cleanly: the call returns status 4 and the process ends normally."""
+    import subprocess
+    import textwrap
+    say("device reserve: NOMEM in a child process")
+    code = textwrap.dedent("""
+        import sys
+        sys.path.insert(0, sys.argv[1])
+        from stda_amd import synth
+        from stda_amd.prep import prepare
+        from stda_amd.core import StdaHip, StdaHipError
+        mol, ints = synth.cluster(12, seed=7, basis="minimal")
+        ci = prepare(mol, 7.0, 0.25, ints)
+        try:
+            with StdaHip(0) as h:
+                h.setup(ci, 7.0, 0.25)
+                h.solve_tda(5)
+            print("status 0")
+        except StdaHipError as e:
+            print("status", e.status, str(e)[:160])
+    """)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, STDA_HIP_DEVICE_RESERVE_MB="1e12")
+    r = subprocess.run([sys.executable, "-c", code, root], env=env, capture_output=True,
+                       text=True, timeout=240, cwd=root)
+    say(r.stdout.strip()[-200:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "status 4" in r.stdout, r.stdout
+
+
 @pytest.mark.parametrize("case", [1, 2, 3])
 def test_water_beta(water, case):
     """SHG beta and beta_HRS (std2 -resp 2) against the reference's BETA_TEST files (its
