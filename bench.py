@@ -167,7 +167,7 @@ def run_single(args):
         h.select()
         h.solve_tda(p["nroots"], velcorr=True, vectors=False)
         log(f"# warmup {w + 1}/{args.warmup} done")
-    lat, stats = [], []
+    lat, stats, hbm = [], [], []
     tstart = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
@@ -177,6 +177,7 @@ def run_single(args):
         stats.append({k: h.kernel_stats(k) for k in ("pt2_rows", "pt2_rows_f32",
                                                        "pt2_rows_recheck", "pt2_cols",
                                                        "ao_matvec", "ao_matvec_f32")})
+        hbm.append({k: h.kernel_stats(k) for k in ("tsgemm_tn", "tsgemm_nn")})
         log(f"# step {s + 1}/{args.steps}: {lat[-1]:.3f}s nci={nci} timings={sol.timings_ms}")
     total = time.perf_counter() - tstart
     root_err = None
@@ -232,6 +233,8 @@ def run_single(args):
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
                         "tflops": round(v["work"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else 0}
                     for k, v in last.items() if v["launches"]},
+        # the solver's tall-skinny products (HBM-bound; one launch in eight timed)
+        "hbm_kernels": plain_rates(hbm[-1]),
         "cpu_baseline": cpu,
     }
 
