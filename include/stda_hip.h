@@ -45,7 +45,8 @@ enum {
     STDA_HIP_ERR_ARG = 1,          /* invalid argument / dimension */
     STDA_HIP_ERR_NO_CSF = 2,       /* 'no CSF, increase energy threshold (-e option)' stda.f:316,644 */
     STDA_HIP_ERR_DEVICE = 3,       /* HIP runtime or library failure */
-    STDA_HIP_ERR_NOMEM = 4,        /* device allocation failed */
+    STDA_HIP_ERR_NOMEM = 4,        /* device allocation failed, or would leave less than the
+                                      runtime's reserve free (4 GiB, STDA_HIP_DEVICE_RESERVE_MB) */
     STDA_HIP_ERR_STATE = 5,        /* call out of order (e.g. solve before setup) */
     STDA_HIP_ERR_NOT_CONVERGED = 6,/* iterative solver hit its iteration limit */
     STDA_HIP_ERR_AMB_NOT_PD = 7,   /* 'matrix power impossible' srpapack.f:229 (A-B not positive definite) */
