@@ -1932,12 +1932,15 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
 // out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32, on the
 // matrix cores (v_mfma_f64_16x16x4f64).  A work-group covers 64 rows, wave w
 // the 16-row block 16w..16w+15, and NB column blocks of 16.  C is staged
-// through LDS in 128-row chunks; each lane loads its 32 A values of a chunk
-// (row 16w + c of the block, inner indices g + 4u: lane l = 16g + c supplies
-// the A value of row c, inner index g, of MFMA step u) before the multiplies,
-// so 32 loads per lane are in flight.  The products take the C values as the
-// first operand (the transposed block), so register i of accumulator j holds
-// row 16w + c, column 16j + 4i + g.  beta == 0 never reads out.
+// through LDS in chunks of NN_CH = 64 rows; each lane loads its 16 A values
+// of a chunk (row 16w + c of the block, inner indices g + 4u: lane l = 16g + c
+// supplies the A value of row c, inner index g, of MFMA step u) before the
+// multiplies, so 16 loads per lane are in flight.  (64-row chunks halve the
+// LDS and registers of 128-row ones, so twice the work-groups share a CU:
+// 4.15 against 3.36 TB/s at n = 1.2 M.)  The products take the C values as
+// the first operand (the transposed block), so register i of accumulator j
+// holds row 16w + c, column 16j + 4i + g.  beta == 0 never reads out.
+constexpr int NN_CH = 64;
 template <int NB>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A0,
@@ -1950,7 +1953,7 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
     const double *__restrict__ A = blockIdx.y ? A1 : A0;
     double *out = blockIdx.y ? out1 : out0;
     constexpr int PJ = 16 * NB;
-    __shared__ double cs[128 * PJ];
+    __shared__ double cs[NN_CH * PJ];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = lane & 15, g = lane >> 4;
@@ -1960,23 +1963,23 @@ __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p,
     v4d acc[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int i0 = 0; i0 < m; i0 += 128) {
-        const int mi = min(128, m - i0);
+    for (int i0 = 0; i0 < m; i0 += NN_CH) {
+        const int mi = min(NN_CH, m - i0);
         __syncthreads();
-        for (int e = tid; e < 128 * PJ; e += 256) {
+        for (int e = tid; e < NN_CH * PJ; e += 256) {
             const int i = e / PJ, j = e % PJ;
             cs[e] = (i < mi && j < p) ? C[(i0 + i) + ldc * j] : 0.0;
         }
-        double a[32];
+        double a[NN_CH / 4];
         const double *ap = A + (live ? row : 0) + lda * (int64_t)(i0 + g);
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
+        for (int u = 0; u < NN_CH / 4; ++u) {
             a[u] = (live && g + 4 * u < mi) ? *ap : 0.0;
             ap += 4 * lda;
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 32; ++u) {
+        for (int u = 0; u < NN_CH / 4; ++u) {
             if (4 * u < mi) {
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
