@@ -1729,14 +1729,16 @@ namespace stda {
 // C (m x p) = A^T B, A (n x m), B (n x p): work-group (s, ib) reduces the
 // rows of slice s for the 32 columns ib*32.. of A into part[s*m*p + i + m*j],
 // on the matrix cores (v_mfma_f64_16x16x4f64, the rows as the reduction
-// index).  Rows are staged through LDS in 64-row chunks with coalesced loads,
-// the next chunk's loads in flight (registers) while the current one is
-// multiplied; the chunk's 16 steps of 4 rows are dealt to the four waves
-// (step s to wave s % 4).  Lane l = 16g + c supplies A^T[16mb + c][row 4s + g]
+// index).  Rows are staged through LDS in TS_CH = 32-row chunks with
+// coalesced loads, the next chunk's loads in flight (registers) while the
+// current one is multiplied; the chunk's 8 steps of 4 rows are dealt to the
+// four waves (step s to wave s % 4).  (32-row chunks leave more work-groups
+// per CU than 64-row ones: 3.39 against 2.71 TB/s at n = 1.2 M.)  Lane
+// l = 16g + c supplies A^T[16mb + c][row 4s + g]
 // and B[row 4s + g][16pb + c]; register i of accumulator (mb, pb) holds
 // C(16mb + 4i + g, 16pb + c).  The wave partials are added in wave order at
 // the end.  tsgemm_reduce_kernel then sums the slices in slice order.
-constexpr int TS_ROWS = 32, TS_CH = 64;
+constexpr int TS_ROWS = 32, TS_CH = 32;
 
 template <int PB>
 __global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
