@@ -1934,15 +1934,15 @@ void launch_tsgemm_tn(hipStream_t s, int64_t n, int m, int p, const double *A, i
 // out (n x p) = alpha A C + beta out, A (n x m), C (m x p), p <= 32, on the
 // matrix cores (v_mfma_f64_16x16x4f64).  A work-group covers 64 rows, wave w
 // the 16-row block 16w..16w+15, and NB column blocks of 16.  C is staged
-// through LDS in chunks of NN_CH = 64 rows; each lane loads its 16 A values
+// through LDS in chunks of NN_CH = 32 rows; each lane loads its 8 A values
 // of a chunk (row 16w + c of the block, inner indices g + 4u: lane l = 16g + c
 // supplies the A value of row c, inner index g, of MFMA step u) before the
-// multiplies, so 16 loads per lane are in flight.  (64-row chunks halve the
-// LDS and registers of 128-row ones, so twice the work-groups share a CU:
-// 4.15 against 3.36 TB/s at n = 1.2 M.)  The products take the C values as
-// the first operand (the transposed block), so register i of accumulator j
-// holds row 16w + c, column 16j + 4i + g.  beta == 0 never reads out.
-constexpr int NN_CH = 64;
+// multiplies.  (Smaller chunks take less LDS and fewer registers, so more
+// work-groups share a CU: 4.50 TB/s at n = 1.2 M against 4.15 with 64-row
+// and 3.36 with 128-row chunks.)  The products take the C values as the
+// first operand (the transposed block), so register i of accumulator j holds
+// row 16w + c, column 16j + 4i + g.  beta == 0 never reads out.
+constexpr int NN_CH = 32;
 template <int NB>
 __global__ __launch_bounds__(256) void tsgemm_nn_kernel(int64_t n, int m, int p, double alpha,
                                                         const double *__restrict__ A0,
