@@ -1740,8 +1740,10 @@ namespace stda {
 // the end.  tsgemm_reduce_kernel then sums the slices in slice order.
 constexpr int TS_ROWS = 32, TS_CH = 32;
 
+// (five waves per SIMD: the PB = 2 form then fits 86 registers without
+// spilling; 3.47 against 3.39 TB/s at n = 1.2 M)
 template <int PB>
-__global__ __launch_bounds__(256) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void tsgemm_tn_partial_kernel(int64_t n, int m, int p,
                                                                 const double *__restrict__ A,
                                                                 int m1,
                                                                 const double *__restrict__ A2,
